@@ -1,0 +1,234 @@
+"""ctypes binding of libfpsi_gpu.so (include/fpsi.h).
+
+The names follow the flowPsi facts each call produces (iflux, vflux, src,
+residual, fjp/fjm, fluid_D/B, dtau, dfluid).  There is no CPU fallback: a
+missing library or a failing HIP call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import grids as _grids
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfpsi_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+# enum values of include/fpsi.h
+FLUX = {"hllc": 0, "roe": 1}
+LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
+PRECISION = {"fp64": 0, "fp32": 1}
+SOLVER = {"sgs": 0}
+ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
+
+# every entry point declared in include/fpsi.h
+EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "fpsi_set_params",
+           "fpsi_set_state", "fpsi_get_state", "fpsi_residual", "fpsi_gradients", "fpsi_assemble",
+           "fpsi_get_system", "fpsi_linear_solve", "fpsi_update", "fpsi_newton_step",
+           "fpsi_synchronize"]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nc", C.c_int), ("nf", C.c_int), ("cl", _ip), ("cr", _ip), ("btype", _ip),
+                ("pface", _ip), ("n", _dp), ("area", _dp), ("fc", _dp), ("vol", _dp), ("cc", _dp),
+                ("pshift", _dp)]
+
+
+class Params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("Rtilde", C.c_double), ("Pambient", C.c_double),
+                ("flux", C.c_int), ("limiter", C.c_int), ("Kl", C.c_double), ("viscous", C.c_int),
+                ("suth", C.c_double * 6), ("Twall", C.c_double), ("pg0", C.c_double),
+                ("T0", C.c_double), ("u0", C.c_double * 3), ("cflmax", C.c_double),
+                ("dtmax", C.c_double), ("urelax", C.c_double), ("linear_solver", C.c_int),
+                ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double)]
+
+
+class Norms(C.Structure):
+    _fields_ = [("rtrms", C.c_double), ("etrms", C.c_double), ("mtrms", C.c_double),
+                ("rtmax", C.c_double), ("etmax", C.c_double), ("mtmax", C.c_double),
+                ("ccmaxr", C.c_double * 3), ("ccmaxe", C.c_double * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in ("rtrms", "etrms", "mtrms", "rtmax", "etmax", "mtmax")}
+
+
+class Info(C.Structure):
+    _fields_ = [("nlevels", C.c_int), ("max_level_width", C.c_int), ("nnz_blocks", C.c_int),
+                ("device_bytes", C.c_double)]
+
+
+class FpsiError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FpsiError(f"{path} is missing: build it with flowpsi_amd.build.build()")
+        L = C.CDLL(path)
+        L.fpsi_last_error.restype = C.c_char_p
+        L.fpsi_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(GridDesc)]
+        for name in EXPORTS:
+            if name not in ("fpsi_last_error", "fpsi_destroy"):
+                getattr(L, name).restype = C.c_int
+        L.fpsi_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def make_params(prm: dict) -> Params:
+    s = Params()
+    s.gamma = prm.get("gamma", 1.4)
+    s.Rtilde = prm.get("Rt", 287.0)
+    s.Pambient = prm.get("Pambient", 0.0)
+    s.flux = FLUX[prm.get("flux", "hllc")]
+    s.limiter = LIMITER[prm.get("limiter", "venkatakrishnan")]
+    s.Kl = prm.get("Kl", 1.0)
+    s.viscous = int(prm.get("viscous", 0))
+    s.suth[:] = prm.get("suth", (1.458e-6, 1.5, 110.4, 2.495e-3, 1.5, 194.0))
+    s.Twall = prm.get("Twall", 300.0)
+    s.pg0 = prm.get("pg0", 101325.0)
+    s.T0 = prm.get("T0", 300.0)
+    s.u0[:] = prm.get("u0", (0.0, 0.0, 0.0))
+    s.cflmax = prm.get("cflmax", 0.0)
+    s.dtmax = prm.get("dtmax", 1e30)
+    s.urelax = prm.get("urelax", 1e30)
+    s.linear_solver = SOLVER[prm.get("solver", "sgs")]
+    s.sgs_iter = prm.get("sgs_iter", 5)
+    s.precision = PRECISION[prm.get("precision", "fp64")]
+    s.minPg = prm.get("minPg", -s.Pambient)
+    return s
+
+
+class Solver:
+    """One device context for one grid (the drop-in for the Newton-step rules)."""
+
+    def __init__(self, grid: "_grids.Grid", prm: dict | None = None):
+        self.L = load()
+        self.grid = grid
+        self._keep = dict(
+            cl=np.ascontiguousarray(grid.cl, np.int32), cr=np.ascontiguousarray(grid.cr, np.int32),
+            btype=np.ascontiguousarray(grid.btype, np.int32),
+            pface=np.ascontiguousarray(grid.pface, np.int32),
+            n=np.ascontiguousarray(grid.n, np.float64), area=np.ascontiguousarray(grid.area, np.float64),
+            fc=np.ascontiguousarray(grid.fc, np.float64), vol=np.ascontiguousarray(grid.vol, np.float64),
+            cc=np.ascontiguousarray(grid.cc, np.float64),
+            pshift=np.ascontiguousarray(grid.pshift, np.float64))
+        k = self._keep
+        gd = GridDesc(grid.nc, grid.nf, _ptr(k["cl"], C.c_int), _ptr(k["cr"], C.c_int),
+                      _ptr(k["btype"], C.c_int), _ptr(k["pface"], C.c_int), _ptr(k["n"]),
+                      _ptr(k["area"]), _ptr(k["fc"]), _ptr(k["vol"]), _ptr(k["cc"]), _ptr(k["pshift"]))
+        h = C.c_void_p()
+        rc = self.L.fpsi_create(C.byref(h), C.byref(gd))
+        self.h = h
+        if rc != 0:
+            msg = self.L.fpsi_last_error(h).decode() if h.value else "fpsi_create failed"
+            self.close()
+            raise FpsiError(f"fpsi_create: {ERRORS.get(rc, rc)}: {msg}")
+        self.prm = None
+        if prm is not None:
+            self.set_params(prm)
+
+    # -- plumbing
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise FpsiError(f"{what}: {ERRORS.get(rc, rc)}: {self.L.fpsi_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.fpsi_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def fj(self):
+        return np.float32 if self.prm and self.prm.get("precision") == "fp32" else np.float64
+
+    def info(self) -> dict:
+        i = Info()
+        self._chk(self.L.fpsi_get_info(self.h, C.byref(i)), "fpsi_get_info")
+        return dict(nlevels=i.nlevels, max_level_width=i.max_level_width, nnz_blocks=i.nnz_blocks,
+                    device_bytes=i.device_bytes)
+
+    # -- rules
+    def set_params(self, prm: dict):
+        self.prm = dict(prm)
+        p = make_params(prm)
+        self._chk(self.L.fpsi_set_params(self.h, C.byref(p)), "fpsi_set_params")
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        assert q.shape == (5, self.grid.nc)
+        self._chk(self.L.fpsi_set_state(self.h, _ptr(q)), "fpsi_set_state")
+
+    def get_state(self):
+        q = np.zeros((5, self.grid.nc))
+        self._chk(self.L.fpsi_get_state(self.h, _ptr(q)), "fpsi_get_state")
+        return q
+
+    def residual(self, want_faces=True):
+        nc, nf = self.grid.nc, self.grid.nf
+        src = np.zeros((5, nc))
+        iflux = np.zeros((5, nf)) if want_faces else None
+        vflux = np.zeros((4, nf)) if want_faces else None
+        nrm = Norms()
+        self._chk(self.L.fpsi_residual(self.h, _ptr(src), _ptr(iflux) if want_faces else None,
+                                       _ptr(vflux) if want_faces else None, C.byref(nrm)),
+                  "fpsi_residual")
+        return src, iflux, vflux, nrm
+
+    def gradients(self):
+        nc = self.grid.nc
+        grad = np.zeros((15, nc)); phi = np.zeros((5, nc))
+        self._chk(self.L.fpsi_gradients(self.h, _ptr(grad), _ptr(phi)), "fpsi_gradients")
+        return grad, phi
+
+    def assemble(self):
+        self._chk(self.L.fpsi_assemble(self.h), "fpsi_assemble")
+
+    def get_system(self):
+        nc, nf = self.grid.nc, self.grid.nf
+        fj = self.fj
+        fjp = np.zeros((nf, 25), fj); fjm = np.zeros((nf, 25), fj)
+        D = np.zeros((nc, 25), fj); piv = np.zeros((nc, 5), np.int32); B = np.zeros((nc, 5), fj)
+        dtau = np.zeros(nc)
+        self._chk(self.L.fpsi_get_system(self.h, fjp.ctypes.data_as(C.c_void_p),
+                                         fjm.ctypes.data_as(C.c_void_p), D.ctypes.data_as(C.c_void_p),
+                                         _ptr(piv, C.c_int), B.ctypes.data_as(C.c_void_p), _ptr(dtau)),
+                  "fpsi_get_system")
+        return dict(fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, dtau=dtau)
+
+    def linear_solve(self, nsweeps):
+        dq = np.zeros((self.grid.nc, 5))
+        self._chk(self.L.fpsi_linear_solve(self.h, int(nsweeps), _ptr(dq)), "fpsi_linear_solve")
+        return dq
+
+    def update(self):
+        self._chk(self.L.fpsi_update(self.h), "fpsi_update")
+
+    def newton(self, nsteps):
+        hist = (Norms * max(nsteps, 1))()
+        self._chk(self.L.fpsi_newton_step(self.h, int(nsteps), hist), "fpsi_newton_step")
+        return [hist[i].as_dict() for i in range(nsteps)]
+
+    def synchronize(self):
+        self._chk(self.L.fpsi_synchronize(self.h), "fpsi_synchronize")

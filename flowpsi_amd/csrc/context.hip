@@ -1,0 +1,634 @@
+// Host side of libfpsi_gpu.so: grid setup, device buffers and the C ABI of
+// include/fpsi.h.  All numerics run in the kernels of kernels.hip; this file
+// only builds index structures, moves data and orders launches on the
+// context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fpsi.h"
+#include "kernels.h"
+
+using namespace fpsi;
+
+struct fpsi_ctx {
+  std::string err;
+  hipStream_t stream = nullptr;
+  int nc = 0, nf = 0, nfi = 0, nper = 0, nb = 0, nnz = 0;
+  std::vector<int> cperm;   // new -> old cell
+  std::vector<int> icperm;  // old -> new cell
+  std::vector<int> fperm;   // new -> old face
+  std::vector<int> ifperm;  // old -> new face
+  std::vector<int> lev_off; // level ranges in new numbering
+  int max_width = 0;
+  double vtot = 0;
+  DevGrid g{};
+  DevState s{};
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  fpsi_params prm{};
+  Prm kp{};
+  bool have_params = false, have_state = false, assembled = false;
+  int fp32 = 0;
+  size_t fj_bytes() const { return fp32 ? 4 : 8; }
+};
+
+static int fail(fpsi_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, FPSI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dalloc(fpsi_ctx *c, T **p, size_t n) {
+  void *v = nullptr;
+  size_t b = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&v, b);
+  if (e != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(v);
+  c->bytes += b;
+  *p = (T *)v;
+  return 0;
+}
+
+template <typename T>
+static int upload(fpsi_ctx *c, const T **dst, const std::vector<T> &h) {
+  T *d = nullptr;
+  int rc = dalloc(c, &d, h.size());
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(c, hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  *dst = d;
+  return 0;
+}
+
+extern "C" const char *fpsi_last_error(const fpsi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocs) hipFree(p);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
+  if (!out || !gd) return FPSI_ERR_ARG;
+  *out = nullptr;
+  fpsi_ctx *c = new fpsi_ctx();
+  const int nc = gd->nc, nf = gd->nf;
+  if (nc <= 0 || nf <= 0 || !gd->cl || !gd->cr || !gd->btype || !gd->n || !gd->area || !gd->fc ||
+      !gd->vol || !gd->cc) {
+    *out = c;
+    return fail(c, FPSI_ERR_ARG, "grid descriptor incomplete");
+  }
+  c->nc = nc;
+  c->nf = nf;
+  // ---- validate and classify faces
+  std::vector<int> kind(nf);  // 0 interior, 1 periodic, 2 boundary
+  for (int f = 0; f < nf; ++f) {
+    const int t = gd->btype[f], l = gd->cl[f], r = gd->cr[f];
+    if (l < 0 || l >= nc) { *out = c; return fail(c, FPSI_ERR_ARG, "cl out of range at face " + std::to_string(f)); }
+    if (t == FPSI_BC_INTERIOR) {
+      if (r <= l || r >= nc) { *out = c; return fail(c, FPSI_ERR_ARG, "interior face needs cl < cr < nc at face " + std::to_string(f)); }
+      kind[f] = 0;
+    } else if (t == FPSI_BC_PERIODIC) {
+      const int pf = gd->pface ? gd->pface[f] : -1;
+      if (pf < 0 || pf >= nf || gd->btype[pf] != FPSI_BC_PERIODIC || gd->pface[pf] != f || !gd->pshift) {
+        *out = c;
+        return fail(c, FPSI_ERR_ARG, "periodic face without a matching partner at face " + std::to_string(f));
+      }
+      kind[f] = 1;
+    } else if (t == FPSI_BC_FARFIELD || t == FPSI_BC_WALL || t == FPSI_BC_SYMMETRY) {
+      if (r != -1) { *out = c; return fail(c, FPSI_ERR_ARG, "boundary face must have cr = -1"); }
+      kind[f] = 2;
+    } else {
+      *out = c;
+      return fail(c, FPSI_ERR_ARG, "unsupported boundary type at face " + std::to_string(f));
+    }
+  }
+  // ---- per-cell face lists (caller order), lower / upper / boundary
+  std::vector<int> lo_off(nc + 1, 0), up_off(nc + 1, 0), bd_off(nc + 1, 0);
+  for (int f = 0; f < nf; ++f) {
+    if (kind[f] == 0) { up_off[gd->cl[f] + 1]++; lo_off[gd->cr[f] + 1]++; }
+    else if (kind[f] == 1) up_off[gd->cl[f] + 1]++;
+    else bd_off[gd->cl[f] + 1]++;
+  }
+  for (int i = 0; i < nc; ++i) { lo_off[i + 1] += lo_off[i]; up_off[i + 1] += up_off[i]; bd_off[i + 1] += bd_off[i]; }
+  std::vector<int> lo(lo_off[nc]), up(up_off[nc]), bd(bd_off[nc]);
+  {
+    std::vector<int> a(lo_off.begin(), lo_off.end() - 1), b(up_off.begin(), up_off.end() - 1),
+        d(bd_off.begin(), bd_off.end() - 1);
+    for (int f = 0; f < nf; ++f) {
+      if (kind[f] == 0) { up[b[gd->cl[f]]++] = f; lo[a[gd->cr[f]]++] = f; }
+      else if (kind[f] == 1) up[b[gd->cl[f]]++] = f;
+      else bd[d[gd->cl[f]]++] = f;
+    }
+  }
+  // ---- Gauss-Seidel levels: longest path in the cl -> cr graph
+  std::vector<int> level(nc, 0);
+  int nlev = 0;
+  for (int i = 0; i < nc; ++i) {
+    int L = 0;
+    for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) L = std::max(L, level[gd->cl[lo[j]]] + 1);
+    level[i] = L;
+    nlev = std::max(nlev, L + 1);
+  }
+  c->lev_off.assign(nlev + 1, 0);
+  for (int i = 0; i < nc; ++i) c->lev_off[level[i] + 1]++;
+  for (int l = 0; l < nlev; ++l) {
+    c->max_width = std::max(c->max_width, c->lev_off[l + 1]);
+    c->lev_off[l + 1] += c->lev_off[l];
+  }
+  c->cperm.resize(nc);
+  c->icperm.resize(nc);
+  {
+    std::vector<int> pos(c->lev_off.begin(), c->lev_off.end() - 1);
+    for (int i = 0; i < nc; ++i) {
+      const int ni = pos[level[i]]++;
+      c->cperm[ni] = i;
+      c->icperm[i] = ni;
+    }
+  }
+  const std::vector<int> &ic = c->icperm;
+  // ---- face order: interior by new cl, periodic by new cl, boundary by type then new cl
+  std::vector<int> fi, fp, fb;
+  for (int f = 0; f < nf; ++f) (kind[f] == 0 ? fi : kind[f] == 1 ? fp : fb).push_back(f);
+  auto by_cl = [&](int a, int b) {
+    const int ca = ic[gd->cl[a]], cb = ic[gd->cl[b]];
+    return ca != cb ? ca < cb : a < b;
+  };
+  std::stable_sort(fi.begin(), fi.end(), by_cl);
+  std::stable_sort(fp.begin(), fp.end(), by_cl);
+  std::stable_sort(fb.begin(), fb.end(), [&](int a, int b) {
+    if (gd->btype[a] != gd->btype[b]) return gd->btype[a] < gd->btype[b];
+    return by_cl(a, b);
+  });
+  c->nfi = (int)fi.size();
+  c->nper = (int)fp.size();
+  c->nb = (int)fb.size();
+  c->fperm.clear();
+  c->fperm.insert(c->fperm.end(), fi.begin(), fi.end());
+  c->fperm.insert(c->fperm.end(), fp.begin(), fp.end());
+  c->fperm.insert(c->fperm.end(), fb.begin(), fb.end());
+  c->ifperm.assign(nf, -1);
+  for (int i = 0; i < nf; ++i) c->ifperm[c->fperm[i]] = i;
+  const std::vector<int> &ifp = c->ifperm;
+  const int nfi = c->nfi, nper = c->nper;
+
+  // ---- face arrays in new order
+  std::vector<double> fnx(nf), fny(nf), fnz(nf), farea(nf), fcx(nf), fcy(nf), fcz(nf);
+  std::vector<int> fcl(nf), frc(nf), fbt(nf);
+  std::vector<double> psx(nper), psy(nper), psz(nper);
+  for (int i = 0; i < nf; ++i) {
+    const int f = c->fperm[i];
+    fnx[i] = gd->n[f]; fny[i] = gd->n[nf + f]; fnz[i] = gd->n[2 * nf + f];
+    farea[i] = gd->area[f];
+    fcx[i] = gd->fc[f]; fcy[i] = gd->fc[nf + f]; fcz[i] = gd->fc[2 * nf + f];
+    fcl[i] = ic[gd->cl[f]];
+    fbt[i] = gd->btype[f];
+    if (kind[f] == 0) frc[i] = ic[gd->cr[f]];
+    else if (kind[f] == 1) {
+      frc[i] = ic[gd->cl[gd->pface[f]]];
+      const int pi = i - nfi;
+      psx[pi] = gd->pshift[f]; psy[pi] = gd->pshift[nf + f]; psz[pi] = gd->pshift[2 * nf + f];
+    } else frc[i] = -1;
+  }
+  // ---- cell arrays in new order
+  std::vector<double> vol(nc), ccx(nc), ccy(nc), ccz(nc);
+  c->vtot = 0;
+  for (int i = 0; i < nc; ++i) c->vtot += gd->vol[i];  // caller order, as the norm join sums
+  for (int ni = 0; ni < nc; ++ni) {
+    const int i = c->cperm[ni];
+    vol[ni] = gd->vol[i];
+    ccx[ni] = gd->cc[i]; ccy[ni] = gd->cc[nc + i]; ccz[ni] = gd->cc[2 * nc + i];
+  }
+  // ---- gather lists (lower, upper, boundary), least-squares inverse
+  std::vector<int> g_off(nc + 1, 0), g_ent;
+  g_ent.reserve(lo.size() + up.size() + bd.size());
+  std::vector<double> lsq(6 * (size_t)nc);
+  for (int ni = 0; ni < nc; ++ni) {
+    const int i = c->cperm[ni];
+    double A[6] = {0, 0, 0, 0, 0, 0};
+    const double xc[3] = {gd->cc[i], gd->cc[nc + i], gd->cc[2 * nc + i]};
+    auto addA = [&](const double *x) {
+      const double dx = x[0] - xc[0], dy = x[1] - xc[1], dz = x[2] - xc[2];
+      A[0] += dx * dx; A[1] += dx * dy; A[2] += dx * dz;
+      A[3] += dy * dy; A[4] += dy * dz; A[5] += dz * dz;
+    };
+    for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) {
+      const int f = lo[j];
+      g_ent.push_back(ifp[f]);
+      const int o = gd->cl[f];
+      const double x[3] = {gd->cc[o], gd->cc[nc + o], gd->cc[2 * nc + o]};
+      addA(x);
+    }
+    for (int j = up_off[i]; j < up_off[i + 1]; ++j) {
+      const int f = up[j];
+      g_ent.push_back(~ifp[f]);
+      double x[3];
+      if (kind[f] == 0) {
+        const int o = gd->cr[f];
+        x[0] = gd->cc[o]; x[1] = gd->cc[nc + o]; x[2] = gd->cc[2 * nc + o];
+      } else {
+        const int o = gd->cl[gd->pface[f]];
+        x[0] = gd->cc[o]; x[1] = gd->cc[nc + o]; x[2] = gd->cc[2 * nc + o];
+        for (int d = 0; d < 3; ++d) x[d] += gd->pshift[d * nf + f];
+      }
+      addA(x);
+    }
+    for (int j = bd_off[i]; j < bd_off[i + 1]; ++j) {
+      const int f = bd[j];
+      g_ent.push_back(~ifp[f]);
+      const double x[3] = {gd->fc[f], gd->fc[nf + f], gd->fc[2 * nf + f]};
+      addA(x);
+    }
+    g_off[ni + 1] = (int)g_ent.size();
+    const double c00 = A[3] * A[5] - A[4] * A[4];
+    const double c01 = A[2] * A[4] - A[1] * A[5];
+    const double c02 = A[1] * A[4] - A[2] * A[3];
+    const double c11 = A[0] * A[5] - A[2] * A[2];
+    const double c12 = A[1] * A[2] - A[0] * A[4];
+    const double c22 = A[0] * A[3] - A[1] * A[1];
+    const double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+    if (!(std::fabs(det) > 0)) {
+      *out = c;
+      return fail(c, FPSI_ERR_ARG, "degenerate least-squares stencil at cell " + std::to_string(i));
+    }
+    const double rdet = 1.0 / det;
+    lsq[ni] = c00 * rdet; lsq[nc + ni] = c01 * rdet; lsq[2 * nc + ni] = c02 * rdet;
+    lsq[3 * nc + ni] = c11 * rdet; lsq[4 * nc + ni] = c12 * rdet; lsq[5 * nc + ni] = c22 * rdet;
+  }
+  // ---- block rows: upper faces then lower faces
+  std::vector<int> m_off(nc + 1, 0), m_nbr, m_part;
+  std::vector<int> slot_cl(nfi + nper, -1), slot_cr(nfi, -1);
+  for (int ni = 0; ni < nc; ++ni) {
+    const int i = c->cperm[ni];
+    for (int j = up_off[i]; j < up_off[i + 1]; ++j) {
+      const int f = up[j];
+      const int nfid = ifp[f];
+      slot_cl[nfid] = (int)m_nbr.size();
+      m_nbr.push_back(kind[f] == 0 ? ic[gd->cr[f]] : nc + (nfid - nfi));
+      m_part.push_back(0);
+    }
+    for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) {
+      const int f = lo[j];
+      slot_cr[ifp[f]] = (int)m_nbr.size();
+      m_nbr.push_back(ic[gd->cl[f]]);
+      m_part.push_back(0);
+    }
+    m_off[ni + 1] = (int)m_nbr.size();
+  }
+  // partner slots: for every stored block, where the face's other block sits
+  for (int nfid = 0; nfid < nfi + nper; ++nfid) {
+    const int s_cl = slot_cl[nfid];
+    if (nfid < nfi) {
+      const int s_cr = slot_cr[nfid];
+      m_part[s_cl] = s_cr;   // upper entry of cl reads L = -fjp (srcJ -= fjp)
+      m_part[s_cr] = s_cl;   // lower entry of cr reads U = fjm  (srcJ += fjm)
+    } else {
+      m_part[s_cl] = -1 - (nfid - nfi);
+    }
+  }
+  c->nnz = (int)m_nbr.size();
+  std::vector<int> old_id(c->cperm);
+
+  // ---- every index the kernels dereference must be in bounds
+  {
+    const int nnz = c->nnz;
+    bool ok = (int)g_ent.size() == g_off[nc] && (int)m_nbr.size() == m_off[nc];
+    for (int e : g_ent) ok = ok && (e < 0 ? ~e : e) < nf;
+    for (int v : m_nbr) ok = ok && v >= 0 && v < nc + nper;
+    for (int v : m_part) ok = ok && (v >= 0 ? v < nnz : (-1 - v) < nper);
+    for (int v : slot_cl) ok = ok && v >= 0 && v < nnz;
+    for (int v : slot_cr) ok = ok && v >= 0 && v < nnz;
+    for (int i = 0; i < nf; ++i) {
+      ok = ok && fcl[i] >= 0 && fcl[i] < nc;
+      if (i < nfi + nper) ok = ok && frc[i] >= 0 && frc[i] < nc;
+    }
+    if (!ok) {
+      *out = c;
+      return fail(c, FPSI_ERR_ARG, "internal index structure out of bounds");
+    }
+  }
+
+  // ---- device upload
+  HIPCHK(c, hipStreamCreate(&c->stream));
+  DevGrid &g = c->g;
+  g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
+  int rc = 0;
+  if ((rc = upload(c, &g.fnx, fnx)) || (rc = upload(c, &g.fny, fny)) || (rc = upload(c, &g.fnz, fnz)) ||
+      (rc = upload(c, &g.farea, farea)) || (rc = upload(c, &g.fcx, fcx)) || (rc = upload(c, &g.fcy, fcy)) ||
+      (rc = upload(c, &g.fcz, fcz)) || (rc = upload(c, &g.fcl, fcl)) || (rc = upload(c, &g.frc, frc)) ||
+      (rc = upload(c, &g.fbt, fbt)) || (rc = upload(c, &g.psx, psx)) || (rc = upload(c, &g.psy, psy)) ||
+      (rc = upload(c, &g.psz, psz)) || (rc = upload(c, &g.vol, vol)) || (rc = upload(c, &g.ccx, ccx)) ||
+      (rc = upload(c, &g.ccy, ccy)) || (rc = upload(c, &g.ccz, ccz)) || (rc = upload(c, &g.lsq, lsq)) ||
+      (rc = upload(c, &g.g_off, g_off)) || (rc = upload(c, &g.g_ent, g_ent)) ||
+      (rc = upload(c, &g.m_off, m_off)) || (rc = upload(c, &g.m_nbr, m_nbr)) ||
+      (rc = upload(c, &g.m_part, m_part)) || (rc = upload(c, &g.slot_cl, slot_cl)) ||
+      (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id))) {
+    *out = c;
+    return rc;
+  }
+  DevState &s = c->s;
+  const size_t nbb = c->nb;
+  if ((rc = dalloc(c, &s.q, 5 * (size_t)nc)) || (rc = dalloc(c, &s.qf, 5 * nbb)) ||
+      (rc = dalloc(c, &s.grad, 15 * (size_t)nc)) || (rc = dalloc(c, &s.phi, 5 * (size_t)nc)) ||
+      (rc = dalloc(c, &s.iflux, 5 * (size_t)nf)) || (rc = dalloc(c, &s.vflux, 4 * (size_t)nf)) ||
+      (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
+      (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
+      (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper))) ||
+      (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12))) {
+    *out = c;
+    return rc;
+  }
+  // real_fj buffers sized for FP64 so the precision can change without reallocation
+  double *tmp = nullptr;
+  if ((rc = dalloc(c, &tmp, 25 * (size_t)c->nnz))) { *out = c; return rc; }
+  s.off = tmp;
+  if ((rc = dalloc(c, &tmp, 25 * (size_t)nper))) { *out = c; return rc; }
+  s.pblk = tmp;
+  if ((rc = dalloc(c, &tmp, 25 * nbb))) { *out = c; return rc; }
+  s.bblk = tmp;
+  if ((rc = dalloc(c, &tmp, 25 * (size_t)nc))) { *out = c; return rc; }
+  s.lu = tmp;
+  if ((rc = dalloc(c, &tmp, 5 * (size_t)nc))) { *out = c; return rc; }
+  s.B = tmp;
+  HIPCHK(c, hipMemsetAsync(s.vflux, 0, 4 * (size_t)nf * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper) * sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = c;
+  return 0;
+}
+
+extern "C" int fpsi_get_info(fpsi_ctx *c, fpsi_info *info) {
+  if (!c || !info) return FPSI_ERR_ARG;
+  info->nlevels = (int)c->lev_off.size() - 1;
+  info->max_level_width = c->max_width;
+  info->nnz_blocks = c->nnz;
+  info->device_bytes = (double)c->bytes;
+  return 0;
+}
+
+extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
+  if (!c || !p) return FPSI_ERR_ARG;
+  if (p->flux != FPSI_FLUX_HLLC && p->flux != FPSI_FLUX_ROE) return fail(c, FPSI_ERR_ARG, "unknown flux");
+  if (p->limiter < 0 || p->limiter > 3) return fail(c, FPSI_ERR_ARG, "unknown limiter");
+  if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
+  if (p->linear_solver != FPSI_SOLVER_SGS) return fail(c, FPSI_ERR_ARG, "unknown linear solver");
+  if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
+  if (!(p->gamma > 1.0) || !(p->Rtilde > 0.0)) return fail(c, FPSI_ERR_ARG, "bad gas constants");
+  c->prm = *p;
+  Prm &k = c->kp;
+  k.gamma = p->gamma; k.Rt = p->Rtilde; k.Pamb = p->Pambient;
+  k.flux = p->flux; k.limiter = p->limiter; k.Kl = p->Kl; k.viscous = p->viscous;
+  k.a1 = p->suth[0]; k.a2 = p->suth[1]; k.a3 = p->suth[2];
+  k.k1 = p->suth[3]; k.k2 = p->suth[4]; k.k3 = p->suth[5];
+  k.Twall = p->Twall; k.pg0 = p->pg0; k.T0 = p->T0;
+  k.u0x = p->u0[0]; k.u0y = p->u0[1]; k.u0z = p->u0[2];
+  k.cflmax = p->cflmax; k.dtmax = p->dtmax; k.urelax = p->urelax; k.minPg = p->minPg;
+  c->fp32 = p->precision == FPSI_PREC_FP32;
+  c->have_params = true;
+  c->assembled = false;
+  return 0;
+}
+
+extern "C" int fpsi_set_state(fpsi_ctx *c, const double *q) {
+  if (!c || !q) return FPSI_ERR_ARG;
+  const int nc = c->nc;
+  std::vector<double> h(5 * (size_t)nc);
+  for (int k = 0; k < 5; ++k)
+    for (int ni = 0; ni < nc; ++ni) h[(size_t)k * nc + ni] = q[(size_t)k * nc + c->cperm[ni]];
+  HIPCHK(c, hipMemcpyAsync(c->s.q, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  c->assembled = false;
+  return 0;
+}
+
+static int copy_cells_out(fpsi_ctx *c, const double *dsrc, int ncomp, double *out) {
+  const int nc = c->nc;
+  std::vector<double> h((size_t)ncomp * nc);
+  HIPCHK(c, hipMemcpyAsync(h.data(), dsrc, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ncomp; ++k)
+    for (int ni = 0; ni < nc; ++ni) out[(size_t)k * nc + c->cperm[ni]] = h[(size_t)k * nc + ni];
+  return 0;
+}
+
+static int copy_faces_out(fpsi_ctx *c, const double *dsrc, int ncomp, double *out) {
+  const int nf = c->nf;
+  std::vector<double> h((size_t)ncomp * nf);
+  HIPCHK(c, hipMemcpyAsync(h.data(), dsrc, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ncomp; ++k)
+    for (int i = 0; i < nf; ++i) out[(size_t)k * nf + c->fperm[i]] = h[(size_t)k * nf + i];
+  return 0;
+}
+
+extern "C" int fpsi_get_state(fpsi_ctx *c, double *q) {
+  if (!c || !q) return FPSI_ERR_ARG;
+  if (!c->have_state) return fail(c, FPSI_ERR_STATE, "no state set");
+  return copy_cells_out(c, c->s.q, 5, q);
+}
+
+static int check_ready(fpsi_ctx *c) {
+  if (!c->have_params) return fail(c, FPSI_ERR_STATE, "fpsi_set_params not called");
+  if (!c->have_state) return fail(c, FPSI_ERR_STATE, "fpsi_set_state not called");
+  return 0;
+}
+
+static void enqueue_residual(fpsi_ctx *c) {
+  launch_bface(c->g, c->kp, c->s, c->stream);
+  launch_grad(c->g, c->kp, c->s, c->stream);
+  launch_face_flux(c->g, c->kp, c->s, c->stream);
+  launch_gather(c->g, c->kp, c->vtot, c->s, c->stream);
+}
+
+static void norms_from(const double *o, fpsi_norms *n) {
+  n->rtrms = o[0]; n->etrms = o[1]; n->mtrms = o[2];
+  n->rtmax = o[3]; n->etmax = o[4]; n->mtmax = o[5];
+  n->ccmaxr[0] = o[6]; n->ccmaxr[1] = o[7]; n->ccmaxr[2] = o[8];
+  n->ccmaxe[0] = o[9]; n->ccmaxe[1] = o[10]; n->ccmaxe[2] = o[11];
+}
+
+extern "C" int fpsi_residual(fpsi_ctx *c, double *src, double *iflux, double *vflux, fpsi_norms *norms) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  enqueue_residual(c);
+  HIPCHK(c, hipGetLastError());
+  if (src) if (int rc = copy_cells_out(c, c->s.src, 5, src)) return rc;
+  if (iflux) if (int rc = copy_faces_out(c, c->s.iflux, 5, iflux)) return rc;
+  if (vflux) if (int rc = copy_faces_out(c, c->s.vflux, 4, vflux)) return rc;
+  double o[12];
+  HIPCHK(c, hipMemcpyAsync(o, c->s.nout, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (norms) norms_from(o, norms);
+  const double tot = o[0] + o[1] + o[2];
+  if (tot != tot) return fail(c, FPSI_ERR_NAN, "NAN residual");
+  return 0;
+}
+
+extern "C" int fpsi_gradients(fpsi_ctx *c, double *grad, double *phi) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  launch_bface(c->g, c->kp, c->s, c->stream);
+  launch_grad(c->g, c->kp, c->s, c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (grad) if (int rc = copy_cells_out(c, c->s.grad, 15, grad)) return rc;
+  if (phi) if (int rc = copy_cells_out(c, c->s.phi, 5, phi)) return rc;
+  return 0;
+}
+
+static void enqueue_assemble(fpsi_ctx *c) {
+  enqueue_residual(c);
+  launch_face_jac(c->g, c->kp, c->fp32, c->s, c->stream);
+  launch_cell_assemble(c->g, c->kp, c->fp32, c->s, c->stream);
+}
+
+extern "C" int fpsi_assemble(fpsi_ctx *c) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  enqueue_assemble(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = true;
+  return 0;
+}
+
+extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *pivot, void *B, double *dtau) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  const size_t e = c->fj_bytes();
+  const int nc = c->nc, nf = c->nf, nfi = c->nfi, nper = c->nper;
+  if (fjp || fjm) {
+    std::vector<char> off((size_t)c->nnz * 25 * e), pb((size_t)nper * 25 * e);
+    HIPCHK(c, hipMemcpy(off.data(), c->s.off, off.size(), hipMemcpyDeviceToHost));
+    if (nper) HIPCHK(c, hipMemcpy(pb.data(), c->s.pblk, pb.size(), hipMemcpyDeviceToHost));
+    std::vector<int> scl(nfi + nper), scr(nfi);
+    HIPCHK(c, hipMemcpy(scl.data(), c->g.slot_cl, scl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (nfi) HIPCHK(c, hipMemcpy(scr.data(), c->g.slot_cr, scr.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (fjp) std::memset(fjp, 0, (size_t)nf * 25 * e);
+    if (fjm) std::memset(fjm, 0, (size_t)nf * 25 * e);
+    for (int i = 0; i < nfi + nper; ++i) {
+      const int f = c->fperm[i];
+      if (fjm) std::memcpy((char *)fjm + (size_t)f * 25 * e, off.data() + (size_t)scl[i] * 25 * e, 25 * e);
+      if (fjp) {
+        char *dst = (char *)fjp + (size_t)f * 25 * e;
+        if (i < nfi) {
+          const char *srcb = off.data() + (size_t)scr[i] * 25 * e;
+          for (int k = 0; k < 25; ++k) {  // stored as L = -fjp
+            if (e == 8) { double v; std::memcpy(&v, srcb + 8 * k, 8); v = -v; std::memcpy(dst + 8 * k, &v, 8); }
+            else { float v; std::memcpy(&v, srcb + 4 * k, 4); v = -v; std::memcpy(dst + 4 * k, &v, 4); }
+          }
+        } else {
+          std::memcpy(dst, pb.data() + (size_t)(i - nfi) * 25 * e, 25 * e);
+        }
+      }
+    }
+  }
+  if (D) {
+    std::vector<char> h((size_t)nc * 25 * e);
+    HIPCHK(c, hipMemcpy(h.data(), c->s.lu, h.size(), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni) std::memcpy((char *)D + (size_t)c->cperm[ni] * 25 * e, h.data() + (size_t)ni * 25 * e, 25 * e);
+  }
+  if (pivot) {
+    std::vector<int> h(nc);
+    HIPCHK(c, hipMemcpy(h.data(), c->s.piv, nc * sizeof(int), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni)
+      for (int k = 0; k < 5; ++k) pivot[(size_t)c->cperm[ni] * 5 + k] = (h[ni] >> (3 * k)) & 7;
+  }
+  if (B) {
+    std::vector<char> h((size_t)nc * 5 * e);
+    HIPCHK(c, hipMemcpy(h.data(), c->s.B, h.size(), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni) std::memcpy((char *)B + (size_t)c->cperm[ni] * 5 * e, h.data() + (size_t)ni * 5 * e, 5 * e);
+  }
+  if (dtau) if (int rc = copy_cells_out(c, c->s.dtau, 1, dtau)) return rc;
+  return 0;
+}
+
+static void enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
+  const int nlev = (int)c->lev_off.size() - 1;
+  for (int it = 0; it < nsweeps; ++it) {
+    launch_ghost(c->g, c->s, c->stream);
+    for (int l = 0; l < nlev; ++l)
+      launch_sgs_level(c->g, c->fp32, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+    launch_ghost(c->g, c->s, c->stream);
+    for (int l = nlev - 1; l >= 0; --l)
+      launch_sgs_level(c->g, c->fp32, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+  }
+}
+
+extern "C" int fpsi_linear_solve(fpsi_ctx *c, int nsweeps, double *dq) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
+  enqueue_sweeps(c, nsweeps);
+  HIPCHK(c, hipGetLastError());
+  if (dq) {
+    const int nc = c->nc;
+    std::vector<double> h(5 * (size_t)nc);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->s.X, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int ni = 0; ni < nc; ++ni)
+      for (int k = 0; k < 5; ++k) dq[(size_t)c->cperm[ni] * 5 + k] = h[(size_t)ni * 5 + k];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;  // X now holds the solution, not the Jacobi start
+  return 0;
+}
+
+extern "C" int fpsi_update(fpsi_ctx *c) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  launch_update(c->g, c->kp, c->s, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;
+  return 0;
+}
+
+extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
+  std::vector<double> h(12 * (size_t)std::max(nsteps, 1));
+  double *dh = nullptr;
+  if (nsteps > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    enqueue_assemble(c);
+    HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    enqueue_sweeps(c, c->prm.sgs_iter);
+    launch_update(c->g, c->kp, c->s, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (nsteps > 0) {
+    HIPCHK(c, hipMemcpyAsync(h.data(), dh, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipFreeAsync(dh, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;
+  for (int s = 0; s < nsteps; ++s) {
+    if (hist) norms_from(&h[12 * (size_t)s], &hist[s]);
+    const double tot = h[12 * s] + h[12 * s + 1] + h[12 * s + 2];
+    if (tot != tot) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
+  }
+  return 0;
+}
+
+extern "C" int fpsi_synchronize(fpsi_ctx *c) {
+  if (!c) return FPSI_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}

@@ -1,0 +1,17 @@
+// Launchers of the Newton-step kernels (kernels.hip).
+#pragma once
+#include "layout.h"
+#include "physics.h"
+
+namespace fpsi {
+void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+int norm_parts(const DevGrid &g);
+void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st);
+void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
+void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
+void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st);
+void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
+void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+}  // namespace fpsi

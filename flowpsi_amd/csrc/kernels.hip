@@ -1,0 +1,908 @@
+// Newton-step kernels for gfx950.  One lane per face or per cell; no atomics:
+// every cell quantity is gathered through the cell's face list in a fixed
+// order, so results are bitwise reproducible run to run.
+#include "kernels.h"
+#include "physics.h"
+
+namespace fpsi {
+
+static constexpr int TPB = 256;
+static inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
+
+// ---------------------------------------------------------------------------
+// boundary-face states: farfield src/bcInflow.loci:311-340, viscous wall
+// src/bcViscousWall.loci:205-214 (stationary), symmetry src/bcImpermeable.loci:48-64
+// ---------------------------------------------------------------------------
+__global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ qf) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.nb) return;
+  const int f = g.bnd_begin + b;
+  const int c = g.fcl[f];
+  const int nc = g.nc;
+  const V3 n = {g.fnx[f], g.fny[f], g.fnz[f]};
+  Q5 i;
+  i.T = q[c];
+  i.u = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+  i.pg = q[4 * nc + c];
+  Q5 o;
+  const int t = g.fbt[f];
+  if (t == 1) {  // farfield
+    const double us_n = 0.0, eta_pf = 1.0;
+    const double uit = dot(i.u, n) - us_n;
+    const double ai = sqrt(p.gamma * p.Rt * i.T);
+    const double a0 = sqrt(p.gamma * p.Rt * p.T0);
+    Q5 r0;
+    r0.T = p.T0;
+    r0.u = {p.u0x, p.u0y, p.u0z};
+    r0.pg = p.pg0;
+    if (uit > 0) {
+      if (uit > ai) o = i;
+      else cv_outflow(o, i, r0, n, us_n, p.Pamb, p.gamma, p.Rt, eta_pf);
+    } else {
+      if (-uit > a0) o = r0;
+      else cv_inflow(o, i, r0, n, us_n, p.Pamb, p.gamma, p.Rt, eta_pf);
+    }
+  } else if (t == 2) {  // viscous wall, uwall = us = 0
+    o.u = {0.0, 0.0, 0.0};
+    o.pg = i.pg;
+    o.T = p.Twall;
+  } else {  // symmetry
+    const double uit = dot(i.u, n);
+    o.u = {i.u.x - uit * n.x, i.u.y - uit * n.y, i.u.z - uit * n.z};
+    const double rho = (i.pg + p.Pamb) / (p.Rt * i.T);
+    const double U = sqrt(dot(i.u, i.u));
+    o.pg = i.pg + 0.5 * rho * U * uit;
+    o.T = i.T;
+  }
+  const int nb = g.nb;
+  qf[b] = o.T;
+  qf[nb + b] = o.u.x;
+  qf[2 * nb + b] = o.u.y;
+  qf[3 * nb + b] = o.u.z;
+  qf[4 * nb + b] = o.pg;
+}
+
+// neighbour across gather entry e of cell c: position and primitive values
+__device__ __forceinline__ void stencil(const DevGrid &g, const double *__restrict__ q,
+                                        const double *__restrict__ qf, int e, double *x, double *v) {
+  const int nc = g.nc;
+  const int f = e < 0 ? ~e : e;
+  if (f < g.nfi) {
+    const int o = e < 0 ? g.frc[f] : g.fcl[f];
+    x[0] = g.ccx[o]; x[1] = g.ccy[o]; x[2] = g.ccz[o];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+  } else if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    const int pi = f - g.nfi;
+    x[0] = g.ccx[o] + g.psx[pi]; x[1] = g.ccy[o] + g.psy[pi]; x[2] = g.ccz[o] + g.psz[pi];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+  } else {
+    const int b = f - g.bnd_begin;
+    x[0] = g.fcx[f]; x[1] = g.fcy[f]; x[2] = g.fcz[f];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = qf[k * g.nb + b];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// least-squares gradient and limiter per cell (Loci FVM grads/limiters;
+// textbook unweighted least squares, Venkatakrishnan / Barth limiters)
+// ---------------------------------------------------------------------------
+__global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+                       double *__restrict__ grad, double *__restrict__ phi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const double xc0 = g.ccx[c], xc1 = g.ccy[c], xc2 = g.ccz[c];
+  double qc[5], b[5][3], qmin[5], qmax[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    qc[k] = q[k * nc + c];
+    b[k][0] = b[k][1] = b[k][2] = 0.0;
+    qmin[k] = qmax[k] = qc[k];
+  }
+  const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
+  for (int j = j0; j < j1; ++j) {
+    double x[3], v[5];
+    stencil(g, q, qf, g.g_ent[j], x, v);
+    const double dx = x[0] - xc0, dy = x[1] - xc1, dz = x[2] - xc2;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double dq = v[k] - qc[k];
+      b[k][0] += dx * dq;
+      b[k][1] += dy * dq;
+      b[k][2] += dz * dq;
+      qmin[k] = dmin(qmin[k], v[k]);
+      qmax[k] = dmax(qmax[k], v[k]);
+    }
+  }
+  const double A0 = g.lsq[c], A1 = g.lsq[nc + c], A2 = g.lsq[2 * nc + c];
+  const double A3 = g.lsq[3 * nc + c], A4 = g.lsq[4 * nc + c], A5 = g.lsq[5 * nc + c];
+  double gr[5][3];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    gr[k][0] = A0 * b[k][0] + A1 * b[k][1] + A2 * b[k][2];
+    gr[k][1] = A1 * b[k][0] + A3 * b[k][1] + A4 * b[k][2];
+    gr[k][2] = A2 * b[k][0] + A4 * b[k][1] + A5 * b[k][2];
+    grad[(3 * k) * nc + c] = gr[k][0];
+    grad[(3 * k + 1) * nc + c] = gr[k][1];
+    grad[(3 * k + 2) * nc + c] = gr[k][2];
+  }
+  double ph[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+  if (p.limiter == 2) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ph[k] = 0.0;
+  } else if (p.limiter == 1 || p.limiter == 3) {
+    const double eps2 = p.Kl * p.Kl * p.Kl * g.vol[c];
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      const int f = e < 0 ? ~e : e;
+      const double dx = g.fcx[f] - xc0, dy = g.fcy[f] - xc1, dz = g.fcz[f] - xc2;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const double qdif = gr[k][0] * dx + gr[k][1] * dy + gr[k][2] * dz;
+        double lim;
+        if (p.limiter == 1) {
+          const double delp = (qdif >= 0.0) ? qmax[k] - qc[k] : qmin[k] - qc[k];
+          lim = venkat(delp, qdif, eps2);
+        } else {
+          if (qdif > 0) lim = dmin(1.0, (qmax[k] - qc[k]) / qdif);
+          else if (qdif < 0) lim = dmin(1.0, (qmin[k] - qc[k]) / qdif);
+          else lim = 1.0;
+        }
+        ph[k] = dmin(ph[k], lim);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ph[k] = dmax(0.0, dmin(1.0, ph[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) phi[k * nc + c] = ph[k];
+}
+
+// limited reconstruction with the floors of leftsP(temperature,Zero) and
+// leftsP(gagePressure,minPg)
+__device__ __forceinline__ Q5 recon(const DevGrid &g, const Prm &p, const double *__restrict__ q,
+                                    const double *__restrict__ grad, const double *__restrict__ phi,
+                                    int c, double xc0, double xc1, double xc2, double xf0, double xf1,
+                                    double xf2) {
+  const int nc = g.nc;
+  const double d0 = xf0 - xc0, d1 = xf1 - xc1, d2 = xf2 - xc2;
+  double o[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double gd = grad[(3 * k) * nc + c] * d0 + grad[(3 * k + 1) * nc + c] * d1 +
+                      grad[(3 * k + 2) * nc + c] * d2;
+    o[k] = q[k * nc + c] + phi[k * nc + c] * gd;
+  }
+  Q5 r;
+  r.T = dmax(o[0], 0.0);
+  r.u = {o[1], o[2], o[3]};
+  r.pg = dmax(o[4], p.minPg);
+  return r;
+}
+
+// face gradient of variable k: interior/periodic faces average the two cell
+// gradients and correct along the centre line; boundary faces correct the
+// cell gradient along x_f - x_c (Loci gradv3d_f / grads_f)
+__device__ __forceinline__ void face_grad_int(const DevGrid &g, const double *__restrict__ q,
+                                              const double *__restrict__ grad, int c, int o, int k,
+                                              const double *d, double dd, double *gout) {
+  const int nc = g.nc;
+  double ga[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ga[j] = 0.5 * (grad[(3 * k + j) * nc + c] + grad[(3 * k + j) * nc + o]);
+  const double corr = (q[k * nc + o] - q[k * nc + c] - (ga[0] * d[0] + ga[1] * d[1] + ga[2] * d[2])) / dd;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) gout[j] = ga[j] + corr * d[j];
+}
+__device__ __forceinline__ void face_grad_bnd(const DevGrid &g, const double *__restrict__ q,
+                                              const double *__restrict__ grad, double vf, int c, int k,
+                                              const double *d, double dd, double *gout) {
+  const int nc = g.nc;
+  double ga[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ga[j] = grad[(3 * k + j) * nc + c];
+  const double corr = (vf - q[k * nc + c] - (ga[0] * d[0] + ga[1] * d[1] + ga[2] * d[2])) / dd;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) gout[j] = ga[j] + corr * d[j];
+}
+
+// viscous face quantities (src/fluxViscous.loci:121-133,145-151,184-197,217-238,272-297)
+__device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double *__restrict__ q,
+                                    const double *__restrict__ qf, const double *__restrict__ grad,
+                                    int f, double *mu_f, double *k_f, V3 *uf, double *vf,
+                                    bool need_flux) {
+  const int nc = g.nc;
+  const int c = g.fcl[f];
+  const V3 an = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double sada = g.farea[f];
+  const double xl0 = g.ccx[c], xl1 = g.ccy[c], xl2 = g.ccz[c];
+  double gu[3][3], gT3[3];
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    double xr0 = g.ccx[o], xr1 = g.ccy[o], xr2 = g.ccz[o];
+    if (f >= g.nfi) {
+      const int pi = f - g.nfi;
+      xr0 += g.psx[pi]; xr1 += g.psy[pi]; xr2 += g.psz[pi];
+    }
+    const double vl = g.vol[c], vr = g.vol[o];
+    const double mul = suth_mu(p, q[c]), mur = suth_mu(p, q[o]);
+    const double kl = suth_k(p, q[c]), kr = suth_k(p, q[o]);
+    *mu_f = (vl * mur + vr * mul) / (vl + vr);
+    *k_f = (vl * kr + vr * kl) / (vl + vr);
+    const double factor = 1. / (vl + vr);
+    uf->x = factor * (vl * q[nc + o] + vr * q[nc + c]);
+    uf->y = factor * (vl * q[2 * nc + o] + vr * q[2 * nc + c]);
+    uf->z = factor * (vl * q[3 * nc + o] + vr * q[3 * nc + c]);
+    if (!need_flux) return;
+    const double d[3] = {xr0 - xl0, xr1 - xl1, xr2 - xl2};
+    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    for (int i = 0; i < 3; ++i) face_grad_int(g, q, grad, c, o, 1 + i, d, dd, gu[i]);
+    face_grad_int(g, q, grad, c, o, 0, d, dd, gT3);
+  } else {
+    const int b = f - g.bnd_begin;
+    const int nb = g.nb;
+    const double Tf = qf[b];
+    *mu_f = suth_mu(p, Tf);
+    *k_f = suth_k(p, Tf);
+    uf->x = qf[nb + b];
+    uf->y = qf[2 * nb + b];
+    uf->z = qf[3 * nb + b];
+    if (!need_flux) return;
+    if (g.fbt[f] == 3) {
+      vf[0] = vf[1] = vf[2] = vf[3] = 0.0;
+      return;
+    }
+    const double d[3] = {g.fcx[f] - xl0, g.fcy[f] - xl1, g.fcz[f] - xl2};
+    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    for (int i = 0; i < 3; ++i) face_grad_bnd(g, q, grad, qf[(1 + i) * nb + b], c, 1 + i, d, dd, gu[i]);
+    face_grad_bnd(g, q, grad, qf[b], c, 0, d, dd, gT3);
+  }
+  const V3 gT = {gT3[0], gT3[1], gT3[2]};
+  viscous_flux(vf, gu, gT, *mu_f, *k_f, *uf, an, sada);
+}
+
+// ---------------------------------------------------------------------------
+// face fluxes: iflux (src/fluxInviscid.loci:1170-1188, src/bcInflow.loci:342-355,
+// src/bcViscousWall.loci:216-229, src/bcImpermeable.loci:66-73), vflux and
+// maxev_f (src/solverSetup.loci:83-105)
+// ---------------------------------------------------------------------------
+__global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+                            const double *__restrict__ grad, const double *__restrict__ phi,
+                            double *__restrict__ iflux, double *__restrict__ vflux,
+                            double *__restrict__ maxev) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.nf) return;
+  const int nc = g.nc, nf = g.nf, nb = g.nb;
+  const V3 an = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double area = g.farea[f];
+  const double xf0 = g.fcx[f], xf1 = g.fcy[f], xf2 = g.fcz[f];
+  const int c = g.fcl[f];
+  const double us_n = 0.0, eta = 1.0;
+  const Q5 L = recon(g, p, q, grad, phi, c, g.ccx[c], g.ccy[c], g.ccz[c], xf0, xf1, xf2);
+  double fl[5];
+  double mev;
+  const double Eta_P = eta + 1., Eta_M = eta - 1.;
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    double xr0 = g.ccx[o], xr1 = g.ccy[o], xr2 = g.ccz[o];
+    if (f >= g.nfi) {
+      const int pi = f - g.nfi;
+      xr0 += g.psx[pi]; xr1 += g.psy[pi]; xr2 += g.psz[pi];
+    }
+    const Q5 R = recon(g, p, q, grad, phi, o, xr0, xr1, xr2, xf0, xf1, xf2);
+    if (p.flux == 1) roe_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+    else hllc_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+    const V3 ul = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+    const V3 ur = {q[nc + o], q[2 * nc + o], q[3 * nc + o]};
+    const double utl = dot(ul, an) - us_n, utr = dot(ur, an) - us_n;
+    const double al = sqrt(p.gamma * p.Rt * q[c]), ar = sqrt(p.gamma * p.Rt * q[o]);
+    const double sigmal = sqrt(utl * utl * Eta_M * Eta_M + 4. * eta * al * al);
+    const double sigmar = sqrt(utr * utr * Eta_M * Eta_M + 4. * eta * ar * ar);
+    const double mevl = 0.5 * dmax(fabs(utl * Eta_P - sigmal), fabs(utl * Eta_P + sigmal));
+    const double mevr = 0.5 * dmax(fabs(utr * Eta_P - sigmar), fabs(utr * Eta_P + sigmar));
+    mev = dmax(mevl, mevr);
+  } else {
+    const int b = f - g.bnd_begin;
+    const int t = g.fbt[f];
+    Q5 R;
+    R.T = qf[b];
+    R.u = {qf[nb + b], qf[2 * nb + b], qf[3 * nb + b]};
+    R.pg = qf[4 * nb + b];
+    if (t == 1) {
+      if (p.flux == 1) roe_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+      else hllc_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+    } else if (t == 2) {
+      const double pg = R.pg;
+      fl[0] = 0;
+      fl[1] = pg * (an.x * area);
+      fl[2] = pg * (an.y * area);
+      fl[3] = pg * (an.z * area);
+      const double pw = pg + p.Pamb;
+      fl[4] = pw * area * us_n;
+    } else {
+      const double pg = R.pg;
+      fl[0] = 0;
+      const double moment = pg * area;
+      fl[1] = an.x * moment;
+      fl[2] = an.y * moment;
+      fl[3] = an.z * moment;
+      fl[4] = area * pg * dot(R.u, an);
+    }
+    const double utf = dot(R.u, an) - us_n;
+    const double af = sqrt(p.gamma * p.Rt * R.T);
+    const double sigmaf = sqrt(utf * utf * Eta_M * Eta_M + 4. * eta * af * af);
+    mev = 0.5 * dmax(fabs(utf * Eta_P - sigmaf), fabs(utf * Eta_P + sigmaf));
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) iflux[k * nf + f] = fl[k];
+  maxev[f] = mev;
+  if (p.viscous) {
+    double mu_f, k_f, vf[4];
+    V3 uf;
+    viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, true);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) vflux[k * nf + f] = vf[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// residual gather (src/fluxInviscid.loci:1253-1268, src/fluxViscous.loci:299-310)
+// and the residual norm terms (src/solverResidual.loci:74-112)
+// ---------------------------------------------------------------------------
+struct NormAcc {
+  double rt, et, mt;
+  double rmax, emax, mmax;
+  int rid, eid, mid;  // caller cell index of the maxima (-1 none)
+  int rcell, ecell;   // stored cell of the maxima
+};
+
+__device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
+  // a replaces b when |b| < |a|; on equal magnitude the lower caller index
+  // wins, which is what a join in caller cell order produces
+  const double fa = fabs(a), fb = fabs(b);
+  if (fb < fa) return true;
+  if (fa == fb) return ia >= 0 && (ib < 0 || ia < ib);
+  return false;
+}
+
+__device__ __forceinline__ void norm_join(NormAcc &r, const NormAcc &s) {
+  r.rt += s.rt;
+  r.et += s.et;
+  r.mt += s.mt;
+  if (better(s.emax, s.eid, r.emax, r.eid)) { r.emax = s.emax; r.eid = s.eid; r.ecell = s.ecell; }
+  if (better(s.rmax, s.rid, r.rmax, r.rid)) { r.rmax = s.rmax; r.rid = s.rid; r.rcell = s.rcell; }
+  if (better(s.mmax, s.mid, r.mmax, r.mid)) { r.mmax = s.mmax; r.mid = s.mid; }
+}
+
+static constexpr int NPART = 12;  // doubles per partial record
+
+__device__ void block_norms(NormAcc a, double *__restrict__ part) {
+  __shared__ NormAcc sh[TPB];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) norm_join(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const NormAcc &r = sh[0];
+    double *o = part + (size_t)blockIdx.x * NPART;
+    o[0] = r.rt; o[1] = r.et; o[2] = r.mt; o[3] = r.rmax; o[4] = r.emax; o[5] = r.mmax;
+    o[6] = r.rid; o[7] = r.eid; o[8] = r.mid; o[9] = r.rcell; o[10] = r.ecell; o[11] = 0;
+  }
+}
+
+__global__ void k_gather_src(DevGrid g, int viscous, double vtot, const double *__restrict__ iflux,
+                             const double *__restrict__ vflux, double *__restrict__ src,
+                             double *__restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  NormAcc a = {0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1};
+  if (c < g.nc) {
+    const int nc = g.nc, nf = g.nf;
+    double s[5] = {0, 0, 0, 0, 0};
+    const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e < 0) {
+        const int f = ~e;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s[k] -= iflux[k * nf + f];
+        if (viscous) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) s[k + 1] += vflux[k * nf + f];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s[k] += iflux[k * nf + e];
+        if (viscous) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) s[k + 1] -= vflux[k * nf + e];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) src[k * nc + c] = s[k];
+    // norm terms of this cell
+    const double vol = g.vol[c];
+    const double rvol = 1. / vol;
+    const double weight = vol / vtot;
+    const int id = g.old_id[c];
+    const double r0 = rvol * s[0];
+    if (fabs(0.0) < fabs(r0)) { a.rmax = r0; a.rid = id; a.rcell = c; }
+    double rt = 0;
+    rt += r0 * r0;
+    rt *= weight;
+    double mt = 0, mmax = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double v = rvol * s[1 + i];
+      if (fabs(mmax) < fabs(v)) mmax = v;
+      mt += v * v;
+    }
+    mt *= weight / 3.0;
+    const double ve = rvol * s[4];
+    a.rt = rt;
+    a.mt = mt;
+    a.et = weight * ve * ve;
+    a.emax = ve; a.eid = id; a.ecell = c;
+    a.mmax = mmax; a.mid = id;
+  }
+  block_norms(a, part);
+}
+
+__global__ void k_norm_final(DevGrid g, int nparts, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ NormAcc sh[TPB];
+  NormAcc a = {0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1};
+  // each thread folds a contiguous slice in order, then a fixed tree
+  const int per = (nparts + TPB - 1) / TPB;
+  const int i0 = threadIdx.x * per, i1 = min(nparts, i0 + per);
+  for (int i = i0; i < i1; ++i) {
+    const double *o = part + (size_t)i * NPART;
+    NormAcc s = {o[0], o[1], o[2], o[3], o[4], o[5], (int)o[6], (int)o[7], (int)o[8], (int)o[9], (int)o[10]};
+    norm_join(a, s);
+  }
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) norm_join(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const NormAcc &r = sh[0];
+    out[0] = r.rt; out[1] = r.et; out[2] = r.mt;
+    out[3] = r.rmax; out[4] = r.emax; out[5] = r.mmax;
+    const int rc = r.rcell, ec = r.ecell;
+    out[6] = (rc >= 0 && r.rmax != 0.0) ? g.ccx[rc] : 0.0;
+    out[7] = (rc >= 0 && r.rmax != 0.0) ? g.ccy[rc] : 0.0;
+    out[8] = (rc >= 0 && r.rmax != 0.0) ? g.ccz[rc] : 0.0;
+    out[9] = (ec >= 0 && r.emax != 0.0) ? g.ccx[ec] : 0.0;
+    out[10] = (ec >= 0 && r.emax != 0.0) ? g.ccy[ec] : 0.0;
+    out[11] = (ec >= 0 && r.emax != 0.0) ? g.ccz[ec] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// face Jacobians: fjp/fjm (src/fluxInviscid.loci:1190-1221 + viscous
+// src/fluxViscous.loci:315-403) written straight into the block rows;
+// boundary faces: their whole srcJ contribution (src/bcInflow.loci:357-375,
+// src/bcViscousWall.loci:230-240, src/bcImpermeable.loci:75-100,
+// src/fluxViscous.loci:405-448)
+// ---------------------------------------------------------------------------
+template <typename F>
+__device__ __forceinline__ void store_blk(F *__restrict__ dst, const F *blk, bool neg) {
+#pragma unroll
+  for (int i = 0; i < 25; ++i) dst[i] = neg ? (F)(-blk[i]) : blk[i];
+}
+
+template <typename F>
+__global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+                           const double *__restrict__ grad, const double *__restrict__ phi,
+                           const double *__restrict__ vflux, F *__restrict__ off, F *__restrict__ pblk,
+                           F *__restrict__ bblk) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.nf) return;
+  const int nc = g.nc, nf = g.nf, nb = g.nb;
+  const V3 an = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double area = g.farea[f];
+  const double xf0 = g.fcx[f], xf1 = g.fcy[f], xf2 = g.fcz[f];
+  const int c = g.fcl[f];
+  const double xl0 = g.ccx[c], xl1 = g.ccy[c], xl2 = g.ccz[c];
+  const double us_n = 0.0, eta = 1.0;
+  const Q5 L = recon(g, p, q, grad, phi, c, xl0, xl1, xl2, xf0, xf1, xf2);
+  F blk[25];
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    double xr0 = g.ccx[o], xr1 = g.ccy[o], xr2 = g.ccz[o];
+    const bool per = f >= g.nfi;
+    if (per) {
+      const int pi = f - g.nfi;
+      xr0 += g.psx[pi]; xr1 += g.psy[pi]; xr2 += g.psz[pi];
+    }
+    const Q5 R = recon(g, p, q, grad, phi, o, xr0, xr1, xr2, xf0, xf1, xf2);
+    const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, eta);
+    double mu_f = 0, k_f = 0, vf[4] = {0, 0, 0, 0}, dist = 1;
+    V3 uf = {0, 0, 0};
+    if (p.viscous) {
+      viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, false);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vf[k] = vflux[k * nf + f];
+      const V3 dv = {xr0 - xl0, xr1 - xl1, xr2 - xl2};
+      dist = dot(an, dv);
+    }
+    // fjp
+#pragma unroll
+    for (int i = 0; i < 25; ++i) blk[i] = (F)0;
+    roe_jacobian<F>(blk, ra, an, area, p.gamma, p.Rt, 1);
+    if (p.viscous) viscous_jacobian<F>(blk, 1.0, an, area, dist, mu_f, k_f, uf, vf);
+    if (!per) store_blk(off + (size_t)g.slot_cr[f] * 25, blk, true);
+    else store_blk(pblk + (size_t)(f - g.nfi) * 25, blk, false);
+    // fjm
+#pragma unroll
+    for (int i = 0; i < 25; ++i) blk[i] = (F)0;
+    roe_jacobian<F>(blk, ra, an, area, p.gamma, p.Rt, -1);
+    if (p.viscous) viscous_jacobian<F>(blk, -1.0, an, area, dist, mu_f, k_f, uf, vf);
+    store_blk(off + (size_t)g.slot_cl[f] * 25, blk, false);
+  } else {
+    const int b = f - g.bnd_begin;
+    const int t = g.fbt[f];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) blk[i] = (F)0;
+    Q5 R;
+    R.T = qf[b];
+    R.u = {qf[nb + b], qf[2 * nb + b], qf[3 * nb + b]};
+    R.pg = qf[4 * nb + b];
+    if (t == 1) {
+      F tmp[25];
+#pragma unroll
+      for (int i = 0; i < 25; ++i) tmp[i] = (F)0;
+      const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, 1.0);
+      roe_jacobian<F>(tmp, ra, an, area, p.gamma, p.Rt, 1);
+#pragma unroll
+      for (int i = 0; i < 25; ++i) blk[i] -= tmp[i];
+    } else if (t == 2) {
+      acc(blk[1 + 5 * 4], -(an.x * area));
+      acc(blk[2 + 5 * 4], -(an.y * area));
+      acc(blk[3 + 5 * 4], -(an.z * area));
+      acc(blk[4 + 5 * 4], -(area * dot(R.u, an)));
+    } else {
+      const double pp = q[4 * nc + c] + p.Pamb;
+      const double T = q[c];
+      const double rho = pp / (p.Rt * T);
+      const double drdt = -rho / T;
+      const double drdp = rho / pp;
+      const double ui[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+      const double n3[3] = {an.x, an.y, an.z};
+      const double U = sqrt(ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
+      const double uit = (n3[0] * ui[0] + n3[1] * ui[1] + n3[2] * ui[2]) - us_n;
+      const double coef1 = -0.5 * rho * area;
+      const double coef2 = -0.5 * area;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double coefj = coef1 * (ui[j] * uit / (U + 1e-30) + n3[j] * U);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc(blk[(1 + i) + 5 * (1 + j)], coefj * n3[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double coefx = 0.5 * U * uit;
+        acc(blk[(1 + i) + 5 * 0], coef2 * (coefx * drdt) * n3[i]);
+        acc(blk[(1 + i) + 5 * 4], coef2 * (1.0 + coefx * drdp) * n3[i]);
+      }
+    }
+    if (p.viscous) {
+      double mu_f, k_f, vf[4];
+      V3 uf;
+      viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, false);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vf[k] = vflux[k * nf + f];
+      const V3 dv = {xf0 - xl0, xf1 - xl1, xf2 - xl2};
+      viscous_jacobian<F>(blk, -1.0, an, area, dot(an, dv), mu_f, k_f, uf, vf);
+    }
+    store_blk(bblk + (size_t)b * 25, blk, false);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// in-register pivoted LU of a 5x5 block and its solve (Loci Mat
+// decompose_lu_pivot / solve_lu_pivot; textbook Doolittle with partial
+// pivoting, first maximum wins)
+// ---------------------------------------------------------------------------
+template <typename F>
+__device__ __forceinline__ int lu5(F a[25]) {
+  int perm[5] = {0, 1, 2, 3, 4};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    int pr = k;
+    F amax = (F)fabs((double)a[k + 5 * k]);
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      const F v = (F)fabs((double)a[i + 5 * k]);
+      if (v > amax) { amax = v; pr = i; }
+    }
+#pragma unroll
+    for (int r = k + 1; r < 5; ++r) {
+      const bool s = (r == pr);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const F t = a[k + 5 * j];
+        a[k + 5 * j] = s ? a[r + 5 * j] : t;
+        a[r + 5 * j] = s ? t : a[r + 5 * j];
+      }
+      const int tp = perm[k];
+      perm[k] = s ? perm[r] : tp;
+      perm[r] = s ? tp : perm[r];
+    }
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      const F l = a[i + 5 * k] / a[k + 5 * k];
+      a[i + 5 * k] = l;
+#pragma unroll
+      for (int j = k + 1; j < 5; ++j) a[i + 5 * j] = a[i + 5 * j] - l * a[k + 5 * j];
+    }
+  }
+  return perm[0] | (perm[1] << 3) | (perm[2] << 6) | (perm[3] << 9) | (perm[4] << 12);
+}
+
+template <typename F>
+__device__ __forceinline__ void lusolve5(const F a[25], int pk, const double b[5], double x[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int pi = (pk >> (3 * i)) & 7;
+    double tmp = pi == 0 ? b[0] : pi == 1 ? b[1] : pi == 2 ? b[2] : pi == 3 ? b[3] : b[4];
+#pragma unroll
+    for (int j = 0; j < i; ++j) tmp -= a[i + 5 * j] * x[j];
+    x[i] = tmp;
+  }
+#pragma unroll
+  for (int i = 4; i >= 0; --i) {
+    double tmp = x[i];
+#pragma unroll
+    for (int j = i + 1; j < 5; ++j) tmp -= a[i + 5 * j] * x[j];
+    x[i] = tmp / a[i + 5 * i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cell assembly: cflpdt (src/solverSetup.loci:107-119), dtcfl/dtau
+// (src/solverTimestep.loci:290-329), srcJ (src/solverLinearSystem.loci:106-120
+// + boundary terms), fluid_D (:176-224), B = rhs in real_fj (:45-54), LU
+// (src/linearSolverSGS.loci:43-50) and the Jacobi start X = D^-1 B (:57-69)
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
+                                const double *__restrict__ src, const double *__restrict__ maxev,
+                                const F *__restrict__ off, const F *__restrict__ pblk,
+                                const F *__restrict__ bblk, double *__restrict__ dtau_out,
+                                F *__restrict__ lu, int *__restrict__ piv, F *__restrict__ Bout,
+                                double *__restrict__ X) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
+  double meva = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    const int f = e < 0 ? ~e : e;
+    meva += maxev[f] * g.farea[f];
+  }
+  const double vol = g.vol[c];
+  const double cflpdt = 0.5 * meva / vol;
+  double dtcfl = p.dtmax;
+  if (p.cflmax > 0) dtcfl = dmin(dtcfl, p.cflmax / cflpdt);
+  double dt = dtcfl;
+  const double rvol = 1. / vol;
+  const double pg = q[4 * nc + c];
+  const double pp = pg + p.Pamb;
+  const double T = q[c];
+  const double Rt = p.Rt;
+  const double rP = 1. / (Rt * T);
+  const double rho = (pg + p.Pamb) * rP;
+  const double rT = -rho / T;
+  const double gm = p.gamma;
+  const V3 U = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+  const double qq = 0.5 * dot(U, U);
+  const double fT = rT * qq;
+  const double fP = (1. / (gm - 1) + rP * qq);
+  const double reta = 1. / 1.0;
+  const double a2 = gm * Rt * T;
+  const double pterm = (1. - reta) * (gm - 1.) / a2;
+  const double cterm = fP * rT * reta - fT * rP * reta;
+  const double q2 = 2. * qq;
+  double s[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s[k] = src[k * nc + c];
+  const double dsu = s[1] * U.x + s[2] * U.y + s[3] * U.z;
+  const double dtemp = rvol * (s[0] * (fP * rT * reta + fT * pterm) + (dsu - s[4] - q2 * s[0]) * (rT * (pterm + rP * reta))) / (cterm * rT);
+  dt = dmin(dt, p.urelax * T / (fabs(dtemp) + 1e-30));
+  const double dp = rvol * (rT * (q2 * s[0] - dsu + s[4]) - fT * s[0]);
+  dt = dmin(dt, p.urelax * pp / (fabs(dp) + 1e-30));
+  dtau_out[c] = dt;
+
+  F sJ[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) sJ[i] = (F)0;
+  // upper faces (srcJ -= fjp) then lower faces (srcJ += fjm); the partner
+  // slot of an upper face holds L = -fjp, of a lower face U = fjm
+  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+  for (int j = m0; j < m1; ++j) {
+    const int pt = g.m_part[j];
+    if (pt >= 0) {
+      const F *blk = off + (size_t)pt * 25;
+#pragma unroll
+      for (int i = 0; i < 25; ++i) sJ[i] += blk[i];
+    } else {
+      const F *blk = pblk + (size_t)(-1 - pt) * 25;
+#pragma unroll
+      for (int i = 0; i < 25; ++i) sJ[i] -= blk[i];
+    }
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    const int f = e < 0 ? ~e : e;
+    if (f >= g.bnd_begin) {
+      const F *blk = bblk + (size_t)(f - g.bnd_begin) * 25;
+#pragma unroll
+      for (int i = 0; i < 25; ++i) sJ[i] += blk[i];
+    }
+  }
+  F D[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) D[i] = -sJ[i];
+  const double coef = vol / dt;
+  acc(D[0 + 5 * 0], coef * rT);
+  acc(D[1 + 5 * 0], coef * rT * U.x);
+  acc(D[2 + 5 * 0], coef * rT * U.y);
+  acc(D[3 + 5 * 0], coef * rT * U.z);
+  acc(D[4 + 5 * 0], coef * fT);
+  acc(D[0 + 5 * 4], coef * (pterm + rP * reta));
+  acc(D[1 + 5 * 4], coef * (pterm + rP * reta) * U.x);
+  acc(D[2 + 5 * 4], coef * (pterm + rP * reta) * U.y);
+  acc(D[3 + 5 * 4], coef * (pterm + rP * reta) * U.z);
+  acc(D[4 + 5 * 4], coef * (fT * pterm / rT + fP * reta));
+  acc(D[1 + 5 * 1], coef * rho);
+  acc(D[4 + 5 * 1], coef * rho * U.x);
+  acc(D[2 + 5 * 2], coef * rho);
+  acc(D[4 + 5 * 2], coef * rho * U.y);
+  acc(D[3 + 5 * 3], coef * rho);
+  acc(D[4 + 5 * 3], coef * rho * U.z);
+  const int pk = lu5<F>(D);
+  F *luc = lu + (size_t)c * 25;
+#pragma unroll
+  for (int i = 0; i < 25; ++i) luc[i] = D[i];
+  piv[c] = pk;
+  double b[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const F bk = (F)s[k];
+    Bout[(size_t)c * 5 + k] = bk;
+    b[k] = (double)bk;
+  }
+  double x[5];
+  lusolve5<F>(D, pk, b, x);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
+}
+
+// ---------------------------------------------------------------------------
+// block SGS (src/linearSolverSGS.loci:124-300): one level of a forward or
+// backward sweep; every cell of a level depends only on other levels.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ void k_sgs_level(DevGrid g, int lo, int hi, const F *__restrict__ off,
+                            const F *__restrict__ lu, const int *__restrict__ piv,
+                            const F *__restrict__ B, double *__restrict__ X) {
+  const int c = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= hi) return;
+  double t[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) t[k] = -(double)B[(size_t)c * 5 + k];
+  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+  for (int j = m0; j < m1; ++j) {
+    const F *M = off + (size_t)j * 25;
+    const double *xn = X + (size_t)g.m_nbr[j] * 5;
+    double xv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) xv[k] = xn[k];
+#pragma unroll
+    for (int jj = 0; jj < 5; ++jj)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) t[i] += M[i + 5 * jj] * xv[jj];
+  }
+  double v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = -t[k];
+  F a[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+  double x[5];
+  lusolve5<F>(a, piv[c], v, x);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
+}
+
+// periodic ghost copies, src/linearSolverSGS.loci:124-127 and :215-219
+__global__ void k_ghost(DevGrid g, double *__restrict__ X) {
+  const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pi >= g.nper) return;
+  const int src = g.frc[g.nfi + pi];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + pi) * 5 + k] = X[(size_t)src * 5 + k];
+}
+
+// damped Newton update, src/solverNewton.loci:62-78
+__global__ void k_update(DevGrid g, double Pamb, const double *__restrict__ X, double *__restrict__ q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const double *dq = X + (size_t)c * 5;
+  const double dT = dq[0], dp = dq[4];
+  double alpha = 1.0;
+  const double T = q[c], pg = q[4 * nc + c];
+  alpha = dmin(alpha, 0.9 * T / (fabs(dT) + 1e-30));
+  alpha = dmin(alpha, 0.9 * (pg + Pamb) / (fabs(dp) + 1e-30));
+  q[c] = T + alpha * dT;
+  q[4 * nc + c] = pg + alpha * dp;
+  q[nc + c] = q[nc + c] + alpha * dq[1];
+  q[2 * nc + c] = q[2 * nc + c] + alpha * dq[2];
+  q[3 * nc + c] = q[3 * nc + c] + alpha * dq[3];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  if (g.nb) hipLaunchKernelGGL(k_bface, dim3(nblk(g.nb)), dim3(TPB), 0, st, g, p, s.q, s.qf);
+}
+void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
+}
+void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_face_flux, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi,
+                     s.iflux, s.vflux, s.maxev);
+}
+int norm_parts(const DevGrid &g) { return nblk(g.nc); }
+void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_src, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.viscous, vtot, s.iflux,
+                     s.vflux, s.src, s.npart);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(TPB), 0, st, g, nblk(g.nc), s.npart, s.nout);
+}
+void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st) {
+  if (fp32)
+    hipLaunchKernelGGL(k_face_jac<float>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
+                       s.phi, s.vflux, (float *)s.off, (float *)s.pblk, (float *)s.bblk);
+  else
+    hipLaunchKernelGGL(k_face_jac<double>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
+                       s.phi, s.vflux, (double *)s.off, (double *)s.pblk, (double *)s.bblk);
+}
+void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st) {
+  if (fp32)
+    hipLaunchKernelGGL(k_cell_assemble<float>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src,
+                       s.maxev, (const float *)s.off, (const float *)s.pblk, (const float *)s.bblk,
+                       s.dtau, (float *)s.lu, s.piv, (float *)s.B, s.X);
+  else
+    hipLaunchKernelGGL(k_cell_assemble<double>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src,
+                       s.maxev, (const double *)s.off, (const double *)s.pblk, (const double *)s.bblk,
+                       s.dtau, (double *)s.lu, s.piv, (double *)s.B, s.X);
+}
+void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st) {
+  const int n = hi - lo;
+  if (n <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(k_sgs_level<float>, dim3(nblk(n)), dim3(TPB), 0, st, g, lo, hi,
+                       (const float *)s.off, (const float *)s.lu, s.piv, (const float *)s.B, s.X);
+  else
+    hipLaunchKernelGGL(k_sgs_level<double>, dim3(nblk(n)), dim3(TPB), 0, st, g, lo, hi,
+                       (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, s.X);
+}
+void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
+  if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);
+}
+void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.Pamb, s.X, s.q);
+}
+
+}  // namespace fpsi

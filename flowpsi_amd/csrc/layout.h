@@ -1,0 +1,62 @@
+// Device layout of one grid and its Newton-step buffers.
+//
+// Cells are stored in level order (Gauss-Seidel dependency level, then the
+// caller's index), so every SGS level is a contiguous cell range and the
+// block rows of the matrix stream sequentially through a sweep.
+// Faces are stored as [interior | periodic | boundary], interior and
+// periodic faces ordered by their (new) cl cell, boundary faces by type
+// then cl, so a wavefront reads neighbouring cells and takes one branch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fpsi {
+
+struct DevGrid {
+  int nc, nf;
+  int nfi;        // interior faces   [0, nfi)
+  int nper;       // periodic faces   [nfi, nfi + nper)
+  int nb;         // boundary faces   [nfi + nper, nf)
+  int bnd_begin;  // nfi + nper
+  // faces
+  const double *fnx, *fny, *fnz, *farea, *fcx, *fcy, *fcz;
+  const int *fcl;    // left cell
+  const int *frc;    // right cell (interior), partner cell (periodic), -1 (boundary)
+  const int *fbt;    // boundary type
+  const double *psx, *psy, *psz;  // [nper] ghost-centre shift
+  // cells
+  const double *vol, *ccx, *ccy, *ccz;
+  const double *lsq;  // [6][nc] inverse least-squares normal matrix
+  // gather lists per cell: lower, upper, boundary faces (each ascending in the
+  // caller's face index); entry = face, or ~face when the cell is its cl.
+  const int *g_off, *g_ent;
+  // block rows: upper faces then lower faces (each ascending); per slot the
+  // neighbour (cell, or nc + periodic index) and the partner slot holding the
+  // same face's other block (>= 0), or -(1 + periodic index).
+  const int *m_off, *m_nbr, *m_part;
+  const int *slot_cl;  // [nfi + nper] slot of U_f = fjm in row cl
+  const int *slot_cr;  // [nfi] slot of L_f = -fjp in row cr
+  const int *old_id;   // [nc] caller's index of each cell (tie-breaks of the norms)
+};
+
+struct DevState {
+  double *q;      // [5][nc]
+  double *qf;     // [5][nb]
+  double *grad;   // [15][nc]
+  double *phi;    // [5][nc]
+  double *iflux;  // [5][nf]
+  double *vflux;  // [4][nf]
+  double *maxev;  // [nf]
+  double *src;    // [5][nc]
+  double *dtau;   // [nc]
+  void *off;      // [nnz][25] real_fj
+  void *pblk;     // [nper][25] real_fj (fjp of periodic faces)
+  void *bblk;     // [nb][25] real_fj (boundary contribution to srcJ)
+  void *lu;       // [nc][25] real_fj
+  int *piv;       // [nc] packed 3 bits per row
+  void *B;        // [nc][5] real_fj
+  double *X;      // [nc + nper][5]
+  double *npart;  // norm partials
+  double *nout;   // final norms
+};
+
+}  // namespace fpsi

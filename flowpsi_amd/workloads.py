@@ -1,0 +1,42 @@
+"""The Newton-step workloads: grid, initial state and options facts.
+
+box32   -- inviscid Euler, 32^3 uniform hex box on the unit cube, farfield on
+           all sides, Mach 0.3 freestream (p = 101325 Pa, T = 300 K) plus
+           seeded 1% Gaussian perturbations of rho and p, Roe flux with
+           Venkatakrishnan-limited second-order reconstruction, CFL 10,
+           5 SGS sweeps, 50 Newton steps.
+channel128 -- laminar Navier-Stokes, 128^3 channel, periodic x/z, isothermal
+           no-slip walls at y = +-h with 1.1 geometric stretching, Mach 0.2
+           Poiseuille flow plus seeded 2% velocity noise, Re = 500 on h,
+           Sutherland viscosity, HLLC + viscous fluxes, CFL 10, 10 SGS sweeps.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import grids
+
+A300 = float(np.sqrt(grids.GAMMA * grids.R_AIR * 300.0))
+
+BASE = dict(gamma=grids.GAMMA, Rt=grids.R_AIR, Pambient=0.0, limiter="venkatakrishnan", Kl=1.0,
+            Twall=300.0, pg0=101325.0, T0=300.0, dtmax=1e30, urelax=1e30, solver="sgs",
+            precision="fp64")
+
+
+def box(n=32, seed=1, **over):
+    g = grids.box_grid(n)
+    q = grids.box_state(g, seed=seed)
+    prm = dict(BASE, flux="roe", viscous=0, u0=(0.3 * A300, 0.0, 0.0), cflmax=10.0, sgs_iter=5)
+    prm.update(over)
+    return g, q, prm
+
+
+def channel(n=128, seed=2, nx=None, ny=None, nz=None, **over):
+    g = grids.channel_grid(nx or n, ny or n, nz or n)
+    q = grids.channel_state(g, seed=seed)
+    prm = dict(BASE, flux="hllc", viscous=1, u0=(0.0, 0.0, 0.0), cflmax=10.0, sgs_iter=10)
+    prm.update(over)
+    return g, q, prm
+
+
+WORKLOADS = {"box32": dict(fn=box, steps=50), "channel128": dict(fn=channel, steps=None)}

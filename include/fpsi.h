@@ -1,0 +1,165 @@
+/*
+ * fpsi.h -- C ABI of libfpsi_gpu.so, the MI355X (gfx950) implementation of
+ * flowPsi's implicit finite-volume Newton step.
+ *
+ * Each entry point stands in for a group of flowPsi Loci rules; the rule
+ * bodies it replaces are cited per function.  A Loci rule class binds them
+ * by forwarding its compute(sequence) to the matching call (see the
+ * adapter listing that ships with this header's documentation).
+ *
+ * Conventions
+ *   - Return 0 on success, a negative FPSI_ERR_* code on failure; the message
+ *     is available from fpsi_last_error().  Nothing throws across the ABI.
+ *   - Host arrays are owned by the caller and copied in/out; the context owns
+ *     all device memory.  Calls are ordered on the context's HIP stream and
+ *     are synchronous at return unless stated otherwise.
+ *   - Arrays are structure-of-arrays in the caller's (reference) cell and
+ *     face order: state q[5][nc] = (T, u, v, w, pg); blocks are 5x5,
+ *     column-major (element (i,j) at i + 5*j), as Loci::Mat stores them.
+ *   - One context per host thread.
+ */
+#ifndef FPSI_H
+#define FPSI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fpsi_ctx fpsi_ctx;
+
+enum {
+  FPSI_OK = 0,
+  FPSI_ERR_ARG = -1,     /* bad argument / inconsistent grid */
+  FPSI_ERR_HIP = -2,     /* a HIP runtime call failed */
+  FPSI_ERR_NAN = -3,     /* NaN residual (src/solverResidual.loci:202-208) */
+  FPSI_ERR_STATE = -4,   /* call out of order (no state, no assembly, ...) */
+  FPSI_ERR_LU = -5       /* singular diagonal block */
+};
+
+/* boundary types of a face (btype) */
+enum { FPSI_BC_INTERIOR = 0, FPSI_BC_FARFIELD = 1, FPSI_BC_WALL = 2,
+       FPSI_BC_SYMMETRY = 3, FPSI_BC_PERIODIC = 4 };
+enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
+enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
+enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
+enum { FPSI_SOLVER_SGS = 0 };
+
+/* Grid: the Loci FVM facts cl, cr, area{n,sada}, facecenter, vol,
+ * cellcenter, pmap (used at src/fluxInviscid.loci:1170-1188,
+ * src/solverLinearSystem.loci:106-120, src/solverPeriodic.loci:25-87).
+ * Interior faces must have cl < cr (cell order is then a topological
+ * order of the Gauss-Seidel dependency graph). */
+typedef struct {
+  int nc, nf;
+  const int *cl;        /* [nf] */
+  const int *cr;        /* [nf] >= 0 interior, -1 boundary (periodic included) */
+  const int *btype;     /* [nf] FPSI_BC_* */
+  const int *pface;     /* [nf] periodic partner face, -1 otherwise */
+  const double *n;      /* [3][nf] unit normal, cl -> cr / outward */
+  const double *area;   /* [nf] */
+  const double *fc;     /* [3][nf] face centre */
+  const double *vol;    /* [nc] */
+  const double *cc;     /* [3][nc] cell centre */
+  const double *pshift; /* [3][nf] periodic ghost centre = cc[cl[pface]] + pshift */
+} fpsi_grid_desc;
+
+/* Parameters, with flowPsi's option names and code defaults:
+ * gamma/molecularMass/p0 src/solverEoS.loci:27-47; flux include/inviscidFlux.h:82-90;
+ * limiter/Kl (Loci FVM); sutherlandParameters src/fluxViscous.loci:59-110;
+ * Twall src/bcViscousWall.loci:152-154; farfield reference state
+ * src/bcInflow.loci:311-340; cflmax/dtmax/urelax src/solverTimestep.loci:281-297;
+ * gauss_seidel_iter src/linearSolverSGS.loci:39-41. */
+typedef struct {
+  double gamma, Rtilde, Pambient;
+  int flux;             /* FPSI_FLUX_* */
+  int limiter;          /* FPSI_LIM_* */
+  double Kl;
+  int viscous;          /* 0 inviscid, 1 laminar */
+  double suth[6];       /* a1 a2 a3 k1 k2 k3 */
+  double Twall;
+  double pg0, T0, u0[3];
+  double cflmax;        /* <= 0: not set */
+  double dtmax, urelax;
+  int linear_solver;    /* FPSI_SOLVER_* */
+  int sgs_iter;         /* symmetric sweeps */
+  int precision;        /* FPSI_PREC_*: storage/arithmetic of real_fj (Jacobians, D, LU, B) */
+  double minPg;         /* floor of the reconstructed gage pressure */
+} fpsi_params;
+
+/* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
+ * rms fields are the volume-weighted sums before the square root. */
+typedef struct {
+  double rtrms, etrms, mtrms, rtmax, etmax, mtmax;
+  double ccmaxr[3], ccmaxe[3];
+} fpsi_norms;
+
+typedef struct {
+  int nlevels;          /* Gauss-Seidel dependency levels */
+  int max_level_width;
+  int nnz_blocks;       /* off-diagonal blocks stored */
+  double device_bytes;  /* device memory held by the context */
+} fpsi_info;
+
+/* Build the context: renumbers cells into level order, builds the cell->face
+ * gather lists and the block-row layout, copies the grid to the device.
+ * Replaces Loci's setup of the FVM maps (createLowerUpper,
+ * src/gridReader.loci:226-229) for this path. */
+int fpsi_create(fpsi_ctx **ctx, const fpsi_grid_desc *grid);
+void fpsi_destroy(fpsi_ctx *ctx);
+const char *fpsi_last_error(const fpsi_ctx *ctx);
+int fpsi_get_info(fpsi_ctx *ctx, fpsi_info *info);
+
+/* Options facts (the $rule default(...) values listed in fpsi_params). */
+int fpsi_set_params(fpsi_ctx *ctx, const fpsi_params *params);
+
+/* temperature, u, gagePressure stores (include/flowPsi.lh:53-65). */
+int fpsi_set_state(fpsi_ctx *ctx, const double *q);
+int fpsi_get_state(fpsi_ctx *ctx, double *q);
+
+/* Residual path: boundary face states (src/bcInflow.loci:311-340,
+ * src/bcViscousWall.loci:205-214, src/bcImpermeable.loci:48-64), gradients
+ * and limiters (Loci FVM), iflux (src/fluxInviscid.loci:1170-1188 and the
+ * boundary iflux rules), vflux (src/fluxViscous.loci:217-292), the src join
+ * (src/fluxInviscid.loci:1253-1268, src/fluxViscous.loci:299-310) and the
+ * residual norms (src/solverResidual.loci:74-112).  Any output may be NULL.
+ * src[5][nc], iflux[5][nf], vflux[4][nf]. */
+int fpsi_residual(fpsi_ctx *ctx, double *src, double *iflux, double *vflux, fpsi_norms *norms);
+
+/* Gradient [15][nc] (d q_k / d x_d at 3k+d) and limiter [5][nc] of the current state. */
+int fpsi_gradients(fpsi_ctx *ctx, double *grad, double *phi);
+
+/* Linear system of one steady Newton step: dtau (src/solverTimestep.loci:290-329),
+ * fjp/fjm (src/fluxInviscid.loci:1190-1221, src/fluxViscous.loci:315-403),
+ * boundary srcJ (src/bcInflow.loci:357-375, src/bcViscousWall.loci:230-240,
+ * src/bcImpermeable.loci:75-100, src/fluxViscous.loci:405-448), srcJ and
+ * fluid_D/L/U/B (src/solverLinearSystem.loci:27-224), LU of D
+ * (src/linearSolverSGS.loci:43-50) and the Jacobi start (:57-69). */
+int fpsi_assemble(fpsi_ctx *ctx);
+
+/* Copy the assembled system out in the reference layout: fjp/fjm [nf][25]
+ * for interior and periodic faces (zero elsewhere), D [nc][25] (LU factors),
+ * pivot [nc][5], B [nc][5] (real_fj of the current precision: 8 or 4 bytes
+ * per element), dtau [nc].  Any output may be NULL. */
+int fpsi_get_system(fpsi_ctx *ctx, void *fjp, void *fjm, void *D, int *pivot, void *B, double *dtau);
+
+/* Block SGS on the assembled system (src/linearSolverSGS.loci:57-331):
+ * Jacobi start then nsweeps symmetric level-scheduled sweeps.
+ * dq[nc][5] may be NULL (the result stays on the device as dfluid). */
+int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
+
+/* Damped update q += alpha dq (src/solverNewton.loci:62-78) with the
+ * current dfluid. */
+int fpsi_update(fpsi_ctx *ctx);
+
+/* nsteps fused steady Newton steps, device resident: residual, assembly,
+ * linear solve, update.  hist[nsteps] (nullable) receives the norms of the
+ * residual evaluated at the start of each step. */
+int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
+
+/* Wait for the context's stream. */
+int fpsi_synchronize(fpsi_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,245 @@
+"""ctypes front end of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg; the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OgGrid(C.Structure):
+    _fields_ = [("nc", C.c_int), ("nf", C.c_int), ("cl", _ip), ("cr", _ip), ("btype", _ip),
+                ("pface", _ip), ("n", _dp), ("area", _dp), ("fc", _dp), ("vol", _dp),
+                ("cc", _dp), ("pshift", _dp)]
+
+
+class OgParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("Rt", C.c_double), ("Pambient", C.c_double),
+                ("flux", C.c_int), ("limiter", C.c_int), ("Kl", C.c_double),
+                ("viscous", C.c_int), ("suth", C.c_double * 6), ("Twall", C.c_double),
+                ("pg0", C.c_double), ("T0", C.c_double), ("u0", C.c_double * 3),
+                ("cflmax", C.c_double), ("dtmax", C.c_double), ("urelax", C.c_double),
+                ("sgs_iter", C.c_int), ("minPg", C.c_double)]
+
+
+class OgNorms(C.Structure):
+    _fields_ = [("rtrms", C.c_double), ("etrms", C.c_double), ("mtrms", C.c_double),
+                ("rtmax", C.c_double), ("etmax", C.c_double), ("mtmax", C.c_double),
+                ("ccmaxr", C.c_double * 3), ("ccmaxe", C.c_double * 3)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=subprocess.DEVNULL)
+
+
+_libs = {}
+
+
+def lib(fj="f64"):
+    if fj not in _libs:
+        path = os.path.join(BUILD, f"liboracle_{fj}.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.og_real_fj_bytes.restype = C.c_int
+        L.og_lu_decompose.restype = C.c_int
+        L.og_newton.restype = C.c_int
+        _libs[fj] = L
+    return _libs[fj]
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OGrid:
+    """Keeps the numpy arrays alive behind an og_grid struct."""
+
+    def __init__(self, g):
+        self.g = g
+        self.arrs = dict(
+            cl=np.ascontiguousarray(g.cl, np.int32), cr=np.ascontiguousarray(g.cr, np.int32),
+            btype=np.ascontiguousarray(g.btype, np.int32), pface=np.ascontiguousarray(g.pface, np.int32),
+            n=np.ascontiguousarray(g.n, np.float64), area=np.ascontiguousarray(g.area, np.float64),
+            fc=np.ascontiguousarray(g.fc, np.float64), vol=np.ascontiguousarray(g.vol, np.float64),
+            cc=np.ascontiguousarray(g.cc, np.float64), pshift=np.ascontiguousarray(g.pshift, np.float64))
+        a = self.arrs
+        self.s = OgGrid(g.nc, g.nf, _ptr(a["cl"], C.c_int), _ptr(a["cr"], C.c_int),
+                        _ptr(a["btype"], C.c_int), _ptr(a["pface"], C.c_int), _ptr(a["n"]),
+                        _ptr(a["area"]), _ptr(a["fc"]), _ptr(a["vol"]), _ptr(a["cc"]),
+                        _ptr(a["pshift"]))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def params(p: dict) -> OgParams:
+    s = OgParams()
+    s.gamma = p.get("gamma", 1.4)
+    s.Rt = p.get("Rt", 287.0)
+    s.Pambient = p.get("Pambient", 0.0)
+    s.flux = {"hllc": 0, "roe": 1}[p.get("flux", "hllc")]
+    s.limiter = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}[p.get("limiter", "venkatakrishnan")]
+    s.Kl = p.get("Kl", 1.0)
+    s.viscous = int(p.get("viscous", 0))
+    s.suth[:] = p.get("suth", (1.458e-6, 1.5, 110.4, 2.495e-3, 1.5, 194.0))
+    s.Twall = p.get("Twall", 300.0)
+    s.pg0 = p.get("pg0", 101325.0)
+    s.T0 = p.get("T0", 300.0)
+    s.u0[:] = p.get("u0", (0.0, 0.0, 0.0))
+    s.cflmax = p.get("cflmax", 0.0)
+    s.dtmax = p.get("dtmax", 1e30)
+    s.urelax = p.get("urelax", 1e30)
+    s.sgs_iter = p.get("sgs_iter", 5)
+    s.minPg = p.get("minPg", -s.Pambient)
+    return s
+
+
+class Oracle:
+    """Whole-grid stages of the oracle on one grid."""
+
+    def __init__(self, g, prm: dict, fj="f64"):
+        self.g = g
+        self.og = OGrid(g)
+        self.p = params(prm)
+        self.L = lib(fj)
+        self.fj = np.float64 if fj == "f64" else np.float32
+
+    def face_states(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        qf = np.zeros((5, self.g.nf))
+        self.L.og_face_states(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf))
+        return qf
+
+    def gradients(self, q, qf=None):
+        q = np.ascontiguousarray(q, np.float64)
+        if qf is None:
+            qf = self.face_states(q)
+        grad = np.zeros((15, self.g.nc)); phi = np.zeros((5, self.g.nc))
+        self.L.og_gradients(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf), _ptr(grad), _ptr(phi))
+        return grad, phi
+
+    def fluxes(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        qf = self.face_states(q)
+        grad, phi = self.gradients(q, qf)
+        nf = self.g.nf
+        iflux = np.zeros((5, nf)); vflux = np.zeros((4, nf)); maxev = np.zeros(nf)
+        self.L.og_fluxes(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf), _ptr(grad), _ptr(phi),
+                         _ptr(iflux), _ptr(vflux), _ptr(maxev))
+        return dict(qf=qf, grad=grad, phi=phi, iflux=iflux, vflux=vflux, maxev=maxev)
+
+    def residual(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        nc, nf = self.g.nc, self.g.nf
+        src = np.zeros((5, nc)); iflux = np.zeros((5, nf)); vflux = np.zeros((4, nf))
+        nrm = OgNorms()
+        self.L.og_residual(self.og.ref, C.byref(self.p), _ptr(q), _ptr(src), _ptr(iflux),
+                           _ptr(vflux), C.byref(nrm))
+        return src, iflux, vflux, nrm
+
+    def assemble(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        nc, nf = self.g.nc, self.g.nf
+        src = np.zeros((5, nc)); dtau = np.zeros(nc)
+        fjp = np.zeros((nf, 25), self.fj); fjm = np.zeros((nf, 25), self.fj)
+        D = np.zeros((nc, 25), self.fj); piv = np.zeros((nc, 5), np.int32)
+        B = np.zeros((nc, 5), self.fj)
+        nrm = OgNorms()
+        self.L.og_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(src), _ptr(dtau),
+                           fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
+                           D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
+                           B.ctypes.data_as(C.c_void_p), C.byref(nrm))
+        return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm)
+
+    def sgs(self, sys, nsweeps):
+        X = np.zeros((self.g.nc, 5))
+        self.L.og_sgs(self.og.ref, nsweeps, sys["fjp"].ctypes.data_as(C.c_void_p),
+                      sys["fjm"].ctypes.data_as(C.c_void_p), sys["D"].ctypes.data_as(C.c_void_p),
+                      _ptr(sys["piv"], C.c_int), sys["B"].ctypes.data_as(C.c_void_p), _ptr(X))
+        return X
+
+    def newton(self, q, nsteps):
+        q = np.array(q, np.float64, copy=True, order="C")
+        hist = np.zeros((nsteps, 6))
+        self.L.og_newton(self.og.ref, C.byref(self.p), _ptr(q), nsteps, _ptr(hist))
+        return q, hist
+
+
+# ---- point functions -------------------------------------------------------
+
+def hllc_flux(qL, qR, n, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4, eta=1.0, fj="f64"):
+    f = np.zeros(5)
+    n = np.asarray(n, float); UL = np.asarray(qL[1:4], float); UR = np.asarray(qR[1:4], float)
+    lib(fj).og_hllc_flux(_ptr(f), C.c_double(qL[4]), C.c_double(qL[0]), _ptr(UL), C.c_double(qR[4]),
+                         C.c_double(qR[0]), _ptr(UR), _ptr(n), C.c_double(area), C.c_double(pamb),
+                         C.c_double(Rt), C.c_double(gamma), C.c_double(0.0), C.c_double(eta))
+    return f
+
+
+def roe_flux(qL, qR, n, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4, eta=1.0, fj="f64"):
+    f = np.zeros(5)
+    n = np.asarray(n, float); UL = np.asarray(qL[1:4], float); UR = np.asarray(qR[1:4], float)
+    lib(fj).og_roe_flux(_ptr(f), C.c_double(qL[4]), C.c_double(qL[0]), _ptr(UL), C.c_double(qR[4]),
+                        C.c_double(qR[0]), _ptr(UR), _ptr(n), C.c_double(area), C.c_double(pamb),
+                        C.c_double(Rt), C.c_double(gamma), C.c_double(0.0), C.c_double(eta))
+    return f
+
+
+def inviscid_flux(q, n, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4):
+    f = np.zeros(5)
+    n = np.asarray(n, float); U = np.asarray(q[1:4], float)
+    lib().og_inviscid_flux(_ptr(f), C.c_double(q[4]), C.c_double(q[0]), _ptr(U), _ptr(n),
+                           C.c_double(area), C.c_double(pamb), C.c_double(Rt), C.c_double(gamma),
+                           C.c_double(0.0))
+    return f
+
+
+def roe_jacobian(qL, qR, n, dir, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4, eta=1.0, fj="f64"):
+    """5x5 block as a numpy [row, col] matrix."""
+    L = lib(fj)
+    dt = np.float64 if fj == "f64" else np.float32
+    blk = np.zeros(25, dt)
+    n = np.asarray(n, float); UL = np.asarray(qL[1:4], float); UR = np.asarray(qR[1:4], float)
+    L.og_roe_jacobian(blk.ctypes.data_as(C.c_void_p), C.c_double(qL[4]), C.c_double(qL[0]), _ptr(UL),
+                      C.c_double(qR[4]), C.c_double(qR[0]), _ptr(UR), _ptr(n), C.c_double(area),
+                      C.c_double(pamb), C.c_double(Rt), C.c_double(gamma), C.c_double(0.0),
+                      C.c_double(eta), C.c_int(dir))
+    return blk.reshape(5, 5).T.astype(np.float64)  # column-major -> [row, col]
+
+
+def lu_decompose(A, fj="f64"):
+    dt = np.float64 if fj == "f64" else np.float32
+    a = np.ascontiguousarray(np.asarray(A, dt).T).ravel().copy()  # column-major
+    piv = np.zeros(5, np.int32)
+    rc = lib(fj).og_lu_decompose(a.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int), 5)
+    return a, piv, rc
+
+
+def lu_solve(a, piv, b, fj="f64"):
+    x = np.zeros(5)
+    b = np.asarray(b, np.float64)
+    lib(fj).og_lu_solve(a.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int), _ptr(b), _ptr(x), 5)
+    return x
+
+
+def cv_state(kind, qi, q0, n, pamb=0.0, Rt=287.0, gamma=1.4, beta=1.0):
+    pgb = C.c_double(); Tb = C.c_double(); ub = np.zeros(3)
+    ui = np.asarray(qi[1:4], float); u0 = np.asarray(q0[1:4], float); n = np.asarray(n, float)
+    fn = lib().og_cv_inflow if kind == "inflow" else lib().og_cv_outflow
+    fn(C.byref(pgb), C.byref(Tb), _ptr(ub), C.c_double(qi[4]), C.c_double(qi[0]), _ptr(ui),
+       C.c_double(q0[4]), C.c_double(q0[0]), _ptr(u0), _ptr(n), C.c_double(0.0), C.c_double(pamb),
+       C.c_double(gamma), C.c_double(Rt), C.c_double(beta))
+    return np.array([Tb.value, *ub, pgb.value])

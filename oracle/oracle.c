@@ -1,0 +1,1289 @@
+/*
+ * CPU oracle for the flowPsi implicit finite-volume Newton step.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg, never by the product library.
+ *
+ * Each function restates the flowPsi rule body it cites (file:line in the
+ * flowPsi sources).  Pieces that flowPsi takes from the Loci FVM module
+ * (least-squares gradients, limiters, face reconstruction, face gradients,
+ * block LU, the order in which apply-rules join) are not in the flowPsi
+ * sources; they are defined here from textbook methods, named where they
+ * occur, and the GPU library implements the same definitions.
+ *
+ * Build: gcc -O2 -ffp-contract=off (no FMA contraction, so the operation
+ * order written here is the order executed).  REAL_FJ_FLOAT selects
+ * real_fj = float (flowPsi's default, include/flowTypes.h:153-159).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double real;
+#ifdef REAL_FJ_FLOAT
+typedef float real_fj;
+#else
+typedef double real_fj;
+#endif
+
+#define NB 5
+#define M_(a, i, j) (a)[(i) + NB * (j)] /* Loci::Mat column-major element (i,j) */
+
+static inline real dot3(const real *a, const real *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline real maxr(real a, real b) { return a > b ? a : b; }
+static inline real minr(real a, real b) { return a < b ? a : b; }
+static inline void addfj(real_fj *a, real v) { *a = (real_fj)((real)*a + v); }
+static inline void subfj(real_fj *a, real v) { *a = (real_fj)((real)*a - v); }
+
+int og_real_fj_bytes(void) { return (int)sizeof(real_fj); }
+
+/* include/inviscidFlux.h:25-45 */
+void og_inviscid_flux(double *iflux, double pg, double T, const double *U, const double *an,
+                      double area, double pambient, double Rt, double gamma, double Us_n) {
+  real gm1 = gamma - 1;
+  real rho = (pg + pambient) / (Rt * T);
+  real h0 = Rt * T * gamma / gm1 + 0.5 * dot3(U, U);
+  const real e0 = h0 - Rt * T;
+  const real uta = dot3(U, an);
+  const real ut = uta - Us_n;
+  real mdot = area * rho * ut;
+  real p = pg + pambient;
+  iflux[0] = mdot;
+  iflux[1] = mdot * U[0] + area * pg * an[0];
+  iflux[2] = mdot * U[1] + area * pg * an[1];
+  iflux[3] = mdot * U[2] + area * pg * an[2];
+  iflux[4] = mdot * e0 + area * p * uta;
+}
+
+/* src/fluxInviscid.loci:157-256 (hllc_flux) */
+void og_hllc_flux(double *iflux, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                  const double *Ur, const double *an, double area, double pambient, double Rt,
+                  double gamma, double Us_n, double Eta) {
+  real gm1 = gamma - 1;
+  real rl = (pgl + pambient) / (Rt * Tl);
+  real rr = (pgr + pambient) / (Rt * Tr);
+  real h0l = Rt * Tl * gamma / gm1 + 0.5 * dot3(Ul, Ul);
+  real h0r = Rt * Tr * gamma / gm1 + 0.5 * dot3(Ur, Ur);
+  const real e0l = h0l - Rt * Tl;
+  const real e0r = h0r - Rt * Tr;
+  const real srl = sqrt(rl), srr = sqrt(rr);
+  const real roe_factor = 1. / (srl + srr);
+  real U_[3];
+  for (int k = 0; k < 3; ++k) U_[k] = (Ul[k] * srl + Ur[k] * srr) * roe_factor;
+  const real ut_ = dot3(U_, an) - Us_n;
+  const real h0_ = (h0l * srl + h0r * srr) * roe_factor;
+  const real a_ = sqrt(maxr(gm1 * (h0_ - 0.5 * dot3(U_, U_)), 0.0));
+  const real EtaP = Eta + 1., EtaM = Eta - 1.;
+  const real sigma = sqrt(maxr(ut_ * ut_ * EtaM * EtaM + 4. * Eta * a_ * a_, 1e-30));
+  const real evl_ = 0.5 * (ut_ * EtaP - sigma);
+  const real evr_ = 0.5 * (ut_ * EtaP + sigma);
+  const real utla = dot3(Ul, an), utra = dot3(Ur, an);
+  const real utl = utla - Us_n, utr = utra - Us_n;
+  const real al2 = gamma * Rt * Tl, ar2 = gamma * Rt * Tr;
+  const real sigmar = sqrt(maxr(utr * utr * EtaM * EtaM + 4. * Eta * ar2, 0.0));
+  const real sigmal = sqrt(maxr(utl * utl * EtaM * EtaM + 4. * Eta * al2, 0.0));
+  real SL = minr(evl_, .5 * (utl * EtaP - sigmal));
+  real SR = maxr(evr_, .5 * (utr * EtaP + sigmar));
+  real lmdot = rl * utl, rmdot = rr * utr;
+  real pl = pgl + pambient, pr = pgr + pambient;
+  real coef = 0.5 * area;
+  real AL = coef * ((SL > 0) ? (SR < 0 ? 1. : 2.) : (SR < 0 ? 0. : 1.));
+  real AR = coef * ((SR < 0) ? (SL > 0 ? 1. : 2.) : (SL > 0 ? 0. : 1.));
+  real AM = coef * ((SL > 0 || SR < 0) ? 0. : 1.);
+  iflux[0] = AL * lmdot + AR * rmdot;
+  iflux[1] = AL * lmdot * Ul[0] + AR * rmdot * Ur[0] + an[0] * (AL * pgl + AR * pgr);
+  iflux[2] = AL * lmdot * Ul[1] + AR * rmdot * Ur[1] + an[1] * (AL * pgl + AR * pgr);
+  iflux[3] = AL * lmdot * Ul[2] + AR * rmdot * Ur[2] + an[2] * (AL * pgl + AR * pgr);
+  iflux[4] = AL * (lmdot * e0l + pl * utla) + AR * (rmdot * e0r + pr * utra);
+  iflux[0] -= AM * (SL * rl + SR * rr);
+  iflux[1] -= AM * (SL * rl * Ul[0] + SR * rr * Ur[0]);
+  iflux[2] -= AM * (SL * rl * Ul[1] + SR * rr * Ur[1]);
+  iflux[3] -= AM * (SL * rl * Ul[2] + SR * rr * Ur[2]);
+  iflux[4] -= AM * (SL * rl * e0l + SR * rr * e0r);
+  const real SM = ((rr * utr * (SR - utr) - rl * utl * (SL - utl) + pgl - pgr) /
+                   (rr * (SR - utr) - rl * (SL - utl)));
+  const real rl_star = rl * (SL - utl) / (SL - SM);
+  const real rr_star = rr * (SR - utr) / (SR - SM);
+  const real pg_star = rl * (utl - SL) * (utl - SM) + pgl;
+  const real p_star = pg_star + pambient;
+  const real rul_star = ((SL - utl) * rl * Ul[0] + (pg_star - pgl) * an[0]) / (SL - SM);
+  const real rvl_star = ((SL - utl) * rl * Ul[1] + (pg_star - pgl) * an[1]) / (SL - SM);
+  const real rwl_star = ((SL - utl) * rl * Ul[2] + (pg_star - pgl) * an[2]) / (SL - SM);
+  const real rur_star = ((SR - utr) * rr * Ur[0] + (pg_star - pgr) * an[0]) / (SR - SM);
+  const real rvr_star = ((SR - utr) * rr * Ur[1] + (pg_star - pgr) * an[1]) / (SR - SM);
+  const real rwr_star = ((SR - utr) * rr * Ur[2] + (pg_star - pgr) * an[2]) / (SR - SM);
+  const real re0l_star = ((SL - utl) * rl * e0l - pl * utl + p_star * SM) / (SL - SM);
+  const real re0r_star = ((SR - utr) * rr * e0r - pr * utr + p_star * SM) / (SR - SM);
+  const real CR = SR - fabs(SM);
+  const real CL = SL + fabs(SM);
+  iflux[0] += AM * (CR * rr_star + CL * rl_star);
+  iflux[1] += AM * (CR * rur_star + CL * rul_star);
+  iflux[2] += AM * (CR * rvr_star + CL * rvl_star);
+  iflux[3] += AM * (CR * rwr_star + CL * rwl_star);
+  iflux[4] += AM * (CR * re0r_star + CL * re0l_star);
+}
+
+/* src/fluxInviscid.loci:869-1014 (roeCT_flux, ENTROPY_FIX defined) */
+void og_roe_flux(double *iflux, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                 const double *Ur, const double *an, double area, double pambient, double Rt,
+                 double gamma, double Us_n, double Eta) {
+  const real pl = pgl + pambient, pr = pgr + pambient;
+  const real rl = pl / (Rt * Tl), rr = pr / (Rt * Tr);
+  const real gm1 = gamma - 1.;
+  const real re0l = rl * (Rt * Tl / gm1 + 0.5 * dot3(Ul, Ul));
+  const real re0r = rr * (Rt * Tr / gm1 + 0.5 * dot3(Ur, Ur));
+  const real h0l = (re0l + pl) / rl, h0r = (re0r + pr) / rr;
+  const real r_ = sqrt(rl * rr);
+  const real srl = sqrt(rl), srr = sqrt(rr);
+  const real roe_factor = 1. / (srl + srr);
+  real U_[3];
+  for (int k = 0; k < 3; ++k) U_[k] = (Ul[k] * srl + Ur[k] * srr) * roe_factor;
+  const real ut_ = dot3(U_, an) - Us_n;
+  const real uta_ = dot3(U_, an);
+  const real q2_ = dot3(U_, U_);
+  const real h0_ = (h0l * srl + h0r * srr) * roe_factor;
+  const real h_ = h0_ - 0.5 * q2_;
+  const real al2 = gamma * Rt * Tl, ar2 = gamma * Rt * Tr;
+  const real utl = dot3(Ul, an) - Us_n, utr = dot3(Ur, an) - Us_n;
+  const real utla = dot3(Ul, an), utra = dot3(Ur, an);
+  if (fabs(pl - pr) / (pl + pr + 1e-33) > .1) Eta = 1.;
+  real af2 = gm1 * h_;
+  af2 = maxr(af2, 1e-30);
+  const real EtaP = Eta + 1., EtaM = Eta - 1.;
+  real a_ = sqrt(af2);
+  const real sigma = sqrt(ut_ * ut_ * EtaM * EtaM + 4. * Eta * a_ * a_);
+  real ev1 = fabs(ut_);
+  real ev2 = fabs(ut_ * EtaP - sigma) / 2.;
+  real ev3 = fabs(ut_ * EtaP + sigma) / 2.;
+  /* Harten-Hyman entropy fix (src/fluxInviscid.loci:940-962) */
+  real sigmal = sqrt(utl * utl * EtaM * EtaM + 4. * Eta * al2);
+  real sigmar = sqrt(utr * utr * EtaM * EtaM + 4. * Eta * ar2);
+  const real e2l = (utl * EtaP - sigmal) / 2., e2r = (utr * EtaP - sigmar) / 2.;
+  const real e3l = (utl * EtaP + sigmal) / 2., e3r = (utr * EtaP + sigmar) / 2.;
+  const real ev2s = (ut_ * EtaP - sigma) / 2., ev3s = (ut_ * EtaP + sigma) / 2.;
+  const real eps2 = maxr(0.0, maxr(ev2s - e2l, e2r - ev2s));
+  const real eps3 = maxr(0.0, maxr(ev3s - e3l, e3r - ev3s));
+  ev2 = ev2 < eps2 ? 0.5 * ((ev2s * ev2s) / eps2 + eps2) : ev2;
+  ev3 = ev3 < eps3 ? 0.5 * ((ev3s * ev3s) / eps3 + eps3) : ev3;
+  real Uj[3];
+  for (int k = 0; k < 3; ++k) Uj[k] = Ur[k] - Ul[k];
+  const real utj = dot3(Uj, an);
+  const real pj = pgr - pgl;
+  const real Tj = Tr - Tl;
+  const real BM = EtaM;
+  const real T1 = ev2 - ev3;
+  const real T2 = (ev2 + ev3 - 2 * ev1);
+  const real T3 = r_ * (BM * BM * ut_ * ut_ - sigma * sigma) / (4. * sigma);
+  const real T4 = T2 * sigma + BM * T1 * ut_;
+  const real T5 = (T2 * sigma - BM * T1 * ut_) / (2. * sigma);
+  const real T6 = ((ev2 + ev3) * sigma - BM * T1 * ut_) / (2. * sigma);
+  const real Cp = gamma * Rt / gm1;
+  const real rT_ = -gamma * Rt * r_ / (h_ * gm1);
+  const real rbeta = 1. / Eta;
+  const real ra2 = 1. / (a_ * a_);
+  const real T7 = rbeta * T1 * T3 * ra2;
+  const real T8 = (((rbeta - 1) + gamma) * T6 - gm1 * T5) * ra2;
+  const real T9 = Cp * r_ + h0_ * rT_;
+  const real x1 = gamma * h0_ * T1 * T3 * ra2 * rbeta;
+  const real x2 = -rbeta * gm1 * T1 * T3 * T9 * ra2 / rT_;
+  const real x3 = -T1 * T3 * rbeta;
+  const real x4 = .5 * r_ * T4 * uta_ / sigma;
+  const real T10 = (x1 + x2 + x3 + x4);
+  const real T11 = (rbeta * ra2 * h0_ + (T9 - r_ * rbeta * Cp) * ra2 * gm1 / rT_ - rbeta) * T6 -
+                   gm1 * ra2 * T9 * T5 / rT_ - uta_ * T1 / sigma;
+  iflux[0] = pj * T8 + utj * T7 + ev1 * Tj * rT_;
+  iflux[1] = U_[0] * T7 * utj + ev1 * (U_[0] * Tj * rT_ + r_ * Uj[0]) +
+             pj * (T8 * U_[0] - an[0] * T1 / sigma) + an[0] * r_ * T4 * utj / (2. * sigma);
+  iflux[2] = U_[1] * T7 * utj + ev1 * (U_[1] * Tj * rT_ + r_ * Uj[1]) +
+             pj * (T8 * U_[1] - an[1] * T1 / sigma) + an[1] * r_ * T4 * utj / (2. * sigma);
+  iflux[3] = U_[2] * T7 * utj + ev1 * (U_[2] * Tj * rT_ + r_ * Uj[2]) +
+             pj * (T8 * U_[2] - an[2] * T1 / sigma) + an[2] * r_ * T4 * utj / (2. * sigma);
+  iflux[4] = pj * T11 + T10 * utj + ev1 * (T9 * Tj + r_ * dot3(U_, Uj));
+  const real Ah = 0.5 * area;
+  iflux[0] = Ah * (rl * utl + rr * utr - iflux[0]);
+  iflux[1] = Ah * (Ul[0] * rl * utl + Ur[0] * rr * utr + an[0] * (pgl + pgr) - iflux[1]);
+  iflux[2] = Ah * (Ul[1] * rl * utl + Ur[1] * rr * utr + an[1] * (pgl + pgr) - iflux[2]);
+  iflux[3] = Ah * (Ul[2] * rl * utl + Ur[2] * rr * utr + an[2] * (pgl + pgr) - iflux[3]);
+  iflux[4] = Ah * ((re0l * utl + pl * utla) + (re0r * utr + pr * utra) - iflux[4]);
+}
+
+/* src/fluxInviscid.loci:1018-1090 (RoeJacobian); Fj accumulated (+=) in real_fj */
+static void roe_jacobian_ev(real_fj *Fj, real ev1, real ev2, real ev3, const real *U, real un,
+                            real a, real beta, real sigma, real gamma, real Rt, real rho, real T,
+                            real h0, const real *n, real area) {
+  (void)T;
+  const int Ti = 0, Pi = 4, mi = 1, ei = 4;
+  const real BM = beta - 1.;
+  const real una = dot3(U, n);
+  const real T1 = ev2 - ev3;
+  const real T2 = (ev2 + ev3 - 2 * ev1);
+  const real T3 = rho * (BM * BM * un * un - sigma * sigma) / (4. * sigma);
+  const real T4 = T2 * sigma + BM * T1 * un;
+  const real T5 = (T2 * sigma - BM * T1 * un) / (2. * sigma);
+  const real T6 = ((ev2 + ev3) * sigma - BM * T1 * un) / (2. * sigma);
+  const real gm1 = gamma - 1.0;
+  const real Cp = gamma * Rt / gm1;
+  const real rT = -gamma * Rt * rho / ((h0 - 0.5 * dot3(U, U)) * gm1);
+  const real rbeta = 1. / beta;
+  const real ra2 = 1. / (a * a);
+  const real T7 = rbeta * T1 * T3 * ra2;
+  const real T8 = (((rbeta - 1) + gamma) * T6 - gm1 * T5) * ra2;
+  const real T9 = Cp * rho + h0 * rT;
+  const real x1 = gamma * h0 * T1 * T3 * ra2 * rbeta;
+  const real x2 = -rbeta * gm1 * T1 * T3 * T9 * ra2 / rT;
+  const real x3 = -T1 * T3 * rbeta;
+  const real x4 = .5 * rho * T4 * una / sigma;
+  const real T10 = (x1 + x2 + x3 + x4);
+  const real T11 = (rbeta * ra2 * h0 + (T9 - rho * rbeta * Cp) * ra2 * gm1 / rT - rbeta) * T6 -
+                   gm1 * ra2 * T9 * T5 / rT - una * T1 / sigma;
+  addfj(&M_(Fj, 0, Ti), area * ev1 * rT);
+  addfj(&M_(Fj, 0, mi + 0), area * n[0] * T7);
+  addfj(&M_(Fj, 0, mi + 1), area * n[1] * T7);
+  addfj(&M_(Fj, 0, mi + 2), area * n[2] * T7);
+  addfj(&M_(Fj, 0, Pi), area * T8);
+  addfj(&M_(Fj, mi + 0, Ti), area * ev1 * rT * U[0]);
+  addfj(&M_(Fj, mi + 1, Ti), area * ev1 * rT * U[1]);
+  addfj(&M_(Fj, mi + 2, Ti), area * ev1 * rT * U[2]);
+  addfj(&M_(Fj, mi + 0, mi + 0), area * (n[0] * (T7 * U[0] + n[0] * rho * T4 / (2. * sigma)) + ev1 * rho));
+  addfj(&M_(Fj, mi + 0, mi + 1), area * n[1] * (T7 * U[0] + n[0] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 0, mi + 2), area * n[2] * (T7 * U[0] + n[0] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 1, mi + 0), area * n[0] * (T7 * U[1] + n[1] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 1, mi + 1), area * (n[1] * (T7 * U[1] + n[1] * rho * T4 / (2. * sigma)) + ev1 * rho));
+  addfj(&M_(Fj, mi + 1, mi + 2), area * n[2] * (T7 * U[1] + n[1] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 2, mi + 0), area * n[0] * (T7 * U[2] + n[2] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 2, mi + 1), area * n[1] * (T7 * U[2] + n[2] * rho * T4 / (2. * sigma)));
+  addfj(&M_(Fj, mi + 2, mi + 2), area * (n[2] * (T7 * U[2] + n[2] * rho * T4 / (2. * sigma)) + ev1 * rho));
+  addfj(&M_(Fj, mi + 0, Pi), area * (T8 * U[0] - n[0] * T1 / sigma));
+  addfj(&M_(Fj, mi + 1, Pi), area * (T8 * U[1] - n[1] * T1 / sigma));
+  addfj(&M_(Fj, mi + 2, Pi), area * (T8 * U[2] - n[2] * T1 / sigma));
+  addfj(&M_(Fj, ei, Ti), area * ev1 * T9);
+  addfj(&M_(Fj, ei, mi + 0), area * (n[0] * T10 + ev1 * rho * U[0]));
+  addfj(&M_(Fj, ei, mi + 1), area * (n[1] * T10 + ev1 * rho * U[1]));
+  addfj(&M_(Fj, ei, mi + 2), area * (n[2] * T10 + ev1 * rho * U[2]));
+  addfj(&M_(Fj, ei, Pi), area * T11);
+}
+
+void og_roe_jacobian_ev(void *fj25, double ev1, double ev2, double ev3, const double *U, double un,
+                        double a, double beta, double sigma, double gamma, double Rt, double rho,
+                        double T, double h0, const double *n, double area) {
+  roe_jacobian_ev((real_fj *)fj25, ev1, ev2, ev3, U, un, a, beta, sigma, gamma, Rt, rho, T, h0, n, area);
+}
+
+/* src/fluxInviscid.loci:1092-1168 (roeCT_jacobian, smoothed eigenvalues) */
+static void roe_jacobian(real_fj *Fj, real pgl, real Tl, const real *Ul, real pgr, real Tr,
+                         const real *Ur, const real *an, real area, real pambient, real Rt,
+                         real gamma, real Us_n, real Eta, int dir) {
+  real gm1 = gamma - 1;
+  const real pl = pgl + pambient, pr = pgr + pambient;
+  const real rl = pl / (Rt * Tl), rr = pr / (Rt * Tr);
+  const real re0l = rl * (Rt * Tl / gm1 + 0.5 * dot3(Ul, Ul));
+  const real re0r = rr * (Rt * Tr / gm1 + 0.5 * dot3(Ur, Ur));
+  const real h0l = (re0l + pl) / rl, h0r = (re0r + pr) / rr;
+  const real r_ = sqrt(rl * rr);
+  const real srl = sqrt(rl), srr = sqrt(rr);
+  const real roe_factor = 1. / (srl + srr);
+  real U_[3];
+  for (int k = 0; k < 3; ++k) U_[k] = (Ul[k] * srl + Ur[k] * srr) * roe_factor;
+  const real ut_ = dot3(U_, an) - Us_n;
+  const real q2_ = dot3(U_, U_);
+  const real h0_ = (h0l * srl + h0r * srr) * roe_factor;
+  const real h_ = h0_ - 0.5 * q2_;
+  if (fabs(pl - pr) / (pl + pr + 1e-33) > .1) Eta = 1.;
+  real af2 = gm1 * h_;
+  af2 = maxr(af2, 1e-30);
+  const real T_ = h_ * gm1 / (Rt * gamma);
+  const real EtaP = Eta + 1., EtaM = Eta - 1.;
+  real a_ = sqrt(af2);
+  const real sigma = sqrt(ut_ * ut_ * EtaM * EtaM + 4. * Eta * a_ * a_);
+  real ev1 = ut_;
+  real ev2 = (ut_ * EtaP - sigma) / 2.;
+  real ev3 = (ut_ * EtaP + sigma) / 2.;
+  real d = (dir < 0) ? -1 : 1;
+  const real mxv = maxr(ev1, maxr(ev2, ev3));
+  const real mnv = minr(ev1, minr(ev2, ev3));
+  const real smooth_factor = 0.02;
+  const real eps = smooth_factor * (mxv - mnv);
+  const real eps2 = eps * eps;
+  ev1 = .5 * (ev1 + d * sqrt(ev1 * ev1 + eps2));
+  ev2 = .5 * (ev2 + d * sqrt(ev2 * ev2 + eps2));
+  ev3 = .5 * (ev3 + d * sqrt(ev3 * ev3 + eps2));
+  roe_jacobian_ev(Fj, ev1, ev2, ev3, U_, ut_, a_, Eta, sigma, gamma, Rt, r_, T_, h0_, an, area);
+}
+
+void og_roe_jacobian(void *fj25, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                     const double *Ur, const double *an, double area, double pamb, double Rt,
+                     double gamma, double us_n, double eta, int dir) {
+  roe_jacobian((real_fj *)fj25, pgl, Tl, Ul, pgr, Tr, Ur, an, area, pamb, Rt, gamma, us_n, eta, dir);
+}
+
+/* include/bcTools.h:25-51 */
+void og_cv_inflow(double *pgb, double *Tb, double *ub, double pgi, double Ti, const double *ui,
+                  double pg0, double T0, const double *u0, const double *n, double us_n,
+                  double Pambient, double gamma, double Rtilde, double Beta) {
+  (void)Ti;
+  const real uit = dot3(ui, n) - us_n;
+  const real u0t = dot3(u0, n) - us_n;
+  const real a02 = gamma * Rtilde * T0;
+  const real a0 = sqrt(a02);
+  const real r0 = (pg0 + Pambient) / (Rtilde * T0);
+  const real MinPressure = 0.1 * (pg0 + Pambient) - Pambient;
+  const real BetaM = 0.5 * (1.0 - Beta);
+  const real uitBeta = uit * (1. - Beta);
+  const real sigma2 = uitBeta * uitBeta + 4. * Beta * a0 * a0;
+  const real sigma = sqrt(sigma2);
+  real p = 0.5 * (pg0 + pgi - 0.5 * r0 * sigma * (u0t - uit)) -
+           ((pgi - pg0) + r0 * (uit - u0t) * uit * BetaM) * uit * BetaM / (2. * sigma);
+  p = maxr(p, MinPressure);
+  for (int k = 0; k < 3; ++k) ub[k] = u0[k] + 2. * n[k] * (p - pg0) / (r0 * (sigma + uit * BetaM));
+  real rhob = r0 + (p - pg0) * (sigma - uit * BetaM) / (Beta * a0 * a0 * (sigma + uit * BetaM));
+  *Tb = (p + Pambient) / (Rtilde * rhob);
+  *pgb = p;
+}
+
+/* include/bcTools.h:52-80 */
+void og_cv_outflow(double *pgb, double *Tb, double *ub, double pgi, double Ti, const double *ui,
+                   double pg0, double T0, const double *u0, const double *n, double us_n,
+                   double Pambient, double gamma, double Rtilde, double Beta) {
+  (void)Ti;
+  const real uit = dot3(ui, n) - us_n;
+  const real u0t = dot3(u0, n) - us_n;
+  const real a02 = gamma * Rtilde * T0;
+  const real a0 = sqrt(a02);
+  const real r0 = (pg0 + Pambient) / (Rtilde * T0);
+  const real MinPressure = 0.1 * (pgi + Pambient) - Pambient;
+  const real BetaM = 0.5 * (1.0 - Beta);
+  const real uitBeta = uit * (1. - Beta);
+  const real sigma2 = uitBeta * uitBeta + 4. * Beta * a0 * a0;
+  const real sigma = sqrt(sigma2);
+  real p = .5 * (pgi + pg0 - 0.5 * r0 * sigma * (u0t - uit)) +
+           ((pgi - pg0) + r0 * (uit - u0t) * uit * BetaM) * uit * BetaM / (2. * sigma);
+  p = maxr(p, MinPressure);
+  for (int k = 0; k < 3; ++k) ub[k] = ui[k] + 2. * n[k] * (pgi - p) / (r0 * (sigma + uit * BetaM));
+  const real rhob = r0 + (p - pgi) * (sigma - uit * BetaM) / (Beta * a0 * a0 * (sigma + uit * BetaM));
+  *Tb = (p + Pambient) / (Rtilde * rhob);
+  *pgb = p;
+}
+
+/* ------------------------------------------------------------------ */
+/* grid helpers                                                        */
+/* ------------------------------------------------------------------ */
+
+/* Per-cell face lists: lower (cr == c), upper (cl == c, periodic faces
+ * included, as their ghost sits on the cr side), boundary (cl == c, not
+ * periodic); each ascending in face index. */
+typedef struct {
+  int *lo_off, *lo, *up_off, *up, *bd_off, *bd;
+} og_adj;
+
+static void build_adj(const og_grid *g, og_adj *a) {
+  int nc = g->nc, nf = g->nf;
+  a->lo_off = calloc(nc + 1, sizeof(int));
+  a->up_off = calloc(nc + 1, sizeof(int));
+  a->bd_off = calloc(nc + 1, sizeof(int));
+  for (int f = 0; f < nf; ++f) {
+    int t = g->btype[f];
+    if (t == OG_INTERIOR) {
+      a->up_off[g->cl[f] + 1]++;
+      a->lo_off[g->cr[f] + 1]++;
+    } else if (t == OG_PERIODIC) {
+      a->up_off[g->cl[f] + 1]++;
+    } else {
+      a->bd_off[g->cl[f] + 1]++;
+    }
+  }
+  for (int c = 0; c < nc; ++c) {
+    a->lo_off[c + 1] += a->lo_off[c];
+    a->up_off[c + 1] += a->up_off[c];
+    a->bd_off[c + 1] += a->bd_off[c];
+  }
+  a->lo = malloc(sizeof(int) * (a->lo_off[nc] + 1));
+  a->up = malloc(sizeof(int) * (a->up_off[nc] + 1));
+  a->bd = malloc(sizeof(int) * (a->bd_off[nc] + 1));
+  int *lp = calloc(nc, sizeof(int)), *upp = calloc(nc, sizeof(int)), *bp = calloc(nc, sizeof(int));
+  for (int f = 0; f < nf; ++f) {
+    int t = g->btype[f];
+    if (t == OG_INTERIOR) {
+      int c = g->cl[f];
+      a->up[a->up_off[c] + upp[c]++] = f;
+      c = g->cr[f];
+      a->lo[a->lo_off[c] + lp[c]++] = f;
+    } else if (t == OG_PERIODIC) {
+      int c = g->cl[f];
+      a->up[a->up_off[c] + upp[c]++] = f;
+    } else {
+      int c = g->cl[f];
+      a->bd[a->bd_off[c] + bp[c]++] = f;
+    }
+  }
+  free(lp);
+  free(upp);
+  free(bp);
+}
+
+static void free_adj(og_adj *a) {
+  free(a->lo_off); free(a->lo); free(a->up_off); free(a->up); free(a->bd_off); free(a->bd);
+}
+
+static inline void fnormal(const og_grid *g, int f, real *n) {
+  n[0] = g->n[f]; n[1] = g->n[g->nf + f]; n[2] = g->n[2 * g->nf + f];
+}
+static inline void fcenter(const og_grid *g, int f, real *x) {
+  x[0] = g->fc[f]; x[1] = g->fc[g->nf + f]; x[2] = g->fc[2 * g->nf + f];
+}
+static inline void ccenter(const og_grid *g, int c, real *x) {
+  x[0] = g->cc[c]; x[1] = g->cc[g->nc + c]; x[2] = g->cc[2 * g->nc + c];
+}
+/* cell on the cr side of an interior or periodic face, and its centre
+ * (src/solverPeriodic.loci:31-35: ghost centre = transformed partner centre) */
+static inline int rcell(const og_grid *g, int f, real *x) {
+  if (g->btype[f] == OG_PERIODIC) {
+    int c = g->cl[g->pface[f]];
+    ccenter(g, c, x);
+    for (int d = 0; d < 3; ++d) x[d] += g->pshift[d * g->nf + f];
+    return c;
+  }
+  ccenter(g, g->cr[f], x);
+  return g->cr[f];
+}
+
+/* Neighbour of cell c across face f for the gradient stencil: its position
+ * and primitive values (cell, periodic ghost, or boundary-face state). */
+static void stencil_nb(const og_grid *g, const double *q, const double *qf, int c, int f,
+                       real *x, real *v) {
+  int nc = g->nc, nf = g->nf;
+  int t = g->btype[f];
+  if (t == OG_INTERIOR) {
+    int o = (g->cl[f] == c) ? g->cr[f] : g->cl[f];
+    ccenter(g, o, x);
+    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+  } else if (t == OG_PERIODIC) {
+    int o = rcell(g, f, x);
+    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+  } else {
+    fcenter(g, f, x);
+    for (int k = 0; k < 5; ++k) v[k] = qf[k * nf + f];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* boundary-face states                                                */
+/* ------------------------------------------------------------------ */
+
+/* farfield: src/bcInflow.loci:311-340; viscous wall: src/bcViscousWall.loci:205-214
+ * (stationary wall, us = 0); symmetry: src/bcImpermeable.loci:48-64 */
+void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf) {
+  int nc = g->nc, nf = g->nf;
+  for (int f = 0; f < nf; ++f) {
+    int t = g->btype[f];
+    if (t == OG_INTERIOR || t == OG_PERIODIC) continue;
+    int c = g->cl[f];
+    real n[3];
+    fnormal(g, f, n);
+    real Ti = q[c], ui[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]}, pgi = q[4 * nc + c];
+    real Tb, pgb, ub[3];
+    if (t == OG_FARFIELD) {
+      const real us_n = 0.0, eta_pf = 1.0;
+      const real uit = dot3(ui, n) - us_n;
+      const real ai = sqrt(p->gamma * p->Rt * Ti);
+      const real a0 = sqrt(p->gamma * p->Rt * p->T0);
+      if (uit > 0) {
+        if (uit > ai) {
+          pgb = pgi; Tb = Ti; ub[0] = ui[0]; ub[1] = ui[1]; ub[2] = ui[2];
+        } else {
+          og_cv_outflow(&pgb, &Tb, ub, pgi, Ti, ui, p->pg0, p->T0, p->u0, n, us_n,
+                        p->Pambient, p->gamma, p->Rt, eta_pf);
+        }
+      } else {
+        if (-uit > a0) {
+          pgb = p->pg0; Tb = p->T0; ub[0] = p->u0[0]; ub[1] = p->u0[1]; ub[2] = p->u0[2];
+        } else {
+          og_cv_inflow(&pgb, &Tb, ub, pgi, Ti, ui, p->pg0, p->T0, p->u0, n, us_n,
+                       p->Pambient, p->gamma, p->Rt, eta_pf);
+        }
+      }
+    } else if (t == OG_WALL) {
+      /* u_f = uwall - (uwall.n) n + us with uwall = 0, us = 0 */
+      ub[0] = ub[1] = ub[2] = 0.0;
+      pgb = pgi;
+      Tb = p->Twall;
+    } else { /* symmetry */
+      const real us_n = 0.0;
+      const real uit = dot3(ui, n) - us_n;
+      for (int k = 0; k < 3; ++k) ub[k] = ui[k] - uit * n[k];
+      const real rho = (pgi + p->Pambient) / (p->Rt * Ti);
+      const real U = sqrt(dot3(ui, ui));
+      pgb = pgi + 0.5 * rho * U * uit;
+      Tb = Ti;
+    }
+    qf[f] = Tb;
+    qf[nf + f] = ub[0];
+    qf[2 * nf + f] = ub[1];
+    qf[3 * nf + f] = ub[2];
+    qf[4 * nf + f] = pgb;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* gradients and limiters (Loci FVM pieces: textbook definitions)      */
+/* ------------------------------------------------------------------ */
+
+/* Inverse of the symmetric 3x3 least-squares normal matrix
+ * A = sum dx dx^T over the face stencil, by the adjugate. */
+static void lsq_inverse(const og_grid *g, const og_adj *a, int c, real *Ai /* 6: xx xy xz yy yz zz */) {
+  real A[6] = {0, 0, 0, 0, 0, 0};
+  real xc[3];
+  ccenter(g, c, xc);
+  int lists[3][2] = {{a->lo_off[c], a->lo_off[c + 1]}, {a->up_off[c], a->up_off[c + 1]}, {a->bd_off[c], a->bd_off[c + 1]}};
+  const int *arr[3] = {a->lo, a->up, a->bd};
+  for (int L = 0; L < 3; ++L)
+    for (int j = lists[L][0]; j < lists[L][1]; ++j) {
+      int f = arr[L][j];
+      real x[3];
+      int t = g->btype[f];
+      if (t == OG_INTERIOR) {
+        int o = (g->cl[f] == c) ? g->cr[f] : g->cl[f];
+        ccenter(g, o, x);
+      } else if (t == OG_PERIODIC) {
+        rcell(g, f, x);
+      } else {
+        fcenter(g, f, x);
+      }
+      real dx = x[0] - xc[0], dy = x[1] - xc[1], dz = x[2] - xc[2];
+      A[0] += dx * dx; A[1] += dx * dy; A[2] += dx * dz;
+      A[3] += dy * dy; A[4] += dy * dz; A[5] += dz * dz;
+    }
+  real c00 = A[3] * A[5] - A[4] * A[4];
+  real c01 = A[2] * A[4] - A[1] * A[5];
+  real c02 = A[1] * A[4] - A[2] * A[3];
+  real c11 = A[0] * A[5] - A[2] * A[2];
+  real c12 = A[1] * A[2] - A[0] * A[4];
+  real c22 = A[0] * A[3] - A[1] * A[1];
+  real det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+  real rdet = 1.0 / det;
+  Ai[0] = c00 * rdet; Ai[1] = c01 * rdet; Ai[2] = c02 * rdet;
+  Ai[3] = c11 * rdet; Ai[4] = c12 * rdet; Ai[5] = c22 * rdet;
+}
+
+/* Venkatakrishnan limiter function (Venkatakrishnan, AIAA 93-0880) with the
+ * threshold eps2 = (Kl h)^3, h = vol^(1/3). */
+static inline real venkat(real delp, real delm, real eps2) {
+  const real num = ((delp * delp + eps2) * delm + 2.0 * delm * delm * delp);
+  const real den = (delm * (delp * delp + 2.0 * delm * delm + delm * delp + eps2));
+  const real e = (den >= 0.0 ? 1.0e-30 : -1.0e-30);
+  return (num + e) / (den + e);
+}
+
+void og_gradients(const og_grid *g, const og_params *p, const double *q, const double *qf,
+                  double *grad, double *phi) {
+  int nc = g->nc;
+  og_adj a;
+  build_adj(g, &a);
+  for (int c = 0; c < nc; ++c) {
+    real Ai[6];
+    lsq_inverse(g, &a, c, Ai);
+    real xc[3];
+    ccenter(g, c, xc);
+    real qc[5];
+    for (int k = 0; k < 5; ++k) qc[k] = q[k * nc + c];
+    real b[5][3];
+    real qmin[5], qmax[5];
+    for (int k = 0; k < 5; ++k) { b[k][0] = b[k][1] = b[k][2] = 0; qmin[k] = qmax[k] = qc[k]; }
+    int lists[3][2] = {{a.lo_off[c], a.lo_off[c + 1]}, {a.up_off[c], a.up_off[c + 1]}, {a.bd_off[c], a.bd_off[c + 1]}};
+    const int *arr[3] = {a.lo, a.up, a.bd};
+    for (int L = 0; L < 3; ++L)
+      for (int j = lists[L][0]; j < lists[L][1]; ++j) {
+        int f = arr[L][j];
+        real x[3], v[5];
+        stencil_nb(g, q, qf, c, f, x, v);
+        real dx = x[0] - xc[0], dy = x[1] - xc[1], dz = x[2] - xc[2];
+        for (int k = 0; k < 5; ++k) {
+          real dq = v[k] - qc[k];
+          b[k][0] += dx * dq; b[k][1] += dy * dq; b[k][2] += dz * dq;
+          qmin[k] = minr(qmin[k], v[k]);
+          qmax[k] = maxr(qmax[k], v[k]);
+        }
+      }
+    real gr[5][3];
+    for (int k = 0; k < 5; ++k) {
+      gr[k][0] = Ai[0] * b[k][0] + Ai[1] * b[k][1] + Ai[2] * b[k][2];
+      gr[k][1] = Ai[1] * b[k][0] + Ai[3] * b[k][1] + Ai[4] * b[k][2];
+      gr[k][2] = Ai[2] * b[k][0] + Ai[4] * b[k][1] + Ai[5] * b[k][2];
+      for (int d = 0; d < 3; ++d) grad[(3 * k + d) * nc + c] = gr[k][d];
+    }
+    /* limiter */
+    real ph[5];
+    for (int k = 0; k < 5; ++k) ph[k] = 1.0;
+    if (p->limiter == 2) {
+      for (int k = 0; k < 5; ++k) ph[k] = 0.0;
+    } else if (p->limiter == 1 || p->limiter == 3) {
+      const real h3 = g->vol[c];
+      const real eps2 = p->Kl * p->Kl * p->Kl * h3;
+      for (int L = 0; L < 3; ++L)
+        for (int j = lists[L][0]; j < lists[L][1]; ++j) {
+          int f = arr[L][j];
+          real xf[3];
+          fcenter(g, f, xf);
+          real dx = xf[0] - xc[0], dy = xf[1] - xc[1], dz = xf[2] - xc[2];
+          for (int k = 0; k < 5; ++k) {
+            real qdif = gr[k][0] * dx + gr[k][1] * dy + gr[k][2] * dz;
+            real lim;
+            if (p->limiter == 1) {
+              real delp = (qdif >= 0.0) ? qmax[k] - qc[k] : qmin[k] - qc[k];
+              lim = venkat(delp, qdif, eps2);
+            } else {
+              if (qdif > 0) lim = minr(1.0, (qmax[k] - qc[k]) / qdif);
+              else if (qdif < 0) lim = minr(1.0, (qmin[k] - qc[k]) / qdif);
+              else lim = 1.0;
+            }
+            ph[k] = minr(ph[k], lim);
+          }
+        }
+      for (int k = 0; k < 5; ++k) ph[k] = maxr(0.0, minr(1.0, ph[k]));
+    }
+    for (int k = 0; k < 5; ++k) phi[k * nc + c] = ph[k];
+  }
+  free_adj(&a);
+}
+
+/* Limited reconstruction q_f = q_c + phi_c (grad q_c . (x_f - x_c)), with the
+ * floors of leftsP(temperature,Zero) and leftsP(gagePressure,minPg). */
+static void recon(const og_grid *g, const og_params *p, const double *q, const double *grad,
+                  const double *phi, int c, const real *xcell, const real *xf, real *out) {
+  int nc = g->nc;
+  real d[3] = {xf[0] - xcell[0], xf[1] - xcell[1], xf[2] - xcell[2]};
+  for (int k = 0; k < 5; ++k) {
+    real gd = grad[(3 * k) * nc + c] * d[0] + grad[(3 * k + 1) * nc + c] * d[1] +
+              grad[(3 * k + 2) * nc + c] * d[2];
+    out[k] = q[k * nc + c] + phi[k * nc + c] * gd;
+  }
+  out[0] = maxr(out[0], 0.0);
+  out[4] = maxr(out[4], p->minPg);
+}
+
+/* Sutherland law: src/fluxViscous.loci:112-119 (mu), :137-143 (kconduct) */
+static inline real suth_mu(const og_params *p, real T) { return p->suth[0] * pow(T, p->suth[1]) / (T + p->suth[2]); }
+static inline real suth_k(const og_params *p, real T) { return p->suth[3] * pow(T, p->suth[4]) / (T + p->suth[5]); }
+
+/* Face gradient of variable k (gradv3d_f / grads_f, Loci FVM):
+ * interior/periodic: average of the two cell gradients corrected along the
+ * cell-centre line; boundary: cell gradient corrected along x_f - x_c. */
+static void face_grad(const og_grid *g, const double *q, const double *qf, const double *grad,
+                      int f, int k, real *gout) {
+  int nc = g->nc, nf = g->nf;
+  int t = g->btype[f];
+  int c = g->cl[f];
+  real xl[3];
+  ccenter(g, c, xl);
+  if (t == OG_INTERIOR || t == OG_PERIODIC) {
+    real xr[3];
+    int o = rcell(g, f, xr);
+    real d[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+    real ga[3];
+    for (int j = 0; j < 3; ++j) ga[j] = 0.5 * (grad[(3 * k + j) * nc + c] + grad[(3 * k + j) * nc + o]);
+    real corr = (q[k * nc + o] - q[k * nc + c] - dot3(ga, d)) / dot3(d, d);
+    for (int j = 0; j < 3; ++j) gout[j] = ga[j] + corr * d[j];
+  } else {
+    real xf[3];
+    fcenter(g, f, xf);
+    real d[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+    real ga[3];
+    for (int j = 0; j < 3; ++j) ga[j] = grad[(3 * k + j) * nc + c];
+    real corr = (qf[k * nf + f] - q[k * nc + c] - dot3(ga, d)) / dot3(d, d);
+    for (int j = 0; j < 3; ++j) gout[j] = ga[j] + corr * d[j];
+  }
+}
+
+/* Face viscous quantities: muTotal_f, kconductTotal, u_f and vflux[4]
+ * (src/fluxViscous.loci:121-133,145-151,184-197,217-238,272-292). */
+static void viscous_face(const og_grid *g, const og_params *p, const double *q, const double *qf,
+                         const double *grad, int f, real *mu_f, real *k_f, real *uf, real *vflux) {
+  int nc = g->nc, nf = g->nf;
+  int t = g->btype[f];
+  real an[3];
+  fnormal(g, f, an);
+  const real sada = g->area[f];
+  if (t == OG_INTERIOR || t == OG_PERIODIC) {
+    int c = g->cl[f];
+    real xr[3];
+    int o = rcell(g, f, xr);
+    const real vl = g->vol[c], vr = g->vol[o];
+    real mul = suth_mu(p, q[c]), mur = suth_mu(p, q[o]);
+    real kl = suth_k(p, q[c]), kr = suth_k(p, q[o]);
+    *mu_f = (vl * mur + vr * mul) / (vl + vr);
+    *k_f = (vl * kr + vr * kl) / (vl + vr);
+    const real factor = 1. / (vl + vr);
+    for (int j = 0; j < 3; ++j) uf[j] = factor * (vl * q[(1 + j) * nc + o] + vr * q[(1 + j) * nc + c]);
+  } else {
+    real Tf = qf[f];
+    *mu_f = suth_mu(p, Tf);
+    *k_f = suth_k(p, Tf);
+    for (int j = 0; j < 3; ++j) uf[j] = qf[(1 + j) * nf + f];
+  }
+  if (t == OG_SYMMETRY) { /* src/fluxViscous.loci:294-297 */
+    vflux[0] = vflux[1] = vflux[2] = vflux[3] = 0;
+    return;
+  }
+  real gu[3][3], gT[3];
+  for (int i = 0; i < 3; ++i) face_grad(g, q, qf, grad, f, 1 + i, gu[i]);
+  face_grad(g, q, qf, grad, f, 0, gT);
+  real mu_total = *mu_f;
+  real dudx = gu[0][0], dvdy = gu[1][1], dwdz = gu[2][2];
+  real dvdx = gu[1][0], dwdx = gu[2][0], dwdy = gu[2][1];
+  real dudy = gu[0][1], dudz = gu[0][2], dvdz = gu[1][2];
+  real divm = (dudx + dvdy + dwdz) * (1. / 3.);
+  real txx = 2.0 * mu_total * (dudx - divm);
+  real tyy = 2.0 * mu_total * (dvdy - divm);
+  real tzz = 2.0 * mu_total * (dwdz - divm);
+  real txy = mu_total * (dudy + dvdx);
+  real txz = mu_total * (dudz + dwdx);
+  real tyz = mu_total * (dvdz + dwdy);
+  vflux[0] = (txx * an[0] + txy * an[1] + txz * an[2]) * sada;
+  vflux[1] = (txy * an[0] + tyy * an[1] + tyz * an[2]) * sada;
+  vflux[2] = (txz * an[0] + tyz * an[1] + tzz * an[2]) * sada;
+  vflux[3] = (uf[0] * vflux[0] + uf[1] * vflux[1] + uf[2] * vflux[2]);
+  const real ktotal = *k_f;
+  vflux[3] += dot3(gT, an) * ktotal * sada;
+}
+
+/* Face fluxes: interior/periodic src/fluxInviscid.loci:1170-1188; farfield
+ * src/bcInflow.loci:342-355; wall src/bcViscousWall.loci:216-229; symmetry
+ * src/bcImpermeable.loci:66-73; maxev_f src/solverSetup.loci:83-105. */
+void og_fluxes(const og_grid *g, const og_params *p, const double *q, const double *qf,
+               const double *grad, const double *phi, double *iflux, double *vflux,
+               double *maxev) {
+  int nc = g->nc, nf = g->nf;
+  const real us_n = 0.0, eta = 1.0;
+  for (int f = 0; f < nf; ++f) {
+    int t = g->btype[f];
+    real an[3], xf[3], xl[3];
+    fnormal(g, f, an);
+    fcenter(g, f, xf);
+    int c = g->cl[f];
+    ccenter(g, c, xl);
+    real fl[5];
+    real qL[5];
+    recon(g, p, q, grad, phi, c, xl, xf, qL);
+    real UL[3] = {qL[1], qL[2], qL[3]};
+    if (t == OG_INTERIOR || t == OG_PERIODIC || t == OG_FARFIELD) {
+      real qR[5];
+      if (t == OG_FARFIELD) {
+        for (int k = 0; k < 5; ++k) qR[k] = qf[k * nf + f];
+      } else {
+        real xr[3];
+        int o = rcell(g, f, xr);
+        recon(g, p, q, grad, phi, o, xr, xf, qR);
+      }
+      real UR[3] = {qR[1], qR[2], qR[3]};
+      if (p->flux == 1)
+        og_roe_flux(fl, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta);
+      else
+        og_hllc_flux(fl, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta);
+    } else if (t == OG_WALL) {
+      const real pg = qf[4 * nf + f];
+      const real A = g->area[f];
+      fl[0] = 0;
+      fl[1] = pg * (an[0] * A);
+      fl[2] = pg * (an[1] * A);
+      fl[3] = pg * (an[2] * A);
+      const real pw = pg + p->Pambient;
+      fl[4] = pw * A * us_n;
+    } else { /* symmetry */
+      const real pg = qf[4 * nf + f];
+      real uf[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+      fl[0] = 0;
+      const real moment = pg * g->area[f];
+      fl[1] = an[0] * moment;
+      fl[2] = an[1] * moment;
+      fl[3] = an[2] * moment;
+      fl[4] = g->area[f] * pg * dot3(uf, an);
+    }
+    for (int k = 0; k < 5; ++k) iflux[k * nf + f] = fl[k];
+    /* viscous */
+    real vf[4] = {0, 0, 0, 0};
+    if (p->viscous) {
+      real mu_f, k_f, uf[3];
+      viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, uf, vf);
+    }
+    for (int k = 0; k < 4; ++k) vflux[k * nf + f] = vf[k];
+    /* maxev_f */
+    const real Eta_P = eta + 1., Eta_M = eta - 1.;
+    if (t == OG_INTERIOR || t == OG_PERIODIC) {
+      real xr[3];
+      int o = rcell(g, f, xr);
+      real ul[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+      real ur[3] = {q[nc + o], q[2 * nc + o], q[3 * nc + o]};
+      const real utl = dot3(ul, an) - us_n, utr = dot3(ur, an) - us_n;
+      const real al = sqrt(p->gamma * p->Rt * q[c]), ar = sqrt(p->gamma * p->Rt * q[o]);
+      const real sigmal = sqrt(utl * utl * Eta_M * Eta_M + 4. * eta * al * al);
+      const real sigmar = sqrt(utr * utr * Eta_M * Eta_M + 4. * eta * ar * ar);
+      const real mevl = 0.5 * maxr(fabs(utl * Eta_P - sigmal), fabs(utl * Eta_P + sigmal));
+      const real mevr = 0.5 * maxr(fabs(utr * Eta_P - sigmar), fabs(utr * Eta_P + sigmar));
+      maxev[f] = maxr(mevl, mevr);
+    } else {
+      real uf[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+      const real utf = dot3(uf, an) - us_n;
+      const real af = sqrt(p->gamma * p->Rt * qf[f]);
+      const real sigmaf = sqrt(utf * utf * Eta_M * Eta_M + 4. * eta * af * af);
+      maxev[f] = 0.5 * maxr(fabs(utf * Eta_P - sigmaf), fabs(utf * Eta_P + sigmaf));
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* residual gather and norms                                           */
+/* ------------------------------------------------------------------ */
+
+/* src join: src/fluxInviscid.loci:1253-1268, src/fluxViscous.loci:299-310.
+ * Per cell the faces join in the order lower, upper, boundary (each
+ * ascending), each face adding its inviscid then its viscous part. */
+static void gather_src(const og_grid *g, const og_params *p, const og_adj *a, const double *iflux,
+                       const double *vflux, double *src) {
+  int nc = g->nc, nf = g->nf;
+  for (int c = 0; c < nc; ++c) {
+    real s[5] = {0, 0, 0, 0, 0};
+    for (int j = a->lo_off[c]; j < a->lo_off[c + 1]; ++j) {
+      int f = a->lo[j];
+      for (int k = 0; k < 5; ++k) s[k] += iflux[k * nf + f];
+      if (p->viscous)
+        for (int k = 0; k < 4; ++k) s[k + 1] -= vflux[k * nf + f];
+    }
+    for (int j = a->up_off[c]; j < a->up_off[c + 1]; ++j) {
+      int f = a->up[j];
+      for (int k = 0; k < 5; ++k) s[k] -= iflux[k * nf + f];
+      if (p->viscous)
+        for (int k = 0; k < 4; ++k) s[k + 1] += vflux[k * nf + f];
+    }
+    for (int j = a->bd_off[c]; j < a->bd_off[c + 1]; ++j) {
+      int f = a->bd[j];
+      for (int k = 0; k < 5; ++k) s[k] -= iflux[k * nf + f];
+      if (p->viscous)
+        for (int k = 0; k < 4; ++k) s[k + 1] += vflux[k * nf + f];
+    }
+    for (int k = 0; k < 5; ++k) src[k * nc + c] = s[k];
+  }
+}
+
+/* src/solverResidual.loci:31-112 (steady residual, resid_join), in cell order */
+static void residual_norms(const og_grid *g, const double *src, og_norms *r) {
+  int nc = g->nc;
+  real vtot = 0;
+  for (int c = 0; c < nc; ++c) vtot += g->vol[c];
+  memset(r, 0, sizeof(*r));
+  for (int c = 0; c < nc; ++c) {
+    const int mi = 1, ei = 4;
+    real rvol = 1. / g->vol[c];
+    real weight = g->vol[c] / vtot;
+    real s0 = src[c];
+    real rtmax = 0, cr3[3] = {0, 0, 0};
+    if (fabs(rtmax) < fabs(rvol * s0)) { rtmax = rvol * s0; ccenter(g, c, cr3); }
+    real rtrms = 0;
+    rtrms += (rvol * s0) * (rvol * s0);
+    rtrms *= weight;
+    real mtrms = 0, mtmax = 0;
+    for (int i = 0; i < 3; ++i) {
+      real v = rvol * src[(mi + i) * nc + c];
+      if (fabs(mtmax) < fabs(v)) mtmax = v;
+      mtrms += v * v;
+    }
+    mtrms *= weight / 3.0;
+    real ve = rvol * src[ei * nc + c];
+    real etrms = weight * ve * ve;
+    real etmax = ve;
+    r->rtrms += rtrms; r->etrms += etrms; r->mtrms += mtrms;
+    if (fabs(r->etmax) < fabs(etmax)) { r->etmax = etmax; ccenter(g, c, r->ccmaxe); }
+    if (fabs(r->rtmax) < fabs(rtmax)) { r->rtmax = rtmax; memcpy(r->ccmaxr, cr3, sizeof(cr3)); }
+    if (fabs(r->mtmax) < fabs(mtmax)) r->mtmax = mtmax;
+  }
+}
+
+void og_residual(const og_grid *g, const og_params *p, const double *q, double *src,
+                 double *iflux, double *vflux, og_norms *nrm) {
+  int nc = g->nc, nf = g->nf;
+  double *qf = calloc((size_t)5 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 15 * nc), *phi = malloc(sizeof(double) * 5 * nc);
+  double *maxev = malloc(sizeof(double) * nf);
+  og_face_states(g, p, q, qf);
+  og_gradients(g, p, q, qf, grad, phi);
+  og_fluxes(g, p, q, qf, grad, phi, iflux, vflux, maxev);
+  og_adj a;
+  build_adj(g, &a);
+  gather_src(g, p, &a, iflux, vflux, src);
+  if (nrm) residual_norms(g, src, nrm);
+  free_adj(&a);
+  free(qf); free(grad); free(phi); free(maxev);
+}
+
+/* ------------------------------------------------------------------ */
+/* block LU (Loci Tools Mat; textbook Doolittle with partial pivoting)  */
+/* ------------------------------------------------------------------ */
+
+/* In place: unit-lower L below the diagonal, U on and above it; rows are
+ * swapped, pivot[i] = original row now at position i.  Returns 0 on success. */
+static int lu_decompose(real_fj *a, int *pivot, int nb) {
+  for (int i = 0; i < nb; ++i) pivot[i] = i;
+  for (int k = 0; k < nb; ++k) {
+    int pr = k;
+    real_fj amax = (real_fj)fabs((double)a[k + nb * k]);
+    for (int i = k + 1; i < nb; ++i) {
+      real_fj v = (real_fj)fabs((double)a[i + nb * k]);
+      if (v > amax) { amax = v; pr = i; }
+    }
+    if (pr != k) {
+      for (int j = 0; j < nb; ++j) {
+        real_fj t = a[k + nb * j]; a[k + nb * j] = a[pr + nb * j]; a[pr + nb * j] = t;
+      }
+      int t = pivot[k]; pivot[k] = pivot[pr]; pivot[pr] = t;
+    }
+    if (a[k + nb * k] == 0) return -1;
+    for (int i = k + 1; i < nb; ++i) {
+      real_fj l = a[i + nb * k] / a[k + nb * k];
+      a[i + nb * k] = l;
+      for (int j = k + 1; j < nb; ++j) a[i + nb * j] = a[i + nb * j] - l * a[k + nb * j];
+    }
+  }
+  return 0;
+}
+
+/* forward/back substitution in real (double) with real_fj factors
+ * (the order of src/linearSolverFSGS.loci:73-106, with the row permutation) */
+static void lu_solve(const real_fj *a, const int *pivot, const real *b, real *x, int nb) {
+  for (int i = 0; i < nb; ++i) {
+    real tmp = b[pivot[i]];
+    for (int j = 0; j < i; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp;
+  }
+  for (int i = nb - 1; i >= 0; --i) {
+    real tmp = x[i];
+    for (int j = i + 1; j < nb; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp / a[i + nb * i];
+  }
+}
+
+int og_lu_decompose(void *a25, int *pivot, int nb) { return lu_decompose((real_fj *)a25, pivot, nb); }
+void og_lu_solve(const void *a25, const int *pivot, const double *b, double *x, int nb) {
+  lu_solve((const real_fj *)a25, pivot, b, x, nb);
+}
+
+/* t[i] += sum_j M(i,j) v[j], columns outer (src/linearSolverFSGS.loci:53-70) */
+static inline void dotprod_accum(const real_fj *M, const real *v, real *t) {
+  for (int j = 0; j < NB; ++j) {
+    const real in = v[j];
+    for (int i = 0; i < NB; ++i) t[i] += M[i + NB * j] * in;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Jacobians and assembly                                              */
+/* ------------------------------------------------------------------ */
+
+/* thin-layer viscous Jacobian terms (src/fluxViscous.loci:315-403), added
+ * with sign s (+1 for fjp, -1 for fjm) */
+static void visc_jac(real_fj *F, real s, const real *an, real sada, real dist, real mu_f, real k_f,
+                     const real *uf, const real *vf) {
+  const int mi = 1, ei = 4, ru = mi, rv = mi + 1, rw = mi + 2;
+  const real coef = mu_f * sada / dist;
+  const real third = 1.0 / 3.0;
+  real c_[3][3];
+  c_[0][0] = coef * (1.0 + an[0] * an[0] * third);
+  c_[0][1] = coef * an[0] * an[1] * third;
+  c_[0][2] = coef * an[0] * an[2] * third;
+  c_[1][0] = coef * an[1] * an[0] * third;
+  c_[1][1] = coef * (1.0 + an[1] * an[1] * third);
+  c_[1][2] = coef * an[1] * an[2] * third;
+  c_[2][0] = coef * an[2] * an[0] * third;
+  c_[2][1] = coef * an[2] * an[1] * third;
+  c_[2][2] = coef * (1.0 + an[2] * an[2] * third);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) addfj(&M_(F, mi + i, mi + j), s * c_[i][j]);
+  /* energy row, literal operand order of the rule bodies:
+   * diagonal u_i*coef*(1 + n_i n_i/3), off-diagonal u_i*coef*n_i*n_j/3 */
+  const int cols[3] = {ru, rv, rw};
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      real term = (i == j) ? uf[i] * coef * (1.0 + an[i] * an[i] * third)
+                           : uf[i] * coef * an[i] * an[j] * third;
+      addfj(&M_(F, ei, cols[j]), s * term);
+    }
+  addfj(&M_(F, ei, ru), s * (.5 * vf[0]));
+  addfj(&M_(F, ei, rv), s * (.5 * vf[1]));
+  addfj(&M_(F, ei, rw), s * (.5 * vf[2]));
+  const real factor = k_f * sada / dist;
+  addfj(&M_(F, ei, 0), s * factor);
+}
+
+/* Boundary-face Jacobian block bj (real_fj, zeroed by the caller), i.e. the
+ * face's whole contribution to srcJ of its interior cell:
+ *   farfield src/bcInflow.loci:357-375, wall src/bcViscousWall.loci:230-240,
+ *   symmetry src/bcImpermeable.loci:75-100, viscous src/fluxViscous.loci:405-448. */
+static void boundary_block(const og_grid *g, const og_params *p, const double *q, const double *qf,
+                           const double *grad, const double *phi, int f, real_fj *bj) {
+  int nc = g->nc, nf = g->nf;
+  int t = g->btype[f];
+  int c = g->cl[f];
+  real an[3], xf[3], xl[3];
+  fnormal(g, f, an);
+  fcenter(g, f, xf);
+  ccenter(g, c, xl);
+  const real sada = g->area[f], us_n = 0.0;
+  if (t == OG_FARFIELD) {
+    real qL[5];
+    recon(g, p, q, grad, phi, c, xl, xf, qL);
+    real UL[3] = {qL[1], qL[2], qL[3]};
+    real UR[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+    real_fj fjp[25];
+    for (int i = 0; i < 25; ++i) fjp[i] = 0;
+    roe_jacobian(fjp, qL[4], qL[0], UL, qf[4 * nf + f], qf[f], UR, an, sada, p->Pambient, p->Rt,
+                 p->gamma, us_n, 1.0, 1);
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) M_(bj, i, j) -= M_(fjp, i, j);
+  } else if (t == OG_WALL) {
+    const int mi = 1, ei = 4;
+    real uf[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+    subfj(&M_(bj, mi + 0, ei), an[0] * sada);
+    subfj(&M_(bj, mi + 1, ei), an[1] * sada);
+    subfj(&M_(bj, mi + 2, ei), an[2] * sada);
+    subfj(&M_(bj, ei, ei), sada * dot3(uf, an));
+  } else if (t == OG_SYMMETRY) {
+    const real pp = q[4 * nc + c] + p->Pambient;
+    const real T = q[c];
+    const real rho = pp / (p->Rt * T);
+    const real drdt = -rho / T;
+    const real drdp = rho / pp;
+    const int mi = 1;
+    real ui[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+    const real U = sqrt(dot3(ui, ui));
+    const real uit = dot3(an, ui) - us_n;
+    const real coef1 = -0.5 * rho * sada;
+    const real coef2 = -0.5 * sada;
+    for (int j = 0; j < 3; ++j) {
+      const real coefj = coef1 * (ui[j] * uit / (U + 1e-30) + an[j] * U);
+      for (int i = 0; i < 3; ++i) addfj(&M_(bj, mi + i, mi + j), coefj * an[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+      real coefx = 0.5 * U * uit;
+      addfj(&M_(bj, mi + i, 0), coef2 * (coefx * drdt) * an[i]);
+      addfj(&M_(bj, mi + i, 4), coef2 * (1.0 + coefx * drdp) * an[i]);
+    }
+  }
+  if (p->viscous) {
+    real mu_f, k_f, uf[3], vf[4];
+    viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, uf, vf);
+    real dv[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+    visc_jac(bj, -1.0, an, sada, dot3(an, dv), mu_f, k_f, uf, vf);
+  }
+}
+
+/* Assemble the linear system of one steady Newton step:
+ *   rhs = src                       src/solverTimestep.loci:337-338
+ *   cflpdt, dtcfl, dtau             src/solverSetup.loci:107-119, src/solverTimestep.loci:290-329
+ *   fjp/fjm (Roe + viscous)         src/fluxInviscid.loci:1190-1221, src/fluxViscous.loci:315-403
+ *   srcJ, fluid_D                   src/solverLinearSystem.loci:106-120,176-224
+ *   B = rhs rounded to real_fj      src/solverLinearSystem.loci:45-54
+ *   LU of D with pivots             src/linearSolverSGS.loci:43-50 */
+void og_assemble(const og_grid *g, const og_params *p, const double *q, double *src,
+                 double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
+                 og_norms *nrm) {
+  int nc = g->nc, nf = g->nf;
+  real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
+  double *qf = calloc((size_t)5 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 15 * nc), *phi = malloc(sizeof(double) * 5 * nc);
+  double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
+  double *maxev = malloc(sizeof(double) * nf);
+  og_face_states(g, p, q, qf);
+  og_gradients(g, p, q, qf, grad, phi);
+  og_fluxes(g, p, q, qf, grad, phi, iflux, vflux, maxev);
+  og_adj a;
+  build_adj(g, &a);
+  gather_src(g, p, &a, iflux, vflux, src);
+  if (nrm) residual_norms(g, src, nrm);
+
+  /* face Jacobians */
+  const real us_n = 0.0, eta = 1.0;
+  real_fj *bblk = calloc((size_t)nf * 25, sizeof(real_fj));
+  for (int f = 0; f < nf; ++f) {
+    real_fj *P = fjp + 25 * (size_t)f, *Mm = fjm + 25 * (size_t)f;
+    for (int i = 0; i < 25; ++i) P[i] = Mm[i] = 0;
+    int t = g->btype[f];
+    if (t == OG_INTERIOR || t == OG_PERIODIC) {
+      real an[3], xf[3], xl[3], xr[3], qL[5], qR[5];
+      fnormal(g, f, an);
+      fcenter(g, f, xf);
+      int c = g->cl[f];
+      ccenter(g, c, xl);
+      int o = rcell(g, f, xr);
+      recon(g, p, q, grad, phi, c, xl, xf, qL);
+      recon(g, p, q, grad, phi, o, xr, xf, qR);
+      real UL[3] = {qL[1], qL[2], qL[3]}, UR[3] = {qR[1], qR[2], qR[3]};
+      roe_jacobian(P, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta, 1);
+      roe_jacobian(Mm, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta, -1);
+      if (p->viscous) {
+        real mu_f, k_f, uf[3], vf[4];
+        viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, uf, vf);
+        real dv[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+        const real dist = dot3(an, dv);
+        visc_jac(Mm, -1.0, an, g->area[f], dist, mu_f, k_f, uf, vf);
+        visc_jac(P, 1.0, an, g->area[f], dist, mu_f, k_f, uf, vf);
+      }
+    } else {
+      boundary_block(g, p, q, qf, grad, phi, f, bblk + 25 * (size_t)f);
+    }
+  }
+
+  /* cell loop: cflpdt, dtau, srcJ, D, B, LU */
+  for (int c = 0; c < nc; ++c) {
+    /* cflpdt: lower, upper, boundary */
+    real meva = 0.0;
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) meva += maxev[a.lo[j]] * g->area[a.lo[j]];
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) meva += maxev[a.up[j]] * g->area[a.up[j]];
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) meva += maxev[a.bd[j]] * g->area[a.bd[j]];
+    const real cflpdt = 0.5 * meva / g->vol[c];
+    real dtcfl = p->dtmax;
+    if (p->cflmax > 0) dtcfl = minr(dtcfl, p->cflmax / cflpdt);
+    /* dtau */
+    real dt = dtcfl;
+    const real rvol = 1. / g->vol[c];
+    const real Pamb = p->Pambient;
+    const real pg = q[4 * nc + c];
+    const real pp = pg + Pamb;
+    const real T = q[c];
+    const real Rt = p->Rt;
+    const real rP = 1. / (Rt * T);
+    const real rho = (pg + Pamb) * rP;
+    const real rT = -rho / T;
+    const real gm = p->gamma;
+    real U[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+    const real qq = 0.5 * dot3(U, U);
+    const real fT = rT * qq;
+    const real fP = (1. / (gm - 1) + rP * qq);
+    const real reta = 1. / 1.0; /* Eta_p = 1 */
+    const real a2 = gm * Rt * T;
+    const real pterm = (1. - reta) * (gm - 1.) / a2;
+    const real cterm = fP * rT * reta - fT * rP * reta;
+    const real q2 = 2. * qq;
+    real s[5];
+    for (int k = 0; k < 5; ++k) s[k] = src[k * nc + c];
+    const real dsu = s[1] * U[0] + s[2] * U[1] + s[3] * U[2];
+    const real dtemp = rvol * (s[0] * (fP * rT * reta + fT * pterm) + (dsu - s[4] - q2 * s[0]) * (rT * (pterm + rP * reta))) / (cterm * rT);
+    dt = minr(dt, p->urelax * T / (fabs(dtemp) + 1e-30));
+    const real dp = rvol * (rT * (q2 * s[0] - dsu + s[4]) - fT * s[0]);
+    dt = minr(dt, p->urelax * pp / (fabs(dp) + 1e-30));
+    if (dtau_out) dtau_out[c] = dt;
+
+    real_fj sJ[25];
+    for (int i = 0; i < 25; ++i) sJ[i] = 0;
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+      const real_fj *P = fjp + 25 * (size_t)a.up[j];
+      for (int i = 0; i < 25; ++i) sJ[i] -= P[i];
+    }
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+      const real_fj *Mm = fjm + 25 * (size_t)a.lo[j];
+      for (int i = 0; i < 25; ++i) sJ[i] += Mm[i];
+    }
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) {
+      const real_fj *bb = bblk + 25 * (size_t)a.bd[j];
+      for (int i = 0; i < 25; ++i) sJ[i] += bb[i];
+    }
+    real_fj *Dc = D + 25 * (size_t)c;
+    for (int i = 0; i < 25; ++i) Dc[i] = -sJ[i];
+    real coef = g->vol[c] / dt;
+    addfj(&M_(Dc, 0, 0), coef * rT);
+    addfj(&M_(Dc, 1, 0), coef * rT * U[0]);
+    addfj(&M_(Dc, 2, 0), coef * rT * U[1]);
+    addfj(&M_(Dc, 3, 0), coef * rT * U[2]);
+    addfj(&M_(Dc, 4, 0), coef * fT);
+    addfj(&M_(Dc, 0, 4), coef * (pterm + rP * reta));
+    addfj(&M_(Dc, 1, 4), coef * (pterm + rP * reta) * U[0]);
+    addfj(&M_(Dc, 2, 4), coef * (pterm + rP * reta) * U[1]);
+    addfj(&M_(Dc, 3, 4), coef * (pterm + rP * reta) * U[2]);
+    addfj(&M_(Dc, 4, 4), coef * (fT * pterm / rT + fP * reta));
+    addfj(&M_(Dc, 1, 1), coef * rho);
+    addfj(&M_(Dc, 4, 1), coef * rho * U[0]);
+    addfj(&M_(Dc, 2, 2), coef * rho);
+    addfj(&M_(Dc, 4, 2), coef * rho * U[1]);
+    addfj(&M_(Dc, 3, 3), coef * rho);
+    addfj(&M_(Dc, 4, 3), coef * rho * U[2]);
+    for (int k = 0; k < 5; ++k) B[5 * (size_t)c + k] = (real_fj)s[k];
+    lu_decompose(Dc, pivot + 5 * (size_t)c, 5);
+  }
+  free(bblk);
+  free_adj(&a);
+  free(qf); free(grad); free(phi); free(iflux); free(vflux); free(maxev);
+}
+
+/* Block SGS, src/linearSolverSGS.loci:57-69 (Jacobi start), :124-127 and
+ * :215-219 (periodic ghost copies), :186-208 (forward), :277-300 (backward).
+ * Cells are visited in index order, a topological order of the cl -> cr
+ * face graph since every interior face has cl < cr. */
+void og_sgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v, const void *D_v,
+            const int *pivot, const void *B_v, double *X) {
+  int nc = g->nc, nf = g->nf;
+  const real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
+  og_adj a;
+  build_adj(g, &a);
+  double *Xg = calloc((size_t)5 * nf, sizeof(double)); /* ghost copies per periodic face */
+  for (int c = 0; c < nc; ++c) {
+    real b[5];
+    for (int k = 0; k < 5; ++k) b[k] = B[5 * (size_t)c + k];
+    lu_solve(D + 25 * (size_t)c, pivot + 5 * (size_t)c, b, X + 5 * (size_t)c, 5);
+  }
+  for (int it = 0; it < nsweeps; ++it) {
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int f = 0; f < nf; ++f)
+        if (g->btype[f] == OG_PERIODIC)
+          for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
+      for (int ii = 0; ii < nc; ++ii) {
+        int c = pass == 0 ? ii : nc - 1 - ii;
+        real t[5], v[5];
+        for (int k = 0; k < 5; ++k) t[k] = -(real)B[5 * (size_t)c + k];
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          int f = a.up[j];
+          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          dotprod_accum(fjm + 25 * (size_t)f, xn, t); /* fluid_U = fjm */
+        }
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          int f = a.lo[j];
+          real_fj L[25];
+          const real_fj *P = fjp + 25 * (size_t)f;
+          for (int i = 0; i < 25; ++i) L[i] = -P[i]; /* fluid_L = -fjp */
+          dotprod_accum(L, X + 5 * (size_t)g->cl[f], t);
+        }
+        for (int k = 0; k < 5; ++k) v[k] = -t[k];
+        lu_solve(D + 25 * (size_t)c, pivot + 5 * (size_t)c, v, X + 5 * (size_t)c, 5);
+      }
+    }
+  }
+  free(Xg);
+  free_adj(&a);
+}
+
+/* Steady Newton steps, src/solverNewton.loci:62-78 (damped update). */
+int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, double *hist) {
+  int nc = g->nc, nf = g->nf;
+  double *src = malloc(sizeof(double) * 5 * nc);
+  real_fj *fjp = malloc(sizeof(real_fj) * 25 * (size_t)nf), *fjm = malloc(sizeof(real_fj) * 25 * (size_t)nf);
+  real_fj *D = malloc(sizeof(real_fj) * 25 * (size_t)nc), *B = malloc(sizeof(real_fj) * 5 * (size_t)nc);
+  int *piv = malloc(sizeof(int) * 5 * (size_t)nc);
+  double *X = malloc(sizeof(double) * 5 * (size_t)nc);
+  for (int s = 0; s < nsteps; ++s) {
+    og_norms nrm;
+    og_assemble(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm);
+    if (hist) {
+      hist[6 * s + 0] = nrm.rtrms; hist[6 * s + 1] = nrm.etrms; hist[6 * s + 2] = nrm.mtrms;
+      hist[6 * s + 3] = nrm.rtmax; hist[6 * s + 4] = nrm.etmax; hist[6 * s + 5] = nrm.mtmax;
+    }
+    og_sgs(g, p->sgs_iter, fjp, fjm, D, piv, B, X);
+    for (int c = 0; c < nc; ++c) {
+      const real *dq = X + 5 * (size_t)c;
+      const real dT = dq[0], dp = dq[4];
+      real alpha = 1.0;
+      alpha = minr(alpha, 0.9 * q[c] / (fabs(dT) + 1e-30));
+      alpha = minr(alpha, 0.9 * (q[4 * nc + c] + p->Pambient) / (fabs(dp) + 1e-30));
+      q[c] = q[c] + alpha * dT;
+      q[4 * nc + c] = q[4 * nc + c] + alpha * dp;
+      for (int k = 1; k < 4; ++k) q[k * nc + c] = q[k * nc + c] + alpha * dq[k];
+    }
+  }
+  free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X);
+  return 0;
+}

@@ -1,0 +1,112 @@
+/*
+ * CPU oracle for the flowPsi implicit finite-volume Newton step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or as the timed CPU baseline).  The product library
+ * (libfpsi_gpu.so) never links, loads or calls it.
+ *
+ * It is a plain-C restatement of the flowPsi rule bodies on the Newton-step
+ * path (citations are per function in oracle.c).  It is compiled twice:
+ *   liboracle_f64.so : real_fj = double  ("plain FP64" Jacobian path)
+ *   liboracle_f32.so : real_fj = float   (flowPsi's default, include/flowTypes.h:43-49)
+ *
+ * Data layout (all arrays structure-of-arrays, reference cell/face order):
+ *   q    [5][nc]  primitive state (T, u, v, w, pg)
+ *   qf   [5][nf]  boundary-face state (T_f, u_f, pg_f); unused on interior faces
+ *   grad [15][nc] gradient of q: grad[3*k+d] = d q_k / d x_d
+ *   phi  [5][nc]  limiter per variable
+ *   flux [5][nf]  iflux;  vflux [4][nf]
+ *   blocks are 5x5 column-major (element (i,j) at i + 5*j), as Loci::Mat.
+ */
+#ifndef FPSI_ORACLE_H
+#define FPSI_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OG_INTERIOR = 0, OG_FARFIELD = 1, OG_WALL = 2, OG_SYMMETRY = 3, OG_PERIODIC = 4 };
+
+typedef struct {
+  int nc, nf;
+  const int *cl, *cr;     /* cr >= 0 interior; -1 on every boundary face (periodic included) */
+  const int *btype;       /* OG_* per face */
+  const int *pface;       /* periodic partner face, -1 otherwise */
+  const double *n;        /* [3][nf] unit normal, cl -> cr (out of the domain on boundaries) */
+  const double *area;     /* [nf] */
+  const double *fc;       /* [3][nf] face centre */
+  const double *vol;      /* [nc] */
+  const double *cc;       /* [3][nc] cell centre */
+  const double *pshift;   /* [3][nf] periodic ghost centre = cc[cl[pface]] + pshift */
+} og_grid;
+
+typedef struct {
+  double gamma, Rt, Pambient;
+  int flux;               /* 0 = HLLC, 1 = Roe (roeCT_flux) */
+  int limiter;            /* 0 = none, 1 = venkatakrishnan, 2 = zero (first order), 3 = barth */
+  double Kl;
+  int viscous;            /* 0 = inviscid, 1 = laminar */
+  double suth[6];         /* a1 a2 a3 k1 k2 k3 */
+  double Twall;
+  double pg0, T0, u0[3];  /* farfield reference state */
+  double cflmax, dtmax, urelax;
+  int sgs_iter;
+  double minPg;           /* floor of the reconstructed gage pressure */
+} og_params;
+
+typedef struct {
+  double rtrms, etrms, mtrms, rtmax, etmax, mtmax;
+  double ccmaxr[3], ccmaxe[3];
+} og_norms;
+
+/* point functions (one face) */
+void og_inviscid_flux(double *f5, double pg, double T, const double *U, const double *an,
+                      double area, double pamb, double Rt, double gamma, double us_n);
+void og_hllc_flux(double *f5, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                  const double *Ur, const double *an, double area, double pamb, double Rt,
+                  double gamma, double us_n, double eta);
+void og_roe_flux(double *f5, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                 const double *Ur, const double *an, double area, double pamb, double Rt,
+                 double gamma, double us_n, double eta);
+/* 5x5 column-major block, real_fj precision of the build; block is accumulated (+=) */
+void og_roe_jacobian(void *fj25, double pgl, double Tl, const double *Ul, double pgr, double Tr,
+                     const double *Ur, const double *an, double area, double pamb, double Rt,
+                     double gamma, double us_n, double eta, int dir);
+void og_roe_jacobian_ev(void *fj25, double ev1, double ev2, double ev3, const double *U, double un,
+                        double a, double beta, double sigma, double gamma, double Rt, double rho,
+                        double T, double h0, const double *n, double area);
+void og_cv_inflow(double *pgb, double *Tb, double *ub, double pgi, double Ti, const double *ui,
+                  double pg0, double T0, const double *u0, const double *n, double us_n,
+                  double pamb, double gamma, double Rt, double beta);
+void og_cv_outflow(double *pgb, double *Tb, double *ub, double pgi, double Ti, const double *ui,
+                   double pg0, double T0, const double *u0, const double *n, double us_n,
+                   double pamb, double gamma, double Rt, double beta);
+int  og_lu_decompose(void *a25, int *pivot, int nb);
+void og_lu_solve(const void *a25, const int *pivot, const double *b, double *x, int nb);
+int  og_real_fj_bytes(void);
+
+/* pipeline stages (whole grid) */
+void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf);
+void og_gradients(const og_grid *g, const og_params *p, const double *q, const double *qf,
+                  double *grad, double *phi);
+void og_fluxes(const og_grid *g, const og_params *p, const double *q, const double *qf,
+               const double *grad, const double *phi, double *iflux, double *vflux,
+               double *maxev);
+void og_residual(const og_grid *g, const og_params *p, const double *q, double *src,
+                 double *iflux, double *vflux, og_norms *nrm);
+/* full assembly: dtau, D (LU-factored in place) with pivots, B, and the
+ * off-diagonal blocks fjp/fjm per face ([nf][25] each, real_fj). */
+void og_assemble(const og_grid *g, const og_params *p, const double *q, double *src,
+                 double *dtau, void *fjp, void *fjm, void *D, int *pivot, void *B,
+                 og_norms *nrm);
+/* block SGS (Jacobi start + sweeps in cell-index order) on an assembled system */
+void og_sgs(const og_grid *g, int nsweeps, const void *fjp, const void *fjm, const void *D,
+            const int *pivot, const void *B, double *X);
+/* one steady Newton step per call; hist[6*k..] = rtrms etrms mtrms (squared sums) per step */
+int  og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, double *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
