@@ -1,0 +1,140 @@
+"""Newton-step throughput of the HIP path on one MI355X.
+
+Default workload: laminar Navier-Stokes on the 128^3 stretched channel
+(2,097,152 cells), HLLC + viscous fluxes, FP64 everywhere (the plain path),
+10 symmetric block-SGS sweeps per steady Newton step.  A step is one full
+Newton step: residual, Jacobian assembly, block LU, SGS, update.  The state
+is resident in HBM before the timed region.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(HIP events on the library's stream, algorithmic bytes from the library's
+byte model) and a CPU baseline (the oracle on this host, one thread, on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="channel128", choices=["channel128", "box32"])
+    ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--sweep", default="persistent", choices=["persistent", "per_level"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+
+    import numpy as np
+    from flowpsi_amd import workloads
+    from flowpsi_amd.capi import Solver
+
+    t0 = time.time()
+    if a.workload == "channel128":
+        n = a.n or 128
+        g, q, prm = workloads.channel(n)
+        wname = f"laminar NS channel {n}^3 ({g.nc} cells), HLLC+viscous, SGS x{prm['sgs_iter']}"
+    else:
+        n = a.n or 32
+        g, q, prm = workloads.box(n)
+        wname = f"Euler box {n}^3 ({g.nc} cells), Roe, SGS x{prm['sgs_iter']}"
+    prm = dict(prm, precision=a.precision, sweep=a.sweep)
+    log(f"grid {g.name}: nc={g.nc} nf={g.nf} built in {time.time() - t0:.1f}s")
+    s = Solver(g, prm)
+    info = s.info()
+    log(f"context: {info}")
+    s.set_state(q)
+    t1 = time.time()
+    s.newton(a.warmup)
+    s.synchronize()
+    log(f"warmup {a.warmup} steps {time.time() - t1:.2f}s")
+
+    s.profile(True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    hist = s.newton(a.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile(False)
+    dt = t1 - t0
+    ms_per_step = 1e3 * dt / a.steps
+    value = g.nc * a.steps / dt
+    log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step; groups " +
+        ", ".join(f"{k}={v['ms'] / max(v['launches'], 1):.3f}ms x{v['launches']}" for k, v in prof.items()))
+
+    # dominant kernel group by time
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_s = d["ms"] / max(d["launches"], 1) / 1e3
+    achieved = d["bytes"] / avg_s / 1e9
+    # whole-step algorithmic bytes (per Newton step) for the step-level roofline
+    sweeps = prm["sgs_iter"] * 2
+    step_bytes = (prof["residual"]["bytes"] + prof["face_jac"]["bytes"] + prof["assemble"]["bytes"] +
+                  sweeps * prof["sgs_sweep"]["bytes"] + prof["update"]["bytes"])
+    step_gbs = step_bytes * a.steps / dt / 1e9
+
+    tot = [h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist]
+    cpu = None
+    if not a.no_cpu_baseline:
+        import oracle
+        fj = "f32" if a.precision == "fp32" else "f64"
+        o = oracle.Oracle(g, prm, fj)
+        log(f"cpu baseline: oracle ({fj}) {a.cpu_sample_steps} Newton step(s) on the same grid, 1 thread")
+        c0 = time.perf_counter()
+        o.newton(q, a.cpu_sample_steps)
+        c1 = time.perf_counter()
+        cpu = {"value": g.nc * a.cpu_sample_steps / (c1 - c0), "unit": "cells*Newton-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{a.cpu_sample_steps} Newton step(s) of the same workload ({g.nc} cells), "
+                         f"oracle real_fj={fj}, single thread, {c1 - c0:.1f}s"}
+        log(f"cpu baseline {cpu['value']:.3e} cells*steps/s")
+
+    out = {
+        "metric": "cells*Newton-steps/s",
+        "value": value,
+        "unit": "cells*Newton-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f64 residual/update, f32 Jacobian/LU/SGS matrices",
+        "data": "synthetic (seeded grid and initial state)",
+        "config": {"workload": wname, "cells": g.nc, "faces": g.nf, "levels": info["nlevels"],
+                   "precision": a.precision, "sweep": a.sweep, "sgs_sweeps": prm["sgs_iter"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_s * 1e3, "bytes_per_launch": d["bytes"]},
+        "step_roofline": {"achieved": step_gbs, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                          "bytes_per_step": step_bytes},
+        "residual_drop": float(np.sqrt(tot[-1] / tot[0])) if tot[0] > 0 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,13 +24,15 @@ FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0}
+SWEEP = {"persistent": 0, "per_level": 1}
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
 # every entry point declared in include/fpsi.h
 EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "fpsi_set_params",
            "fpsi_set_state", "fpsi_get_state", "fpsi_residual", "fpsi_gradients", "fpsi_assemble",
            "fpsi_get_system", "fpsi_linear_solve", "fpsi_update", "fpsi_newton_step",
-           "fpsi_synchronize"]
+           "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read"]
+KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
 class GridDesc(C.Structure):
@@ -45,7 +47,8 @@ class Params(C.Structure):
                 ("suth", C.c_double * 6), ("Twall", C.c_double), ("pg0", C.c_double),
                 ("T0", C.c_double), ("u0", C.c_double * 3), ("cflmax", C.c_double),
                 ("dtmax", C.c_double), ("urelax", C.c_double), ("linear_solver", C.c_int),
-                ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double)]
+                ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double),
+                ("sweep_mode", C.c_int)]
 
 
 class Norms(C.Structure):
@@ -60,6 +63,10 @@ class Norms(C.Structure):
 class Info(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("max_level_width", C.c_int), ("nnz_blocks", C.c_int),
                 ("device_bytes", C.c_double)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 5), ("launches", C.c_long * 5), ("bytes", C.c_double * 5)]
 
 
 class FpsiError(RuntimeError):
@@ -111,6 +118,7 @@ def make_params(prm: dict) -> Params:
     s.sgs_iter = prm.get("sgs_iter", 5)
     s.precision = PRECISION[prm.get("precision", "fp64")]
     s.minPg = prm.get("minPg", -s.Pambient)
+    s.sweep_mode = SWEEP[prm.get("sweep", "persistent")]
     return s
 
 
@@ -229,6 +237,14 @@ class Solver:
         hist = (Norms * max(nsteps, 1))()
         self._chk(self.L.fpsi_newton_step(self.h, int(nsteps), hist), "fpsi_newton_step")
         return [hist[i].as_dict() for i in range(nsteps)]
+
+    def profile(self, on: bool):
+        self._chk(self.L.fpsi_profile_enable(self.h, int(on)), "fpsi_profile_enable")
+
+    def profile_read(self) -> dict:
+        p = Profile()
+        self._chk(self.L.fpsi_profile_read(self.h, C.byref(p)), "fpsi_profile_read")
+        return {k: dict(ms=p.ms[i], launches=p.launches[i], bytes=p.bytes[i]) for i, k in enumerate(KGROUPS)}
 
     def synchronize(self):
         self._chk(self.L.fpsi_synchronize(self.h), "fpsi_synchronize")

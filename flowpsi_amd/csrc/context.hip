@@ -26,6 +26,8 @@ struct fpsi_ctx {
   std::vector<int> fperm;   // new -> old face
   std::vector<int> ifperm;  // old -> new face
   std::vector<int> lev_off; // level ranges in new numbering
+  int nchunks = 0;          // sweep chunks (<= 64 cells, within one level)
+  const int *chunk_lo = nullptr, *chunk_lev = nullptr, *lev_size = nullptr;
   int max_width = 0;
   double vtot = 0;
   DevGrid g{};
@@ -36,6 +38,12 @@ struct fpsi_ctx {
   Prm kp{};
   bool have_params = false, have_state = false, assembled = false;
   int fp32 = 0;
+  // profiling: event pairs per kernel group, resolved at fpsi_profile_read
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double prof_ms[FPSI_K_NGROUPS] = {0};
+  long prof_n[FPSI_K_NGROUPS] = {0};
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
 };
 
@@ -75,9 +83,37 @@ static int upload(fpsi_ctx *c, const T **dst, const std::vector<T> &h) {
 
 extern "C" const char *fpsi_last_error(const fpsi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+static hipEvent_t ev_get(fpsi_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// record the start of a timed kernel group; returns the event (or null)
+static hipEvent_t prof_begin(fpsi_ctx *c) {
+  if (!c->prof) return nullptr;
+  hipEvent_t e = ev_get(c);
+  if (e) hipEventRecord(e, c->stream);
+  return e;
+}
+static void prof_end(fpsi_ctx *c, int grp, hipEvent_t start) {
+  if (!c->prof || !start) return;
+  hipEvent_t e = ev_get(c);
+  if (!e) return;
+  hipEventRecord(e, c->stream);
+  c->ev_pending.push_back({grp, {start, e}});
+}
+
 extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (auto &p : ctx->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+  for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void *p : ctx->allocs) hipFree(p);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -327,6 +363,17 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   HIPCHK(c, hipStreamCreate(&c->stream));
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
+  g.nlev = nlev;
+  std::vector<int> chunk_lo, chunk_lev, lev_size(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    lev_size[l] = c->lev_off[l + 1] - c->lev_off[l];
+    for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 64) {
+      chunk_lo.push_back(a);
+      chunk_lev.push_back(l);
+    }
+  }
+  c->nchunks = (int)chunk_lo.size();
+  chunk_lo.push_back(nc);
   int rc = 0;
   if ((rc = upload(c, &g.fnx, fnx)) || (rc = upload(c, &g.fny, fny)) || (rc = upload(c, &g.fnz, fnz)) ||
       (rc = upload(c, &g.farea, farea)) || (rc = upload(c, &g.fcx, fcx)) || (rc = upload(c, &g.fcy, fcy)) ||
@@ -337,7 +384,9 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = upload(c, &g.g_off, g_off)) || (rc = upload(c, &g.g_ent, g_ent)) ||
       (rc = upload(c, &g.m_off, m_off)) || (rc = upload(c, &g.m_nbr, m_nbr)) ||
       (rc = upload(c, &g.m_part, m_part)) || (rc = upload(c, &g.slot_cl, slot_cl)) ||
-      (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id))) {
+      (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id)) ||
+      (rc = upload(c, &c->chunk_lo, chunk_lo)) || (rc = upload(c, &c->chunk_lev, chunk_lev)) ||
+      (rc = upload(c, &c->lev_size, lev_size))) {
     *out = c;
     return rc;
   }
@@ -349,7 +398,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
       (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper))) ||
-      (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12))) {
+      (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
+      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev)) || (rc = dalloc(c, &s.tick, 1))) {
     *out = c;
     return rc;
   }
@@ -388,6 +438,8 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
   if (p->linear_solver != FPSI_SOLVER_SGS) return fail(c, FPSI_ERR_ARG, "unknown linear solver");
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
+  if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL)
+    return fail(c, FPSI_ERR_ARG, "unknown sweep mode");
   if (!(p->gamma > 1.0) || !(p->Rtilde > 0.0)) return fail(c, FPSI_ERR_ARG, "bad gas constants");
   c->prm = *p;
   Prm &k = c->kp;
@@ -450,10 +502,12 @@ static int check_ready(fpsi_ctx *c) {
 }
 
 static void enqueue_residual(fpsi_ctx *c) {
+  hipEvent_t e = prof_begin(c);
   launch_bface(c->g, c->kp, c->s, c->stream);
   launch_grad(c->g, c->kp, c->s, c->stream);
   launch_face_flux(c->g, c->kp, c->s, c->stream);
   launch_gather(c->g, c->kp, c->vtot, c->s, c->stream);
+  prof_end(c, FPSI_K_RESIDUAL, e);
 }
 
 static void norms_from(const double *o, fpsi_norms *n) {
@@ -493,8 +547,12 @@ extern "C" int fpsi_gradients(fpsi_ctx *c, double *grad, double *phi) {
 
 static void enqueue_assemble(fpsi_ctx *c) {
   enqueue_residual(c);
+  hipEvent_t e = prof_begin(c);
   launch_face_jac(c->g, c->kp, c->fp32, c->s, c->stream);
+  prof_end(c, FPSI_K_FACE_JAC, e);
+  e = prof_begin(c);
   launch_cell_assemble(c->g, c->kp, c->fp32, c->s, c->stream);
+  prof_end(c, FPSI_K_ASSEMBLE, e);
 }
 
 extern "C" int fpsi_assemble(fpsi_ctx *c) {
@@ -560,6 +618,21 @@ extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *
 
 static void enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
   const int nlev = (int)c->lev_off.size() - 1;
+  if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int nwaves = std::max(ncu, 1) * 8;
+    for (int it = 0; it < nsweeps; ++it)
+      for (int dir = 0; dir < 2; ++dir) {
+        launch_sweep_prep(c->g, c->s, c->stream);
+        hipEvent_t e = prof_begin(c);
+        launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, c->chunk_lev, c->lev_size, nwaves,
+                         c->s, c->stream);
+        prof_end(c, FPSI_K_SGS_SWEEP, e);
+      }
+    return;
+  }
   for (int it = 0; it < nsweeps; ++it) {
     launch_ghost(c->g, c->s, c->stream);
     for (int l = 0; l < nlev; ++l)
@@ -610,7 +683,9 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
     enqueue_assemble(c);
     HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     enqueue_sweeps(c, c->prm.sgs_iter);
+    hipEvent_t e = prof_begin(c);
     launch_update(c->g, c->kp, c->s, c->stream);
+    prof_end(c, FPSI_K_UPDATE, e);
   }
   HIPCHK(c, hipGetLastError());
   if (nsteps > 0) {
@@ -630,5 +705,55 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
 extern "C" int fpsi_synchronize(fpsi_ctx *c) {
   if (!c) return FPSI_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fpsi_profile_enable(fpsi_ctx *c, int on) {
+  if (!c) return FPSI_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.second.first); c->ev_pool.push_back(p.second.second); }
+  c->ev_pending.clear();
+  for (int k = 0; k < FPSI_K_NGROUPS; ++k) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+  c->prof = on != 0;
+  return 0;
+}
+
+extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
+  if (!c || !out) return FPSI_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &p : c->ev_pending) {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    c->prof_ms[p.first] += ms;
+    c->prof_n[p.first] += 1;
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->ev_pending.clear();
+  // algorithmic bytes of one launch of each group (each array once, gathers
+  // cache-perfect), from the buffer sizes of this context
+  const double nc = c->nc, nf = c->nf, nb = c->nb, nfi = c->nfi, nper = c->nper, nnz = c->nnz;
+  const double ps = 8, pj = (double)c->fj_bytes();
+  const double nent = nfi * 2 + nper + nb;  // gather-list entries
+  const double visc = c->prm.viscous ? 1 : 0;
+  const double face_geo = 64 * nf;            // n, area, fc (56) + cl, rc (8)
+  const double cell_rec = (5 + 15 + 5 + 3 + 1) * ps * nc;  // q, grad, phi, cc, vol
+  double b[FPSI_K_NGROUPS];
+  b[FPSI_K_RESIDUAL] =
+      /* bface */ nb * (5 * ps + 24 + 4 + 4) + nb * 5 * ps +
+      /* grad  */ nc * (5 * ps + 3 * ps + 6 * ps + 4 + ps) + 4 * nent + 24 * nf + 5 * ps * nb + nc * 20 * ps +
+      /* flux  */ face_geo + cell_rec + 5 * ps * nb + nf * (5 + 4 * visc + 1) * ps +
+      /* gather*/ 4 * nc + 4 * nent + nf * (5 + 4 * visc) * ps + nc * (ps + 4) + 5 * ps * nc;
+  b[FPSI_K_FACE_JAC] = face_geo + cell_rec + 5 * ps * nb + visc * 4 * ps * nf + 4 * (2 * nfi + nper) +
+                       (2 * nfi + 2 * nper + nb) * 25 * pj;
+  b[FPSI_K_ASSEMBLE] = 4 * nc + 4 * nent + 16 * nf + (5 + 5 + 1) * ps * nc + 4 * nc + 4 * nnz +
+                       (nnz + nper + nb) * 25 * pj + nc * (ps + 25 * pj + 4 + 5 * pj + 5 * ps);
+  b[FPSI_K_SGS_SWEEP] = nnz * (25 * pj + 4) + nc * (4 + 25 * pj + 4 + 5 * pj + 2 * 5 * ps);
+  b[FPSI_K_UPDATE] = nc * (5 * ps + 2 * 5 * ps);
+  for (int k = 0; k < FPSI_K_NGROUPS; ++k) {
+    out->ms[k] = c->prof_ms[k];
+    out->launches[k] = c->prof_n[k];
+    out->bytes[k] = b[k];
+  }
   return 0;
 }

@@ -4,6 +4,8 @@
 #include "kernels.h"
 #include "physics.h"
 
+#include <algorithm>
+
 namespace fpsi {
 
 static constexpr int TPB = 256;
@@ -824,6 +826,96 @@ __global__ void k_sgs_level(DevGrid g, int lo, int hi, const F *__restrict__ off
   for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
 }
 
+// One whole half-sweep in a single launch.  Cells are cut into chunks of at
+// most 64 cells that never straddle a level; waves take chunks in sweep order
+// from a ticket counter.  A chunk of level L waits until every cell of the
+// level it depends on (L-1 forward, L+1 backward) has published its X
+// (release by the producer, acquire by the consumer, agent scope), so every
+// dependency of flowPsi's sequential sweep is honoured.  A wave only takes a
+// new ticket after finishing its chunk and all earlier tickets belong to
+// running waves, so the wait always ends.
+template <typename F>
+__global__ void __launch_bounds__(256) k_sgs_sweep(DevGrid g, int backward, int nchunks,
+                                                   const int *__restrict__ chunk_lo,
+                                                   const int *__restrict__ chunk_lev,
+                                                   const int *__restrict__ lev_size,
+                                                   int *__restrict__ lev_cnt, int *__restrict__ tick,
+                                                   const F *__restrict__ off, const F *__restrict__ lu,
+                                                   const int *__restrict__ piv, const F *__restrict__ B,
+                                                   double *__restrict__ X) {
+  const int lane = threadIdx.x & 63;
+  const int nlev = g.nlev;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = __hip_atomic_fetch_add(tick, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = __shfl(t, 0, 64);
+    if (t >= nchunks) return;
+    const int ch = backward ? nchunks - 1 - t : t;
+    const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
+    const int L = chunk_lev[ch];
+    const int dep = backward ? L + 1 : L - 1;
+    const int c = lo + lane;
+    const bool act = c < hi;
+    // independent of the wait: right-hand side and pivots
+    double t5[5];
+    int pk = 0;
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) t5[k] = -(double)B[(size_t)c * 5 + k];
+      pk = piv[c];
+    }
+    if (dep >= 0 && dep < nlev) {
+      // every lane polls the same word (one request per wave)
+      const int need = lev_size[dep];
+      while (__hip_atomic_load(&lev_cnt[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+        __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (act) {
+      const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+      for (int j = m0; j < m1; ++j) {
+        const F *M = off + (size_t)j * 25;
+        const double *xn = X + (size_t)g.m_nbr[j] * 5;
+        double xv[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) xv[k] = xn[k];
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+#pragma unroll
+          for (int i = 0; i < 5; ++i) t5[i] += M[i + 5 * jj] * xv[jj];
+      }
+      double v[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[k] = -t5[k];
+      F a[25];
+#pragma unroll
+      for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+      double x[5];
+      lusolve5<F>(a, pk, v, x);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
+    }
+    // the fence completes every store of the wave before lane 0 publishes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) __hip_atomic_fetch_add(&lev_cnt[L], hi - lo, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Before each half-sweep: zero the ticket and level counters and refresh the
+// periodic ghost copies (src/linearSolverSGS.loci:124-127 and :215-219).
+__global__ void k_sweep_prep(DevGrid g, int *__restrict__ lev_cnt, int *__restrict__ tick,
+                             double *__restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.nlev) lev_cnt[i] = 0;
+  if (i == 0) *tick = 0;
+  if (i < g.nper) {
+    const int src = g.frc[g.nfi + i];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + i) * 5 + k] = X[(size_t)src * 5 + k];
+  }
+}
+
 // periodic ghost copies, src/linearSolverSGS.loci:124-127 and :215-219
 __global__ void k_ghost(DevGrid g, double *__restrict__ X) {
   const int pi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -897,6 +989,22 @@ void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, h
   else
     hipLaunchKernelGGL(k_sgs_level<double>, dim3(nblk(n)), dim3(TPB), 0, st, g, lo, hi,
                        (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, s.X);
+}
+void launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
+                      const int *chunk_lev, const int *lev_size, int nwaves, DevState &s, hipStream_t st) {
+  const int blocks = (std::min(nwaves, nchunks) + 3) / 4;
+  if (fp32)
+    hipLaunchKernelGGL(k_sgs_sweep<float>, dim3(blocks), dim3(256), 0, st, g, backward, nchunks, chunk_lo,
+                       chunk_lev, lev_size, s.lev_cnt, s.tick, (const float *)s.off, (const float *)s.lu,
+                       s.piv, (const float *)s.B, s.X);
+  else
+    hipLaunchKernelGGL(k_sgs_sweep<double>, dim3(blocks), dim3(256), 0, st, g, backward, nchunks, chunk_lo,
+                       chunk_lev, lev_size, s.lev_cnt, s.tick, (const double *)s.off, (const double *)s.lu,
+                       s.piv, (const double *)s.B, s.X);
+}
+void launch_sweep_prep(const DevGrid &g, DevState &s, hipStream_t st) {
+  const int n = std::max(g.nlev, g.nper);
+  hipLaunchKernelGGL(k_sweep_prep, dim3(nblk(std::max(n, 1))), dim3(TPB), 0, st, g, s.lev_cnt, s.tick, s.X);
 }
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);

@@ -17,6 +17,7 @@ struct DevGrid {
   int nper;       // periodic faces   [nfi, nfi + nper)
   int nb;         // boundary faces   [nfi + nper, nf)
   int bnd_begin;  // nfi + nper
+  int nlev;       // Gauss-Seidel levels
   // faces
   const double *fnx, *fny, *fnz, *farea, *fcx, *fcy, *fcz;
   const int *fcl;    // left cell
@@ -57,6 +58,8 @@ struct DevState {
   double *X;      // [nc + nper][5]
   double *npart;  // norm partials
   double *nout;   // final norms
+  int *lev_cnt;   // [nlev] cells published per level (persistent sweep)
+  int *tick;      // chunk ticket counter
 };
 
 }  // namespace fpsi

@@ -43,6 +43,8 @@ enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
 enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
 enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
 enum { FPSI_SOLVER_SGS = 0 };
+/* one persistent launch per half-sweep (level counters), or one launch per level */
+enum { FPSI_SWEEP_PERSISTENT = 0, FPSI_SWEEP_PER_LEVEL = 1 };
 
 /* Grid: the Loci FVM facts cl, cr, area{n,sada}, facecenter, vol,
  * cellcenter, pmap (used at src/fluxInviscid.loci:1170-1188,
@@ -84,6 +86,7 @@ typedef struct {
   int sgs_iter;         /* symmetric sweeps */
   int precision;        /* FPSI_PREC_*: storage/arithmetic of real_fj (Jacobians, D, LU, B) */
   double minPg;         /* floor of the reconstructed gage pressure */
+  int sweep_mode;       /* FPSI_SWEEP_*: how a Gauss-Seidel half-sweep is launched */
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -155,6 +158,24 @@ int fpsi_update(fpsi_ctx *ctx);
  * linear solve, update.  hist[nsteps] (nullable) receives the norms of the
  * residual evaluated at the start of each step. */
 int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
+
+/* Kernel groups timed with HIP events on the context's stream while
+ * profiling is on, and their algorithmic bytes per launch (compulsory
+ * traffic, each array counted once per launch, computed from the actual
+ * buffer sizes). */
+enum { FPSI_K_RESIDUAL = 0,   /* boundary states, gradients+limiters, face fluxes, gather+norms */
+       FPSI_K_FACE_JAC = 1,   /* face Jacobian blocks */
+       FPSI_K_ASSEMBLE = 2,   /* cell assembly + LU + Jacobi start */
+       FPSI_K_SGS_SWEEP = 3,  /* one Gauss-Seidel half-sweep */
+       FPSI_K_UPDATE = 4,     /* Newton update */
+       FPSI_K_NGROUPS = 5 };
+typedef struct {
+  double ms[FPSI_K_NGROUPS];        /* summed event time */
+  long launches[FPSI_K_NGROUPS];    /* timed launches */
+  double bytes[FPSI_K_NGROUPS];     /* algorithmic bytes of one launch */
+} fpsi_profile;
+int fpsi_profile_enable(fpsi_ctx *ctx, int on);   /* clears the accumulators */
+int fpsi_profile_read(fpsi_ctx *ctx, fpsi_profile *out);
 
 /* Wait for the context's stream. */
 int fpsi_synchronize(fpsi_ctx *ctx);
