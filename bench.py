@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--workload", default="channel128", choices=["channel128", "box32"])
     ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--sweep", default="persistent", choices=["persistent", "per_level"])
+    ap.add_argument("--sweep", default="per_level", choices=["per_level", "persistent"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     a = ap.parse_args()

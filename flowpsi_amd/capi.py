@@ -24,7 +24,7 @@ FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0}
-SWEEP = {"persistent": 0, "per_level": 1}
+SWEEP = {"per_level": 0, "persistent": 1}
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
 # every entry point declared in include/fpsi.h
@@ -118,7 +118,7 @@ def make_params(prm: dict) -> Params:
     s.sgs_iter = prm.get("sgs_iter", 5)
     s.precision = PRECISION[prm.get("precision", "fp64")]
     s.minPg = prm.get("minPg", -s.Pambient)
-    s.sweep_mode = SWEEP[prm.get("sweep", "persistent")]
+    s.sweep_mode = SWEEP[prm.get("sweep", "per_level")]
     return s
 
 

@@ -26,7 +26,8 @@ struct fpsi_ctx {
   std::vector<int> fperm;   // new -> old face
   std::vector<int> ifperm;  // old -> new face
   std::vector<int> lev_off; // level ranges in new numbering
-  int nchunks = 0;          // sweep chunks (<= 64 cells, within one level)
+  int nchunks = 0;          // sweep chunks (<= 12 cells, within one level)
+  int sweep_waves = 0;      // co-resident waves of the cooperative sweep
   const int *chunk_lo = nullptr, *chunk_lev = nullptr, *lev_size = nullptr;
   int max_width = 0;
   double vtot = 0;
@@ -339,6 +340,11 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   }
   c->nnz = (int)m_nbr.size();
   std::vector<int> old_id(c->cperm);
+  // fixed-width neighbour rows for the sweeps; pads name the zero X entry
+  const int zslot = nc + nper;
+  std::vector<int> nbr_pad(6 * (size_t)nc, zslot);
+  for (int ni = 0; ni < nc; ++ni)
+    for (int j = m_off[ni]; j < m_off[ni + 1] && j - m_off[ni] < 6; ++j) nbr_pad[6 * (size_t)ni + (j - m_off[ni])] = m_nbr[j];
 
   // ---- every index the kernels dereference must be in bounds
   {
@@ -349,6 +355,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     for (int v : m_part) ok = ok && (v >= 0 ? v < nnz : (-1 - v) < nper);
     for (int v : slot_cl) ok = ok && v >= 0 && v < nnz;
     for (int v : slot_cr) ok = ok && v >= 0 && v < nnz;
+    for (int v : nbr_pad) ok = ok && v >= 0 && v <= nc + nper;
     for (int i = 0; i < nf; ++i) {
       ok = ok && fcl[i] >= 0 && fcl[i] < nc;
       if (i < nfi + nper) ok = ok && frc[i] >= 0 && frc[i] < nc;
@@ -361,13 +368,21 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
 
   // ---- device upload
   HIPCHK(c, hipStreamCreate(&c->stream));
+  {
+    int dev = 0, ncu = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // two co-resident waves per CU at most: enough to keep a level in flight
+    const int cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
+    c->sweep_waves = std::max(1, std::min(cap, 2 * ncu));
+  }
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
   g.nlev = nlev;
   std::vector<int> chunk_lo, chunk_lev, lev_size(nlev);
   for (int l = 0; l < nlev; ++l) {
     lev_size[l] = c->lev_off[l + 1] - c->lev_off[l];
-    for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 64) {
+    for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 12) {  // 12 cells per wave
       chunk_lo.push_back(a);
       chunk_lev.push_back(l);
     }
@@ -385,6 +400,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = upload(c, &g.m_off, m_off)) || (rc = upload(c, &g.m_nbr, m_nbr)) ||
       (rc = upload(c, &g.m_part, m_part)) || (rc = upload(c, &g.slot_cl, slot_cl)) ||
       (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id)) ||
+      (rc = upload(c, &g.nbr_pad, nbr_pad)) ||
       (rc = upload(c, &c->chunk_lo, chunk_lo)) || (rc = upload(c, &c->chunk_lev, chunk_lev)) ||
       (rc = upload(c, &c->lev_size, lev_size))) {
     *out = c;
@@ -397,15 +413,15 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.iflux, 5 * (size_t)nf)) || (rc = dalloc(c, &s.vflux, 4 * (size_t)nf)) ||
       (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
-      (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper))) ||
+      (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper + 1))) ||
       (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
-      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev)) || (rc = dalloc(c, &s.tick, 1))) {
+      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev * 32))) {
     *out = c;
     return rc;
   }
   // real_fj buffers sized for FP64 so the precision can change without reallocation
   double *tmp = nullptr;
-  if ((rc = dalloc(c, &tmp, 25 * (size_t)c->nnz))) { *out = c; return rc; }
+  if ((rc = dalloc(c, &tmp, 25 * (size_t)std::max(c->nnz, 1)))) { *out = c; return rc; }
   s.off = tmp;
   if ((rc = dalloc(c, &tmp, 25 * (size_t)nper))) { *out = c; return rc; }
   s.pblk = tmp;
@@ -416,7 +432,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   if ((rc = dalloc(c, &tmp, 5 * (size_t)nc))) { *out = c; return rc; }
   s.B = tmp;
   HIPCHK(c, hipMemsetAsync(s.vflux, 0, 4 * (size_t)nf * sizeof(double), c->stream));
-  HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper) * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.lev_cnt, 0, (size_t)nlev * 32 * sizeof(int), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *out = c;
   return 0;
@@ -616,38 +633,40 @@ extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *
   return 0;
 }
 
-static void enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
+static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
   const int nlev = (int)c->lev_off.size() - 1;
   if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
-    int dev = 0, ncu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const int nwaves = std::max(ncu, 1) * 8;
     for (int it = 0; it < nsweeps; ++it)
       for (int dir = 0; dir < 2; ++dir) {
         launch_sweep_prep(c->g, c->s, c->stream);
         hipEvent_t e = prof_begin(c);
-        launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, c->chunk_lev, c->lev_size, nwaves,
-                         c->s, c->stream);
+        const hipError_t he = launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, c->chunk_lev,
+                                               c->lev_size, c->sweep_waves, c->s, c->stream);
         prof_end(c, FPSI_K_SGS_SWEEP, e);
+        if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
       }
-    return;
+    return 0;
   }
   for (int it = 0; it < nsweeps; ++it) {
     launch_ghost(c->g, c->s, c->stream);
+    hipEvent_t e = prof_begin(c);
     for (int l = 0; l < nlev; ++l)
       launch_sgs_level(c->g, c->fp32, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
     launch_ghost(c->g, c->s, c->stream);
+    e = prof_begin(c);
     for (int l = nlev - 1; l >= 0; --l)
       launch_sgs_level(c->g, c->fp32, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
   }
+  return 0;
 }
 
 extern "C" int fpsi_linear_solve(fpsi_ctx *c, int nsweeps, double *dq) {
   if (!c) return FPSI_ERR_ARG;
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
-  enqueue_sweeps(c, nsweeps);
+  if (int rc = enqueue_sweeps(c, nsweeps)) return rc;
   HIPCHK(c, hipGetLastError());
   if (dq) {
     const int nc = c->nc;
@@ -682,7 +701,10 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    enqueue_sweeps(c, c->prm.sgs_iter);
+    if (int rc = enqueue_sweeps(c, c->prm.sgs_iter)) {
+      hipStreamSynchronize(c->stream);
+      return rc;
+    }
     hipEvent_t e = prof_begin(c);
     launch_update(c->g, c->kp, c->s, c->stream);
     prof_end(c, FPSI_K_UPDATE, e);

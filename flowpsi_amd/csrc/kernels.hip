@@ -790,125 +790,161 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
 }
 
 // ---------------------------------------------------------------------------
-// block SGS (src/linearSolverSGS.loci:124-300): one level of a forward or
-// backward sweep; every cell of a level depends only on other levels.
+// block SGS (src/linearSolverSGS.loci:124-300).  One cell per lane:
+//   t = -B; t += M_j X[nbr_j] over the row (upper faces, then lower faces);
+//   X = LU^-1 (-t).
+// Rows are read through a fixed-width neighbour list (pads point at a zero
+// X entry), so every block and neighbour load of a row issues at once.
 // ---------------------------------------------------------------------------
-template <typename F>
-__global__ void k_sgs_level(DevGrid g, int lo, int hi, const F *__restrict__ off,
-                            const F *__restrict__ lu, const int *__restrict__ piv,
-                            const F *__restrict__ B, double *__restrict__ X) {
-  const int c = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= hi) return;
-  double t[5];
+static constexpr int MAXR = 6;
+
+template <bool COHERENT>
+__device__ __forceinline__ double ldx(const double *p) {
+  if (COHERENT) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return *p;
+}
+template <bool COHERENT>
+__device__ __forceinline__ void stx(double *p, double v) {
+  if (COHERENT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
+// Five lanes per cell: lane i of the group owns equation i.  Each lane
+// accumulates row i of every block of the row against the neighbour's whole
+// X (the five lanes read the same X words), the five partial sums are
+// exchanged with cross-lane reads, and every lane of the group repeats the
+// small LU solve and keeps its own component.  The summation order of each
+// t_i is the one-lane order (blocks in row order, columns ascending), so the
+// result is bitwise the same as the sequential sweep.
+static constexpr int CPW = 12;  // cells per wave (60 of 64 lanes)
+
+template <typename F, bool COHERENT>
+__device__ __forceinline__ void sgs_cell5(const DevGrid &g, int c, int i, int gbase, bool act,
+                                          const F *__restrict__ off, const F *__restrict__ lu,
+                                          const int *__restrict__ piv, const F *__restrict__ B,
+                                          double *__restrict__ X) {
+  double t = 0.0;
+  int pk = 0;
+  F a[25];
+  if (act) {
+    t = -(double)B[(size_t)c * 5 + i];
+    pk = piv[c];
+    const int m0 = g.m_off[c];
+    const int len = g.m_off[c + 1] - m0;
+    int nb[MAXR];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) t[k] = -(double)B[(size_t)c * 5 + k];
-  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
-  for (int j = m0; j < m1; ++j) {
-    const F *M = off + (size_t)j * 25;
-    const double *xn = X + (size_t)g.m_nbr[j] * 5;
-    double xv[5];
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)c * MAXR + j];
+    F mrow[MAXR][5];
+    double xv[MAXR][5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) xv[k] = xn[k];
+    for (int j = 0; j < MAXR; ++j) {
+      const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
+      const F *M = off + (size_t)slot * 25;
 #pragma unroll
-    for (int jj = 0; jj < 5; ++jj)
+      for (int jj = 0; jj < 5; ++jj) {
+        mrow[j][jj] = M[i + 5 * jj];
+        xv[j][jj] = ldx<COHERENT>(X + (size_t)nb[j] * 5 + jj);
+      }
+    }
 #pragma unroll
-      for (int i = 0; i < 5; ++i) t[i] += M[i + 5 * jj] * xv[jj];
+    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu[(size_t)c * 25 + i2];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
+    for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+      const F *M = off + (size_t)(m0 + j) * 25;
+      const double *xn = X + (size_t)g.m_nbr[m0 + j] * 5;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += M[i + 5 * jj] * ldx<COHERENT>(xn + jj);
+    }
   }
   double v[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) v[k] = -t[k];
-  F a[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
-  double x[5];
-  lusolve5<F>(a, piv[c], v, x);
-#pragma unroll
-  for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
+  for (int k = 0; k < 5; ++k) v[k] = -__shfl(t, gbase + k, 64);
+  if (act) {
+    double x[5];
+    lusolve5<F>(a, pk, v, x);
+    const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+    stx<COHERENT>(X + (size_t)c * 5 + i, xi);
+  }
 }
 
-// One whole half-sweep in a single launch.  Cells are cut into chunks of at
-// most 64 cells that never straddle a level; waves take chunks in sweep order
-// from a ticket counter.  A chunk of level L waits until every cell of the
-// level it depends on (L-1 forward, L+1 backward) has published its X
-// (release by the producer, acquire by the consumer, agent scope), so every
-// dependency of flowPsi's sequential sweep is honoured.  A wave only takes a
-// new ticket after finishing its chunk and all earlier tickets belong to
-// running waves, so the wait always ends.
+// one level of a forward or backward sweep; every cell of a level depends only
+// on cells of other levels
 template <typename F>
-__global__ void __launch_bounds__(256) k_sgs_sweep(DevGrid g, int backward, int nchunks,
-                                                   const int *__restrict__ chunk_lo,
-                                                   const int *__restrict__ chunk_lev,
-                                                   const int *__restrict__ lev_size,
-                                                   int *__restrict__ lev_cnt, int *__restrict__ tick,
-                                                   const F *__restrict__ off, const F *__restrict__ lu,
-                                                   const int *__restrict__ piv, const F *__restrict__ B,
-                                                   double *__restrict__ X) {
-  const int lane = threadIdx.x & 63;
+__global__ void __launch_bounds__(64) k_sgs_level(DevGrid g, int lo, int hi, const F *__restrict__ off,
+                                                  const F *__restrict__ lu, const int *__restrict__ piv,
+                                                  const F *__restrict__ B, double *__restrict__ X) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int c = lo + blockIdx.x * CPW + q;
+  const bool act = q < CPW && c < hi;
+  sgs_cell5<F, false>(g, c, i, 5 * (q < CPW ? q : 0), act, off, lu, piv, B, X);
+}
+
+// One whole half-sweep in a single cooperative launch of W one-wave
+// workgroups, all co-resident.  Cells are cut into chunks of at most CPW cells
+// that never straddle a level, numbered in sweep order; wave w takes chunks
+// w, w+W, w+2W, ... in order.  A chunk of level L first waits until every cell
+// of the level it depends on (L-1 forward, L+1 backward) has been published,
+// so every dependency of flowPsi's sequential sweep is honoured.  The wave
+// holding the smallest unfinished chunk has finished all its earlier chunks
+// and all of that chunk's dependencies are smaller chunks, so it always
+// proceeds: every wait ends.  The loop has a uniform trip count and no
+// lane-specific branches.  X is written and read with device-coherent
+// accesses; a wave's stores complete before it publishes its cells with a
+// release atomic.  Counters sit 128 bytes apart.
+static constexpr int CNT_STRIDE = 32;
+
+template <typename F>
+__global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int backward, int nchunks, int nwaves,
+                                                  const int *__restrict__ chunk_lo,
+                                                  const int *__restrict__ chunk_lev,
+                                                  const int *__restrict__ lev_size,
+                                                  int *__restrict__ lev_cnt, const F *__restrict__ off,
+                                                  const F *__restrict__ lu, const int *__restrict__ piv,
+                                                  const F *__restrict__ B, double *__restrict__ X, int never) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
   const int nlev = g.nlev;
-  for (;;) {
-    int t = 0;
-    if (lane == 0) t = __hip_atomic_fetch_add(tick, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = __shfl(t, 0, 64);
-    if (t >= nchunks) return;
+  for (int t = blockIdx.x; t < nchunks; t += nwaves) {
     const int ch = backward ? nchunks - 1 - t : t;
     const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
     const int L = chunk_lev[ch];
     const int dep = backward ? L + 1 : L - 1;
-    const int c = lo + lane;
-    const bool act = c < hi;
-    // independent of the wait: right-hand side and pivots
-    double t5[5];
-    int pk = 0;
+    const int c = lo + q;
+    const bool act = q < CPW && c < hi;
     if (act) {
-#pragma unroll
-      for (int k = 0; k < 5; ++k) t5[k] = -(double)B[(size_t)c * 5 + k];
-      pk = piv[c];
+      // pull the row's blocks toward the caches while the level is pending:
+      // touch one word per 128-byte line; the sum is kept alive by a store
+      // that never happens (never == 0)
+      const int m0 = g.m_off[c];
+      const char *p0 = (const char *)(off + (size_t)m0 * 25);
+      const int nbytes = (g.m_off[c + 1] - m0) * 25 * (int)sizeof(F);
+      float touch = 0.f;
+      for (int o = i * 128; o < nbytes; o += 5 * 128) touch += *(const float *)(p0 + o);
+      if (never) X[0] = (double)touch;
     }
     if (dep >= 0 && dep < nlev) {
-      // every lane polls the same word (one request per wave)
       const int need = lev_size[dep];
-      while (__hip_atomic_load(&lev_cnt[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
-        __builtin_amdgcn_s_sleep(1);
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const int *cnt = lev_cnt + (size_t)dep * CNT_STRIDE;
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
+        __builtin_amdgcn_s_sleep(2);
     }
-    if (act) {
-      const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
-      for (int j = m0; j < m1; ++j) {
-        const F *M = off + (size_t)j * 25;
-        const double *xn = X + (size_t)g.m_nbr[j] * 5;
-        double xv[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) xv[k] = xn[k];
-#pragma unroll
-        for (int jj = 0; jj < 5; ++jj)
-#pragma unroll
-          for (int i = 0; i < 5; ++i) t5[i] += M[i + 5 * jj] * xv[jj];
-      }
-      double v[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) v[k] = -t5[k];
-      F a[25];
-#pragma unroll
-      for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
-      double x[5];
-      lusolve5<F>(a, pk, v, x);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
-    }
-    // the fence completes every store of the wave before lane 0 publishes
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) __hip_atomic_fetch_add(&lev_cnt[L], hi - lo, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    sgs_cell5<F, true>(g, c, i, 5 * (q < CPW ? q : 0), act, off, lu, piv, B, X);
+    // every store of the wave has completed (vmcnt = 0) before the publish
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __hip_atomic_fetch_add(lev_cnt + (size_t)L * CNT_STRIDE, (act && i == 0) ? 1 : 0, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Before each half-sweep: zero the ticket and level counters and refresh the
+// Before each half-sweep: zero the level counters and refresh the
 // periodic ghost copies (src/linearSolverSGS.loci:124-127 and :215-219).
-__global__ void k_sweep_prep(DevGrid g, int *__restrict__ lev_cnt, int *__restrict__ tick,
-                             double *__restrict__ X) {
+__global__ void k_sweep_prep(DevGrid g, int *__restrict__ lev_cnt, double *__restrict__ X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < g.nlev) lev_cnt[i] = 0;
-  if (i == 0) *tick = 0;
+  if (i < g.nlev) lev_cnt[(size_t)i * CNT_STRIDE] = 0;
   if (i < g.nper) {
     const int src = g.frc[g.nfi + i];
 #pragma unroll
@@ -983,28 +1019,40 @@ void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, DevState &s,
 void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st) {
   const int n = hi - lo;
   if (n <= 0) return;
+  const int nb = (n + CPW - 1) / CPW;
   if (fp32)
-    hipLaunchKernelGGL(k_sgs_level<float>, dim3(nblk(n)), dim3(TPB), 0, st, g, lo, hi,
+    hipLaunchKernelGGL(k_sgs_level<float>, dim3(nb), dim3(64), 0, st, g, lo, hi,
                        (const float *)s.off, (const float *)s.lu, s.piv, (const float *)s.B, s.X);
   else
-    hipLaunchKernelGGL(k_sgs_level<double>, dim3(nblk(n)), dim3(TPB), 0, st, g, lo, hi,
+    hipLaunchKernelGGL(k_sgs_level<double>, dim3(nb), dim3(64), 0, st, g, lo, hi,
                        (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, s.X);
 }
-void launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
-                      const int *chunk_lev, const int *lev_size, int nwaves, DevState &s, hipStream_t st) {
-  const int blocks = (std::min(nwaves, nchunks) + 3) / 4;
-  if (fp32)
-    hipLaunchKernelGGL(k_sgs_sweep<float>, dim3(blocks), dim3(256), 0, st, g, backward, nchunks, chunk_lo,
-                       chunk_lev, lev_size, s.lev_cnt, s.tick, (const float *)s.off, (const float *)s.lu,
-                       s.piv, (const float *)s.B, s.X);
-  else
-    hipLaunchKernelGGL(k_sgs_sweep<double>, dim3(blocks), dim3(256), 0, st, g, backward, nchunks, chunk_lo,
-                       chunk_lev, lev_size, s.lev_cnt, s.tick, (const double *)s.off, (const double *)s.lu,
-                       s.piv, (const double *)s.B, s.X);
+int sgs_sweep_waves(int fp32, int ncu) {
+  int per_cu = 0;
+  hipError_t e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgs_sweep<float>, 64, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgs_sweep<double>, 64, 0);
+  if (e != hipSuccess || per_cu <= 0) return 0;
+  return per_cu * ncu;
+}
+
+hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
+                            const int *chunk_lev, const int *lev_size, int nwaves, DevState &s, hipStream_t st) {
+  int nw = std::max(1, std::min(nwaves, nchunks));
+  int never = 0;
+  if (fp32) {
+    const float *off = (const float *)s.off, *lu = (const float *)s.lu, *B = (const float *)s.B;
+    void *args[] = {(void *)&g, &backward, &nchunks, &nw, &chunk_lo, &chunk_lev, &lev_size, &s.lev_cnt,
+                    &off, &lu, &s.piv, &B, &s.X, &never};
+    return hipLaunchCooperativeKernel((const void *)k_sgs_sweep<float>, dim3(nw), dim3(64), args, 0, st);
+  }
+  const double *off = (const double *)s.off, *lu = (const double *)s.lu, *B = (const double *)s.B;
+  void *args[] = {(void *)&g, &backward, &nchunks, &nw, &chunk_lo, &chunk_lev, &lev_size, &s.lev_cnt,
+                  &off, &lu, &s.piv, &B, &s.X, &never};
+  return hipLaunchCooperativeKernel((const void *)k_sgs_sweep<double>, dim3(nw), dim3(64), args, 0, st);
 }
 void launch_sweep_prep(const DevGrid &g, DevState &s, hipStream_t st) {
   const int n = std::max(g.nlev, g.nper);
-  hipLaunchKernelGGL(k_sweep_prep, dim3(nblk(std::max(n, 1))), dim3(TPB), 0, st, g, s.lev_cnt, s.tick, s.X);
+  hipLaunchKernelGGL(k_sweep_prep, dim3(nblk(std::max(n, 1))), dim3(TPB), 0, st, g, s.lev_cnt, s.X);
 }
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);

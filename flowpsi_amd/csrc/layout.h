@@ -34,6 +34,7 @@ struct DevGrid {
   // neighbour (cell, or nc + periodic index) and the partner slot holding the
   // same face's other block (>= 0), or -(1 + periodic index).
   const int *m_off, *m_nbr, *m_part;
+  const int *nbr_pad;  // [nc][6] row neighbours padded with the zero X entry nc + nper
   const int *slot_cl;  // [nfi + nper] slot of U_f = fjm in row cl
   const int *slot_cr;  // [nfi] slot of L_f = -fjp in row cr
   const int *old_id;   // [nc] caller's index of each cell (tie-breaks of the norms)
@@ -55,11 +56,10 @@ struct DevState {
   void *lu;       // [nc][25] real_fj
   int *piv;       // [nc] packed 3 bits per row
   void *B;        // [nc][5] real_fj
-  double *X;      // [nc + nper][5]
+  double *X;      // [nc + nper + 1][5], last entry stays zero
   double *npart;  // norm partials
   double *nout;   // final norms
-  int *lev_cnt;   // [nlev] cells published per level (persistent sweep)
-  int *tick;      // chunk ticket counter
+  int *lev_cnt;   // [nlev * 32] cells published per level (persistent sweep)
 };
 
 }  // namespace fpsi

@@ -43,8 +43,9 @@ enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
 enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
 enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
 enum { FPSI_SOLVER_SGS = 0 };
-/* one persistent launch per half-sweep (level counters), or one launch per level */
-enum { FPSI_SWEEP_PERSISTENT = 0, FPSI_SWEEP_PER_LEVEL = 1 };
+/* one launch per dependency level (default), or one cooperative launch per
+ * half-sweep with per-level completion counters */
+enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1 };
 
 /* Grid: the Loci FVM facts cl, cr, area{n,sada}, facecenter, vol,
  * cellcenter, pmap (used at src/fluxInviscid.loci:1170-1188,
@@ -166,7 +167,7 @@ int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
 enum { FPSI_K_RESIDUAL = 0,   /* boundary states, gradients+limiters, face fluxes, gather+norms */
        FPSI_K_FACE_JAC = 1,   /* face Jacobian blocks */
        FPSI_K_ASSEMBLE = 2,   /* cell assembly + LU + Jacobi start */
-       FPSI_K_SGS_SWEEP = 3,  /* one Gauss-Seidel half-sweep */
+       FPSI_K_SGS_SWEEP = 3,  /* one Gauss-Seidel half-sweep (all of its level launches) */
        FPSI_K_UPDATE = 4,     /* Newton update */
        FPSI_K_NGROUPS = 5 };
 typedef struct {

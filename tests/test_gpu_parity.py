@@ -192,7 +192,7 @@ def test_bitwise_reproducible():
 
 
 def test_sweep_modes_agree():
-    """The persistent half-sweep kernel and one launch per level give the same bits."""
+    """One launch per level and the cooperative half-sweep kernel give the same bits."""
     g, q, prm = workloads.channel(nx=12, ny=16, nz=10)
     outs = []
     for mode in ("persistent", "per_level"):
