@@ -31,7 +31,10 @@ ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_
 EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "fpsi_set_params",
            "fpsi_set_state", "fpsi_get_state", "fpsi_residual", "fpsi_gradients", "fpsi_assemble",
            "fpsi_get_system", "fpsi_linear_solve", "fpsi_update", "fpsi_newton_step",
-           "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read"]
+           "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read", "fpsi_linear_residual",
+           "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
+           "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
+           "fpsi_mlp_eval", "fpsi_mlp_get_sample"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -65,6 +68,14 @@ class Info(C.Structure):
                 ("device_bytes", C.c_double)]
 
 
+class LinStats(C.Structure):
+    _fields_ = [("r0", C.c_double), ("rguess", C.c_double), ("rfinal", C.c_double),
+                ("sweeps", C.c_int), ("used_guess", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Profile(C.Structure):
     _fields_ = [("ms", C.c_double * 5), ("launches", C.c_long * 5), ("bytes", C.c_double * 5)]
 
@@ -89,6 +100,11 @@ def load(path: str = LIB_PATH):
             if name not in ("fpsi_last_error", "fpsi_destroy"):
                 getattr(L, name).restype = C.c_int
         L.fpsi_destroy.restype = None
+        L.fpsi_mlp_init.argtypes = [C.c_void_p, C.c_ulonglong]
+        L.fpsi_mlp_train.argtypes = [C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_double)]
+        L.fpsi_linear_solve_tol.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(LinStats)]
+        L.fpsi_newton_adaptive.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p]
         _lib = L
     return _lib
 
@@ -237,6 +253,71 @@ class Solver:
         hist = (Norms * max(nsteps, 1))()
         self._chk(self.L.fpsi_newton_step(self.h, int(nsteps), hist), "fpsi_newton_step")
         return [hist[i].as_dict() for i in range(nsteps)]
+
+    # -- linear tolerance and learned guess
+    def linear_residual(self):
+        v = C.c_double()
+        self._chk(self.L.fpsi_linear_residual(self.h, C.byref(v)), "fpsi_linear_residual")
+        return v.value
+
+    def linear_solve_tol(self, rtol, max_sweeps, use_guess=False):
+        st = LinStats()
+        self._chk(self.L.fpsi_linear_solve_tol(self.h, rtol, int(max_sweeps), int(use_guess), C.byref(st)),
+                  "fpsi_linear_solve_tol")
+        return st.as_dict()
+
+    def newton_adaptive(self, nsteps, rtol, max_sweeps, use_guess=False):
+        hist = (Norms * max(nsteps, 1))()
+        lin = (LinStats * max(nsteps, 1))()
+        self._chk(self.L.fpsi_newton_adaptive(self.h, int(nsteps), rtol, int(max_sweeps), int(use_guess), hist, lin),
+                  "fpsi_newton_adaptive")
+        return [hist[i].as_dict() for i in range(nsteps)], [lin[i].as_dict() for i in range(nsteps)]
+
+    def mlp_init(self, seed=1):
+        self._chk(self.L.fpsi_mlp_init(self.h, seed), "fpsi_mlp_init")
+
+    def mlp_num_params(self):
+        return self.L.fpsi_mlp_num_params()
+
+    def mlp_params(self):
+        p = np.zeros(self.mlp_num_params(), np.float32)
+        self._chk(self.L.fpsi_mlp_get_params(self.h, _ptr(p, C.c_float)), "fpsi_mlp_get_params")
+        return p
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._chk(self.L.fpsi_mlp_set_params(self.h, _ptr(p, C.c_float)), "fpsi_mlp_set_params")
+
+    def mlp_harvest(self, target_sweeps):
+        slot = C.c_int()
+        self._chk(self.L.fpsi_mlp_harvest(self.h, int(target_sweeps), C.byref(slot)), "fpsi_mlp_harvest")
+        return slot.value
+
+    def mlp_train(self, epochs, lr=1e-3):
+        loss = C.c_double()
+        self._chk(self.L.fpsi_mlp_train(self.h, int(epochs), float(lr), C.byref(loss)), "fpsi_mlp_train")
+        return loss.value
+
+    def mlp_eval(self, slot):
+        nc = self.grid.nc
+        out = np.zeros((nc, 16), np.float32)
+        grad = np.zeros(self.mlp_num_params(), np.float32)
+        loss = C.c_double()
+        self._chk(self.L.fpsi_mlp_eval(self.h, int(slot), _ptr(out, C.c_float), _ptr(grad, C.c_float),
+                                       C.byref(loss)), "fpsi_mlp_eval")
+        return out, grad, loss.value
+
+    def mlp_sample(self, slot):
+        nc = self.grid.nc
+        x = np.zeros((nc, 32), np.float32)
+        y = np.zeros((nc, 8), np.float32)
+        off = np.zeros(nc + 1, np.int32)
+        self._chk(self.L.fpsi_mlp_get_sample(self.h, int(slot), _ptr(x, C.c_float), _ptr(y, C.c_float),
+                                             _ptr(off, C.c_int), None), "fpsi_mlp_get_sample")
+        idx = np.zeros(int(off[-1]), np.int32)
+        self._chk(self.L.fpsi_mlp_get_sample(self.h, int(slot), None, None, _ptr(off, C.c_int),
+                                             _ptr(idx, C.c_int)), "fpsi_mlp_get_sample")
+        return x, y, off, idx
 
     def profile(self, on: bool):
         self._chk(self.L.fpsi_profile_enable(self.h, int(on)), "fpsi_profile_enable")

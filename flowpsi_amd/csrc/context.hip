@@ -14,6 +14,7 @@
 
 #include "../../include/fpsi.h"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace fpsi;
 
@@ -45,6 +46,16 @@ struct fpsi_ctx {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   double prof_ms[FPSI_K_NGROUPS] = {0};
   long prof_n[FPSI_K_NGROUPS] = {0};
+  // linear residual scratch
+  double *lr_parts = nullptr, *lr_bsum = nullptr, *lr_out = nullptr, *Xj = nullptr;
+  // learned guess: graph and geometry (new cell order), buffers, samples
+  std::vector<int> mlp_nb_off, mlp_nb_idx;
+  std::vector<float> mlp_geo;
+  bool mlp_alloc = false, mlp_norm = false;
+  MlpBufs mb{};
+  float *mlp_xs[4] = {nullptr, nullptr, nullptr, nullptr}, *mlp_ys[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *mlp_xcur = nullptr;
+  int mlp_nsamp = 0, mlp_next = 0, mlp_step = 0;
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
 };
 
@@ -340,6 +351,37 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   }
   c->nnz = (int)m_nbr.size();
   std::vector<int> old_id(c->cperm);
+  // neighbour graph of the learned guess (periodic ghosts mapped to cells)
+  c->mlp_nb_off.assign(nc + 1, 0);
+  c->mlp_nb_idx.clear();
+  c->mlp_nb_idx.reserve(m_nbr.size());
+  for (int ni = 0; ni < nc; ++ni) {
+    for (int j = m_off[ni]; j < m_off[ni + 1]; ++j) {
+      const int v = m_nbr[j];
+      c->mlp_nb_idx.push_back(v < nc ? v : frc[nfi + (v - nc)]);
+    }
+    c->mlp_nb_off[ni + 1] = (int)c->mlp_nb_idx.size();
+  }
+  // geometry features: sum_f A |n_d| / vol^(2/3) per axis, log vol - mean
+  c->mlp_geo.assign(4 * (size_t)nc, 0.f);
+  {
+    double lmean = 0;
+    for (int i = 0; i < nc; ++i) lmean += std::log(gd->vol[i]);
+    lmean /= nc;
+    for (int ni = 0; ni < nc; ++ni) {
+      const int i = c->cperm[ni];
+      double sh[3] = {0, 0, 0};
+      auto addf = [&](int f) {
+        for (int d = 0; d < 3; ++d) sh[d] += gd->area[f] * std::fabs(gd->n[(size_t)d * nf + f]);
+      };
+      for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) addf(lo[j]);
+      for (int j = up_off[i]; j < up_off[i + 1]; ++j) addf(up[j]);
+      for (int j = bd_off[i]; j < bd_off[i + 1]; ++j) addf(bd[j]);
+      const double s23 = std::pow(gd->vol[i], 2.0 / 3.0);
+      for (int d = 0; d < 3; ++d) c->mlp_geo[4 * (size_t)ni + d] = (float)(sh[d] / s23);
+      c->mlp_geo[4 * (size_t)ni + 3] = (float)(std::log(gd->vol[i]) - lmean);
+    }
+  }
   // fixed-width neighbour rows for the sweeps; pads name the zero X entry
   const int zslot = nc + nper;
   std::vector<int> nbr_pad(6 * (size_t)nc, zslot);
@@ -416,6 +458,11 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper + 1))) ||
       (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
       (rc = dalloc(c, &s.lev_cnt, (size_t)nlev * 32))) {
+    *out = c;
+    return rc;
+  }
+  if ((rc = dalloc(c, &c->lr_parts, 5 * (size_t)norm_parts(g) + 5)) || (rc = dalloc(c, &c->lr_bsum, 8)) ||
+      (rc = dalloc(c, &c->lr_out, 8)) || (rc = dalloc(c, &c->Xj, 5 * ((size_t)nc + nper + 1)))) {
     *out = c;
     return rc;
   }
@@ -776,6 +823,306 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
     out->ms[k] = c->prof_ms[k];
     out->launches[k] = c->prof_n[k];
     out->bytes[k] = b[k];
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// linear residual, tolerance-driven solve, learned guess
+// ---------------------------------------------------------------------------
+static int linres(fpsi_ctx *c, double *out) {
+  launch_linres(c->g, c->fp32, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
+  double v = 0;
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(&v, c->lr_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = std::sqrt(v);
+  return 0;
+}
+
+extern "C" int fpsi_linear_residual(fpsi_ctx *c, double *norm) {
+  if (!c || !norm) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  return linres(c, norm);
+}
+
+static int mlp_guess(fpsi_ctx *c) {
+  MlpBufs m = c->mb;
+  const int nc = c->nc;
+  mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
+  mlp_normalise(m, nc, c->mlp_xcur, c->stream);
+  m.x = c->mlp_xcur;
+  mlp_forward(m, nc, c->stream);
+  mlp_apply_guess(m, nc, c->s.X, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st) {
+  fpsi_lin_stats s{};
+  double rcur = 0;
+  if (int rc = linres(c, &s.r0)) return rc;
+  rcur = s.r0;
+  if (use_guess) {
+    if (!c->mlp_alloc || !c->mlp_norm) return fail(c, FPSI_ERR_STATE, "learned guess requested before fpsi_mlp_init/harvest");
+    const long n = 5 * ((long)c->nc + c->nper + 1);
+    launch_copy_d(n, c->s.X, c->Xj, c->stream);
+    if (int rc = mlp_guess(c)) return rc;
+    if (int rc = linres(c, &s.rguess)) return rc;
+    if (s.rguess < s.r0) {
+      s.used_guess = 1;
+      rcur = s.rguess;
+    } else {
+      launch_copy_d(n, c->Xj, c->s.X, c->stream);  // fall back to the plain start
+    }
+  }
+  while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
+    if (int rc = enqueue_sweeps(c, 1)) return rc;
+    ++s.sweeps;
+    if (int rc = linres(c, &rcur)) return rc;
+  }
+  s.rfinal = rcur;
+  if (st) *st = s;
+  return 0;
+}
+
+extern "C" int fpsi_linear_solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  const int rc = solve_tol(c, rtol, max_sweeps, use_guess, st);
+  c->assembled = false;
+  return rc;
+}
+
+extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int max_sweeps, int use_guess,
+                                    fpsi_norms *hist, fpsi_lin_stats *lin) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    enqueue_assemble(c);
+    double o[12];
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(o, c->s.nout, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hist) norms_from(o, &hist[s]);
+    if (o[0] + o[1] + o[2] != o[0] + o[1] + o[2]) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
+    fpsi_lin_stats st{};
+    if (int rc = solve_tol(c, rtol, max_sweeps, use_guess, &st)) return rc;
+    if (lin) lin[s] = st;
+    launch_update(c->g, c->kp, c->s, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;
+  return 0;
+}
+
+// splitmix64 + Box-Muller: deterministic initial weights
+static unsigned long long sm64(unsigned long long &s) {
+  unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double gauss(unsigned long long &s) {
+  const double u1 = ((sm64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((sm64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+extern "C" int fpsi_mlp_num_params(void) { return mlp_layout().n; }
+
+extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
+  if (!c) return FPSI_ERR_ARG;
+  const MlpLayout L = mlp_layout();
+  const size_t nc = c->nc;
+  MlpBufs &m = c->mb;
+  if (!c->mlp_alloc) {
+    int rc = 0;
+    const int np = mlp_parts((int)nc);
+    const int *d_off = nullptr, *d_idx = nullptr;
+    const float *d_geo = nullptr;
+    std::vector<unsigned char> mask(L.n, 1);
+    for (int k = 5; k < 16; ++k) {
+      for (int h = 0; h < 64; ++h) mask[L.w3 + k * 64 + h] = 0;
+      mask[L.b3 + k] = 0;
+    }
+    const unsigned char *d_mask = nullptr;
+    std::vector<float> zero64(64, 0.f);
+    const float *d_zero = nullptr;
+    if ((rc = upload(c, &d_off, c->mlp_nb_off)) || (rc = upload(c, &d_idx, c->mlp_nb_idx)) ||
+        (rc = upload(c, &d_geo, c->mlp_geo)) || (rc = upload(c, &d_mask, mask)) || (rc = upload(c, &d_zero, zero64)) ||
+        (rc = dalloc(c, &m.raw, 16 * nc)) || (rc = dalloc(c, &m.norm, 16)) || (rc = dalloc(c, &m.h0, 64 * nc)) ||
+        (rc = dalloc(c, &m.a1, 64 * nc)) || (rc = dalloc(c, &m.h1, 64 * nc)) || (rc = dalloc(c, &m.a2, 64 * nc)) ||
+        (rc = dalloc(c, &m.h2, 64 * nc)) || (rc = dalloc(c, &m.out, 16 * nc)) || (rc = dalloc(c, &m.dout, 32 * nc)) ||
+        (rc = dalloc(c, &m.g0, 64 * nc)) || (rc = dalloc(c, &m.g1, 64 * nc)) || (rc = dalloc(c, &m.g2, 64 * nc)) ||
+        (rc = dalloc(c, &m.da, 64 * nc)) || (rc = dalloc(c, &m.dcat, 128 * nc)) || (rc = dalloc(c, &m.w, (size_t)L.n)) ||
+        (rc = dalloc(c, &m.grad, (size_t)L.n)) || (rc = dalloc(c, &m.am, (size_t)L.n)) ||
+        (rc = dalloc(c, &m.av, (size_t)L.n)) || (rc = dalloc(c, &m.wb, (size_t)L.n)) ||
+        (rc = dalloc(c, &m.wt1, 64 * 128)) || (rc = dalloc(c, &m.wt2, 64 * 128)) || (rc = dalloc(c, &m.wt3, 64 * 32)) ||
+        (rc = dalloc(c, &m.wpart, (size_t)np * 64 * 128)) || (rc = dalloc(c, &m.bpart, (size_t)np * 64)) ||
+        (rc = dalloc(c, &m.lpart, (size_t)std::max(norm_parts(c->g), 1) * 16 + 16)) || (rc = dalloc(c, &m.loss, 16)) ||
+        (rc = dalloc(c, &c->mlp_xcur, 32 * nc)))
+      return rc;
+    for (int s = 0; s < 4; ++s)
+      if ((rc = dalloc(c, &c->mlp_xs[s], 32 * nc)) || (rc = dalloc(c, &c->mlp_ys[s], 8 * nc))) return rc;
+    m.nb_off = d_off;
+    m.nb_idx = d_idx;
+    m.geo = d_geo;
+    m.mask = d_mask;
+    m.zero64 = d_zero;
+    c->mlp_alloc = true;
+  }
+  // He-normal weights, zero biases; the output layer starts small so the
+  // first guess is close to the Jacobi start
+  std::vector<float> w(L.n, 0.f);
+  unsigned long long st = seed ? seed : 0x5eedull;
+  auto fill = [&](int off, int rows, int cols, double scale) {
+    for (int i = 0; i < rows * cols; ++i) w[off + i] = (float)(scale * gauss(st));
+  };
+  fill(L.w0, 64, 32, std::sqrt(2.0 / 19.0));
+  fill(L.w1, 64, 128, std::sqrt(2.0 / 128.0));
+  fill(L.w2, 64, 128, std::sqrt(2.0 / 128.0));
+  fill(L.w3, 5, 64, 0.1 * std::sqrt(1.0 / 64.0));
+  HIPCHK(c, hipMemcpyAsync(m.w, w.data(), L.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(m.am, 0, L.n * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(m.av, 0, L.n * sizeof(float), c->stream));
+  mlp_refresh_bf16(m, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mlp_norm = false;
+  c->mlp_nsamp = 0;
+  c->mlp_next = 0;
+  c->mlp_step = 0;
+  return 0;
+}
+
+extern "C" int fpsi_mlp_get_params(fpsi_ctx *c, float *p) {
+  if (!c || !p) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "fpsi_mlp_init not called");
+  HIPCHK(c, hipMemcpy(p, c->mb.w, mlp_layout().n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int fpsi_mlp_set_params(fpsi_ctx *c, const float *p) {
+  if (!c || !p) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "fpsi_mlp_init not called");
+  HIPCHK(c, hipMemcpy(c->mb.w, p, mlp_layout().n * sizeof(float), hipMemcpyHostToDevice));
+  mlp_refresh_bf16(c->mb, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fpsi_mlp_harvest(fpsi_ctx *c, int target_sweeps, int *slot) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "fpsi_mlp_init not called");
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  const int nc = c->nc;
+  MlpBufs m = c->mb;
+  mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
+  if (!c->mlp_norm) {
+    float sums[16];
+    mlp_feature_stats(m, nc, m.loss, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(sums, m.loss, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float nrm[16];
+    for (int k = 0; k < 10; ++k) nrm[k] = std::sqrt(sums[k] / (float)nc) + 1e-30f;
+    for (int k = 10; k < 15; ++k) nrm[k] = sums[k] / (float)nc;
+    nrm[15] = 0.f;
+    HIPCHK(c, hipMemcpyAsync(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
+    c->mlp_norm = true;
+  }
+  const int sl = c->mlp_next;
+  mlp_normalise(m, nc, c->mlp_xs[sl], c->stream);
+  if (int rc = enqueue_sweeps(c, target_sweeps)) return rc;
+  mlp_target(m, nc, c->s.X, c->mlp_ys[sl], c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mlp_next = (sl + 1) % 4;
+  c->mlp_nsamp = std::min(c->mlp_nsamp + 1, 4);
+  c->assembled = false;
+  if (slot) *slot = sl;
+  return 0;
+}
+
+extern "C" int fpsi_mlp_train(fpsi_ctx *c, int epochs, float lr, double *loss) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc || c->mlp_nsamp == 0) return fail(c, FPSI_ERR_STATE, "no harvested samples");
+  const int nc = c->nc;
+  double last = 0;
+  for (int e = 0; e < epochs; ++e) {
+    last = 0;
+    for (int s = 0; s < c->mlp_nsamp; ++s) {
+      MlpBufs m = c->mb;
+      m.x = c->mlp_xs[s];
+      m.y = c->mlp_ys[s];
+      mlp_forward(m, nc, c->stream);
+      mlp_backward(m, nc, c->stream);
+      mlp_adam(m, lr, ++c->mlp_step, c->stream);
+      if (e == epochs - 1) {
+        float l = 0;
+        HIPCHK(c, hipMemcpyAsync(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        last += l / (5.0 * nc);
+      }
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = last / c->mlp_nsamp;
+  return 0;
+}
+
+extern "C" int fpsi_mlp_eval(fpsi_ctx *c, int slot, float *out, float *grad, double *loss) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc || slot < 0 || slot >= c->mlp_nsamp) return fail(c, FPSI_ERR_ARG, "no such sample");
+  const int nc = c->nc;
+  MlpBufs m = c->mb;
+  m.x = c->mlp_xs[slot];
+  m.y = c->mlp_ys[slot];
+  mlp_forward(m, nc, c->stream);
+  mlp_backward(m, nc, c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (out) {
+    std::vector<float> h(16 * (size_t)nc);
+    HIPCHK(c, hipMemcpyAsync(h.data(), m.out, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int ni = 0; ni < nc; ++ni) std::memcpy(out + 16 * (size_t)c->cperm[ni], &h[16 * (size_t)ni], 16 * sizeof(float));
+  }
+  if (grad) HIPCHK(c, hipMemcpy(grad, m.grad, mlp_layout().n * sizeof(float), hipMemcpyDeviceToHost));
+  float l = 0;
+  HIPCHK(c, hipMemcpy(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) *loss = l / (5.0 * nc);
+  return 0;
+}
+
+extern "C" int fpsi_mlp_get_sample(fpsi_ctx *c, int slot, float *x, float *y, int *nb_off, int *nb_idx) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->mlp_alloc || slot < 0 || slot >= c->mlp_nsamp) return fail(c, FPSI_ERR_ARG, "no such sample");
+  const int nc = c->nc;
+  if (x) {
+    std::vector<float> h(32 * (size_t)nc);
+    HIPCHK(c, hipMemcpy(h.data(), c->mlp_xs[slot], h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni) std::memcpy(x + 32 * (size_t)c->cperm[ni], &h[32 * (size_t)ni], 32 * sizeof(float));
+  }
+  if (y) {
+    std::vector<float> h(8 * (size_t)nc);
+    HIPCHK(c, hipMemcpy(h.data(), c->mlp_ys[slot], h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni) std::memcpy(y + 8 * (size_t)c->cperm[ni], &h[8 * (size_t)ni], 8 * sizeof(float));
+  }
+  if (nb_off) {
+    nb_off[0] = 0;
+    for (int i = 0; i < nc; ++i) {
+      const int ni = c->icperm[i];
+      nb_off[i + 1] = nb_off[i] + (c->mlp_nb_off[ni + 1] - c->mlp_nb_off[ni]);
+    }
+    if (nb_idx)
+      for (int i = 0; i < nc; ++i) {
+        const int ni = c->icperm[i];
+        int o = nb_off[i];
+        for (int j = c->mlp_nb_off[ni]; j < c->mlp_nb_off[ni + 1]; ++j) nb_idx[o++] = c->cperm[c->mlp_nb_idx[j]];
+      }
   }
   return 0;
 }

@@ -1061,4 +1061,132 @@ void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) 
   hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.Pamb, s.X, s.q);
 }
 
+
+// ---------------------------------------------------------------------------
+// linear residual of the assembled system, r = B - A X, A = D + L + U, in the
+// norm sum_c sum_k (r_ck / sB_k)^2 with sB_k the RMS of B_k over the cells.
+// D X is formed from the packed LU factors: P D = L U.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ void k_b_stats(int nc, const F *__restrict__ B, double *__restrict__ part) {
+  __shared__ double sh[TPB][5];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double b = c < nc ? (double)B[(size_t)c * 5 + k] : 0.0;
+    sh[threadIdx.x][k] = b * b;
+  }
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + w][k];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[(size_t)blockIdx.x * 5 + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+// out[i] = sum_p part[p][i] in block order (one thread per i)
+__global__ void k_sum_parts_d(int nparts, int len, const double *__restrict__ part, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * len + i];
+  out[i] = s;
+}
+
+template <typename F>
+__global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__restrict__ off,
+                         const F *__restrict__ lu, const int *__restrict__ piv, const F *__restrict__ B,
+                         const double *__restrict__ X, double *__restrict__ part) {
+  __shared__ double sh[TPB];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (c < g.nc) {
+    double x[5], y[5], z[5], r[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) x[k] = X[(size_t)c * 5 + k];
+    F a[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = i; j < 5; ++j) s += a[i + 5 * j] * x[j];
+      y[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double s = y[i];
+#pragma unroll
+      for (int j = 0; j < i; ++j) s += a[i + 5 * j] * y[j];
+      z[i] = s;
+    }
+    const int pk = piv[c];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) r[k] = (double)B[(size_t)c * 5 + k];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const int p = (pk >> (3 * i)) & 7;
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        if (k == p) r[k] -= z[i];
+    }
+    const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+    for (int j = m0; j < m1; ++j) {
+      const F *M = off + (size_t)j * 25;
+      const double *xn = X + (size_t)g.m_nbr[j] * 5;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) {
+        const double xv = xn[jj];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) r[i] -= M[i + 5 * jj] * xv;
+      }
+    }
+    const double ncd = (double)g.nc;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double sB = sqrt(bsum[k] / ncd) + 1e-300;
+      const double e = r[k] / sB;
+      acc += e * e;
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// linear residual norm squared into out[0]; needs the ghost copies current
+void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
+                   hipStream_t st) {
+  const int nb = nblk(g.nc);
+  launch_ghost(g, s, st);
+  if (fp32) {
+    hipLaunchKernelGGL(k_b_stats<float>, dim3(nb), dim3(TPB), 0, st, g.nc, (const float *)s.B, parts);
+    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+    hipLaunchKernelGGL(k_linres<float>, dim3(nb), dim3(TPB), 0, st, g, bsum, (const float *)s.off,
+                       (const float *)s.lu, s.piv, (const float *)s.B, s.X, parts);
+  } else {
+    hipLaunchKernelGGL(k_b_stats<double>, dim3(nb), dim3(TPB), 0, st, g.nc, (const double *)s.B, parts);
+    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+    hipLaunchKernelGGL(k_linres<double>, dim3(nb), dim3(TPB), 0, st, g, bsum, (const double *)s.off,
+                       (const double *)s.lu, s.piv, (const double *)s.B, s.X, parts);
+  }
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 1, parts, out);
+}
+
+void launch_copy_d(long n, const double *a, double *b, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, a, b);
+}
+
 }  // namespace fpsi

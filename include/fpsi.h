@@ -178,6 +178,53 @@ typedef struct {
 int fpsi_profile_enable(fpsi_ctx *ctx, int on);   /* clears the accumulators */
 int fpsi_profile_read(fpsi_ctx *ctx, fpsi_profile *out);
 
+/* ---- linear-solve tolerance and the learned initial guess ------------- */
+
+/* Linear residual ||B - A X|| of the assembled system for the current X
+ * (dfluid), each equation scaled by the RMS of its right-hand side.  The
+ * reference's SGS runs a fixed sweep count and has no such test
+ * (src/linearSolverSGS.loci:315-328); it is used to run the solve to a
+ * tolerance and to judge a learned guess. */
+int fpsi_linear_residual(fpsi_ctx *ctx, double *norm);
+
+typedef struct {
+  double r0;       /* residual of the plain Jacobi start X = D^-1 B */
+  double rguess;   /* residual of the learned guess (0 if not tried) */
+  double rfinal;   /* residual after the sweeps */
+  int sweeps;      /* symmetric sweeps run */
+  int used_guess;  /* 1 if the guess beat the Jacobi start and was kept */
+} fpsi_lin_stats;
+
+/* SGS from the Jacobi start, or from the learned guess when use_guess is set
+ * and the guess has a smaller residual than the Jacobi start (otherwise the
+ * plain start is kept), sweeping until ||r|| <= rtol * r0 or max_sweeps. */
+int fpsi_linear_solve_tol(fpsi_ctx *ctx, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st);
+
+/* Steady Newton steps whose linear solves run to a tolerance (see above);
+ * hist[nsteps] and lin[nsteps] are nullable. */
+int fpsi_newton_adaptive(fpsi_ctx *ctx, int nsteps, double rtol, int max_sweeps, int use_guess,
+                         fpsi_norms *hist, fpsi_lin_stats *lin);
+
+/* Learned initial guess: a two-hop message-passing MLP of width 64 (bf16
+ * GEMM operands on MFMA, fp32 accumulation) mapping per-cell features
+ * (Jacobi start, right-hand side, LU pivots, face geometry) to a correction
+ * of the Jacobi start.  No counterpart in the reference. */
+int fpsi_mlp_num_params(void);
+int fpsi_mlp_init(fpsi_ctx *ctx, unsigned long long seed);
+int fpsi_mlp_get_params(fpsi_ctx *ctx, float *p);
+int fpsi_mlp_set_params(fpsi_ctx *ctx, const float *p);
+/* On an assembled system: record features and, as the target, the SGS
+ * solution after target_sweeps sweeps (left in dfluid); *slot receives the
+ * sample slot (the oldest of 4 is overwritten). */
+int fpsi_mlp_harvest(fpsi_ctx *ctx, int target_sweeps, int *slot);
+/* Adam over all recorded samples, epochs passes, mean loss of the last pass. */
+int fpsi_mlp_train(fpsi_ctx *ctx, int epochs, float lr, double *loss);
+/* Forward (and backward) on one sample: out[nc][16], grad[nparams], loss. */
+int fpsi_mlp_eval(fpsi_ctx *ctx, int slot, float *out, float *grad, double *loss);
+/* One sample in caller cell order: x[nc][32], y[nc][8]; the neighbour lists
+ * nb_off[nc+1], nb_idx[nb_off[nc]] the network aggregates over. */
+int fpsi_mlp_get_sample(fpsi_ctx *ctx, int slot, float *x, float *y, int *nb_off, int *nb_idx);
+
 /* Wait for the context's stream. */
 int fpsi_synchronize(fpsi_ctx *ctx);
 

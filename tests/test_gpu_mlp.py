@@ -1,0 +1,84 @@
+"""The learned SGS initial guess on the GPU: forward and gradient against the
+numpy checker (oracle/mlp.py), training reduces the loss, the plain method
+judges and bounds the guess (fallback), and what the trained network gains:
+symmetric sweeps needed to reach the same linear tolerance with and without
+the guess, on Newton steps it was not trained on."""
+import numpy as np
+import pytest
+
+from flowpsi_amd import workloads
+from flowpsi_amd.capi import Solver
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(n=8):
+    g, q, prm = workloads.box(n)
+    s = Solver(g, prm)
+    s.set_state(q)
+    return g, s
+
+
+def test_forward_and_gradient_match_numpy():
+    g, s = _solver(6)
+    s.mlp_init(seed=3)
+    s.assemble()
+    slot = s.mlp_harvest(20)
+    out, grad, loss = s.mlp_eval(slot)
+    x, y, off, idx = s.mlp_sample(slot)
+    p = s.mlp_params()
+    l_ref, g_ref, o_ref = mlp.loss_and_grad(p, x, y, off, idx, round_bf16=True)
+    e_out = np.abs(out[:, :5] - o_ref[:, :5]).max() / (np.abs(o_ref[:, :5]).max() + 1e-30)
+    e_grad = np.abs(grad - g_ref).max() / (np.abs(g_ref).max() + 1e-30)
+    print(f"\nmlp out err {e_out:.2e} loss {loss:.6e} vs {l_ref:.6e} grad err {e_grad:.2e}")
+    assert e_out < 2e-2
+    assert abs(loss - l_ref) <= 2e-2 * l_ref
+    assert e_grad < 3e-2
+
+
+def test_training_reduces_loss_and_guess_saves_sweeps():
+    g, s = _solver(12)
+    s.mlp_init(seed=7)
+    # harvest from the first Newton steps (targets: 30 sweeps), advance the state
+    for _ in range(3):
+        s.assemble()
+        s.mlp_harvest(30)
+        s.update()
+    l0 = s.mlp_train(1, lr=0.0)
+    l1 = s.mlp_train(300, lr=2e-3)
+    print(f"\nloss before {l0:.4e} after {l1:.4e}")
+    assert l1 < 0.7 * l0
+    # held-out steps: same linear tolerance with and without the guess
+    plain, learned = [], []
+    q = s.get_state()
+    for use in (False, True):
+        s.set_state(q)
+        _, lin = s.newton_adaptive(3, 1e-3, 60, use_guess=use)
+        (learned if use else plain).append(lin)
+    sw_plain = sum(l["sweeps"] for l in plain[0])
+    sw_learn = sum(l["sweeps"] for l in learned[0])
+    print(f"sweeps to 1e-3: plain {sw_plain}, learned {sw_learn}; "
+          f"guess kept {[l['used_guess'] for l in learned[0]]}, r_guess/r0 "
+          f"{[round(l['rguess'] / l['r0'], 3) for l in learned[0]]}")
+    for l in plain[0] + learned[0]:
+        assert l["rfinal"] <= 1e-3 * l["r0"] or l["sweeps"] == 60
+    assert sw_learn <= sw_plain
+
+
+def test_bad_guess_falls_back_to_plain_start():
+    g, s = _solver(6)
+    s.mlp_init(seed=1)
+    s.assemble()
+    s.mlp_harvest(5)
+    p = s.mlp_params()
+    o, n = mlp.LAYOUT["b3"]
+    p[o:o + 5] = 1e3  # a guess far from the solution
+    s.mlp_set_params(p)
+    s.assemble()
+    st = s.linear_solve_tol(1e-3, 40, use_guess=True)
+    s.assemble()
+    st0 = s.linear_solve_tol(1e-3, 40, use_guess=False)
+    print(f"\nbad guess: {st}; plain: {st0}")
+    assert st["used_guess"] == 0 and st["rguess"] >= st["r0"]
+    assert st["sweeps"] == st0["sweeps"] and st["rfinal"] == st0["rfinal"]
