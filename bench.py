@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--sweep", default="per_level", choices=["per_level", "persistent"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the time-to-1e-8 comparison on the 32^3 box")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -109,6 +111,16 @@ def main():
                          f"oracle real_fj={fj}, single thread, {c1 - c0:.1f}s"}
         log(f"cpu baseline {cpu['value']:.3e} cells*steps/s")
 
+    conv = None
+    if not a.no_converge:
+        from flowpsi_amd import converge
+        gb, qb, pb = workloads.box(32)
+        log("time to 1e-8 drop on the 32^3 Euler box: fp64 fixed / fp32 fixed / fp32 tol / fp32+NN")
+        conv = converge.compare(gb, qb, pb)
+        conv["workload"] = "Euler box 32^3 (32768 cells), Roe, CFL 10, SGS x5 (fixed paths)"
+        log(f"converge: " + ", ".join(f"{k}={v.get('time_to_target_s')}" for k, v in conv.items()
+                                      if isinstance(v, dict) and "time_to_target_s" in v))
+
     out = {
         "metric": "cells*Newton-steps/s",
         "value": value,
@@ -132,6 +144,7 @@ def main():
                           "bytes_per_step": step_bytes},
         "residual_drop": float(np.sqrt(tot[-1] / tot[0])) if tot[0] > 0 else None,
         "cpu_baseline": cpu,
+        "time_to_1e-8": conv,
     }
     print(json.dumps(out), flush=True)
 

@@ -846,10 +846,32 @@ extern "C" int fpsi_linear_residual(fpsi_ctx *c, double *norm) {
   return linres(c, norm);
 }
 
+// Feature scales of the current system: RMS of the Jacobi start and of the
+// right-hand side per equation, mean log pivot.  The linear solve is
+// equivariant under scaling of B, so normalising per system keeps the
+// network's inputs and targets in range as the Newton residual falls.
+static int mlp_set_norm(fpsi_ctx *c) {
+  MlpBufs m = c->mb;
+  const int nc = c->nc;
+  float sums[16];
+  mlp_feature_stats(m, nc, m.loss, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(sums, m.loss, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float nrm[16];
+  for (int k = 0; k < 10; ++k) nrm[k] = std::sqrt(sums[k] / (float)nc) + 1e-30f;
+  for (int k = 10; k < 15; ++k) nrm[k] = sums[k] / (float)nc;
+  nrm[15] = 0.f;
+  HIPCHK(c, hipMemcpyAsync(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
+  c->mlp_norm = true;
+  return 0;
+}
+
 static int mlp_guess(fpsi_ctx *c) {
   MlpBufs m = c->mb;
   const int nc = c->nc;
   mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
+  if (int rc = mlp_set_norm(c)) return rc;
   mlp_normalise(m, nc, c->mlp_xcur, c->stream);
   m.x = c->mlp_xcur;
   mlp_forward(m, nc, c->stream);
@@ -864,7 +886,7 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
   if (int rc = linres(c, &s.r0)) return rc;
   rcur = s.r0;
   if (use_guess) {
-    if (!c->mlp_alloc || !c->mlp_norm) return fail(c, FPSI_ERR_STATE, "learned guess requested before fpsi_mlp_init/harvest");
+    if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "learned guess requested before fpsi_mlp_init");
     const long n = 5 * ((long)c->nc + c->nper + 1);
     launch_copy_d(n, c->s.X, c->Xj, c->stream);
     if (int rc = mlp_guess(c)) return rc;
@@ -1020,19 +1042,7 @@ extern "C" int fpsi_mlp_harvest(fpsi_ctx *c, int target_sweeps, int *slot) {
   const int nc = c->nc;
   MlpBufs m = c->mb;
   mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
-  if (!c->mlp_norm) {
-    float sums[16];
-    mlp_feature_stats(m, nc, m.loss, c->stream);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(sums, m.loss, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    float nrm[16];
-    for (int k = 0; k < 10; ++k) nrm[k] = std::sqrt(sums[k] / (float)nc) + 1e-30f;
-    for (int k = 10; k < 15; ++k) nrm[k] = sums[k] / (float)nc;
-    nrm[15] = 0.f;
-    HIPCHK(c, hipMemcpyAsync(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
-    c->mlp_norm = true;
-  }
+  if (int rc = mlp_set_norm(c)) return rc;
   const int sl = c->mlp_next;
   mlp_normalise(m, nc, c->mlp_xs[sl], c->stream);
   if (int rc = enqueue_sweeps(c, target_sweeps)) return rc;
