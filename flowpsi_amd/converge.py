@@ -22,6 +22,16 @@ import numpy as np
 from .capi import Solver
 
 
+def state_err(q, ref):
+    """max relative difference with T, |u| and p scales (the velocity
+    components share the speed scale: they vanish at a rest state)"""
+    sT = np.abs(ref[0]).max()
+    su = np.sqrt((ref[1:4] ** 2).sum(0)).max() + 1e-300
+    sp = np.abs(ref[4]).max()
+    scale = np.array([sT, su, su, su, sp])[:, None]
+    return float((np.abs(q - ref) / scale).max())
+
+
 def _drop(hist):
     tot = np.array([h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist])
     return np.sqrt(tot / tot[0])
@@ -101,9 +111,8 @@ def compare(g, q, prm, rtol=0.05, max_sweeps=20, target=1e-8, max_steps=400):
     out["fp32_nn"] = {k: v for k, v in rn.items() if k != "state"}
     out["fp32_nn"]["training"] = tr
     ref = r64["state"]
-    scale = np.abs(ref).max(axis=1, keepdims=True)
     for k, r in (("fp32_fixed", r32), ("fp32_tol", rt), ("fp32_nn", rn)):
-        out[k]["final_state_err_vs_fp64"] = float((np.abs(r["state"] - ref) / scale).max())
+        out[k]["final_state_err_vs_fp64"] = state_err(r["state"], ref)
     t64 = out["fp64_fixed"]["time_to_target_s"]
     tnn = out["fp32_nn"]["time_to_target_s"]
     out["speedup_fp32_nn_vs_fp64"] = (t64 / tnn) if (t64 and tnn) else None
