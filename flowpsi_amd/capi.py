@@ -23,7 +23,7 @@ _ip = C.POINTER(C.c_int)
 FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
-SOLVER = {"sgs": 0}
+SOLVER = {"sgs": 0, "lsgs": 1}
 SWEEP = {"per_level": 0, "persistent": 1}
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
@@ -34,7 +34,7 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read", "fpsi_linear_residual",
            "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
-           "fpsi_mlp_eval", "fpsi_mlp_get_sample"]
+           "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -51,7 +51,8 @@ class Params(C.Structure):
                 ("T0", C.c_double), ("u0", C.c_double * 3), ("cflmax", C.c_double),
                 ("dtmax", C.c_double), ("urelax", C.c_double), ("linear_solver", C.c_int),
                 ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double),
-                ("sweep_mode", C.c_int)]
+                ("sweep_mode", C.c_int), ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double),
+                ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -65,7 +66,7 @@ class Norms(C.Structure):
 
 class Info(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("max_level_width", C.c_int), ("nnz_blocks", C.c_int),
-                ("device_bytes", C.c_double)]
+                ("device_bytes", C.c_double), ("lsgs_nlevels", C.c_int), ("lsgs_contiguous", C.c_int)]
 
 
 class LinStats(C.Structure):
@@ -135,6 +136,10 @@ def make_params(prm: dict) -> Params:
     s.precision = PRECISION[prm.get("precision", "fp64")]
     s.minPg = prm.get("minPg", -s.Pambient)
     s.sweep_mode = SWEEP[prm.get("sweep", "per_level")]
+    s.lsgs_maxiter = prm.get("lsgs_maxiter", 10)
+    s.lsgs_reltol = prm.get("lsgs_reltol", 0.1)
+    s.lsgs_abstol = prm.get("lsgs_abstol", 1e-10)
+    s.lsgs_relax = prm.get("lsgs_relax", 0.25)
     return s
 
 
@@ -191,7 +196,8 @@ class Solver:
         i = Info()
         self._chk(self.L.fpsi_get_info(self.h, C.byref(i)), "fpsi_get_info")
         return dict(nlevels=i.nlevels, max_level_width=i.max_level_width, nnz_blocks=i.nnz_blocks,
-                    device_bytes=i.device_bytes)
+                    device_bytes=i.device_bytes, lsgs_nlevels=i.lsgs_nlevels,
+                    lsgs_contiguous=bool(i.lsgs_contiguous))
 
     # -- rules
     def set_params(self, prm: dict):
@@ -245,6 +251,11 @@ class Solver:
         dq = np.zeros((self.grid.nc, 5))
         self._chk(self.L.fpsi_linear_solve(self.h, int(nsweeps), _ptr(dq)), "fpsi_linear_solve")
         return dq
+
+    def lsgs_stats(self) -> dict:
+        it = C.c_int(); w = C.c_double(); r = C.c_double()
+        self._chk(self.L.fpsi_lsgs_stats(self.h, C.byref(it), C.byref(w), C.byref(r)), "fpsi_lsgs_stats")
+        return dict(iters=it.value, w=w.value, resid=r.value)
 
     def update(self):
         self._chk(self.L.fpsi_update(self.h), "fpsi_update")

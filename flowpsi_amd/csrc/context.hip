@@ -56,7 +56,16 @@ struct fpsi_ctx {
   float *mlp_xs[4] = {nullptr, nullptr, nullptr, nullptr}, *mlp_ys[4] = {nullptr, nullptr, nullptr, nullptr};
   float *mlp_xcur = nullptr;
   int mlp_nsamp = 0, mlp_next = 0, mlp_step = 0;
+  // line SGS: levels of the breadth-first line order (new cell numbering);
+  // lsgs_cells is null when every level is a contiguous cell range
+  std::vector<int> lsgs_lev_off;
+  const int *lsgs_cells = nullptr;
+  bool lsgs_alloc = false;
+  double *lsgs_host = nullptr;  // pinned mirror of DevState::lst
+  int lsgs_iters = 0;
+  double lsgs_w = 0, lsgs_resid = 0;
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
+  bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
 };
 
 static int fail(fpsi_ctx *c, int code, const std::string &m) {
@@ -127,6 +136,7 @@ extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
   for (auto &p : ctx->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void *p : ctx->allocs) hipFree(p);
+  if (ctx->lsgs_host) hipHostFree(ctx->lsgs_host);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -388,6 +398,83 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   for (int ni = 0; ni < nc; ++ni)
     for (int j = m_off[ni]; j < m_off[ni + 1] && j - m_off[ni] < 6; ++j) nbr_pad[6 * (size_t)ni + (j - m_off[ni])] = m_nbr[j];
 
+  // ---- line SGS.  Rows read lower slots first: slot j of the rotated row is
+  // slot (j + nup) mod len of the stored row.
+  std::vector<int> m_nup(nc), lnbr_pad(6 * (size_t)nc, zslot);
+  for (int ni = 0; ni < nc; ++ni) {
+    const int i = c->cperm[ni];
+    const int nup = up_off[i + 1] - up_off[i], len = m_off[ni + 1] - m_off[ni];
+    m_nup[ni] = nup;
+    for (int j = 0; j < len && j < 6; ++j) {
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      lnbr_pad[6 * (size_t)ni + j] = m_nbr[m_off[ni] + jr];
+    }
+  }
+  // Breadth-first order of the one-cell lines (src/linearSolverLSGS.loci:902-937):
+  // from the lowest unvisited cell, layer by layer over interior faces, each
+  // layer ascending.  Its sweep dependencies run from earlier to later cells;
+  // level = longest path, cells of a level sorted by new index.
+  std::vector<int> lsgs_cells;
+  {
+    std::vector<char> vis(nc, 0);
+    std::vector<int> order, cur, nxt;
+    order.reserve(nc);
+    int sp = 0;
+    while ((int)order.size() < nc) {
+      if (cur.empty()) {
+        while (sp < nc && vis[sp]) ++sp;
+        if (sp == nc) break;
+        cur.push_back(sp);
+        vis[sp] = 1;
+      }
+      nxt.clear();
+      for (int i : cur) {
+        for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) {
+          const int o = gd->cl[lo[j]];
+          if (!vis[o]) { vis[o] = 1; nxt.push_back(o); }
+        }
+        for (int j = up_off[i]; j < up_off[i + 1]; ++j) {
+          const int f = up[j];
+          if (kind[f] != 0) continue;
+          const int o = gd->cr[f];
+          if (!vis[o]) { vis[o] = 1; nxt.push_back(o); }
+        }
+        order.push_back(i);
+      }
+      std::sort(nxt.begin(), nxt.end());
+      cur.swap(nxt);
+    }
+    std::vector<int> pos(nc), lv(nc, 0);
+    for (int k = 0; k < nc; ++k) pos[order[k]] = k;
+    int nl = 0;
+    for (int k = 0; k < nc; ++k) {
+      const int i = order[k];
+      int L = 0;
+      for (int j = lo_off[i]; j < lo_off[i + 1]; ++j) {
+        const int o = gd->cl[lo[j]];
+        if (pos[o] < k) L = std::max(L, lv[o] + 1);
+      }
+      for (int j = up_off[i]; j < up_off[i + 1]; ++j) {
+        const int f = up[j];
+        if (kind[f] != 0) continue;
+        const int o = gd->cr[f];
+        if (pos[o] < k) L = std::max(L, lv[o] + 1);
+      }
+      lv[i] = L;
+      nl = std::max(nl, L + 1);
+    }
+    c->lsgs_lev_off.assign(nl + 1, 0);
+    for (int i = 0; i < nc; ++i) c->lsgs_lev_off[lv[i] + 1]++;
+    for (int l = 0; l < nl; ++l) c->lsgs_lev_off[l + 1] += c->lsgs_lev_off[l];
+    lsgs_cells.assign(nc, 0);
+    std::vector<int> fill(c->lsgs_lev_off.begin(), c->lsgs_lev_off.end() - 1);
+    for (int ni = 0; ni < nc; ++ni) lsgs_cells[fill[lv[c->cperm[ni]]]++] = ni;
+    bool ident = true;
+    for (int k = 0; k < nc && ident; ++k) ident = lsgs_cells[k] == k;
+    if (ident) lsgs_cells.clear();
+  }
+
   // ---- every index the kernels dereference must be in bounds
   {
     const int nnz = c->nnz;
@@ -398,6 +485,10 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     for (int v : slot_cl) ok = ok && v >= 0 && v < nnz;
     for (int v : slot_cr) ok = ok && v >= 0 && v < nnz;
     for (int v : nbr_pad) ok = ok && v >= 0 && v <= nc + nper;
+    for (int v : lnbr_pad) ok = ok && v >= 0 && v <= nc + nper;
+    for (int ni = 0; ni < nc; ++ni) ok = ok && m_nup[ni] >= 0 && m_nup[ni] <= m_off[ni + 1] - m_off[ni];
+    for (int v : lsgs_cells) ok = ok && v >= 0 && v < nc;
+    ok = ok && c->lsgs_lev_off.back() == nc;
     for (int i = 0; i < nf; ++i) {
       ok = ok && fcl[i] >= 0 && fcl[i] < nc;
       if (i < nfi + nper) ok = ok && frc[i] >= 0 && frc[i] < nc;
@@ -442,7 +533,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = upload(c, &g.m_off, m_off)) || (rc = upload(c, &g.m_nbr, m_nbr)) ||
       (rc = upload(c, &g.m_part, m_part)) || (rc = upload(c, &g.slot_cl, slot_cl)) ||
       (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id)) ||
-      (rc = upload(c, &g.nbr_pad, nbr_pad)) ||
+      (rc = upload(c, &g.nbr_pad, nbr_pad)) || (rc = upload(c, &g.m_nup, m_nup)) ||
+      (rc = upload(c, &g.lnbr_pad, lnbr_pad)) ||
       (rc = upload(c, &c->chunk_lo, chunk_lo)) || (rc = upload(c, &c->chunk_lev, chunk_lev)) ||
       (rc = upload(c, &c->lev_size, lev_size))) {
     *out = c;
@@ -482,7 +574,23 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.lev_cnt, 0, (size_t)nlev * 32 * sizeof(int), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!lsgs_cells.empty()) {
+    if ((rc = upload(c, &c->lsgs_cells, lsgs_cells))) { *out = c; return rc; }
+  }
   *out = c;
+  return 0;
+}
+
+// line-SGS buffers, allocated the first time that solver is selected
+static int lsgs_alloc(fpsi_ctx *c) {
+  if (c->lsgs_alloc) return 0;
+  DevState &s = c->s;
+  int rc = 0;
+  if ((rc = dalloc(c, &s.lu64, 25 * (size_t)c->nc)) || (rc = dalloc(c, &s.xt, 5 * (size_t)c->nc)) ||
+      (rc = dalloc(c, &s.lparts, (size_t)lsgs_parts(c->g))) || (rc = dalloc(c, &s.lst, 8)))
+    return rc;
+  HIPCHK(c, hipHostMalloc((void **)&c->lsgs_host, 8 * sizeof(double), hipHostMallocDefault));
+  c->lsgs_alloc = true;
   return 0;
 }
 
@@ -492,6 +600,8 @@ extern "C" int fpsi_get_info(fpsi_ctx *c, fpsi_info *info) {
   info->max_level_width = c->max_width;
   info->nnz_blocks = c->nnz;
   info->device_bytes = (double)c->bytes;
+  info->lsgs_nlevels = (int)c->lsgs_lev_off.size() - 1;
+  info->lsgs_contiguous = c->lsgs_cells == nullptr;
   return 0;
 }
 
@@ -500,7 +610,13 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->flux != FPSI_FLUX_HLLC && p->flux != FPSI_FLUX_ROE) return fail(c, FPSI_ERR_ARG, "unknown flux");
   if (p->limiter < 0 || p->limiter > 3) return fail(c, FPSI_ERR_ARG, "unknown limiter");
   if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
-  if (p->linear_solver != FPSI_SOLVER_SGS) return fail(c, FPSI_ERR_ARG, "unknown linear solver");
+  if (p->linear_solver != FPSI_SOLVER_SGS && p->linear_solver != FPSI_SOLVER_LSGS)
+    return fail(c, FPSI_ERR_ARG, "unknown linear solver");
+  if (p->linear_solver == FPSI_SOLVER_LSGS) {
+    if (p->lsgs_maxiter < 0) return fail(c, FPSI_ERR_ARG, "LSGSMaxIter < 0");
+    if (!(p->lsgs_relax > 0.0)) return fail(c, FPSI_ERR_ARG, "LSGSRelaxation must be > 0");
+    if (int rc = lsgs_alloc(c)) return rc;
+  }
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
   if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL)
     return fail(c, FPSI_ERR_ARG, "unknown sweep mode");
@@ -615,7 +731,7 @@ static void enqueue_assemble(fpsi_ctx *c) {
   launch_face_jac(c->g, c->kp, c->fp32, c->s, c->stream);
   prof_end(c, FPSI_K_FACE_JAC, e);
   e = prof_begin(c);
-  launch_cell_assemble(c->g, c->kp, c->fp32, c->s, c->stream);
+  launch_cell_assemble(c->g, c->kp, c->fp32, c->lsgs() ? 1 : 0, c->s, c->stream);
   prof_end(c, FPSI_K_ASSEMBLE, e);
 }
 
@@ -709,11 +825,64 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
   return 0;
 }
 
+// LSGSBlockedSSolve after its line-Jacobi start (made by the assembly):
+// relaxed symmetric sweeps in the breadth-first line order, periodic ghosts
+// refreshed before each (src/linearSolverLSGS.loci:1738-1948).  The stopping
+// test needs the update norm on the host, so each sweep ends with a 40-byte
+// read-back.
+static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
+  const int nl = (int)c->lsgs_lev_off.size() - 1;
+  const fpsi_params &p = c->prm;
+  launch_lsgs_init(c->s, p.lsgs_relax, c->stream);
+  c->lsgs_iters = 0;
+  c->lsgs_w = p.lsgs_relax;
+  c->lsgs_resid = 0;
+  for (int it = 0; it < maxiter; ++it) {
+    launch_ghost(c->g, c->s, c->stream);
+    hipEvent_t e = prof_begin(c);
+    for (int l = 0; l < nl; ++l)
+      launch_lsgs_level(c->g, c->fp32, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                        c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
+    e = prof_begin(c);
+    for (int l = nl - 1; l >= 0; --l)
+      launch_lsgs_level(c->g, c->fp32, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                        c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
+    launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lsgs_w = c->lsgs_host[0];
+    c->lsgs_resid = c->lsgs_host[2];
+    c->lsgs_iters = (int)c->lsgs_host[3];
+    if (c->lsgs_host[4] != 0.0) break;
+  }
+  return 0;
+}
+
+extern "C" int fpsi_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *resid) {
+  if (!c) return FPSI_ERR_ARG;
+  if (iters) *iters = c->lsgs_iters;
+  if (relax) *relax = c->lsgs_w;
+  if (resid) *resid = c->lsgs_resid;
+  return 0;
+}
+
+static int enqueue_solve(fpsi_ctx *c, int nsweeps) {
+  return c->lsgs() ? enqueue_lsgs(c, nsweeps) : enqueue_sweeps(c, nsweeps);
+}
+
+static int need_sgs(fpsi_ctx *c) {
+  if (c->lsgs()) return fail(c, FPSI_ERR_STATE, "this call needs linear_solver = FPSI_SOLVER_SGS");
+  return 0;
+}
+
 extern "C" int fpsi_linear_solve(fpsi_ctx *c, int nsweeps, double *dq) {
   if (!c) return FPSI_ERR_ARG;
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
-  if (int rc = enqueue_sweeps(c, nsweeps)) return rc;
+  if (int rc = enqueue_solve(c, nsweeps)) return rc;
   HIPCHK(c, hipGetLastError());
   if (dq) {
     const int nc = c->nc;
@@ -748,7 +917,7 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (int rc = enqueue_sweeps(c, c->prm.sgs_iter)) {
+    if (int rc = enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
       hipStreamSynchronize(c->stream);
       return rc;
     }
@@ -843,6 +1012,7 @@ static int linres(fpsi_ctx *c, double *out) {
 extern "C" int fpsi_linear_residual(fpsi_ctx *c, double *norm) {
   if (!c || !norm) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
+  if (int rc = need_sgs(c)) return rc;
   return linres(c, norm);
 }
 
@@ -910,6 +1080,7 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
 
 extern "C" int fpsi_linear_solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st) {
   if (!c) return FPSI_ERR_ARG;
+  if (int rc = need_sgs(c)) return rc;
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   const int rc = solve_tol(c, rtol, max_sweeps, use_guess, st);
   c->assembled = false;
@@ -920,6 +1091,7 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
                                     fpsi_norms *hist, fpsi_lin_stats *lin) {
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
+  if (int rc = need_sgs(c)) return rc;
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     double o[12];
@@ -1037,6 +1209,7 @@ extern "C" int fpsi_mlp_set_params(fpsi_ctx *c, const float *p) {
 
 extern "C" int fpsi_mlp_harvest(fpsi_ctx *c, int target_sweeps, int *slot) {
   if (!c) return FPSI_ERR_ARG;
+  if (int rc = need_sgs(c)) return rc;
   if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "fpsi_mlp_init not called");
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   const int nc = c->nc;

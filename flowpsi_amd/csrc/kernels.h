@@ -10,7 +10,7 @@ void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t s
 int norm_parts(const DevGrid &g);
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st);
 void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
-void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
+void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st);
 void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st);
 int sgs_sweep_waves(int fp32, int ncu);
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
@@ -21,4 +21,10 @@ void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
                    hipStream_t st);
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st);
+// line SGS
+int lsgs_parts(const DevGrid &g);
+void launch_lsgs_init(DevState &s, double w0, hipStream_t st);
+void launch_lsgs_level(const DevGrid &g, int fp32, int backward, const int *cells, int lo, int hi, DevState &s,
+                       hipStream_t st);
+void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st);
 }  // namespace fpsi

@@ -674,13 +674,31 @@ __device__ __forceinline__ void lusolve5(const F a[25], int pk, const double b[5
 // + boundary terms), fluid_D (:176-224), B = rhs in real_fj (:45-54), LU
 // (src/linearSolverSGS.loci:43-50) and the Jacobi start X = D^-1 B (:57-69)
 // ---------------------------------------------------------------------------
-template <typename F>
+// LS: line-SGS diagonal.  fluid_D is kept unfactored in lu (identity
+// pivots) and copied to FP64 and LU-factored without pivoting into lu64
+// (src/linearSolverLSGS.loci:1633-1640, preprocessLine :1465-1470 for a line
+// of one cell); the line-Jacobi start uses those factors (:1700-1733).
+static constexpr int PIV_ID = 0 | (1 << 3) | (2 << 6) | (3 << 9) | (4 << 12);
+
+__device__ __forceinline__ void lu5np(double a[25]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      const double l = a[i + 5 * k] / a[k + 5 * k];
+      a[i + 5 * k] = l;
+#pragma unroll
+      for (int j = k + 1; j < 5; ++j) a[i + 5 * j] = a[i + 5 * j] - l * a[k + 5 * j];
+    }
+}
+
+template <typename F, bool LS>
 __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const double *__restrict__ src, const double *__restrict__ maxev,
                                 const F *__restrict__ off, const F *__restrict__ pblk,
                                 const F *__restrict__ bblk, double *__restrict__ dtau_out,
                                 F *__restrict__ lu, int *__restrict__ piv, F *__restrict__ Bout,
-                                double *__restrict__ X) {
+                                double *__restrict__ X, double *__restrict__ lu64) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -771,11 +789,6 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   acc(D[4 + 5 * 2], coef * rho * U.y);
   acc(D[3 + 5 * 3], coef * rho);
   acc(D[4 + 5 * 3], coef * rho * U.z);
-  const int pk = lu5<F>(D);
-  F *luc = lu + (size_t)c * 25;
-#pragma unroll
-  for (int i = 0; i < 25; ++i) luc[i] = D[i];
-  piv[c] = pk;
   double b[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -784,7 +797,27 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
     b[k] = (double)bk;
   }
   double x[5];
-  lusolve5<F>(D, pk, b, x);
+  F *luc = lu + (size_t)c * 25;
+  if constexpr (LS) {
+    double Dd[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+      luc[i] = D[i];
+      Dd[i] = (double)D[i];
+    }
+    piv[c] = PIV_ID;
+    lu5np(Dd);
+    double *l64 = lu64 + (size_t)c * 25;
+#pragma unroll
+    for (int i = 0; i < 25; ++i) l64[i] = Dd[i];
+    lusolve5<double>(Dd, PIV_ID, b, x);
+  } else {
+    const int pk = lu5<F>(D);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) luc[i] = D[i];
+    piv[c] = pk;
+    lusolve5<F>(D, pk, b, x);
+  }
 #pragma unroll
   for (int k = 0; k < 5; ++k) X[(size_t)c * 5 + k] = x[k];
 }
@@ -961,6 +994,136 @@ __global__ void k_ghost(DevGrid g, double *__restrict__ X) {
   for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + pi) * 5 + k] = X[(size_t)src * 5 + k];
 }
 
+// ---------------------------------------------------------------------------
+// line SGS, src/linearSolverLSGS.loci:1692-1954.  The line search of that
+// file links cells only through duplicate couples, which it removes first
+// (:625-632 then :642-664), so every line is one cell: a line solve is
+// X_c = D_c^-1 (B_c - sum_lower L X[cl] - sum_upper U X[cr]) with D_c's FP64
+// unpivoted factors, in the breadth-first line order (:902-937).  Levels of
+// that order run as one launch each; 5 lanes per cell as in k_sgs_level, with
+// the row read lower faces first (linerhs :1498-1516 over the outer entries
+// built at :1036-1066).  The backward sweep also keeps X after the forward
+// sweep in xt (:1827-1829) for the relaxation.
+// ---------------------------------------------------------------------------
+template <typename F, bool BACK>
+__global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi,
+                                                   const F *__restrict__ off, const double *__restrict__ lu64,
+                                                   const F *__restrict__ B, double *__restrict__ X,
+                                                   double *__restrict__ xt) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int k = lo + blockIdx.x * CPW + q;
+  const bool act = q < CPW && k < hi;
+  const int gbase = 5 * (q < CPW ? q : 0);
+  double t = 0.0;
+  double a[25];
+  int c = 0;
+  if (act) {
+    c = cells ? cells[k] : k;
+    t = -(double)B[(size_t)c * 5 + i];
+    const int m0 = g.m_off[c];
+    const int len = g.m_off[c + 1] - m0;
+    const int nup = g.m_nup[c];
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)c * MAXR + j];
+    F mrow[MAXR][5];
+    double xv[MAXR][5];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) {
+        mrow[j][jj] = M[i + 5 * jj];
+        xv[j][jj] = X[(size_t)nb[j] * 5 + jj];
+      }
+    }
+#pragma unroll
+    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu64[(size_t)c * 25 + i2];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
+    for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const F *M = off + (size_t)(m0 + jr) * 25;
+      const double *xn = X + (size_t)g.m_nbr[m0 + jr] * 5;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += M[i + 5 * jj] * xn[jj];
+    }
+  }
+  double v[5];
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) v[kk] = -__shfl(t, gbase + kk, 64);
+  if (act) {
+    if constexpr (BACK) xt[(size_t)c * 5 + i] = X[(size_t)c * 5 + i];
+    double x[5];
+    lusolve5<double>(a, PIV_ID, v, x);
+    const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+    X[(size_t)c * 5 + i] = xi;
+  }
+}
+
+// st: w, zero_resid, resid, iterations run, done
+__global__ void k_lsgs_init(double *__restrict__ st, double w0) {
+  if (threadIdx.x == 0) {
+    st[0] = w0; st[1] = 1.0; st[2] = 0.0; st[3] = 0.0; st[4] = 0.0;
+  }
+}
+
+// X = w X + (1-w) xt and the block sums of (X - xt)^2 (:1869-1874)
+__global__ void k_lsgs_relax(int n, const double *__restrict__ st, double *__restrict__ X,
+                             const double *__restrict__ xt, double *__restrict__ part) {
+  __shared__ double sh[TPB];
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  const double w = st[0];
+  double r2 = 0.0;
+  if (i < n) {
+    const double xo = xt[i];
+    const double x = w * X[i] + (1. - w) * xo;
+    X[i] = x;
+    const double r = x - xo;
+    r2 = r * r;
+  }
+  sh[threadIdx.x] = r2;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// update norm, relaxation control and stopping test (:1927-1948)
+__global__ void k_lsgs_check(int nparts, const double *__restrict__ part, double *__restrict__ st, int iter,
+                             double abstol, double reltol) {
+  __shared__ double sh[TPB];
+  double a = 0.0;
+  for (int j = threadIdx.x; j < nparts; j += TPB) a += part[j];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double resid = sqrt(sh[0]);
+    const double zero = iter == 0 ? (resid > 1e-16 ? resid : 1e-16) : st[1];
+    const double rel = resid / zero;
+    double w = st[0];
+    if (rel > 1.0) w *= 1. / rel;
+    st[0] = w;
+    st[1] = zero;
+    st[2] = resid;
+    st[3] = (double)(iter + 1);
+    st[4] = (resid < abstol || rel < reltol) ? 1.0 : 0.0;
+  }
+}
+
 // damped Newton update, src/solverNewton.loci:62-78
 __global__ void k_update(DevGrid g, double Pamb, const double *__restrict__ X, double *__restrict__ q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1006,15 +1169,20 @@ void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipS
     hipLaunchKernelGGL(k_face_jac<double>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
                        s.phi, s.vflux, (double *)s.off, (double *)s.pblk, (double *)s.bblk);
 }
-void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st) {
-  if (fp32)
-    hipLaunchKernelGGL(k_cell_assemble<float>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src,
-                       s.maxev, (const float *)s.off, (const float *)s.pblk, (const float *)s.bblk,
-                       s.dtau, (float *)s.lu, s.piv, (float *)s.B, s.X);
-  else
-    hipLaunchKernelGGL(k_cell_assemble<double>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src,
-                       s.maxev, (const double *)s.off, (const double *)s.pblk, (const double *)s.bblk,
-                       s.dtau, (double *)s.lu, s.piv, (double *)s.B, s.X);
+template <typename F, bool LS>
+static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL((k_cell_assemble<F, LS>), dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src, s.maxev,
+                     (const F *)s.off, (const F *)s.pblk, (const F *)s.bblk, s.dtau, (F *)s.lu, s.piv,
+                     (F *)s.B, s.X, s.lu64);
+}
+void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st) {
+  if (fp32) {
+    if (lsgs) cell_assemble_t<float, true>(g, p, s, st);
+    else cell_assemble_t<float, false>(g, p, s, st);
+  } else {
+    if (lsgs) cell_assemble_t<double, true>(g, p, s, st);
+    else cell_assemble_t<double, false>(g, p, s, st);
+  }
 }
 void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st) {
   const int n = hi - lo;
@@ -1183,6 +1351,35 @@ void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, doubl
                        (const double *)s.lu, s.piv, (const double *)s.B, s.X, parts);
   }
   hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 1, parts, out);
+}
+
+int lsgs_parts(const DevGrid &g) { return (int)(((long)g.nc * 5 + TPB - 1) / TPB); }
+void launch_lsgs_init(DevState &s, double w0, hipStream_t st) {
+  hipLaunchKernelGGL(k_lsgs_init, dim3(1), dim3(64), 0, st, s.lst, w0);
+}
+template <typename F, bool BACK>
+static void lsgs_level_t(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st) {
+  const int nb = (hi - lo + CPW - 1) / CPW;
+  hipLaunchKernelGGL((k_lsgs_level<F, BACK>), dim3(nb), dim3(64), 0, st, g, cells, lo, hi, (const F *)s.off,
+                     (const double *)s.lu64, (const F *)s.B, s.X, s.xt);
+}
+void launch_lsgs_level(const DevGrid &g, int fp32, int backward, const int *cells, int lo, int hi, DevState &s,
+                       hipStream_t st) {
+  if (hi <= lo) return;
+  if (fp32) {
+    if (backward) lsgs_level_t<float, true>(g, cells, lo, hi, s, st);
+    else lsgs_level_t<float, false>(g, cells, lo, hi, s, st);
+  } else {
+    if (backward) lsgs_level_t<double, true>(g, cells, lo, hi, s, st);
+    else lsgs_level_t<double, false>(g, cells, lo, hi, s, st);
+  }
+}
+void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st) {
+  const int n = g.nc * 5, np = lsgs_parts(g);
+  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, n, (const double *)s.lst, s.X,
+                     (const double *)s.xt, s.lparts);
+  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
+                     reltol);
 }
 
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st) {

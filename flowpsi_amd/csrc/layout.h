@@ -38,6 +38,10 @@ struct DevGrid {
   const int *slot_cl;  // [nfi + nper] slot of U_f = fjm in row cl
   const int *slot_cr;  // [nfi] slot of L_f = -fjp in row cr
   const int *old_id;   // [nc] caller's index of each cell (tie-breaks of the norms)
+  // line SGS: rows read lower faces first, then upper faces.  m_nup[c] is the
+  // number of upper slots of row c; lnbr_pad is nbr_pad in that order.
+  const int *m_nup;    // [nc]
+  const int *lnbr_pad; // [nc][6]
 };
 
 struct DevState {
@@ -60,6 +64,11 @@ struct DevState {
   double *npart;  // norm partials
   double *nout;   // final norms
   int *lev_cnt;   // [nlev * 32] cells published per level (persistent sweep)
+  // line SGS (allocated when that solver is selected)
+  double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
+  double *xt;     // [nc][5] X after the forward sweep
+  double *lparts; // update-norm partials
+  double *lst;    // [8] w, zero_resid, resid, iterations, done
 };
 
 }  // namespace fpsi

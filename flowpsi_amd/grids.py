@@ -63,6 +63,33 @@ class Grid:
         assert np.abs(s).max() < 1e-9 * scale, "cells are not closed"
 
 
+def renumber_cells(g: Grid, perm, q=None):
+    """The same grid with cell i renamed perm[i]; interior faces whose new cl
+    exceeds the new cr are turned round (cl and cr swapped, normal negated)
+    so that cl < cr still holds.  Returns (grid, q renumbered or None)."""
+    perm = np.asarray(perm, np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(g.nc))
+    inter = g.cr >= 0
+    cl = perm[g.cl]
+    cr = np.where(inter, perm[np.where(inter, g.cr, 0)], -1)
+    flip = inter & (cl > cr)
+    n = g.n.copy()
+    n[:, flip] *= -1.0
+    cl2 = np.where(flip, cr, cl).astype(np.int32)
+    cr2 = np.where(flip, cl, cr).astype(np.int32)
+    vol = np.empty_like(g.vol)
+    vol[perm] = g.vol
+    cc = np.empty_like(g.cc)
+    cc[:, perm] = g.cc
+    out = Grid(g.nc, g.nf, cl2, cr2, g.btype.copy(), g.pface.copy(), n, g.area.copy(), g.fc.copy(), vol, cc,
+               g.pshift.copy(), name=g.name + "_renumbered", meta=dict(g.meta))
+    q2 = None
+    if q is not None:
+        q2 = np.empty_like(q)
+        q2[:, perm] = q
+    return out, q2
+
+
 def _axis_coords(n, lo, hi, kind="uniform", ratio=1.0):
     if kind == "uniform":
         return np.linspace(lo, hi, n + 1)

@@ -42,7 +42,9 @@ enum { FPSI_BC_INTERIOR = 0, FPSI_BC_FARFIELD = 1, FPSI_BC_WALL = 2,
 enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
 enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
 enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
-enum { FPSI_SOLVER_SGS = 0 };
+/* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, or line SGS
+ * (src/linearSolverLSGS.loci) */
+enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
 /* one launch per dependency level (default), or one cooperative launch per
  * half-sweep with per-level completion counters */
 enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1 };
@@ -71,7 +73,8 @@ typedef struct {
  * limiter/Kl (Loci FVM); sutherlandParameters src/fluxViscous.loci:59-110;
  * Twall src/bcViscousWall.loci:152-154; farfield reference state
  * src/bcInflow.loci:311-340; cflmax/dtmax/urelax src/solverTimestep.loci:281-297;
- * gauss_seidel_iter src/linearSolverSGS.loci:39-41. */
+ * gauss_seidel_iter src/linearSolverSGS.loci:39-41; LSGSMaxIter/RelTol/AbsTol/
+ * Relaxation src/linearSolverLSGS.loci:96-107 (10, 0.1, 1e-10, 0.25). */
 typedef struct {
   double gamma, Rtilde, Pambient;
   int flux;             /* FPSI_FLUX_* */
@@ -87,7 +90,11 @@ typedef struct {
   int sgs_iter;         /* symmetric sweeps */
   int precision;        /* FPSI_PREC_*: storage/arithmetic of real_fj (Jacobians, D, LU, B) */
   double minPg;         /* floor of the reconstructed gage pressure */
-  int sweep_mode;       /* FPSI_SWEEP_*: how a Gauss-Seidel half-sweep is launched */
+  int sweep_mode;       /* FPSI_SWEEP_*: how a Gauss-Seidel half-sweep is launched (SGS) */
+  int lsgs_maxiter;     /* line SGS: at most this many relaxed symmetric sweeps */
+  double lsgs_reltol;   /* stop when the update norm falls below reltol x the first one */
+  double lsgs_abstol;   /* ... or below abstol */
+  double lsgs_relax;    /* relaxation w: X = w X_sweep + (1-w) X_forward */
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -102,6 +109,8 @@ typedef struct {
   int max_level_width;
   int nnz_blocks;       /* off-diagonal blocks stored */
   double device_bytes;  /* device memory held by the context */
+  int lsgs_nlevels;     /* dependency levels of the line-SGS order */
+  int lsgs_contiguous;  /* 1 when every line-SGS level is a contiguous cell range */
 } fpsi_info;
 
 /* Build the context: renumbers cells into level order, builds the cell->face
@@ -141,22 +150,32 @@ int fpsi_gradients(fpsi_ctx *ctx, double *grad, double *phi);
 int fpsi_assemble(fpsi_ctx *ctx);
 
 /* Copy the assembled system out in the reference layout: fjp/fjm [nf][25]
- * for interior and periodic faces (zero elsewhere), D [nc][25] (LU factors),
- * pivot [nc][5], B [nc][5] (real_fj of the current precision: 8 or 4 bytes
- * per element), dtau [nc].  Any output may be NULL. */
+ * for interior and periodic faces (zero elsewhere), D [nc][25] (LU factors
+ * under FPSI_SOLVER_SGS; the unfactored fluid_D under FPSI_SOLVER_LSGS, whose
+ * factors are kept in FP64), pivot [nc][5] (identity under LSGS), B [nc][5]
+ * (real_fj of the current precision: 8 or 4 bytes per element), dtau [nc].
+ * Any output may be NULL. */
 int fpsi_get_system(fpsi_ctx *ctx, void *fjp, void *fjm, void *D, int *pivot, void *B, double *dtau);
 
-/* Block SGS on the assembled system (src/linearSolverSGS.loci:57-331):
- * Jacobi start then nsweeps symmetric level-scheduled sweeps.
- * dq[nc][5] may be NULL (the result stays on the device as dfluid). */
+/* Linear solve of the assembled system; dq[nc][5] may be NULL (the result
+ * stays on the device as dfluid).
+ * FPSI_SOLVER_SGS: block SGS (src/linearSolverSGS.loci:57-331), Jacobi start
+ *   then nsweeps symmetric level-scheduled sweeps.
+ * FPSI_SOLVER_LSGS: LSGSBlockedSSolve (src/linearSolverLSGS.loci:1692-1954),
+ *   line-Jacobi start then at most nsweeps relaxed symmetric sweeps in the
+ *   breadth-first line order, stopping on lsgs_abstol / lsgs_reltol. */
 int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
+
+/* Outcome of the last line-SGS solve: sweeps run, final relaxation w and
+ * update norm (the diagnostics of src/linearSolverLSGS.loci:1934-1940). */
+int fpsi_lsgs_stats(fpsi_ctx *ctx, int *iters, double *relax, double *resid);
 
 /* Damped update q += alpha dq (src/solverNewton.loci:62-78) with the
  * current dfluid. */
 int fpsi_update(fpsi_ctx *ctx);
 
 /* nsteps fused steady Newton steps, device resident: residual, assembly,
- * linear solve, update.  hist[nsteps] (nullable) receives the norms of the
+ * linear solve (sgs_iter sweeps, or lsgs_maxiter under LSGS), update.  hist[nsteps] (nullable) receives the norms of the
  * residual evaluated at the start of each step. */
 int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
 

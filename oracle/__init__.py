@@ -30,7 +30,9 @@ class OgParams(C.Structure):
                 ("viscous", C.c_int), ("suth", C.c_double * 6), ("Twall", C.c_double),
                 ("pg0", C.c_double), ("T0", C.c_double), ("u0", C.c_double * 3),
                 ("cflmax", C.c_double), ("dtmax", C.c_double), ("urelax", C.c_double),
-                ("sgs_iter", C.c_int), ("minPg", C.c_double)]
+                ("sgs_iter", C.c_int), ("minPg", C.c_double), ("linear_solver", C.c_int),
+                ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double), ("lsgs_abstol", C.c_double),
+                ("lsgs_relax", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -104,6 +106,11 @@ def params(p: dict) -> OgParams:
     s.urelax = p.get("urelax", 1e30)
     s.sgs_iter = p.get("sgs_iter", 5)
     s.minPg = p.get("minPg", -s.Pambient)
+    s.linear_solver = {"sgs": 0, "lsgs": 1}[p.get("solver", "sgs")]
+    s.lsgs_maxiter = p.get("lsgs_maxiter", 10)
+    s.lsgs_reltol = p.get("lsgs_reltol", 0.1)
+    s.lsgs_abstol = p.get("lsgs_abstol", 1e-10)
+    s.lsgs_relax = p.get("lsgs_relax", 0.25)
     return s
 
 
@@ -156,13 +163,29 @@ class Oracle:
         src = np.zeros((5, nc)); dtau = np.zeros(nc)
         fjp = np.zeros((nf, 25), self.fj); fjm = np.zeros((nf, 25), self.fj)
         D = np.zeros((nc, 25), self.fj); piv = np.zeros((nc, 5), np.int32)
-        B = np.zeros((nc, 5), self.fj)
+        B = np.zeros((nc, 5), self.fj); Draw = np.zeros((nc, 25), self.fj)
         nrm = OgNorms()
-        self.L.og_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(src), _ptr(dtau),
-                           fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
-                           D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
-                           B.ctypes.data_as(C.c_void_p), C.byref(nrm))
-        return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm)
+        self.L.og_assemble2(self.og.ref, C.byref(self.p), _ptr(q), _ptr(src), _ptr(dtau),
+                            fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
+                            D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
+                            B.ctypes.data_as(C.c_void_p), C.byref(nrm), Draw.ctypes.data_as(C.c_void_p))
+        return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm, Draw=Draw)
+
+    def lsgs_order(self):
+        order = np.zeros(self.g.nc, np.int32)
+        n = self.L.og_lsgs_order(self.og.ref, _ptr(order, C.c_int))
+        assert n == self.g.nc
+        return order
+
+    def lsgs(self, sys, maxiter=10, abstol=1e-10, reltol=0.1, w=0.25):
+        X = np.zeros((self.g.nc, 5))
+        wo = C.c_double(); ro = C.c_double()
+        self.L.og_lsgs.restype = C.c_int
+        it = self.L.og_lsgs(self.og.ref, int(maxiter), C.c_double(abstol), C.c_double(reltol), C.c_double(w),
+                            sys["fjp"].ctypes.data_as(C.c_void_p), sys["fjm"].ctypes.data_as(C.c_void_p),
+                            sys["Draw"].ctypes.data_as(C.c_void_p), sys["B"].ctypes.data_as(C.c_void_p),
+                            _ptr(X), C.byref(wo), C.byref(ro))
+        return X, dict(iters=it, w=wo.value, resid=ro.value)
 
     def sgs(self, sys, nsweeps):
         X = np.zeros((self.g.nc, 5))

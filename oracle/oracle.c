@@ -1084,6 +1084,13 @@ static void boundary_block(const og_grid *g, const og_params *p, const double *q
 void og_assemble(const og_grid *g, const og_params *p, const double *q, double *src,
                  double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
                  og_norms *nrm) {
+  og_assemble2(g, p, q, src, dtau_out, fjp_v, fjm_v, D_v, pivot, B_v, nrm, NULL);
+}
+
+void og_assemble2(const og_grid *g, const og_params *p, const double *q, double *src,
+                  double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
+                  og_norms *nrm, void *Draw_v) {
+  real_fj *Draw = Draw_v;
   int nc = g->nc, nf = g->nf;
   real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
   double *qf = calloc((size_t)5 * nf, sizeof(double));
@@ -1204,6 +1211,7 @@ void og_assemble(const og_grid *g, const og_params *p, const double *q, double *
     addfj(&M_(Dc, 3, 3), coef * rho);
     addfj(&M_(Dc, 4, 3), coef * rho * U[2]);
     for (int k = 0; k < 5; ++k) B[5 * (size_t)c + k] = (real_fj)s[k];
+    if (Draw) memcpy(Draw + 25 * (size_t)c, Dc, 25 * sizeof(real_fj));
     lu_decompose(Dc, pivot + 5 * (size_t)c, 5);
   }
   free(bblk);
@@ -1265,14 +1273,18 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
   real_fj *D = malloc(sizeof(real_fj) * 25 * (size_t)nc), *B = malloc(sizeof(real_fj) * 5 * (size_t)nc);
   int *piv = malloc(sizeof(int) * 5 * (size_t)nc);
   double *X = malloc(sizeof(double) * 5 * (size_t)nc);
+  real_fj *Draw = p->linear_solver == 1 ? malloc(sizeof(real_fj) * 25 * (size_t)nc) : NULL;
   for (int s = 0; s < nsteps; ++s) {
     og_norms nrm;
-    og_assemble(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm);
+    og_assemble2(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
     if (hist) {
       hist[6 * s + 0] = nrm.rtrms; hist[6 * s + 1] = nrm.etrms; hist[6 * s + 2] = nrm.mtrms;
       hist[6 * s + 3] = nrm.rtmax; hist[6 * s + 4] = nrm.etmax; hist[6 * s + 5] = nrm.mtmax;
     }
-    og_sgs(g, p->sgs_iter, fjp, fjm, D, piv, B, X);
+    if (p->linear_solver == 1)  /* fluidLinearSolver lsgs, src/solverNewton.loci:256-259 */
+      og_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, fjp, fjm, Draw, B, X, NULL, NULL);
+    else
+      og_sgs(g, p->sgs_iter, fjp, fjm, D, piv, B, X);
     for (int c = 0; c < nc; ++c) {
       const real *dq = X + 5 * (size_t)c;
       const real dT = dq[0], dp = dq[4];
@@ -1284,6 +1296,164 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       for (int k = 1; k < 4; ++k) q[k * nc + c] = q[k * nc + c] + alpha * dq[k];
     }
   }
-  free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X);
+  free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X); free(Draw);
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* line SGS (src/linearSolverLSGS.loci)                                */
+/* ------------------------------------------------------------------ */
+
+/* Equation order of the LSGS topology (:361-1007).  In this revision the
+ * line search only links cells through duplicate couples, which are removed
+ * just before it (:625-632), so every line is a single cell and the order is
+ * the breadth-first order of the lines (:902-937): start at cell 0, visit the
+ * interior-face neighbours layer by layer, each layer in ascending order. */
+int og_lsgs_order(const og_grid *g, int *order) {
+  int nc = g->nc;
+  og_adj a;
+  build_adj(g, &a);
+  char *vis = calloc(nc, 1);
+  int *cur = malloc(sizeof(int) * nc), *nxt = malloc(sizeof(int) * nc);
+  int ncur = 0, nout = 0, sp = 0;
+  while (nout < nc) {
+    if (ncur == 0) {
+      while (sp < nc && vis[sp]) sp++;
+      if (sp == nc) break;
+      cur[ncur++] = sp;
+      vis[sp] = 1;
+    }
+    int nn = 0;
+    for (int t = 0; t < ncur; ++t) {
+      int c = cur[t];
+      for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+        int o = g->cl[a.lo[j]];
+        if (!vis[o]) { vis[o] = 1; nxt[nn++] = o; }
+      }
+      for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+        int f = a.up[j];
+        if (g->btype[f] != OG_INTERIOR) continue; /* ghosts do not join the search */
+        int o = g->cr[f];
+        if (!vis[o]) { vis[o] = 1; nxt[nn++] = o; }
+      }
+      order[nout++] = c;
+    }
+    /* the next layer is a std::set: ascending */
+    for (int i = 1; i < nn; ++i) {
+      int v = nxt[i], k = i - 1;
+      while (k >= 0 && nxt[k] > v) { nxt[k + 1] = nxt[k]; --k; }
+      nxt[k + 1] = v;
+    }
+    memcpy(cur, nxt, sizeof(int) * nn);
+    ncur = nn;
+  }
+  free(vis); free(cur); free(nxt);
+  free_adj(&a);
+  return nout;
+}
+
+/* unpivoted Doolittle LU in place (Loci Mat decompose_lu) and its solve */
+static void lu_nopiv(real *a, int nb) {
+  for (int k = 0; k < nb; ++k)
+    for (int i = k + 1; i < nb; ++i) {
+      real l = a[i + nb * k] / a[k + nb * k];
+      a[i + nb * k] = l;
+      for (int j = k + 1; j < nb; ++j) a[i + nb * j] = a[i + nb * j] - l * a[k + nb * j];
+    }
+}
+static void lu_nopiv_solve(const real *a, const real *b, real *x, int nb) {
+  for (int i = 0; i < nb; ++i) {
+    real tmp = b[i];
+    for (int j = 0; j < i; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp;
+  }
+  for (int i = nb - 1; i >= 0; --i) {
+    real tmp = x[i];
+    for (int j = i + 1; j < nb; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp / a[i + nb * i];
+  }
+}
+/* t[i] += sum_j M(i,j) v[j] for a double block */
+static inline void dotprod_accum_d(const real *M, const real *v, real *t) {
+  for (int j = 0; j < NB; ++j) {
+    const real in = v[j];
+    for (int i = 0; i < NB; ++i) t[i] += M[i + NB * j] * in;
+  }
+}
+
+/* LSGSBlockedSSolve (:1420-1495 matrix and diagonal, :1692-1954 solve):
+ * D copied to double and LU-factored without pivots, line-Jacobi start,
+ * then up to maxiter relaxed symmetric sweeps in line order; per cell
+ * t = -B + sum_lower L X[cl] + sum_upper U X[cr] (lower faces first), the
+ * periodic ghost values refreshed once per iteration; after both sweeps
+ * X = w X + (1-w) X_forward, stopping on the update norm.  Returns the
+ * iterations run; *w_out and *resid_out receive the final relaxation and
+ * update norm. */
+int og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const void *fjp_v,
+            const void *fjm_v, const void *Draw_v, const void *B_v, double *X, double *w_out,
+            double *resid_out) {
+  int nc = g->nc, nf = g->nf;
+  const real_fj *fjp = fjp_v, *fjm = fjm_v, *Draw = Draw_v, *B = B_v;
+  og_adj a;
+  build_adj(g, &a);
+  int *order = malloc(sizeof(int) * nc);
+  og_lsgs_order(g, order);
+  real *Dl = malloc(sizeof(real) * 25 * (size_t)nc);
+  for (size_t i = 0; i < 25 * (size_t)nc; ++i) Dl[i] = (real)Draw[i];
+  for (int c = 0; c < nc; ++c) lu_nopiv(Dl + 25 * (size_t)c, 5);
+  double *Xg = calloc((size_t)5 * nf, sizeof(double));
+  double *xt = malloc(sizeof(double) * 5 * (size_t)nc);
+  /* line-Jacobi start */
+  for (int c = 0; c < nc; ++c) {
+    real b[5];
+    for (int k = 0; k < 5; ++k) b[k] = B[5 * (size_t)c + k];
+    lu_nopiv_solve(Dl + 25 * (size_t)c, b, X + 5 * (size_t)c, 5);
+  }
+  real w = w0, zero = 1.0, resid = 0.0;
+  int iter = 0;
+  for (iter = 0; iter < maxiter; ++iter) {
+    for (int f = 0; f < nf; ++f)
+      if (g->btype[f] == OG_PERIODIC)
+        for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int ii = 0; ii < nc; ++ii) {
+        int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
+        real t[5], v[5];
+        for (int k = 0; k < 5; ++k) t[k] = -(real)B[5 * (size_t)c + k];
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          int f = a.lo[j];
+          real_fj L[25];
+          for (int i = 0; i < 25; ++i) L[i] = -fjp[25 * (size_t)f + i];
+          dotprod_accum(L, X + 5 * (size_t)g->cl[f], t);
+        }
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          int f = a.up[j];
+          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          dotprod_accum(fjm + 25 * (size_t)f, xn, t);
+        }
+        for (int k = 0; k < 5; ++k) {
+          xt[5 * (size_t)c + k] = X[5 * (size_t)c + k];
+          v[k] = -t[k];
+        }
+        lu_nopiv_solve(Dl + 25 * (size_t)c, v, X + 5 * (size_t)c, 5);
+      }
+    }
+    double residl = 0;
+    for (size_t i = 0; i < 5 * (size_t)nc; ++i) {
+      X[i] = w * X[i] + (1. - w) * xt[i];
+      double r = X[i] - xt[i];
+      residl += r * r;
+    }
+    resid = sqrt(residl);
+    if (iter == 0) zero = resid > 1e-16 ? resid : 1e-16;
+    double rel = resid / zero;
+    if (rel > 1.0) w *= 1. / rel;
+    if (resid < abstol) { ++iter; break; }
+    if (rel < reltol) { ++iter; break; }
+  }
+  if (w_out) *w_out = w;
+  if (resid_out) *resid_out = resid;
+  free(order); free(Dl); free(Xg); free(xt);
+  free_adj(&a);
+  return iter;
 }

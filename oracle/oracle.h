@@ -53,6 +53,9 @@ typedef struct {
   double cflmax, dtmax, urelax;
   int sgs_iter;
   double minPg;           /* floor of the reconstructed gage pressure */
+  int linear_solver;      /* 0 = block SGS, 1 = line SGS */
+  int lsgs_maxiter;
+  double lsgs_reltol, lsgs_abstol, lsgs_relax;
 } og_params;
 
 typedef struct {
@@ -100,6 +103,15 @@ void og_residual(const og_grid *g, const og_params *p, const double *q, double *
 void og_assemble(const og_grid *g, const og_params *p, const double *q, double *src,
                  double *dtau, void *fjp, void *fjm, void *D, int *pivot, void *B,
                  og_norms *nrm);
+/* as og_assemble, also returning D before its LU (Draw, [nc][25] real_fj) */
+void og_assemble2(const og_grid *g, const og_params *p, const double *q, double *src,
+                  double *dtau, void *fjp, void *fjm, void *D, int *pivot, void *B,
+                  og_norms *nrm, void *Draw);
+/* line SGS: equation order and the relaxed solve (returns iterations run) */
+int  og_lsgs_order(const og_grid *g, int *order);
+int  og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const void *fjp,
+             const void *fjm, const void *Draw, const void *B, double *X, double *w_out,
+             double *resid_out);
 /* block SGS (Jacobi start + sweeps in cell-index order) on an assembled system */
 void og_sgs(const og_grid *g, int nsweeps, const void *fjp, const void *fjm, const void *D,
             const int *pivot, const void *B, double *X);
