@@ -52,7 +52,8 @@ class Params(C.Structure):
                 ("dtmax", C.c_double), ("urelax", C.c_double), ("linear_solver", C.c_int),
                 ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double),
                 ("sweep_mode", C.c_int), ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double),
-                ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double)]
+                ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double), ("Minf", C.c_double),
+                ("etaT", C.c_double), ("PLimPC", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -140,6 +141,9 @@ def make_params(prm: dict) -> Params:
     s.lsgs_reltol = prm.get("lsgs_reltol", 0.1)
     s.lsgs_abstol = prm.get("lsgs_abstol", 1e-10)
     s.lsgs_relax = prm.get("lsgs_relax", 0.25)
+    s.Minf = prm.get("Minf", 1.0)
+    s.etaT = prm.get("etaT", 0.25)
+    s.PLimPC = prm.get("PLimPC", 10.0)
     return s
 
 

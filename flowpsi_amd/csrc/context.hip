@@ -549,7 +549,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
       (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper + 1))) ||
       (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
-      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev * 32))) {
+      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev * 32)) || (rc = dalloc(c, &s.eta, (size_t)nc)) ||
+      (rc = dalloc(c, &s.eta_t, (size_t)nc))) {
     *out = c;
     return rc;
   }
@@ -630,6 +631,15 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   k.Twall = p->Twall; k.pg0 = p->pg0; k.T0 = p->T0;
   k.u0x = p->u0[0]; k.u0y = p->u0[1]; k.u0z = p->u0[2];
   k.cflmax = p->cflmax; k.dtmax = p->dtmax; k.urelax = p->urelax; k.minPg = p->minPg;
+  {  // Mlim2, src/fluxInviscid.loci:98-104; unset (<= 0) options take flowPsi's defaults
+    const double Minf = p->Minf > 0 ? p->Minf : 1.0, etaT = p->etaT > 0 ? p->etaT : 0.25;
+    const double M2 = Minf * Minf;
+    double ml2 = 1.0;
+    if (Minf < 0.5) ml2 = etaT * 2. * M2 / (1. - 2. * M2);
+    k.Mlim2 = ml2;
+    k.PLimPC = p->PLimPC > 0 ? p->PLimPC : 10.0;
+    k.pre = ml2 < 1.0 ? 1 : 0;
+  }
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -685,6 +695,7 @@ static void enqueue_residual(fpsi_ctx *c) {
   hipEvent_t e = prof_begin(c);
   launch_bface(c->g, c->kp, c->s, c->stream);
   launch_grad(c->g, c->kp, c->s, c->stream);
+  launch_eta(c->g, c->kp, c->s, c->stream);
   launch_face_flux(c->g, c->kp, c->s, c->stream);
   launch_gather(c->g, c->kp, c->vtot, c->s, c->stream);
   prof_end(c, FPSI_K_RESIDUAL, e);

@@ -6,6 +6,7 @@
 namespace fpsi {
 void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+void launch_eta(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 int norm_parts(const DevGrid &g);
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st);

@@ -15,6 +15,15 @@ static inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 // boundary-face states: farfield src/bcInflow.loci:311-340, viscous wall
 // src/bcViscousWall.loci:205-214 (stationary), symmetry src/bcImpermeable.loci:48-64
 // ---------------------------------------------------------------------------
+// Eta_pf of a boundary face from its interior cell, src/fluxInviscid.loci:151-155
+__device__ __forceinline__ double eta_face(const Prm &p, double T, const V3 &U) {
+  if (!p.pre) return 1.0;
+  const double a = sqrt(p.gamma * p.Rt * T);
+  const double M = sqrt(dot(U, U)) / a;
+  const double M2 = 2. * dmin(0.25, M * M);
+  return dmin(1.0, dmax(M2 / (1. - M2), p.Mlim2));
+}
+
 __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ qf) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.nb) return;
@@ -29,7 +38,7 @@ __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *
   Q5 o;
   const int t = g.fbt[f];
   if (t == 1) {  // farfield
-    const double us_n = 0.0, eta_pf = 1.0;
+    const double us_n = 0.0, eta_pf = eta_face(p, i.T, i.u);
     const double uit = dot(i.u, n) - us_n;
     const double ai = sqrt(p.gamma * p.Rt * i.T);
     const double a0 = sqrt(p.gamma * p.Rt * p.T0);
@@ -268,14 +277,66 @@ __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double
 }
 
 // ---------------------------------------------------------------------------
+// low-Mach preconditioning, src/fluxInviscid.loci:35-142.  Eta_l per cell from
+// Mref = |u|/a raised by the pressure-jump and laminar viscous limits of every
+// interior and periodic face (a periodic face counts on its cl side, with the
+// partner's state as its ghost, src/solverPeriodic.loci:39-70), then two
+// passes of the maximum over the face neighbours (Eta_ll, Eta_p).  Maxima
+// are exact, so the gather order is free.
+// ---------------------------------------------------------------------------
+__global__ void k_eta_l(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ el) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const double T = q[c], pg = q[4 * nc + c];
+  const V3 U = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+  const double rho = (pg + p.Pamb) / (p.Rt * T);
+  const double a = sqrt(p.gamma * p.Rt * T);
+  const double mu = p.viscous ? suth_mu(p, T) : 0.0;
+  double mref = sqrt(dot(U, U)) / a;
+  for (int j = g.g_off[c]; j < g.g_off[c + 1]; ++j) {
+    const int e = g.g_ent[j];
+    const int f = e < 0 ? ~e : e;
+    if (f >= g.bnd_begin) continue;
+    const int o = e < 0 ? g.frc[f] : g.fcl[f];
+    const double To = q[o], pgo = q[4 * nc + o];
+    const double dP = fabs(pg - pgo);
+    mref = dmax(mref, p.PLimPC * dP / (rho * a * a));
+    if (p.viscous) {
+      const double area = dmax(g.farea[f], 1e-30);
+      const double mum = dmax(mu, suth_mu(p, To));
+      const double rhoo = (pgo + p.Pamb) / (p.Rt * To);
+      const double rhom = 0.5 * (rho + rhoo);
+      const double nu = mum / rhom;
+      mref = dmax(mref, nu / ((g.vol[c] / area) * a));
+    }
+  }
+  const double M2 = 2. * dmin(0.25, mref * mref);
+  el[c] = dmax(M2 / (1. - M2), p.Mlim2);
+}
+
+__global__ void k_eta_max(DevGrid g, const double *__restrict__ src, double *__restrict__ dst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  double b = src[c];
+  for (int j = g.g_off[c]; j < g.g_off[c + 1]; ++j) {
+    const int e = g.g_ent[j];
+    const int f = e < 0 ? ~e : e;
+    if (f >= g.bnd_begin) continue;
+    b = dmax(b, src[e < 0 ? g.frc[f] : g.fcl[f]]);
+  }
+  dst[c] = b;
+}
+
+// ---------------------------------------------------------------------------
 // face fluxes: iflux (src/fluxInviscid.loci:1170-1188, src/bcInflow.loci:342-355,
 // src/bcViscousWall.loci:216-229, src/bcImpermeable.loci:66-73), vflux and
 // maxev_f (src/solverSetup.loci:83-105)
 // ---------------------------------------------------------------------------
 __global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                             const double *__restrict__ grad, const double *__restrict__ phi,
-                            double *__restrict__ iflux, double *__restrict__ vflux,
-                            double *__restrict__ maxev) {
+                            const double *__restrict__ etap, double *__restrict__ iflux,
+                            double *__restrict__ vflux, double *__restrict__ maxev) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= g.nf) return;
   const int nc = g.nc, nf = g.nf, nb = g.nb;
@@ -283,7 +344,11 @@ __global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, cons
   const double area = g.farea[f];
   const double xf0 = g.fcx[f], xf1 = g.fcy[f], xf2 = g.fcz[f];
   const int c = g.fcl[f];
-  const double us_n = 0.0, eta = 1.0;
+  const double us_n = 0.0;
+  // interior/periodic: max over the two cells (src/fluxInviscid.loci:1187);
+  // boundary: the interior cell
+  double eta = 1.0;
+  if (p.pre) eta = f < g.bnd_begin ? dmax(etap[c], etap[g.frc[f]]) : etap[c];
   const Q5 L = recon(g, p, q, grad, phi, c, g.ccx[c], g.ccy[c], g.ccz[c], xf0, xf1, xf2);
   double fl[5];
   double mev;
@@ -503,8 +568,8 @@ __device__ __forceinline__ void store_blk(F *__restrict__ dst, const F *blk, boo
 template <typename F>
 __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                            const double *__restrict__ grad, const double *__restrict__ phi,
-                           const double *__restrict__ vflux, F *__restrict__ off, F *__restrict__ pblk,
-                           F *__restrict__ bblk) {
+                           const double *__restrict__ vflux, const double *__restrict__ etap,
+                           F *__restrict__ off, F *__restrict__ pblk, F *__restrict__ bblk) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= g.nf) return;
   const int nc = g.nc, nf = g.nf, nb = g.nb;
@@ -513,7 +578,9 @@ __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const
   const double xf0 = g.fcx[f], xf1 = g.fcy[f], xf2 = g.fcz[f];
   const int c = g.fcl[f];
   const double xl0 = g.ccx[c], xl1 = g.ccy[c], xl2 = g.ccz[c];
-  const double us_n = 0.0, eta = 1.0;
+  const double us_n = 0.0;
+  double eta = 1.0;  // src/fluxInviscid.loci:1204,1220; src/bcInflow.loci:369
+  if (p.pre) eta = f < g.bnd_begin ? dmax(etap[c], etap[g.frc[f]]) : etap[c];
   const Q5 L = recon(g, p, q, grad, phi, c, xl0, xl1, xl2, xf0, xf1, xf2);
   F blk[25];
   if (f < g.bnd_begin) {
@@ -561,7 +628,7 @@ __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const
       F tmp[25];
 #pragma unroll
       for (int i = 0; i < 25; ++i) tmp[i] = (F)0;
-      const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, 1.0);
+      const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, eta);
       roe_jacobian<F>(tmp, ra, an, area, p.gamma, p.Rt, 1);
 #pragma unroll
       for (int i = 0; i < 25; ++i) blk[i] -= tmp[i];
@@ -698,7 +765,8 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const F *__restrict__ off, const F *__restrict__ pblk,
                                 const F *__restrict__ bblk, double *__restrict__ dtau_out,
                                 F *__restrict__ lu, int *__restrict__ piv, F *__restrict__ Bout,
-                                double *__restrict__ X, double *__restrict__ lu64) {
+                                double *__restrict__ X, double *__restrict__ lu64,
+                                const double *__restrict__ etap) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -727,7 +795,7 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   const double qq = 0.5 * dot(U, U);
   const double fT = rT * qq;
   const double fP = (1. / (gm - 1) + rP * qq);
-  const double reta = 1. / 1.0;
+  const double reta = 1. / (p.pre ? etap[c] : 1.0);  // src/solverLinearSystem.loci:195
   const double a2 = gm * Rt * T;
   const double pterm = (1. - reta) * (gm - 1.) / a2;
   const double cterm = fP * rT * reta - fT * rP * reta;
@@ -1151,9 +1219,15 @@ void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
 void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_grad, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
 }
+void launch_eta(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  if (!p.pre) return;
+  hipLaunchKernelGGL(k_eta_l, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.eta);
+  hipLaunchKernelGGL(k_eta_max, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, (const double *)s.eta, s.eta_t);
+  hipLaunchKernelGGL(k_eta_max, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, (const double *)s.eta_t, s.eta);
+}
 void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_face_flux, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi,
-                     s.iflux, s.vflux, s.maxev);
+                     (const double *)s.eta, s.iflux, s.vflux, s.maxev);
 }
 int norm_parts(const DevGrid &g) { return nblk(g.nc); }
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st) {
@@ -1164,16 +1238,17 @@ void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hip
 void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st) {
   if (fp32)
     hipLaunchKernelGGL(k_face_jac<float>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
-                       s.phi, s.vflux, (float *)s.off, (float *)s.pblk, (float *)s.bblk);
+                       s.phi, s.vflux, (const double *)s.eta, (float *)s.off, (float *)s.pblk, (float *)s.bblk);
   else
     hipLaunchKernelGGL(k_face_jac<double>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
-                       s.phi, s.vflux, (double *)s.off, (double *)s.pblk, (double *)s.bblk);
+                       s.phi, s.vflux, (const double *)s.eta, (double *)s.off, (double *)s.pblk,
+                       (double *)s.bblk);
 }
 template <typename F, bool LS>
 static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL((k_cell_assemble<F, LS>), dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src, s.maxev,
                      (const F *)s.off, (const F *)s.pblk, (const F *)s.bblk, s.dtau, (F *)s.lu, s.piv,
-                     (F *)s.B, s.X, s.lu64);
+                     (F *)s.B, s.X, s.lu64, (const double *)s.eta);
 }
 void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st) {
   if (fp32) {

@@ -64,6 +64,8 @@ struct DevState {
   double *npart;  // norm partials
   double *nout;   // final norms
   int *lev_cnt;   // [nlev * 32] cells published per level (persistent sweep)
+  double *eta;    // [nc] Eta_p (read only when Prm::pre)
+  double *eta_t;  // [nc] Eta_l / Eta_ll scratch
   // line SGS (allocated when that solver is selected)
   double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
   double *xt;     // [nc][5] X after the forward sweep

@@ -17,6 +17,9 @@ struct Prm {
   double a1, a2, a3, k1, k2, k3;
   double Twall, pg0, T0, u0x, u0y, u0z;
   double cflmax, dtmax, urelax, minPg;
+  // low-Mach preconditioning: pre = 0 when Mlim2 >= 1 (eta = 1 everywhere)
+  int pre;
+  double Mlim2, PLimPC;
 };
 
 struct V3 {

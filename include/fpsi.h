@@ -95,6 +95,9 @@ typedef struct {
   double lsgs_reltol;   /* stop when the update norm falls below reltol x the first one */
   double lsgs_abstol;   /* ... or below abstol */
   double lsgs_relax;    /* relaxation w: X = w X_sweep + (1-w) X_forward */
+  /* low-Mach preconditioning (src/fluxInviscid.loci:39-155); a value <= 0
+   * takes flowPsi's default: Minf 1 (no preconditioning), etaT 0.25, PLimPC 10 */
+  double Minf, etaT, PLimPC;
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);

@@ -32,7 +32,8 @@ class OgParams(C.Structure):
                 ("cflmax", C.c_double), ("dtmax", C.c_double), ("urelax", C.c_double),
                 ("sgs_iter", C.c_int), ("minPg", C.c_double), ("linear_solver", C.c_int),
                 ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double), ("lsgs_abstol", C.c_double),
-                ("lsgs_relax", C.c_double)]
+                ("lsgs_relax", C.c_double), ("Minf", C.c_double), ("etaT", C.c_double),
+                ("PLimPC", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -57,6 +58,7 @@ def lib(fj="f64"):
         L.og_real_fj_bytes.restype = C.c_int
         L.og_lu_decompose.restype = C.c_int
         L.og_newton.restype = C.c_int
+        L.og_eta_face.restype = C.c_double
         _libs[fj] = L
     return _libs[fj]
 
@@ -111,6 +113,9 @@ def params(p: dict) -> OgParams:
     s.lsgs_reltol = p.get("lsgs_reltol", 0.1)
     s.lsgs_abstol = p.get("lsgs_abstol", 1e-10)
     s.lsgs_relax = p.get("lsgs_relax", 0.25)
+    s.Minf = p.get("Minf", 1.0)
+    s.etaT = p.get("etaT", 0.25)
+    s.PLimPC = p.get("PLimPC", 10.0)
     return s
 
 
@@ -123,6 +128,13 @@ class Oracle:
         self.p = params(prm)
         self.L = lib(fj)
         self.fj = np.float64 if fj == "f64" else np.float32
+
+    def eta(self, q):
+        """Eta_p per cell."""
+        q = np.ascontiguousarray(q, np.float64)
+        e = np.zeros(self.g.nc)
+        self.L.og_eta(self.og.ref, C.byref(self.p), _ptr(q), _ptr(e))
+        return e
 
     def face_states(self, q):
         q = np.ascontiguousarray(q, np.float64)

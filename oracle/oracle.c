@@ -467,6 +467,90 @@ static void stencil_nb(const og_grid *g, const double *q, const double *qf, int 
 }
 
 /* ------------------------------------------------------------------ */
+/* low-Mach preconditioning parameter, src/fluxInviscid.loci:35-155     */
+/* ------------------------------------------------------------------ */
+
+static inline real suth_mu_(const og_params *p, real T) { return p->suth[0] * pow(T, p->suth[1]) / (T + p->suth[2]); }
+
+/* Mlim2 (:98-104) */
+static real mlim2(const og_params *p) {
+  const real M2 = p->Minf * p->Minf;
+  real r = 1.0;
+  if (p->Minf < 0.5) r = p->etaT * 2. * M2 / (1. - 2. * M2);
+  return r;
+}
+
+/* Eta_pf / Eta_bc of a boundary face from its interior cell (:145-155) */
+double og_eta_face(const og_params *p, double T, const double *U) {
+  const real a = sqrt(p->gamma * p->Rt * T);
+  const real M = sqrt(dot3(U, U)) / a;
+  const real M2 = 2. * minr(0.25, M * M);
+  return minr(1.0, maxr(M2 / (1. - M2), mlim2(p)));
+}
+
+/* Eta_p per cell: Mref = |u|/a (:35-37) raised by the pressure-jump and
+ * laminar viscous limits over every interior and periodic face (:43-79;
+ * the periodic ghost carries its partner's state and volume,
+ * src/solverPeriodic.loci:39-66), Eta_l (:106-110), then the maximum over
+ * the face neighbours twice (Eta_ll :116-126, Eta_p :132-142).  Maxima are
+ * exact, so the join order does not matter. */
+void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p) {
+  int nc = g->nc, nf = g->nf;
+  const real ml2 = mlim2(p);
+  if (ml2 >= 1.0) { /* Eta_l = max(M2/(1-M2), 1) = 1 for M2 <= 1/2 */
+    for (int c = 0; c < nc; ++c) eta_p[c] = 1.0;
+    return;
+  }
+  real *mref = malloc(sizeof(real) * nc), *el = malloc(sizeof(real) * nc), *ell = malloc(sizeof(real) * nc);
+  real *rho = malloc(sizeof(real) * nc), *a = malloc(sizeof(real) * nc), *mu = malloc(sizeof(real) * nc);
+  for (int c = 0; c < nc; ++c) {
+    const real T = q[c], pg = q[4 * nc + c];
+    real U[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+    rho[c] = (pg + p->Pambient) / (p->Rt * T);
+    a[c] = sqrt(p->gamma * p->Rt * T);
+    mu[c] = suth_mu_(p, T);
+    mref[c] = sqrt(dot3(U, U)) / a[c];
+  }
+  for (int f = 0; f < nf; ++f) {
+    const int t = g->btype[f];
+    if (t != OG_INTERIOR && t != OG_PERIODIC) continue;
+    const int l = g->cl[f];
+    const int r = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
+    const real dP = fabs(q[4 * nc + l] - q[4 * nc + r]);
+    /* cl side; the cr side of an interior face (a periodic ghost's own
+     * value is never used: each partner joins from its own face) */
+    mref[l] = maxr(mref[l], p->PLimPC * dP / (rho[l] * a[l] * a[l]));
+    if (t == OG_INTERIOR) mref[r] = maxr(mref[r], p->PLimPC * dP / (rho[r] * a[r] * a[r]));
+    if (p->viscous) {
+      const real area = maxr(g->area[f], 1e-30);
+      const real mum = maxr(mu[l], mu[r]);
+      const real rhom = 0.5 * (rho[l] + rho[r]);
+      const real nu = mum / rhom;
+      mref[l] = maxr(mref[l], nu / ((g->vol[l] / area) * a[l]));
+      if (t == OG_INTERIOR) mref[r] = maxr(mref[r], nu / ((g->vol[r] / area) * a[r]));
+    }
+  }
+  for (int c = 0; c < nc; ++c) {
+    const real M2 = 2. * minr(0.25, mref[c] * mref[c]);
+    el[c] = maxr(M2 / (1. - M2), ml2);
+  }
+  real *src = el, *dst = ell;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int c = 0; c < nc; ++c) dst[c] = src[c];
+    for (int f = 0; f < nf; ++f) {
+      const int t = g->btype[f];
+      if (t != OG_INTERIOR && t != OG_PERIODIC) continue;
+      const int l = g->cl[f];
+      const int r = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
+      dst[l] = maxr(dst[l], src[r]);
+      if (t == OG_INTERIOR) dst[r] = maxr(dst[r], src[l]);
+    }
+    if (pass == 0) { src = ell; dst = eta_p; }
+  }
+  free(mref); free(el); free(ell); free(rho); free(a); free(mu);
+}
+
+/* ------------------------------------------------------------------ */
 /* boundary-face states                                                */
 /* ------------------------------------------------------------------ */
 
@@ -483,7 +567,7 @@ void og_face_states(const og_grid *g, const og_params *p, const double *q, doubl
     real Ti = q[c], ui[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]}, pgi = q[4 * nc + c];
     real Tb, pgb, ub[3];
     if (t == OG_FARFIELD) {
-      const real us_n = 0.0, eta_pf = 1.0;
+      const real us_n = 0.0, eta_pf = og_eta_face(p, Ti, ui);
       const real uit = dot3(ui, n) - us_n;
       const real ai = sqrt(p->gamma * p->Rt * Ti);
       const real a0 = sqrt(p->gamma * p->Rt * p->T0);
@@ -753,7 +837,9 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
                const double *grad, const double *phi, double *iflux, double *vflux,
                double *maxev) {
   int nc = g->nc, nf = g->nf;
-  const real us_n = 0.0, eta = 1.0;
+  const real us_n = 0.0;
+  real *eta_p = malloc(sizeof(real) * nc);
+  og_eta(g, p, q, eta_p);
   for (int f = 0; f < nf; ++f) {
     int t = g->btype[f];
     real an[3], xf[3], xl[3];
@@ -761,6 +847,11 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
     fcenter(g, f, xf);
     int c = g->cl[f];
     ccenter(g, c, xl);
+    /* interior and periodic: max of the two cells (:1176-1187, the periodic
+     * ghost holds its partner's Eta_p); boundary faces: the interior cell */
+    real eta = eta_p[c];
+    if (t == OG_INTERIOR) eta = maxr(eta_p[c], eta_p[g->cr[f]]);
+    else if (t == OG_PERIODIC) eta = maxr(eta_p[c], eta_p[g->cl[g->pface[f]]]);
     real fl[5];
     real qL[5];
     recon(g, p, q, grad, phi, c, xl, xf, qL);
@@ -828,6 +919,7 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
       maxev[f] = 0.5 * maxr(fabs(utf * Eta_P - sigmaf), fabs(utf * Eta_P + sigmaf));
     }
   }
+  free(eta_p);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1017,7 +1109,7 @@ static void visc_jac(real_fj *F, real s, const real *an, real sada, real dist, r
  *   farfield src/bcInflow.loci:357-375, wall src/bcViscousWall.loci:230-240,
  *   symmetry src/bcImpermeable.loci:75-100, viscous src/fluxViscous.loci:405-448. */
 static void boundary_block(const og_grid *g, const og_params *p, const double *q, const double *qf,
-                           const double *grad, const double *phi, int f, real_fj *bj) {
+                           const double *grad, const double *phi, int f, real eta_c, real_fj *bj) {
   int nc = g->nc, nf = g->nf;
   int t = g->btype[f];
   int c = g->cl[f];
@@ -1034,7 +1126,7 @@ static void boundary_block(const og_grid *g, const og_params *p, const double *q
     real_fj fjp[25];
     for (int i = 0; i < 25; ++i) fjp[i] = 0;
     roe_jacobian(fjp, qL[4], qL[0], UL, qf[4 * nf + f], qf[f], UR, an, sada, p->Pambient, p->Rt,
-                 p->gamma, us_n, 1.0, 1);
+                 p->gamma, us_n, eta_c, 1);
     for (int i = 0; i < 5; ++i)
       for (int j = 0; j < 5; ++j) M_(bj, i, j) -= M_(fjp, i, j);
   } else if (t == OG_WALL) {
@@ -1106,13 +1198,17 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   if (nrm) residual_norms(g, src, nrm);
 
   /* face Jacobians */
-  const real us_n = 0.0, eta = 1.0;
+  const real us_n = 0.0;
+  real *eta_p = malloc(sizeof(real) * nc);
+  og_eta(g, p, q, eta_p);
   real_fj *bblk = calloc((size_t)nf * 25, sizeof(real_fj));
   for (int f = 0; f < nf; ++f) {
     real_fj *P = fjp + 25 * (size_t)f, *Mm = fjm + 25 * (size_t)f;
     for (int i = 0; i < 25; ++i) P[i] = Mm[i] = 0;
     int t = g->btype[f];
     if (t == OG_INTERIOR || t == OG_PERIODIC) {
+      const int rc = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
+      const real eta = maxr(eta_p[g->cl[f]], eta_p[rc]); /* :1196-1220 */
       real an[3], xf[3], xl[3], xr[3], qL[5], qR[5];
       fnormal(g, f, an);
       fcenter(g, f, xf);
@@ -1133,7 +1229,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
         visc_jac(P, 1.0, an, g->area[f], dist, mu_f, k_f, uf, vf);
       }
     } else {
-      boundary_block(g, p, q, qf, grad, phi, f, bblk + 25 * (size_t)f);
+      boundary_block(g, p, q, qf, grad, phi, f, eta_p[g->cl[f]], bblk + 25 * (size_t)f);
     }
   }
 
@@ -1163,7 +1259,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
     const real qq = 0.5 * dot3(U, U);
     const real fT = rT * qq;
     const real fP = (1. / (gm - 1) + rP * qq);
-    const real reta = 1. / 1.0; /* Eta_p = 1 */
+    const real reta = 1. / eta_p[c]; /* src/solverLinearSystem.loci:195 */
     const real a2 = gm * Rt * T;
     const real pterm = (1. - reta) * (gm - 1.) / a2;
     const real cterm = fP * rT * reta - fT * rP * reta;
@@ -1215,6 +1311,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
     lu_decompose(Dc, pivot + 5 * (size_t)c, 5);
   }
   free(bblk);
+  free(eta_p);
   free_adj(&a);
   free(qf); free(grad); free(phi); free(iflux); free(vflux); free(maxev);
 }

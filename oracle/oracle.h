@@ -56,6 +56,7 @@ typedef struct {
   int linear_solver;      /* 0 = block SGS, 1 = line SGS */
   int lsgs_maxiter;
   double lsgs_reltol, lsgs_abstol, lsgs_relax;
+  double Minf, etaT, PLimPC;  /* low-Mach preconditioning (Minf >= 0.5: eta = 1) */
 } og_params;
 
 typedef struct {
@@ -88,6 +89,10 @@ void og_cv_outflow(double *pgb, double *Tb, double *ub, double pgi, double Ti, c
 int  og_lu_decompose(void *a25, int *pivot, int nb);
 void og_lu_solve(const void *a25, const int *pivot, const double *b, double *x, int nb);
 int  og_real_fj_bytes(void);
+
+/* preconditioning parameter per cell (Eta_p) and per boundary face (Eta_pf) */
+void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p);
+double og_eta_face(const og_params *p, double T, const double *U);
 
 /* pipeline stages (whole grid) */
 void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf);

@@ -24,6 +24,9 @@ def _cases():
         "box_hllc_barth": lambda: workloads.box(6, flux="hllc", limiter="barth"),
         "channel_visc": lambda: workloads.channel(nx=6, ny=10, nz=4),
         "channel_visc_nolim": lambda: workloads.channel(nx=5, ny=8, nz=6, limiter="none"),
+        # low-Mach preconditioning active (Minf < 0.5 makes eta vary per cell)
+        "box_roe_lowmach": lambda: workloads.box(8, Minf=0.1),
+        "channel_visc_lowmach": lambda: workloads.channel(nx=6, ny=10, nz=4, Minf=0.2),
     }
 
 
