@@ -53,7 +53,7 @@ def main():
     ap.add_argument("--workload", default="channel128", choices=["channel128", "box32"])
     ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--sweep", default="per_level", choices=["per_level", "persistent"])
+    ap.add_argument("--sweep", default="per_level", choices=["per_level", "tiled", "persistent"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--calib-copy", action="store_true",
@@ -111,16 +111,22 @@ def main():
     avg_s = d["ms"] / max(d["launches"], 1) / 1e3
     bytes_launch = d["bytes"]
     kname = dom
-    if dom == "sgs_sweep" and a.sweep == "per_level":
-        avg_s /= info["nlevels"]
-        bytes_launch /= info["nlevels"]
-        kname = "k_sgs_level"
+    tiled = a.sweep == "tiled"
+    if dom == "sgs_sweep":
+        if tiled:
+            kname = "k_sgs_tiles"          # one cooperative launch per half-sweep
+        elif a.sweep == "persistent":
+            kname = "k_sgs_sweep"          # one cooperative launch per half-sweep
+        else:
+            avg_s /= info["nlevels"]       # nlevels launches per half-sweep
+            bytes_launch /= info["nlevels"]
+            kname = "k_sgs_level"
     achieved = bytes_launch / avg_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sgs_level_channel128_traffic.json")
-    if kname == "k_sgs_level" and a.workload == "channel128" and (a.n or 128) == 128 and os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", f"{kname[2:]}_channel128_traffic.json")
+    if a.workload == "channel128" and (a.n or 128) == 128 and os.path.exists(tpath):
         t = json.load(open(tpath))
-        key = "fpsi::k_sgs_level<double>" if a.precision == "fp64" else "fpsi::k_sgs_level<float>"
+        key = f"fpsi::{kname}<double>" if a.precision == "fp64" else f"fpsi::{kname}<float>"
         if t.get("precision") == a.precision and key in t["kernels"]:
             traffic = t["kernels"][key]["bytes"]
     # whole-step algorithmic bytes (per Newton step) for the step-level roofline
@@ -173,7 +179,8 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f64 residual/update, f32 Jacobian/LU/SGS matrices",
         "data": "synthetic (seeded grid and initial state)",
         "config": {"workload": wname, "cells": g.nc, "faces": g.nf, "levels": info["nlevels"],
-                   "precision": a.precision, "sweep": a.sweep, "sgs_sweeps": prm["sgs_iter"],
+                   "precision": a.precision, "sweep": "tiled" if tiled else a.sweep,
+                   "tiles": info["sweep_tiles"], "sgs_sweeps": prm["sgs_iter"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

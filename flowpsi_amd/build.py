@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfpsi_gpu.so")
-SOURCES = ["csrc/kernels.hip", "csrc/mlp.hip", "csrc/context.hip"]
+SOURCES = ["csrc/kernels.hip", "csrc/tiles.hip", "csrc/mlp.hip", "csrc/context.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

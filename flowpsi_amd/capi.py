@@ -24,7 +24,7 @@ FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0, "lsgs": 1}
-SWEEP = {"per_level": 0, "persistent": 1}
+SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2}
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
 # every entry point declared in include/fpsi.h
@@ -67,7 +67,8 @@ class Norms(C.Structure):
 
 class Info(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("max_level_width", C.c_int), ("nnz_blocks", C.c_int),
-                ("device_bytes", C.c_double), ("lsgs_nlevels", C.c_int), ("lsgs_contiguous", C.c_int)]
+                ("device_bytes", C.c_double), ("lsgs_nlevels", C.c_int), ("lsgs_contiguous", C.c_int),
+                ("sweep_tiles", C.c_int)]
 
 
 class LinStats(C.Structure):
@@ -201,7 +202,7 @@ class Solver:
         self._chk(self.L.fpsi_get_info(self.h, C.byref(i)), "fpsi_get_info")
         return dict(nlevels=i.nlevels, max_level_width=i.max_level_width, nnz_blocks=i.nnz_blocks,
                     device_bytes=i.device_bytes, lsgs_nlevels=i.lsgs_nlevels,
-                    lsgs_contiguous=bool(i.lsgs_contiguous))
+                    lsgs_contiguous=bool(i.lsgs_contiguous), sweep_tiles=i.sweep_tiles)
 
     # -- rules
     def set_params(self, prm: dict):

@@ -15,6 +15,7 @@
 #include "../../include/fpsi.h"
 #include "kernels.h"
 #include "mlp.h"
+#include "tiles.h"
 
 using namespace fpsi;
 
@@ -66,6 +67,15 @@ struct fpsi_ctx {
   double lsgs_w = 0, lsgs_resid = 0;
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
   bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
+  // tiled sweeps (none when the grid admits no acyclic tiling)
+  TileGrid tg{};
+  bool have_tiles = false;
+  std::string tile_msg;
+  int tile_groups = 0;
+  int epoch = 0;
+  bool tiled() const {
+    return have_tiles && prm.sweep_mode == FPSI_SWEEP_TILED;
+  }
 };
 
 static int fail(fpsi_ctx *c, int code, const std::string &m) {
@@ -475,6 +485,15 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     if (ident) lsgs_cells.clear();
   }
 
+  // ---- tiles of the tiled sweep
+  TileHost th;
+  {
+    std::vector<int> lvl_new(nc);
+    for (int ni = 0; ni < nc; ++ni) lvl_new[ni] = level[c->cperm[ni]];
+    c->tile_msg = build_tiles(nc, ccx, ccy, ccz, lvl_new, m_off, m_nbr, nbr_pad, th);
+    c->have_tiles = c->tile_msg.empty();
+  }
+
   // ---- every index the kernels dereference must be in bounds
   {
     const int nnz = c->nnz;
@@ -488,6 +507,24 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     for (int v : lnbr_pad) ok = ok && v >= 0 && v <= nc + nper;
     for (int ni = 0; ni < nc; ++ni) ok = ok && m_nup[ni] >= 0 && m_nup[ni] <= m_off[ni + 1] - m_off[ni];
     for (int v : lsgs_cells) ok = ok && v >= 0 && v < nc;
+    if (c->have_tiles) {
+      ok = ok && (int)th.cells.size() == nc && th.cell_off.back() == nc && (int)th.nbr.size() == 6 * nc &&
+           (int)th.row.size() == nc && (int)th.len.size() == nc && (int)th.lvi.size() == nc;
+      for (int k = 0; k < th.ntiles; ++k)
+        for (int x = th.cell_off[k]; x < th.cell_off[k + 1]; ++x)
+          ok = ok && th.lvi[x] >= 0 && th.lvi[x] < th.lev_off[k + 1] - th.lev_off[k] - 1;
+      for (int x = 0; x < nc; ++x)
+        ok = ok && th.len[x] >= 0 && th.len[x] <= 6 && th.row[x] >= 0 && th.row[x] + th.len[x] <= c->nnz;
+      for (int v : th.cells) ok = ok && v >= 0 && v < nc;
+      for (int k = 0; k < th.ntiles; ++k) {
+        const int n = th.cell_off[k + 1] - th.cell_off[k];
+        ok = ok && n >= 1 && n <= TILE_MAX && th.lev[th.lev_off[k]] == 0 && th.lev[th.lev_off[k + 1] - 1] == n;
+        for (int x = th.cell_off[k] * 6; x < th.cell_off[k + 1] * 6; ++x)
+          ok = ok && (th.nbr[x] >= 0 ? th.nbr[x] <= nc + nper : ~th.nbr[x] < n);
+      }
+      for (int v : th.up) ok = ok && v >= 0 && v < th.ntiles;
+      for (int v : th.dn) ok = ok && v >= 0 && v < th.ntiles;
+    }
     ok = ok && c->lsgs_lev_off.back() == nc;
     for (int i = 0; i < nf; ++i) {
       ok = ok && fcl[i] >= 0 && fcl[i] < nc;
@@ -578,6 +615,30 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   if (!lsgs_cells.empty()) {
     if ((rc = upload(c, &c->lsgs_cells, lsgs_cells))) { *out = c; return rc; }
   }
+  if (c->have_tiles) {
+    TileGrid &t = c->tg;
+    t.ntiles = th.ntiles;
+    if ((rc = upload(c, &t.cell_off, th.cell_off)) || (rc = upload(c, &t.cells, th.cells)) ||
+        (rc = upload(c, &t.lev_off, th.lev_off)) || (rc = upload(c, &t.lev, th.lev)) ||
+        (rc = upload(c, &t.up_off, th.up_off)) || (rc = upload(c, &t.up, th.up)) ||
+        (rc = upload(c, &t.dn_off, th.dn_off)) || (rc = upload(c, &t.dn, th.dn)) ||
+        (rc = upload(c, &t.nbr, th.nbr)) || (rc = upload(c, &t.row, th.row)) ||
+        (rc = upload(c, &t.len, th.len)) || (rc = upload(c, &t.lvi, th.lvi)) ||
+        (rc = dalloc(c, &t.flag, (size_t)th.ntiles))) {
+      *out = c;
+      return rc;
+    }
+    HIPCHK(c, hipMemset(t.flag, 0, (size_t)th.ntiles * sizeof(int)));
+    int dev = 0, ncu = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int cap = std::min(sgs_tiles_groups(0, ncu), sgs_tiles_groups(1, ncu));
+    c->tile_groups = std::min(cap, th.ntiles);
+    if (c->tile_groups <= 0) {
+      c->have_tiles = false;
+      c->tile_msg = "no co-resident workgroups for the tiled sweep";
+    }
+  }
   *out = c;
   return 0;
 }
@@ -603,6 +664,7 @@ extern "C" int fpsi_get_info(fpsi_ctx *c, fpsi_info *info) {
   info->device_bytes = (double)c->bytes;
   info->lsgs_nlevels = (int)c->lsgs_lev_off.size() - 1;
   info->lsgs_contiguous = c->lsgs_cells == nullptr;
+  info->sweep_tiles = c->have_tiles ? c->tg.ntiles : 0;
   return 0;
 }
 
@@ -619,8 +681,11 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     if (int rc = lsgs_alloc(c)) return rc;
   }
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
-  if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL)
+  if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL &&
+      p->sweep_mode != FPSI_SWEEP_TILED)
     return fail(c, FPSI_ERR_ARG, "unknown sweep mode");
+  if (p->sweep_mode == FPSI_SWEEP_TILED && !c->have_tiles)
+    return fail(c, FPSI_ERR_ARG, "no tiled sweep on this grid: " + c->tile_msg);
   if (!(p->gamma > 1.0) || !(p->Rtilde > 0.0)) return fail(c, FPSI_ERR_ARG, "bad gas constants");
   c->prm = *p;
   Prm &k = c->kp;
@@ -809,6 +874,17 @@ extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *
 
 static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
   const int nlev = (int)c->lev_off.size() - 1;
+  if (c->tiled()) {
+    for (int it = 0; it < nsweeps; ++it)
+      for (int dir = 0; dir < 2; ++dir) {
+        launch_ghost(c->g, c->s, c->stream);
+        hipEvent_t e = prof_begin(c);
+        const hipError_t he = launch_sgs_tiles(c->g, c->tg, c->fp32, dir, ++c->epoch, c->tile_groups, c->s, c->stream);
+        prof_end(c, FPSI_K_SGS_SWEEP, e);
+        if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("tiled sweep launch: ") + hipGetErrorString(he));
+      }
+    return 0;
+  }
   if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
     for (int it = 0; it < nsweeps; ++it)
       for (int dir = 0; dir < 2; ++dir) {

@@ -45,9 +45,13 @@ enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
 /* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, or line SGS
  * (src/linearSolverLSGS.loci) */
 enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
-/* one launch per dependency level (default), or one cooperative launch per
- * half-sweep with per-level completion counters */
-enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1 };
+/* How a Gauss-Seidel half-sweep runs (same result bit for bit):
+ * PER_LEVEL  one launch per dependency level (default);
+ * PERSISTENT one cooperative launch, per-level completion counters;
+ * TILED      one cooperative launch over tiles of <= 32 cells whose levels run
+ *            in LDS (needs a grid whose cell boxes form an acyclic tile graph;
+ *            fpsi_info.sweep_tiles > 0). */
+enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1, FPSI_SWEEP_TILED = 2 };
 
 /* Grid: the Loci FVM facts cl, cr, area{n,sada}, facecenter, vol,
  * cellcenter, pmap (used at src/fluxInviscid.loci:1170-1188,
@@ -114,6 +118,7 @@ typedef struct {
   double device_bytes;  /* device memory held by the context */
   int lsgs_nlevels;     /* dependency levels of the line-SGS order */
   int lsgs_contiguous;  /* 1 when every line-SGS level is a contiguous cell range */
+  int sweep_tiles;      /* tiles of the tiled sweep (0: the grid admits none) */
 } fpsi_info;
 
 /* Build the context: renumbers cells into level order, builds the cell->face

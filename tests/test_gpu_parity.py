@@ -195,12 +195,35 @@ def test_bitwise_reproducible():
 
 
 def test_sweep_modes_agree():
-    """One launch per level and the cooperative half-sweep kernel give the same bits."""
+    """One launch per level, the cooperative per-level kernel and the tiled
+    kernel give the same bits."""
     g, q, prm = workloads.channel(nx=12, ny=16, nz=10)
     outs = []
-    for mode in ("persistent", "per_level"):
+    for mode in ("persistent", "per_level", "tiled"):
         s = Solver(g, dict(prm, sweep=mode))
+        assert s.info()["sweep_tiles"] > 0
         s.set_state(q)
         s.newton(2)
         outs.append(s.get_state())
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[1], outs[2])
+
+
+@pytest.mark.parametrize("mk,prec", [("box40", "fp64"), ("channel64", "fp64"), ("channel64", "fp32")])
+def test_tiled_sweep_bitwise_at_size(mk, prec):
+    """More tiles than co-resident workgroups (each workgroup walks many tiles):
+    the tiled sweep equals the level-per-launch sweep bit for bit."""
+    if mk == "box40":
+        g, q, prm = workloads.box(40)
+    else:
+        g, q, prm = workloads.channel(64)
+    prm = dict(prm, precision=prec)
+    outs = []
+    for mode in ("per_level", "tiled"):
+        s = Solver(g, dict(prm, sweep=mode))
+        info = s.info()
+        s.set_state(q)
+        s.assemble()
+        outs.append(s.linear_solve(3))
+    print(f"{mk} {prec}: {info['sweep_tiles']} tiles, {info['nlevels']} levels")
     assert np.array_equal(outs[0], outs[1])
