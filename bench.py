@@ -45,6 +45,34 @@ def calib_copy(nbytes):
     hip.hipFree(b)
 
 
+def cpu_baseline(g, q, prm, steps):
+    """The oracle (C restatement of the reference path) timed on the host:
+    real_fj = float as the reference's default build, face and cell loops
+    on OMP_NUM_THREADS threads (16 at most), sweeps sequential as in the
+    reference; the same sample again on one thread."""
+    import ctypes
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(16, threads))
+    o = oracle.Oracle(g, prm, "f32", omp=True)
+    gomp = ctypes.CDLL("libgomp.so.1")
+    res = {}
+    for n in (threads, 1):
+        gomp.omp_set_num_threads(n)
+        log(f"cpu baseline: oracle real_fj=float, {steps} Newton step(s) on the same grid, {n} thread(s)")
+        c0 = time.perf_counter()
+        o.newton(q, steps)
+        res[n] = time.perf_counter() - c0
+    gomp.omp_set_num_threads(threads)
+    cpu = {"value": g.nc * steps / res[threads], "unit": "cells*Newton-steps/s", "cores": threads,
+           "kind": "port", "value_1core": g.nc * steps / res[1],
+           "sample": f"{steps} Newton step(s) of the same workload ({g.nc} cells) from the same initial "
+                     f"state, oracle with real_fj=float (the reference default), OpenMP face/cell loops, "
+                     f"sequential SGS: {res[threads]:.1f}s on {threads} threads, {res[1]:.1f}s on 1"}
+    log(f"cpu baseline {cpu['value']:.3e} cells*steps/s ({threads} threads), {cpu['value_1core']:.3e} (1 thread)")
+    return cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,18 +170,7 @@ def main():
     tot = [h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist]
     cpu = None
     if not a.no_cpu_baseline:
-        import oracle
-        fj = "f32" if a.precision == "fp32" else "f64"
-        o = oracle.Oracle(g, prm, fj)
-        log(f"cpu baseline: oracle ({fj}) {a.cpu_sample_steps} Newton step(s) on the same grid, 1 thread")
-        c0 = time.perf_counter()
-        o.newton(q, a.cpu_sample_steps)
-        c1 = time.perf_counter()
-        cpu = {"value": g.nc * a.cpu_sample_steps / (c1 - c0), "unit": "cells*Newton-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{a.cpu_sample_steps} Newton step(s) of the same workload ({g.nc} cells), "
-                         f"oracle real_fj={fj}, single thread, {c1 - c0:.1f}s"}
-        log(f"cpu baseline {cpu['value']:.3e} cells*steps/s")
+        cpu = cpu_baseline(g, q, prm, a.cpu_sample_steps)
 
     conv = None
     if not a.no_converge:

@@ -122,11 +122,13 @@ def params(p: dict) -> OgParams:
 class Oracle:
     """Whole-grid stages of the oracle on one grid."""
 
-    def __init__(self, g, prm: dict, fj="f64"):
+    def __init__(self, g, prm: dict, fj="f64", omp=False):
+        """omp: the OpenMP build (face/cell loops on all OMP_NUM_THREADS
+        threads), used only to time the CPU baseline."""
         self.g = g
         self.og = OGrid(g)
         self.p = params(prm)
-        self.L = lib(fj)
+        self.L = lib(("omp_" if omp else "") + fj)
         self.fj = np.float64 if fj == "f64" else np.float32
 
     def eta(self, q):

@@ -36,6 +36,12 @@ static inline real minr(real a, real b) { return a < b ? a : b; }
 static inline void addfj(real_fj *a, real v) { *a = (real_fj)((real)*a + v); }
 static inline void subfj(real_fj *a, real v) { *a = (real_fj)((real)*a - v); }
 
+/* Face and cell loops whose iterations are independent run in parallel in
+ * the OpenMP build that times the CPU baseline (liboracle_omp_*.so); the
+ * pragma is inert in the serial builds the tests use.  Sweeps stay
+ * sequential, as in the reference. */
+#define OG_PAR _Pragma("omp parallel for schedule(static)")
+
 int og_real_fj_bytes(void) { return (int)sizeof(real_fj); }
 
 /* include/inviscidFlux.h:25-45 */
@@ -558,6 +564,7 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
  * (stationary wall, us = 0); symmetry: src/bcImpermeable.loci:48-64 */
 void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf) {
   int nc = g->nc, nf = g->nf;
+  OG_PAR
   for (int f = 0; f < nf; ++f) {
     int t = g->btype[f];
     if (t == OG_INTERIOR || t == OG_PERIODIC) continue;
@@ -663,6 +670,7 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
   int nc = g->nc;
   og_adj a;
   build_adj(g, &a);
+  OG_PAR
   for (int c = 0; c < nc; ++c) {
     real Ai[6];
     lsq_inverse(g, &a, c, Ai);
@@ -840,6 +848,7 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
   const real us_n = 0.0;
   real *eta_p = malloc(sizeof(real) * nc);
   og_eta(g, p, q, eta_p);
+  OG_PAR
   for (int f = 0; f < nf; ++f) {
     int t = g->btype[f];
     real an[3], xf[3], xl[3];
@@ -932,6 +941,7 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
 static void gather_src(const og_grid *g, const og_params *p, const og_adj *a, const double *iflux,
                        const double *vflux, double *src) {
   int nc = g->nc, nf = g->nf;
+  OG_PAR
   for (int c = 0; c < nc; ++c) {
     real s[5] = {0, 0, 0, 0, 0};
     for (int j = a->lo_off[c]; j < a->lo_off[c + 1]; ++j) {
@@ -1202,6 +1212,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   real *eta_p = malloc(sizeof(real) * nc);
   og_eta(g, p, q, eta_p);
   real_fj *bblk = calloc((size_t)nf * 25, sizeof(real_fj));
+  OG_PAR
   for (int f = 0; f < nf; ++f) {
     real_fj *P = fjp + 25 * (size_t)f, *Mm = fjm + 25 * (size_t)f;
     for (int i = 0; i < 25; ++i) P[i] = Mm[i] = 0;
@@ -1234,6 +1245,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   }
 
   /* cell loop: cflpdt, dtau, srcJ, D, B, LU */
+  OG_PAR
   for (int c = 0; c < nc; ++c) {
     /* cflpdt: lower, upper, boundary */
     real meva = 0.0;
@@ -1382,6 +1394,7 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       og_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, fjp, fjm, Draw, B, X, NULL, NULL);
     else
       og_sgs(g, p->sgs_iter, fjp, fjm, D, piv, B, X);
+    OG_PAR
     for (int c = 0; c < nc; ++c) {
       const real *dq = X + 5 * (size_t)c;
       const real dT = dq[0], dp = dq[4];
