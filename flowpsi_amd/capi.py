@@ -73,7 +73,7 @@ class Info(C.Structure):
 
 class LinStats(C.Structure):
     _fields_ = [("r0", C.c_double), ("rguess", C.c_double), ("rfinal", C.c_double),
-                ("sweeps", C.c_int), ("used_guess", C.c_int)]
+                ("sweeps", C.c_int), ("used_guess", C.c_int), ("alpha", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -48,7 +48,7 @@ struct fpsi_ctx {
   double prof_ms[FPSI_K_NGROUPS] = {0};
   long prof_n[FPSI_K_NGROUPS] = {0};
   // linear residual scratch
-  double *lr_parts = nullptr, *lr_bsum = nullptr, *lr_out = nullptr, *Xj = nullptr;
+  double *lr_parts = nullptr, *lr_bsum = nullptr, *lr_out = nullptr, *Xj = nullptr, *Dg = nullptr;
   // learned guess: graph and geometry (new cell order), buffers, samples
   std::vector<int> mlp_nb_off, mlp_nb_idx;
   std::vector<float> mlp_geo;
@@ -592,7 +592,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     return rc;
   }
   if ((rc = dalloc(c, &c->lr_parts, 5 * (size_t)norm_parts(g) + 5)) || (rc = dalloc(c, &c->lr_bsum, 8)) ||
-      (rc = dalloc(c, &c->lr_out, 8)) || (rc = dalloc(c, &c->Xj, 5 * ((size_t)nc + nper + 1)))) {
+      (rc = dalloc(c, &c->lr_out, 8)) || (rc = dalloc(c, &c->Xj, 5 * ((size_t)nc + nper + 1))) ||
+      (rc = dalloc(c, &c->Dg, 5 * ((size_t)nc + nper + 1)))) {
     *out = c;
     return rc;
   }
@@ -610,6 +611,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   s.B = tmp;
   HIPCHK(c, hipMemsetAsync(s.vflux, 0, 4 * (size_t)nf * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->Dg, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.lev_cnt, 0, (size_t)nlev * 32 * sizeof(int), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!lsgs_cells.empty()) {
@@ -1087,6 +1089,7 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
 // linear residual, tolerance-driven solve, learned guess
 // ---------------------------------------------------------------------------
 static int linres(fpsi_ctx *c, double *out) {
+  launch_ghost(c->g, c->s, c->stream);  // the periodic couplings read current ghosts
   launch_linres(c->g, c->fp32, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
   double v = 0;
   HIPCHK(c, hipGetLastError());
@@ -1147,6 +1150,18 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
     const long n = 5 * ((long)c->nc + c->nper + 1);
     launch_copy_d(n, c->s.X, c->Xj, c->stream);
     if (int rc = mlp_guess(c)) return rc;
+    // step along the learned correction D = X_guess - X0 by the alpha that
+    // minimises |B - A (X0 + alpha D)| in the residual norm: never worse
+    // than the plain start
+    launch_sub_d(5 * (long)c->nc, c->s.X, c->Xj, c->Dg, c->stream);
+    launch_ghost_of(c->g, c->Dg, c->stream);
+    launch_linres_pair(c->g, c->fp32, c->s, c->Xj, c->Dg, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
+    double h3[3] = {0, 0, 0};
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h3, c->lr_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s.alpha = h3[2] > 0 ? h3[1] / h3[2] : 0.0;
+    launch_axpy_d(5 * (long)c->nc, c->Xj, s.alpha, c->Dg, c->s.X, c->stream);
     if (int rc = linres(c, &s.rguess)) return rc;
     if (s.rguess < s.r0) {
       s.used_guess = 1;

@@ -1307,6 +1307,101 @@ __global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Weighted inner products of r = B - A X0 and q = A D (same operator and
+// per-component weights 1/rms(B_k)^2 as k_linres): part[3 * block + {0,1,2}]
+// = block sums of r.r, r.q, q.q.  Both vectors need current ghost copies.
+template <typename F>
+__device__ __forceinline__ void apply_a(const DevGrid &g, const F *__restrict__ off, const F *__restrict__ lu,
+                                        int pk, int c, const double *__restrict__ V, double out[5]) {
+  double x[5], y[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[k] = V[(size_t)c * 5 + k];
+  F a[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = i; j < 5; ++j) s += a[i + 5 * j] * x[j];
+    y[i] = s;
+  }
+  double z[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double s = y[i];
+#pragma unroll
+    for (int j = 0; j < i; ++j) s += a[i + 5 * j] * y[j];
+    z[i] = s;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) out[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int p = (pk >> (3 * i)) & 7;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k == p) out[k] += z[i];
+  }
+  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+  for (int j = m0; j < m1; ++j) {
+    const F *M = off + (size_t)j * 25;
+    const double *xn = V + (size_t)g.m_nbr[j] * 5;
+#pragma unroll
+    for (int jj = 0; jj < 5; ++jj) {
+      const double xv = xn[jj];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) out[i] += M[i + 5 * jj] * xv;
+    }
+  }
+}
+
+template <typename F>
+__global__ void k_linres_pair(DevGrid g, const double *__restrict__ bsum, const F *__restrict__ off,
+                              const F *__restrict__ lu, const int *__restrict__ piv, const F *__restrict__ B,
+                              const double *__restrict__ X0, const double *__restrict__ D,
+                              double *__restrict__ part) {
+  __shared__ double sh[TPB][3];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rq = 0.0, qq = 0.0;
+  if (c < g.nc) {
+    const int pk = piv[c];
+    double ax[5], ad[5];
+    apply_a<F>(g, off, lu, pk, c, X0, ax);
+    apply_a<F>(g, off, lu, pk, c, D, ad);
+    const double ncd = (double)g.nc;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double w = 1.0 / (bsum[k] / ncd + 1e-300);
+      const double r = (double)B[(size_t)c * 5 + k] - ax[k];
+      rr += w * r * r;
+      rq += w * r * ad[k];
+      qq += w * ad[k] * ad[k];
+    }
+  }
+  sh[threadIdx.x][0] = rr; sh[threadIdx.x][1] = rq; sh[threadIdx.x][2] = qq;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + w][k];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[(size_t)blockIdx.x * 3 + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+// d = a - b
+__global__ void k_sub_d(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ d) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = a[i] - b[i];
+}
+// x = a + alpha d
+__global__ void k_axpy_d(long n, const double *__restrict__ a, double alpha, const double *__restrict__ d,
+                         double *__restrict__ x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = a[i] + alpha * d[i];
+}
+
 __global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -1358,6 +1453,32 @@ void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol,
                      (const double *)s.xt, s.lparts);
   hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
                      reltol);
+}
+
+void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
+                        double *bsum, double *out, hipStream_t st) {
+  const int nb = nblk(g.nc);
+  if (fp32) {
+    hipLaunchKernelGGL(k_b_stats<float>, dim3(nb), dim3(TPB), 0, st, g.nc, (const float *)s.B, parts);
+    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+    hipLaunchKernelGGL(k_linres_pair<float>, dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
+                       (const float *)s.off, (const float *)s.lu, s.piv, (const float *)s.B, X0, D, parts);
+  } else {
+    hipLaunchKernelGGL(k_b_stats<double>, dim3(nb), dim3(TPB), 0, st, g.nc, (const double *)s.B, parts);
+    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+    hipLaunchKernelGGL(k_linres_pair<double>, dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
+                       (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, X0, D, parts);
+  }
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 3, parts, out);
+}
+void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream_t st) {
+  hipLaunchKernelGGL(k_sub_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, a, b, d);
+}
+void launch_axpy_d(long n, const double *a, double alpha, const double *d, double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, a, alpha, d, x);
+}
+void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st) {
+  if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, V);
 }
 
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st) {

@@ -220,6 +220,7 @@ typedef struct {
   double rfinal;   /* residual after the sweeps */
   int sweeps;      /* symmetric sweeps run */
   int used_guess;  /* 1 if the guess beat the Jacobi start and was kept */
+  double alpha;    /* step taken along the learned correction (X0 + alpha (guess - X0)) */
 } fpsi_lin_stats;
 
 /* SGS from the Jacobi start, or from the learned guess when use_guess is set

@@ -60,13 +60,19 @@ def test_training_reduces_loss_and_guess_saves_sweeps():
     sw_learn = sum(l["sweeps"] for l in learned[0])
     print(f"sweeps to 1e-3: plain {sw_plain}, learned {sw_learn}; "
           f"guess kept {[l['used_guess'] for l in learned[0]]}, r_guess/r0 "
-          f"{[round(l['rguess'] / l['r0'], 3) for l in learned[0]]}")
+          f"{[round(l['rguess'] / l['r0'], 3) for l in learned[0]]}, alpha "
+          f"{[round(l['alpha'], 3) for l in learned[0]]}")
+    for l in learned[0]:
+        assert l["rguess"] <= l["r0"] * (1 + 1e-12)
     for l in plain[0] + learned[0]:
         assert l["rfinal"] <= 1e-3 * l["r0"] or l["sweeps"] == 60
     assert sw_learn <= sw_plain
 
 
-def test_bad_guess_falls_back_to_plain_start():
+def test_bad_guess_is_never_worse_than_plain_start():
+    """A guess far from the solution: the step along it is scaled to the
+    residual-minimising alpha, so the start is never worse than the plain
+    one; when it is not better the plain start is kept bit for bit."""
     g, s = _solver(6)
     s.mlp_init(seed=1)
     s.assemble()
@@ -80,5 +86,8 @@ def test_bad_guess_falls_back_to_plain_start():
     s.assemble()
     st0 = s.linear_solve_tol(1e-3, 40, use_guess=False)
     print(f"\nbad guess: {st}; plain: {st0}")
-    assert st["used_guess"] == 0 and st["rguess"] >= st["r0"]
-    assert st["sweeps"] == st0["sweeps"] and st["rfinal"] == st0["rfinal"]
+    assert st["rguess"] <= st["r0"] * (1 + 1e-12)
+    assert abs(st["alpha"]) < 1e-2
+    assert st["rfinal"] <= 1e-3 * st["r0"]
+    if not st["used_guess"]:
+        assert st["sweeps"] == st0["sweeps"] and st["rfinal"] == st0["rfinal"]
