@@ -34,7 +34,8 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read", "fpsi_linear_residual",
            "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
-           "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats"]
+           "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats", "fpsi_set_wall_distance",
+           "fpsi_sa_get_system", "fpsi_sa_solve"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -53,7 +54,8 @@ class Params(C.Structure):
                 ("sgs_iter", C.c_int), ("precision", C.c_int), ("minPg", C.c_double),
                 ("sweep_mode", C.c_int), ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double),
                 ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double), ("Minf", C.c_double),
-                ("etaT", C.c_double), ("PLimPC", C.c_double)]
+                ("etaT", C.c_double), ("PLimPC", C.c_double), ("turb", C.c_int), ("Prt", C.c_double),
+                ("nut_bc", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -145,6 +147,9 @@ def make_params(prm: dict) -> Params:
     s.Minf = prm.get("Minf", 1.0)
     s.etaT = prm.get("etaT", 0.25)
     s.PLimPC = prm.get("PLimPC", 10.0)
+    s.turb = {"none": 0, "sa": 1}[prm.get("turb", "none")]
+    s.Prt = prm.get("Prt", 0.9)
+    s.nut_bc = prm.get("nut_bc", 5e-9)
     return s
 
 
@@ -176,6 +181,9 @@ class Solver:
         self.prm = None
         if prm is not None:
             self.set_params(prm)
+        dist = getattr(grid, "meta", {}).get("dist")
+        if dist is not None:
+            self.set_wall_distance(dist)
 
     # -- plumbing
     def _chk(self, rc, what):
@@ -210,15 +218,38 @@ class Solver:
         p = make_params(prm)
         self._chk(self.L.fpsi_set_params(self.h, C.byref(p)), "fpsi_set_params")
 
+    @property
+    def nvar(self):
+        return 6 if self.prm and self.prm.get("turb", "none") == "sa" else 5
+
     def set_state(self, q):
         q = np.ascontiguousarray(q, np.float64)
-        assert q.shape == (5, self.grid.nc)
+        assert q.shape == (self.nvar, self.grid.nc)
         self._chk(self.L.fpsi_set_state(self.h, _ptr(q)), "fpsi_set_state")
 
     def get_state(self):
-        q = np.zeros((5, self.grid.nc))
+        q = np.zeros((self.nvar, self.grid.nc))
         self._chk(self.L.fpsi_get_state(self.h, _ptr(q)), "fpsi_get_state")
         return q
+
+    def set_wall_distance(self, dist):
+        d = np.ascontiguousarray(dist, np.float64)
+        self._keep["dist"] = d
+        self._chk(self.L.fpsi_set_wall_distance(self.h, _ptr(d)), "fpsi_set_wall_distance")
+
+    def sa_get_system(self):
+        nc, nf = self.grid.nc, self.grid.nf
+        out = {k: np.zeros(n) for k, n in (("src", nc), ("fjp", nf), ("fjm", nf), ("D", nc), ("B", nc),
+                                            ("tmu", nc))}
+        self._chk(self.L.fpsi_sa_get_system(self.h, _ptr(out["src"]), _ptr(out["fjp"]), _ptr(out["fjm"]),
+                                            _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"])),
+                  "fpsi_sa_get_system")
+        return out
+
+    def sa_solve(self, nsweeps):
+        dX = np.zeros(self.grid.nc)
+        self._chk(self.L.fpsi_sa_solve(self.h, int(nsweeps), _ptr(dX)), "fpsi_sa_solve")
+        return dX
 
     def residual(self, want_faces=True):
         nc, nf = self.grid.nc, self.grid.nf

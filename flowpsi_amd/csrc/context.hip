@@ -67,6 +67,9 @@ struct fpsi_ctx {
   double lsgs_w = 0, lsgs_resid = 0;
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
   bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
+  // Spalart-Allmaras: buffers allocated on first selection, wall distance set by the caller
+  bool sa_alloc = false, have_dist = false;
+  int nvar() const { return prm.turb == FPSI_TURB_SA ? 6 : 5; }
   // tiled sweeps (none when the grid admits no acyclic tiling)
   TileGrid tg{};
   bool have_tiles = false;
@@ -579,8 +582,9 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   }
   DevState &s = c->s;
   const size_t nbb = c->nb;
-  if ((rc = dalloc(c, &s.q, 5 * (size_t)nc)) || (rc = dalloc(c, &s.qf, 5 * nbb)) ||
-      (rc = dalloc(c, &s.grad, 15 * (size_t)nc)) || (rc = dalloc(c, &s.phi, 5 * (size_t)nc)) ||
+  // state, face states, gradients and limiters hold a 6th variable (nuTilde) for turbulent runs
+  if ((rc = dalloc(c, &s.q, 6 * (size_t)nc)) || (rc = dalloc(c, &s.qf, 6 * nbb)) ||
+      (rc = dalloc(c, &s.grad, 18 * (size_t)nc)) || (rc = dalloc(c, &s.phi, 6 * (size_t)nc)) ||
       (rc = dalloc(c, &s.iflux, 5 * (size_t)nf)) || (rc = dalloc(c, &s.vflux, 4 * (size_t)nf)) ||
       (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
@@ -670,6 +674,8 @@ extern "C" int fpsi_get_info(fpsi_ctx *c, fpsi_info *info) {
   return 0;
 }
 
+static int sa_alloc(fpsi_ctx *c);
+
 extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (!c || !p) return FPSI_ERR_ARG;
   if (p->flux != FPSI_FLUX_HLLC && p->flux != FPSI_FLUX_ROE) return fail(c, FPSI_ERR_ARG, "unknown flux");
@@ -677,6 +683,11 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
   if (p->linear_solver != FPSI_SOLVER_SGS && p->linear_solver != FPSI_SOLVER_LSGS)
     return fail(c, FPSI_ERR_ARG, "unknown linear solver");
+  if (p->turb != FPSI_TURB_NONE && p->turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_ARG, "unknown turbulence model");
+  if (p->turb == FPSI_TURB_SA) {
+    if (!p->viscous) return fail(c, FPSI_ERR_ARG, "the Spalart-Allmaras model needs viscous = 1");
+    if (int rc = sa_alloc(c)) return rc;
+  }
   if (p->linear_solver == FPSI_SOLVER_LSGS) {
     if (p->lsgs_maxiter < 0) return fail(c, FPSI_ERR_ARG, "LSGSMaxIter < 0");
     if (!(p->lsgs_relax > 0.0)) return fail(c, FPSI_ERR_ARG, "LSGSRelaxation must be > 0");
@@ -707,6 +718,15 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     k.PLimPC = p->PLimPC > 0 ? p->PLimPC : 10.0;
     k.pre = ml2 < 1.0 ? 1 : 0;
   }
+  // Spa_All_param defaults (turbulence/SA/SA_param.h:25-34), Cp
+  // (src/solverEoS.loci:76-78), turbulentPrandtlNumber 0.9
+  // (src/fluxViscous.loci:268-270), inflow nu_t 5e-9 (SA_turb.loci:167-174)
+  k.turb = p->turb == FPSI_TURB_SA ? 1 : 0;
+  k.sa_cb1 = 0.1355; k.sa_k = 0.41; k.sa_cv1 = 7.1; k.sa_cb2 = 0.622;
+  k.sa_cw2 = 0.3; k.sa_cw3 = 2.0; k.sa_sigma = 2. / 3.;
+  k.Cp = p->Rtilde * p->gamma / (p->gamma - 1.0);
+  k.Prt = p->Prt > 0 ? p->Prt : 0.9;
+  k.nut_bc = p->nut_bc > 0 ? p->nut_bc : 5e-9;
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -716,8 +736,10 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
 extern "C" int fpsi_set_state(fpsi_ctx *c, const double *q) {
   if (!c || !q) return FPSI_ERR_ARG;
   const int nc = c->nc;
-  std::vector<double> h(5 * (size_t)nc);
-  for (int k = 0; k < 5; ++k)
+  if (!c->have_params) return fail(c, FPSI_ERR_STATE, "fpsi_set_params must precede fpsi_set_state");
+  const int nv = c->nvar();
+  std::vector<double> h((size_t)nv * nc);
+  for (int k = 0; k < nv; ++k)
     for (int ni = 0; ni < nc; ++ni) h[(size_t)k * nc + ni] = q[(size_t)k * nc + c->cperm[ni]];
   HIPCHK(c, hipMemcpyAsync(c->s.q, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -749,17 +771,37 @@ static int copy_faces_out(fpsi_ctx *c, const double *dsrc, int ncomp, double *ou
 extern "C" int fpsi_get_state(fpsi_ctx *c, double *q) {
   if (!c || !q) return FPSI_ERR_ARG;
   if (!c->have_state) return fail(c, FPSI_ERR_STATE, "no state set");
-  return copy_cells_out(c, c->s.q, 5, q);
+  return copy_cells_out(c, c->s.q, c->nvar(), q);
 }
 
 static int check_ready(fpsi_ctx *c) {
   if (!c->have_params) return fail(c, FPSI_ERR_STATE, "fpsi_set_params not called");
   if (!c->have_state) return fail(c, FPSI_ERR_STATE, "fpsi_set_state not called");
+  if (c->prm.turb == FPSI_TURB_SA && !c->have_dist)
+    return fail(c, FPSI_ERR_STATE, "the Spalart-Allmaras model needs fpsi_set_wall_distance");
+  return 0;
+}
+
+extern "C" int fpsi_set_wall_distance(fpsi_ctx *c, const double *dist) {
+  if (!c || !dist) return FPSI_ERR_ARG;
+  std::vector<double> h(c->nc);
+  for (int ni = 0; ni < c->nc; ++ni) {
+    h[ni] = dist[c->cperm[ni]];
+    if (!(h[ni] > 0)) return fail(c, FPSI_ERR_ARG, "wall distance must be positive");
+  }
+  if (!c->g.dist) {
+    double *d = nullptr;
+    if (int rc = dalloc(c, &d, (size_t)c->nc)) return rc;
+    c->g.dist = d;
+  }
+  HIPCHK(c, hipMemcpy((void *)c->g.dist, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->have_dist = true;
   return 0;
 }
 
 static void enqueue_residual(fpsi_ctx *c) {
   hipEvent_t e = prof_begin(c);
+  launch_tmu(c->g, c->kp, c->s, c->stream);
   launch_bface(c->g, c->kp, c->s, c->stream);
   launch_grad(c->g, c->kp, c->s, c->stream);
   launch_eta(c->g, c->kp, c->s, c->stream);
@@ -810,7 +852,78 @@ static void enqueue_assemble(fpsi_ctx *c) {
   prof_end(c, FPSI_K_FACE_JAC, e);
   e = prof_begin(c);
   launch_cell_assemble(c->g, c->kp, c->fp32, c->lsgs() ? 1 : 0, c->s, c->stream);
+  if (c->kp.turb) launch_sa_assemble(c->g, c->kp, c->s, c->stream);
   prof_end(c, FPSI_K_ASSEMBLE, e);
+}
+
+// scalar SGS sweeps of the nuTilde system (src/linearSolverSGS.loci:333-500)
+static void enqueue_sa_sweeps(fpsi_ctx *c, int nsweeps) {
+  const int nlev = (int)c->lev_off.size() - 1;
+  for (int it = 0; it < nsweeps; ++it) {
+    launch_ghost1(c->g, c->s.sX, c->stream);
+    for (int l = 0; l < nlev; ++l) launch_sa_level(c->g, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+    launch_ghost1(c->g, c->s.sX, c->stream);
+    for (int l = nlev - 1; l >= 0; --l)
+      launch_sa_level(c->g, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+  }
+}
+
+static int sa_alloc(fpsi_ctx *c) {
+  if (c->sa_alloc) return 0;
+  DevState &s = c->s;
+  const size_t nc = c->nc, nf = c->nf;
+  int rc = 0;
+  if ((rc = dalloc(c, &s.tmu, nc)) || (rc = dalloc(c, &s.saf, 10 * nf)) ||
+      (rc = dalloc(c, &s.soff, (size_t)std::max(c->nnz, 1))) || (rc = dalloc(c, &s.ssrc, nc)) ||
+      (rc = dalloc(c, &s.sdinv, nc)) || (rc = dalloc(c, &s.sB, nc)) ||
+      (rc = dalloc(c, &s.sX, nc + c->nper + 1)))
+    return rc;
+  HIPCHK(c, hipMemset(s.sX, 0, (nc + c->nper + 1) * sizeof(double)));
+  HIPCHK(c, hipMemset(s.soff, 0, (size_t)std::max(c->nnz, 1) * sizeof(double)));
+  c->sa_alloc = true;
+  return 0;
+}
+
+extern "C" int fpsi_sa_get_system(fpsi_ctx *c, double *src, double *fjp, double *fjm, double *D, double *B,
+                                  double *tmu) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
+  const int nc = c->nc, nf = c->nf;
+  if (src) if (int rc = copy_cells_out(c, c->s.ssrc, 1, src)) return rc;
+  if (tmu) if (int rc = copy_cells_out(c, c->s.tmu, 1, tmu)) return rc;
+  if (D || B) {
+    std::vector<double> di(nc), b(nc);
+    HIPCHK(c, hipMemcpy(di.data(), c->s.sdinv, nc * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), c->s.sB, nc * sizeof(double), hipMemcpyDeviceToHost));
+    for (int ni = 0; ni < nc; ++ni) {
+      if (D) D[c->cperm[ni]] = 1. / di[ni];  // the kernel keeps 1/D
+      if (B) B[c->cperm[ni]] = b[ni];
+    }
+  }
+  if (fjp || fjm) {
+    std::vector<double> h(10 * (size_t)nf);
+    HIPCHK(c, hipMemcpy(h.data(), c->s.saf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nf; ++i) {
+      const int f = c->fperm[i];
+      const bool inner = i < c->nfi + c->nper;
+      if (fjp) fjp[f] = inner ? h[(size_t)3 * nf + i] : 0.0;
+      if (fjm) fjm[f] = inner ? h[(size_t)4 * nf + i] : 0.0;
+    }
+  }
+  return 0;
+}
+
+extern "C" int fpsi_sa_solve(fpsi_ctx *c, int nsweeps, double *dX) {
+  if (!c) return FPSI_ERR_ARG;
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
+  if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
+  enqueue_sa_sweeps(c, nsweeps);
+  HIPCHK(c, hipGetLastError());
+  if (dX) if (int rc = copy_cells_out(c, c->s.sX, 1, dX)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 extern "C" int fpsi_assemble(fpsi_ctx *c) {
@@ -1010,8 +1123,14 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
       hipStreamSynchronize(c->stream);
       return rc;
     }
+    if (c->kp.turb) {
+      hipEvent_t e2 = prof_begin(c);
+      enqueue_sa_sweeps(c, c->prm.sgs_iter);
+      prof_end(c, FPSI_K_SGS_SWEEP, e2);
+    }
     hipEvent_t e = prof_begin(c);
     launch_update(c->g, c->kp, c->s, c->stream);
+    if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
     prof_end(c, FPSI_K_UPDATE, e);
   }
   HIPCHK(c, hipGetLastError());

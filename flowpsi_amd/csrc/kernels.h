@@ -22,12 +22,20 @@ void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
                    hipStream_t st);
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st);
+// Spalart-Allmaras
+void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st);
+void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // weighted r.r, r.q, q.q of r = B - A X0, q = A D into out[0..2] (needs 3 * nblk(nc) parts)
 void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
                         double *bsum, double *out, hipStream_t st);
 void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream_t st);
 void launch_axpy_d(long n, const double *a, double alpha, const double *d, double *x, hipStream_t st);
+// periodic ghost entries V[nc + i] = V[partner cell], 5-wide rows
 void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st);
+// the same for a scalar per cell (V holds nc + nper entries)
+void launch_ghost1(const DevGrid &g, double *V, hipStream_t st);
 // line SGS
 int lsgs_parts(const DevGrid &g);
 void launch_lsgs_init(DevState &s, double w0, hipStream_t st);

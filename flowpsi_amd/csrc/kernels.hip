@@ -72,9 +72,17 @@ __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *
   qf[2 * nb + b] = o.u.y;
   qf[3 * nb + b] = o.u.z;
   qf[4 * nb + b] = o.pg;
+  if (p.turb) {  // nuTilde_f: turbulence/SA/SA_turb.loci:181-183, src/solverScalarTransport.loci:423-463
+    const double Xi = q[5 * nc + c];
+    double Xb = Xi;
+    if (t == 2) Xb = 0.0;
+    else if (t == 1) Xb = (dot(n, o.u) - 0.0 >= -0.0) ? Xi : p.nut_bc;
+    qf[5 * nb + b] = Xb;
+  }
 }
 
 // neighbour across gather entry e of cell c: position and primitive values
+template <int NQ>
 __device__ __forceinline__ void stencil(const DevGrid &g, const double *__restrict__ q,
                                         const double *__restrict__ qf, int e, double *x, double *v) {
   const int nc = g.nc;
@@ -83,18 +91,18 @@ __device__ __forceinline__ void stencil(const DevGrid &g, const double *__restri
     const int o = e < 0 ? g.frc[f] : g.fcl[f];
     x[0] = g.ccx[o]; x[1] = g.ccy[o]; x[2] = g.ccz[o];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+    for (int k = 0; k < NQ; ++k) v[k] = q[k * nc + o];
   } else if (f < g.bnd_begin) {
     const int o = g.frc[f];
     const int pi = f - g.nfi;
     x[0] = g.ccx[o] + g.psx[pi]; x[1] = g.ccy[o] + g.psy[pi]; x[2] = g.ccz[o] + g.psz[pi];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+    for (int k = 0; k < NQ; ++k) v[k] = q[k * nc + o];
   } else {
     const int b = f - g.bnd_begin;
     x[0] = g.fcx[f]; x[1] = g.fcy[f]; x[2] = g.fcz[f];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = qf[k * g.nb + b];
+    for (int k = 0; k < NQ; ++k) v[k] = qf[k * g.nb + b];
   }
 }
 
@@ -102,26 +110,27 @@ __device__ __forceinline__ void stencil(const DevGrid &g, const double *__restri
 // least-squares gradient and limiter per cell (Loci FVM grads/limiters;
 // textbook unweighted least squares, Venkatakrishnan / Barth limiters)
 // ---------------------------------------------------------------------------
+template <int NQ>
 __global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                        double *__restrict__ grad, double *__restrict__ phi) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
   const double xc0 = g.ccx[c], xc1 = g.ccy[c], xc2 = g.ccz[c];
-  double qc[5], b[5][3], qmin[5], qmax[5];
+  double qc[NQ], b[NQ][3], qmin[NQ], qmax[NQ];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     qc[k] = q[k * nc + c];
     b[k][0] = b[k][1] = b[k][2] = 0.0;
     qmin[k] = qmax[k] = qc[k];
   }
   const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
   for (int j = j0; j < j1; ++j) {
-    double x[3], v[5];
-    stencil(g, q, qf, g.g_ent[j], x, v);
+    double x[3], v[NQ];
+    stencil<NQ>(g, q, qf, g.g_ent[j], x, v);
     const double dx = x[0] - xc0, dy = x[1] - xc1, dz = x[2] - xc2;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NQ; ++k) {
       const double dq = v[k] - qc[k];
       b[k][0] += dx * dq;
       b[k][1] += dy * dq;
@@ -132,9 +141,9 @@ __global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const dou
   }
   const double A0 = g.lsq[c], A1 = g.lsq[nc + c], A2 = g.lsq[2 * nc + c];
   const double A3 = g.lsq[3 * nc + c], A4 = g.lsq[4 * nc + c], A5 = g.lsq[5 * nc + c];
-  double gr[5][3];
+  double gr[NQ][3];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     gr[k][0] = A0 * b[k][0] + A1 * b[k][1] + A2 * b[k][2];
     gr[k][1] = A1 * b[k][0] + A3 * b[k][1] + A4 * b[k][2];
     gr[k][2] = A2 * b[k][0] + A4 * b[k][1] + A5 * b[k][2];
@@ -142,10 +151,12 @@ __global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const dou
     grad[(3 * k + 1) * nc + c] = gr[k][1];
     grad[(3 * k + 2) * nc + c] = gr[k][2];
   }
-  double ph[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+  double ph[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) ph[k] = 1.0;
   if (p.limiter == 2) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) ph[k] = 0.0;
+    for (int k = 0; k < NQ; ++k) ph[k] = 0.0;
   } else if (p.limiter == 1 || p.limiter == 3) {
     const double eps2 = p.Kl * p.Kl * p.Kl * g.vol[c];
     for (int j = j0; j < j1; ++j) {
@@ -153,7 +164,7 @@ __global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const dou
       const int f = e < 0 ? ~e : e;
       const double dx = g.fcx[f] - xc0, dy = g.fcy[f] - xc1, dz = g.fcz[f] - xc2;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < NQ; ++k) {
         const double qdif = gr[k][0] * dx + gr[k][1] * dy + gr[k][2] * dz;
         double lim;
         if (p.limiter == 1) {
@@ -168,10 +179,10 @@ __global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const dou
       }
     }
 #pragma unroll
-    for (int k = 0; k < 5; ++k) ph[k] = dmax(0.0, dmin(1.0, ph[k]));
+    for (int k = 0; k < NQ; ++k) ph[k] = dmax(0.0, dmin(1.0, ph[k]));
   }
 #pragma unroll
-  for (int k = 0; k < 5; ++k) phi[k * nc + c] = ph[k];
+  for (int k = 0; k < NQ; ++k) phi[k * nc + c] = ph[k];
 }
 
 // limited reconstruction with the floors of leftsP(temperature,Zero) and
@@ -223,10 +234,30 @@ __device__ __forceinline__ void face_grad_bnd(const DevGrid &g, const double *__
 }
 
 // viscous face quantities (src/fluxViscous.loci:121-133,145-151,184-197,217-238,272-297)
+// tmu_f (src/fluxViscous.loci:192-201): volume-weighted interior average,
+// the interior cell on boundary faces, zero on no-slip walls
+__device__ __forceinline__ double tmu_face(const DevGrid &g, const double *__restrict__ tmu, int f) {
+  const int c = g.fcl[f];
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    const double vl = g.vol[c], vr = g.vol[o];
+    return (tmu[o] * vl + tmu[c] * vr) / (vr + vl);
+  }
+  return g.fbt[f] == 2 ? 0.0 : tmu[c];
+}
+// muTotal_f, kconductTotal (src/fluxViscous.loci:182-189,275-278)
+__device__ __forceinline__ void add_tmu(const DevGrid &g, const Prm &p, const double *__restrict__ tmu, int f,
+                                        double *mu_f, double *k_f) {
+  if (!p.turb) return;
+  const double tmf = tmu_face(g, tmu, f);
+  *k_f = *k_f + tmf * p.Cp / p.Prt;
+  *mu_f = *mu_f + tmf;
+}
+
 __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double *__restrict__ q,
                                     const double *__restrict__ qf, const double *__restrict__ grad,
-                                    int f, double *mu_f, double *k_f, V3 *uf, double *vf,
-                                    bool need_flux) {
+                                    const double *__restrict__ tmu, int f, double *mu_f, double *k_f, V3 *uf,
+                                    double *vf, bool need_flux) {
   const int nc = g.nc;
   const int c = g.fcl[f];
   const V3 an = {g.fnx[f], g.fny[f], g.fnz[f]};
@@ -245,6 +276,7 @@ __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double
     const double kl = suth_k(p, q[c]), kr = suth_k(p, q[o]);
     *mu_f = (vl * mur + vr * mul) / (vl + vr);
     *k_f = (vl * kr + vr * kl) / (vl + vr);
+    add_tmu(g, p, tmu, f, mu_f, k_f);
     const double factor = 1. / (vl + vr);
     uf->x = factor * (vl * q[nc + o] + vr * q[nc + c]);
     uf->y = factor * (vl * q[2 * nc + o] + vr * q[2 * nc + c]);
@@ -260,6 +292,7 @@ __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double
     const double Tf = qf[b];
     *mu_f = suth_mu(p, Tf);
     *k_f = suth_k(p, Tf);
+    add_tmu(g, p, tmu, f, mu_f, k_f);
     uf->x = qf[nb + b];
     uf->y = qf[2 * nb + b];
     uf->z = qf[3 * nb + b];
@@ -285,7 +318,8 @@ __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double
 // passes of the maximum over the face neighbours (Eta_ll, Eta_p).  Maxima
 // are exact, so the gather order is free.
 // ---------------------------------------------------------------------------
-__global__ void k_eta_l(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ el) {
+__global__ void k_eta_l(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ tmu,
+                        double *__restrict__ el) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -305,9 +339,13 @@ __global__ void k_eta_l(DevGrid g, Prm p, const double *__restrict__ q, double *
     mref = dmax(mref, p.PLimPC * dP / (rho * a * a));
     if (p.viscous) {
       const double area = dmax(g.farea[f], 1e-30);
-      const double mum = dmax(mu, suth_mu(p, To));
       const double rhoo = (pgo + p.Pamb) / (p.Rt * To);
       const double rhom = 0.5 * (rho + rhoo);
+      double mum = dmax(mu, suth_mu(p, To));
+      if (p.turb) {  // src/fluxInviscid.loci:81-96: the cl side joins max(tmu_cl, tmu_cl)
+        const double tl = e < 0 ? tmu[c] : tmu[o], tr = e < 0 ? tmu[o] : tmu[c];
+        mum = mum + (e < 0 ? dmax(tl, tl) : dmax(tl, tr));
+      }
       const double nu = mum / rhom;
       mref = dmax(mref, nu / ((g.vol[c] / area) * a));
     }
@@ -336,8 +374,8 @@ __global__ void k_eta_max(DevGrid g, const double *__restrict__ src, double *__r
 // ---------------------------------------------------------------------------
 __global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                             const double *__restrict__ grad, const double *__restrict__ phi,
-                            const double *__restrict__ etap, double *__restrict__ iflux,
-                            double *__restrict__ vflux, double *__restrict__ maxev) {
+                            const double *__restrict__ etap, const double *__restrict__ tmu,
+                            double *__restrict__ iflux, double *__restrict__ vflux, double *__restrict__ maxev) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= g.nf) return;
   const int nc = g.nc, nf = g.nf, nb = g.nb;
@@ -411,7 +449,7 @@ __global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, cons
   if (p.viscous) {
     double mu_f, k_f, vf[4];
     V3 uf;
-    viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, true);
+    viscous_face(g, p, q, qf, grad, tmu, f, &mu_f, &k_f, &uf, vf, true);
 #pragma unroll
     for (int k = 0; k < 4; ++k) vflux[k * nf + f] = vf[k];
   }
@@ -570,7 +608,8 @@ template <typename F>
 __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                            const double *__restrict__ grad, const double *__restrict__ phi,
                            const double *__restrict__ vflux, const double *__restrict__ etap,
-                           F *__restrict__ off, F *__restrict__ pblk, F *__restrict__ bblk) {
+                           const double *__restrict__ tmu, F *__restrict__ off, F *__restrict__ pblk,
+                           F *__restrict__ bblk) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= g.nf) return;
   const int nc = g.nc, nf = g.nf, nb = g.nb;
@@ -597,7 +636,7 @@ __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const
     double mu_f = 0, k_f = 0, vf[4] = {0, 0, 0, 0}, dist = 1;
     V3 uf = {0, 0, 0};
     if (p.viscous) {
-      viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, false);
+      viscous_face(g, p, q, qf, grad, tmu, f, &mu_f, &k_f, &uf, vf, false);
 #pragma unroll
       for (int k = 0; k < 4; ++k) vf[k] = vflux[k * nf + f];
       const V3 dv = {xr0 - xl0, xr1 - xl1, xr2 - xl2};
@@ -666,7 +705,7 @@ __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const
     if (p.viscous) {
       double mu_f, k_f, vf[4];
       V3 uf;
-      viscous_face(g, p, q, qf, grad, f, &mu_f, &k_f, &uf, vf, false);
+      viscous_face(g, p, q, qf, grad, tmu, f, &mu_f, &k_f, &uf, vf, false);
 #pragma unroll
       for (int k = 0; k < 4; ++k) vf[k] = vflux[k * nf + f];
       const V3 dv = {xf0 - xl0, xf1 - xl1, xf2 - xl2};
@@ -965,6 +1004,12 @@ __global__ void k_ghost(DevGrid g, double *__restrict__ X) {
   for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + pi) * 5 + k] = X[(size_t)src * 5 + k];
 }
 
+__global__ void k_ghost1(DevGrid g, double *__restrict__ X) {
+  const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pi >= g.nper) return;
+  X[g.nc + pi] = X[g.frc[g.nfi + pi]];
+}
+
 // ---------------------------------------------------------------------------
 // line SGS, src/linearSolverLSGS.loci:1692-1954.  The line search of that
 // file links cells only through duplicate couples, which it removes first
@@ -1095,6 +1140,300 @@ __global__ void k_lsgs_check(int nparts, const double *__restrict__ part, double
   }
 }
 
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras nuTilde equation (turbulence/SA/SA_turb.loci, baseline
+// model, standard production; src/solverScalarTransport.loci), segregated
+// as in the reference: its own scalar system and scalar SGS per Newton step.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sa_fv1(const Prm &p, double x) {
+  return pow(x, 3.0) / (pow(x, 3.0) + pow(p.sa_cv1, 3.0));
+}
+
+// eddy viscosity tmu = nuTilde fv1 rho (:93-107)
+__global__ void k_tmu(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ tmu) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const double T = q[c], pg = q[4 * nc + c], nut = q[5 * nc + c];
+  const double rho = (pg + p.Pamb) / (p.Rt * T);
+  const double x = rho * nut / suth_mu(p, T);
+  tmu[c] = nut * sa_fv1(p, x) * rho;
+}
+
+// production - destruction and the destruction Jacobian of one cell
+// (:109-165, 258-315, 347-351)
+__device__ __forceinline__ void sa_core(const Prm &p, double T, double pg, double nut, double vortMag, double y,
+                                        double vol, double *srcv, double *srcJ) {
+  const double rho = (pg + p.Pamb) / (p.Rt * T);
+  const double mu = suth_mu(p, T);
+  const double xcoeff = rho * nut / mu;
+  const double fv1 = sa_fv1(p, xcoeff);
+  const double fv2 = 1.0 - (xcoeff / (1.0 + xcoeff * fv1));
+  const double k = p.sa_k;
+  const double denom = k * k * y * y;
+  double S_tilda = 0.0;
+  S_tilda += vortMag;
+  S_tilda += nut * fv2 / denom;
+  const double S_t = dmax(S_tilda, 0.3 * vortMag);
+  const double rclip = 10;
+  const double r = (S_t < nut / (rclip * denom)) ? rclip : nut / (S_t * denom);
+  const double r2 = r * r, r6 = r2 * r2 * r2;
+  const double gc = r + p.sa_cw2 * (r6 - r);
+  const double cw36 = pow(p.sa_cw3, 6.0);
+  const double g2 = gc * gc, g6 = g2 * g2 * g2;
+  const double fw = gc * pow((1.0 + cw36) / (g6 + cw36), 1.0 / 6.0);
+  const double cw1 = (p.sa_cb1) / (k * k) + (1.0 + (p.sa_cb2)) / (p.sa_sigma);
+  double sv = 0.0;
+  sv += rho * nut * S_t * p.sa_cb1 * vol;
+  const double t = nut / y;
+  sv -= rho * fw * cw1 * t * t * vol;
+  double rp;
+  const double rclip2 = 9e6;
+  if (S_t < nut / (rclip2 * denom)) {
+    rp = 0;
+  } else {
+    const double chi = xcoeff, chiP = 1. / mu;
+    const double chi3 = chi * chi * chi, chi4 = chi3 * chi, chi6 = chi3 * chi3;
+    const double cv1 = p.sa_cv1, cv13 = cv1 * cv1 * cv1;
+    const double fv2p = chi * (4 * (chi3 * chiP) / (chi3 + cv13) - 3 * (chi6 * chiP) / pow(chi3 + cv13, 2.0)) /
+                            pow(chi4 / (chi3 + cv13) + 1, 2.0) -
+                        chiP / (chi4 / (chi3 + cv13) + 1);
+    const double Stp = (fv2 / rho + nut * fv2p) / (denom);
+    rp = 1. / (rho * denom * S_t) - nut * Stp / (denom * S_t * S_t);
+  }
+  const double gp = rp + p.sa_cw2 * (6. * pow(r, 5.0) * rp - rp);
+  const double term = pow((cw36 + 1.) / (cw36 + g6), 1. / 6.);
+  const double term2 = term * term, term5 = term2 * term2 * term;
+  const double rterm5 = 1. / term5, rsum = 1. / (g6 + cw36);
+  const double fwp = gp * term - gp * ((cw36 + 1) * g6) * rsum * rsum * rterm5;
+  *srcv = sv;
+  *srcJ = -(vol * cw1 * rho * nut * (rho * nut * fwp + 2. * fw) / (y * y));
+}
+
+// Per-face terms of the nuTilde system, saf[k][nf]:
+//   0 convective flux mdot max(X_upwind, 0), 1 diff1_f, 2 div_nuTilde_f,
+//   3 fjp, 4 fjm (interior/periodic), 5 srcJ_L, 6 srcJ_U, 7/8 cb2 diagonal
+//   parts of cl / cr, 9 boundary srcJ of the interior cell;
+// the scalar rows get U = fjm in row cl and L = -fjp in row cr
+// (src/solverScalarTransport.loci:50-135,520-535; SA_turb.loci:354-482,737-745).
+__global__ void k_sa_face(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+                          const double *__restrict__ grad, const double *__restrict__ phi,
+                          const double *__restrict__ iflux, double *__restrict__ saf, double *__restrict__ soff) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.nf) return;
+  const int nc = g.nc, nf = g.nf, nb = g.nb;
+  const int c = g.fcl[f];
+  const double an[3] = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double xf[3] = {g.fcx[f], g.fcy[f], g.fcz[f]};
+  const double xl[3] = {g.ccx[c], g.ccy[c], g.ccz[c]};
+  const double area = g.farea[f];
+  const double mdot = iflux[f];
+  const double sig = p.sa_sigma, cb2 = p.sa_cb2;
+  const double factorx = 2, factory = .5;
+  const double rhol = (q[4 * nc + c] + p.Pamb) / (p.Rt * q[c]);
+  const double nul = q[5 * nc + c];
+  auto recon6 = [&](int cc, const double *xc) {
+    const double d0 = xf[0] - xc[0], d1 = xf[1] - xc[1], d2 = xf[2] - xc[2];
+    const double gd = grad[15 * nc + cc] * d0 + grad[16 * nc + cc] * d1 + grad[17 * nc + cc] * d2;
+    return q[5 * nc + cc] + phi[5 * nc + cc] * gd;
+  };
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    double xr[3] = {g.ccx[o], g.ccy[o], g.ccz[o]};
+    if (f >= g.nfi) {
+      const int pi = f - g.nfi;
+      xr[0] += g.psx[pi]; xr[1] += g.psy[pi]; xr[2] += g.psz[pi];
+    }
+    const double XL = recon6(c, xl), XR = recon6(o, xr);
+    const double Xval = (mdot < 0.0) ? XR : XL;
+    v[0] = mdot * dmax(Xval, 0.0);
+    const double vl = g.vol[c], vr = g.vol[o];
+    const double nur = q[5 * nc + o];
+    const double vrvl = 1.0 / (vr + vl);
+    const double nuf = (vr * nul + vl * nur) * vrvl;
+    const double dvec[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+    const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+    double gfx[3];
+    face_grad_int(g, q, grad, c, o, 5, dvec, ddsq, gfx);
+    const double rhor = (q[4 * nc + o] + p.Pamb) / (p.Rt * q[o]);
+    const double factor = 1. / (vl + vr);
+    const double rho_f = factor * (vl * rhor + vr * rhol);
+    const double mu_f = (vl * suth_mu(p, q[o]) + vr * suth_mu(p, q[c])) / (vl + vr);
+    const double gdn = gfx[0] * an[0] + gfx[1] * an[1] + gfx[2] * an[2];
+    const double d1 = gdn * area;
+    v[1] = (d1 * (mu_f / rho_f + (1. + cb2) * nuf)) / (sig);
+    v[2] = gdn * area;
+    double fjp = 0.0, fjm = 0.0;
+    fjp += dmax(mdot, 0.0);
+    fjm += dmin(mdot, 0.0);
+    const double rhom = .5 * (rhol + rhor);
+    const double C = (mu_f + (1. + cb2) * rhom * nuf) * area;
+    const double dd = dvec[0] * an[0] + dvec[1] * an[1] + dvec[2] * an[2];
+    fjp -= -C / (dd * sig);
+    fjm += -C / (dd * sig);
+    const double C2 = cb2 * area;
+    const double d2 = sig * dd;
+    v[6] = -factorx * (C2 / d2) * rhol * nul;
+    v[5] = -factorx * (C2 / d2) * rhor * nur;
+    v[7] = factorx * factory * (C2 / d2) * rhol * nul;
+    v[8] = factorx * factory * (C2 / d2) * rhor * nur;
+    fjp += v[5];
+    fjm -= v[6];
+    v[3] = fjp;
+    v[4] = fjm;
+    soff[g.slot_cl[f]] = fjm;
+    if (f < g.nfi) soff[g.slot_cr[f]] = -fjp;
+  } else {
+    const int b = f - g.bnd_begin;
+    const double XL = recon6(c, xl);
+    const double Xf = qf[5 * nb + b];
+    const double Xval = (mdot < 0.0) ? Xf : XL;
+    v[0] = mdot * dmax(Xval, 0.0);
+    const double Tf = qf[b];
+    const double rho_f = (qf[4 * nb + b] + p.Pamb) / (p.Rt * Tf);
+    const double mu_f = suth_mu(p, Tf);
+    const double dvec[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+    const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+    double gfx[3];
+    face_grad_bnd(g, q, grad, Xf, c, 5, dvec, ddsq, gfx);
+    const double gdn = gfx[0] * an[0] + gfx[1] * an[1] + gfx[2] * an[2];
+    const double d1 = gdn * area;
+    v[1] = (d1 * (mu_f / rho_f + (1. + cb2) * Xf)) / (sig);
+    v[2] = gdn * area;
+    double bj = 0.0;
+    bj -= dmax(mdot, 0.0);
+    if (g.fbt[f] != 3) {
+      const double C = (mu_f + (1. + cb2) * rho_f * Xf) * area;
+      const double dd = dvec[0] * an[0] + dvec[1] * an[1] + dvec[2] * an[2];
+      bj += -C / (dd * sig);
+      const double C2 = cb2 * area;
+      const double d2 = sig * dd;
+      bj += factorx * factory * (C2 / d2) * rhol * nul;
+    }
+    v[9] = bj;
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) saf[(size_t)k * nf + f] = v[k];
+}
+
+// Per-cell gather of the nuTilde system in a fixed order (convection and
+// continuity over lower, upper, boundary faces; stabilisation; SA sources;
+// diffusion and its divergence; the cb2 term), srcJ, D = -srcJ + rho V/dtau,
+// B = src, 1/D and the Jacobi start X = B/D (src/linearSolverSGS.loci:343-352).
+__global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ grad,
+                          const double *__restrict__ dtau, const double *__restrict__ iflux,
+                          const double *__restrict__ saf, double *__restrict__ ssrc, double *__restrict__ sdinv,
+                          double *__restrict__ sB, double *__restrict__ sX) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc, nf = g.nf;
+  const double rho = (q[4 * nc + c] + p.Pamb) / (p.Rt * q[c]);
+  const double X = q[5 * nc + c];
+  const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
+  double sv = 0.0, sJ = 0.0, rhodot = 0.0;
+  // g_ent order is lower faces (e >= 0), upper faces, boundary faces
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) {
+      sv += saf[e];
+      rhodot += iflux[e];
+    } else {
+      const int f = ~e;
+      sv -= saf[f];
+      rhodot -= iflux[f];
+    }
+  }
+  sv -= X * rhodot;
+  sJ += -rhodot;
+#define GU(i, jj) grad[(3 * (1 + (i)) + (jj)) * nc + c]
+  const double wx = GU(2, 1) - GU(1, 2), wy = GU(0, 2) - GU(2, 0), wz = GU(1, 0) - GU(0, 1);
+#undef GU
+  const double vortMag = sqrt(wx * wx + wy * wy + wz * wz);
+  double ssa, sdJ;
+  sa_core(p, q[c], q[4 * nc + c], X, vortMag, g.dist[c], g.vol[c], &ssa, &sdJ);
+  sv += ssa;
+  sJ += sdJ;
+  double div = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) {
+      sv -= rho * saf[(size_t)nf + e];
+      div -= saf[(size_t)2 * nf + e];
+    } else {
+      const int f = ~e;
+      sv += rho * saf[(size_t)nf + f];
+      div += saf[(size_t)2 * nf + f];
+    }
+  }
+  sv += -rho * X * div * p.sa_cb2 / p.sa_sigma;
+  // srcJ: cb2 diagonal parts (lower: cr part, upper: cl part), boundary terms,
+  // then -fjp over upper and +fjm over lower faces, then the L/U cb2 parts
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) sJ += saf[(size_t)8 * nf + e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e < 0 && ~e < g.bnd_begin) sJ += saf[(size_t)7 * nf + ~e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e < 0 && ~e >= g.bnd_begin) sJ += saf[(size_t)9 * nf + ~e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e < 0 && ~e < g.bnd_begin) sJ -= saf[(size_t)3 * nf + ~e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) sJ += saf[(size_t)4 * nf + e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e < 0 && ~e < g.bnd_begin) sJ += saf[(size_t)5 * nf + ~e];
+  }
+  for (int j = j0; j < j1; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) sJ += saf[(size_t)6 * nf + e];
+  }
+  ssrc[c] = sv;
+  const double sd = rho * (g.vol[c] / dtau[c]);
+  const double D = -sJ + sd;
+  const double rhs = 0.0 - sv;
+  const double b = -rhs;
+  const double dinv = 1. / D;
+  sB[c] = b;
+  sdinv[c] = dinv;
+  sX[c] = dinv * b;
+}
+
+// one level of a scalar SGS half-sweep, one lane per cell (the row holds
+// U = fjm for upper faces, then L = -fjp for lower faces)
+__global__ void k_sa_sgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi,
+                               const double *__restrict__ soff, const double *__restrict__ sdinv,
+                               const double *__restrict__ sB, double *__restrict__ sX) {
+  const int k = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= hi) return;
+  const int c = cells ? cells[k] : k;
+  double t = -sB[c];
+  for (int j = g.m_off[c]; j < g.m_off[c + 1]; ++j) t += sX[g.m_nbr[j]] * soff[j];
+  const double v = -t;
+  sX[c] = sdinv[c] * v;
+}
+
+// X_next = X + min(1, urelax X/(|dX| + 1e-15)) dX, floored at 1e-15
+// (src/solverScalarTransport.loci:400-408)
+__global__ void k_sa_update(DevGrid g, Prm p, const double *__restrict__ sX, double *__restrict__ q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const double Xc = q[5 * (size_t)g.nc + c], dX = sX[c];
+  double tscale = 1.0;
+  tscale = dmin(tscale, p.urelax * Xc / (fabs(dX) + 1e-15));
+  const double Xn = Xc + tscale * dX;
+  q[5 * (size_t)g.nc + c] = dmax(Xn, 1e-15);
+}
+
 // damped Newton update, src/solverNewton.loci:62-78
 __global__ void k_update(DevGrid g, double Pamb, const double *__restrict__ X, double *__restrict__ q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1120,17 +1459,18 @@ void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   if (g.nb) hipLaunchKernelGGL(k_bface, dim3(nblk(g.nb)), dim3(TPB), 0, st, g, p, s.q, s.qf);
 }
 void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_grad, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
+  if (p.turb) hipLaunchKernelGGL(k_grad<6>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
+  else hipLaunchKernelGGL(k_grad<5>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
 }
 void launch_eta(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   if (!p.pre) return;
-  hipLaunchKernelGGL(k_eta_l, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.eta);
+  hipLaunchKernelGGL(k_eta_l, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, (const double *)s.tmu, s.eta);
   hipLaunchKernelGGL(k_eta_max, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, (const double *)s.eta, s.eta_t);
   hipLaunchKernelGGL(k_eta_max, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, (const double *)s.eta_t, s.eta);
 }
 void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_face_flux, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi,
-                     (const double *)s.eta, s.iflux, s.vflux, s.maxev);
+                     (const double *)s.eta, (const double *)s.tmu, s.iflux, s.vflux, s.maxev);
 }
 int norm_parts(const DevGrid &g) { return nblk(g.nc); }
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st) {
@@ -1141,11 +1481,12 @@ void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hip
 void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st) {
   if (fp32)
     hipLaunchKernelGGL(k_face_jac<float>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
-                       s.phi, s.vflux, (const double *)s.eta, (float *)s.off, (float *)s.pblk, (float *)s.bblk);
+                       s.phi, s.vflux, (const double *)s.eta, (const double *)s.tmu, (float *)s.off,
+                       (float *)s.pblk, (float *)s.bblk);
   else
     hipLaunchKernelGGL(k_face_jac<double>, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad,
-                       s.phi, s.vflux, (const double *)s.eta, (double *)s.off, (double *)s.pblk,
-                       (double *)s.bblk);
+                       s.phi, s.vflux, (const double *)s.eta, (const double *)s.tmu, (double *)s.off,
+                       (double *)s.pblk, (double *)s.bblk);
 }
 template <typename F, bool LS>
 static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
@@ -1479,6 +1820,29 @@ void launch_axpy_d(long n, const double *a, double alpha, const double *d, doubl
 }
 void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, V);
+}
+void launch_ghost1(const DevGrid &g, double *V, hipStream_t st) {
+  if (g.nper) hipLaunchKernelGGL(k_ghost1, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, V);
+}
+
+void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  if (p.turb) hipLaunchKernelGGL(k_tmu, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q, s.tmu);
+}
+void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_face, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, (const double *)s.q,
+                     (const double *)s.qf, (const double *)s.grad, (const double *)s.phi, (const double *)s.iflux,
+                     s.saf, s.soff);
+  hipLaunchKernelGGL(k_sa_cell, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q,
+                     (const double *)s.grad, (const double *)s.dtau, (const double *)s.iflux,
+                     (const double *)s.saf, s.ssrc, s.sdinv, s.sB, s.sX);
+}
+void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st) {
+  if (hi <= lo) return;
+  hipLaunchKernelGGL(k_sa_sgs_level, dim3((hi - lo + TPB - 1) / TPB), dim3(TPB), 0, st, g, cells, lo, hi,
+                     (const double *)s.soff, (const double *)s.sdinv, (const double *)s.sB, s.sX);
+}
+void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.sX, s.q);
 }
 
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st) {

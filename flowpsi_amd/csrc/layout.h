@@ -42,6 +42,7 @@ struct DevGrid {
   // number of upper slots of row c; lnbr_pad is nbr_pad in that order.
   const int *m_nup;    // [nc]
   const int *lnbr_pad; // [nc][6]
+  const double *dist;  // [nc] wall distance (turbulent runs)
 };
 
 struct DevState {
@@ -66,6 +67,14 @@ struct DevState {
   int *lev_cnt;   // [nlev * 32] cells published per level (persistent sweep)
   double *eta;    // [nc] Eta_p (read only when Prm::pre)
   double *eta_t;  // [nc] Eta_l / Eta_ll scratch
+  // Spalart-Allmaras (allocated when selected): q holds nuTilde as its 6th row
+  double *tmu;    // [nc] eddy viscosity
+  double *saf;    // [10][nf] per-face terms of the nuTilde system
+  double *soff;   // [nnz] scalar row coefficients (U = fjm, L = -fjp)
+  double *ssrc;   // [nc] X_src
+  double *sdinv;  // [nc] 1 / D
+  double *sB;     // [nc]
+  double *sX;     // [nc + nper + 1] scalar solution with ghosts and the zero entry
   // line SGS (allocated when that solver is selected)
   double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
   double *xt;     // [nc][5] X after the forward sweep

@@ -20,6 +20,10 @@ struct Prm {
   // low-Mach preconditioning: pre = 0 when Mlim2 >= 1 (eta = 1 everywhere)
   int pre;
   double Mlim2, PLimPC;
+  // Spalart-Allmaras (turb = 1): constants, Cp, turbulent Prandtl number,
+  // nuTilde of inflow faces
+  int turb;
+  double sa_cb1, sa_k, sa_cv1, sa_cb2, sa_cw2, sa_cw3, sa_sigma, Cp, Prt, nut_bc;
 };
 
 struct V3 {

@@ -272,6 +272,21 @@ def channel_grid(nx=128, ny=128, nz=128, ratio=1.1, re=500.0, mach=0.2, T=300.0,
     return g
 
 
+def wall_distance(g: Grid) -> np.ndarray:
+    """dist_noslip: projected distance |(x_f - x_c) . n_f| to the no-slip face
+    whose centre is nearest the cell centre (src/solverTurbulence.loci:133-137,
+    the default "projected" distance function)."""
+    from scipy.spatial import cKDTree
+    w = np.flatnonzero(g.btype == WALL)
+    if len(w) == 0:
+        return np.full(g.nc, 1e30)
+    tree = cKDTree(g.fc[:, w].T)
+    _, idx = tree.query(g.cc.T)
+    f = w[idx]
+    d = g.fc[:, f] - g.cc
+    return np.abs(np.einsum("ij,ij->j", d, g.n[:, f]))
+
+
 def channel_state(g, T=300.0, p=101325.0, amp=0.02, seed=2):
     """Poiseuille u = U_c (1 - y^2/h^2) plus seeded N(0, amp U_c) velocity noise."""
     rng = np.random.default_rng(seed)

@@ -39,4 +39,20 @@ def channel(n=128, seed=2, nx=None, ny=None, nz=None, **over):
     return g, q, prm
 
 
+def channel_sa(n=32, seed=2, nx=None, ny=None, nz=None, nut_ratio=3.0, **over):
+    """The channel with the Spalart-Allmaras model: nuTilde = nut_ratio x the
+    laminar kinematic viscosity plus 10 % seeded noise, wall distance in
+    g.meta["dist"]."""
+    g, q5, prm = channel(n, seed, nx, ny, nz)
+    rng = __import__("numpy").random.default_rng(seed + 100)
+    np_ = __import__("numpy")
+    rho = q5[4] / (grids.R_AIR * q5[0])
+    nu = grids.sutherland_mu(q5[0]) / rho
+    q = np_.vstack([q5, (nut_ratio * nu * (1.0 + 0.1 * rng.standard_normal(g.nc)))[None, :]])
+    g.meta["dist"] = grids.wall_distance(g)
+    prm = dict(prm, turb="sa")
+    prm.update(over)
+    return g, q, prm
+
+
 WORKLOADS = {"box32": dict(fn=box, steps=50), "channel128": dict(fn=channel, steps=None)}

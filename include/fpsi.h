@@ -45,6 +45,10 @@ enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
 /* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, or line SGS
  * (src/linearSolverLSGS.loci) */
 enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
+/* turbulence model (turbulence/SA/SA_turb.loci): none, or Spalart-Allmaras
+ * (baseline, standard production) solved segregated like the reference;
+ * the state then carries nuTilde as its 6th row */
+enum { FPSI_TURB_NONE = 0, FPSI_TURB_SA = 1 };
 /* How a Gauss-Seidel half-sweep runs (same result bit for bit):
  * PER_LEVEL  one launch per dependency level (default);
  * PERSISTENT one cooperative launch, per-level completion counters;
@@ -102,6 +106,9 @@ typedef struct {
   /* low-Mach preconditioning (src/fluxInviscid.loci:39-155); a value <= 0
    * takes flowPsi's default: Minf 1 (no preconditioning), etaT 0.25, PLimPC 10 */
   double Minf, etaT, PLimPC;
+  int turb;             /* FPSI_TURB_* */
+  double Prt;           /* turbulentPrandtlNumber (<= 0: 0.9) */
+  double nut_bc;        /* nuTilde of inflow faces (<= 0: 5e-9, the BC default) */
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -133,9 +140,15 @@ int fpsi_get_info(fpsi_ctx *ctx, fpsi_info *info);
 /* Options facts (the $rule default(...) values listed in fpsi_params). */
 int fpsi_set_params(fpsi_ctx *ctx, const fpsi_params *params);
 
-/* temperature, u, gagePressure stores (include/flowPsi.lh:53-65). */
+/* temperature, u, gagePressure stores (include/flowPsi.lh:53-65):
+ * q[5][nc] (T, u, v, w, pg), or q[6][nc] with nuTilde under FPSI_TURB_SA;
+ * fpsi_set_params first. */
 int fpsi_set_state(fpsi_ctx *ctx, const double *q);
 int fpsi_get_state(fpsi_ctx *ctx, double *q);
+
+/* dist_noslip[nc] (src/solverTurbulence.loci:133-142, Loci's nearest
+ * no-slip face): needed by the turbulence model. */
+int fpsi_set_wall_distance(fpsi_ctx *ctx, const double *dist);
 
 /* Residual path: boundary face states (src/bcInflow.loci:311-340,
  * src/bcViscousWall.loci:205-214, src/bcImpermeable.loci:48-64), gradients
@@ -173,6 +186,13 @@ int fpsi_get_system(fpsi_ctx *ctx, void *fjp, void *fjm, void *D, int *pivot, vo
  *   line-Jacobi start then at most nsweeps relaxed symmetric sweeps in the
  *   breadth-first line order, stopping on lsgs_abstol / lsgs_reltol. */
 int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
+
+/* nuTilde system of the assembled step (FPSI_TURB_SA): src[nc] (X_src),
+ * fjp/fjm[nf] (zero on boundary faces; L = -fjp, U = fjm), D[nc], B[nc],
+ * tmu[nc]; any output may be NULL.  fpsi_sa_solve runs nsweeps scalar SGS
+ * sweeps from the Jacobi start and returns dX[nc]. */
+int fpsi_sa_get_system(fpsi_ctx *ctx, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu);
+int fpsi_sa_solve(fpsi_ctx *ctx, int nsweeps, double *dX);
 
 /* Outcome of the last line-SGS solve: sweeps run, final relaxation w and
  * update norm (the diagnostics of src/linearSolverLSGS.loci:1934-1940). */

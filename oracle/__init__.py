@@ -21,7 +21,7 @@ _ip = C.POINTER(C.c_int)
 class OgGrid(C.Structure):
     _fields_ = [("nc", C.c_int), ("nf", C.c_int), ("cl", _ip), ("cr", _ip), ("btype", _ip),
                 ("pface", _ip), ("n", _dp), ("area", _dp), ("fc", _dp), ("vol", _dp),
-                ("cc", _dp), ("pshift", _dp)]
+                ("cc", _dp), ("pshift", _dp), ("dist", _dp)]
 
 
 class OgParams(C.Structure):
@@ -33,7 +33,9 @@ class OgParams(C.Structure):
                 ("sgs_iter", C.c_int), ("minPg", C.c_double), ("linear_solver", C.c_int),
                 ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double), ("lsgs_abstol", C.c_double),
                 ("lsgs_relax", C.c_double), ("Minf", C.c_double), ("etaT", C.c_double),
-                ("PLimPC", C.c_double)]
+                ("PLimPC", C.c_double), ("turb", C.c_int), ("sa_cb1", C.c_double), ("sa_k", C.c_double),
+                ("sa_cv1", C.c_double), ("sa_cb2", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double),
+                ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -78,11 +80,13 @@ class OGrid:
             n=np.ascontiguousarray(g.n, np.float64), area=np.ascontiguousarray(g.area, np.float64),
             fc=np.ascontiguousarray(g.fc, np.float64), vol=np.ascontiguousarray(g.vol, np.float64),
             cc=np.ascontiguousarray(g.cc, np.float64), pshift=np.ascontiguousarray(g.pshift, np.float64))
+        dist = g.meta.get("dist") if getattr(g, "meta", None) else None
+        self.arrs["dist"] = np.ascontiguousarray(dist if dist is not None else np.ones(g.nc), np.float64)
         a = self.arrs
         self.s = OgGrid(g.nc, g.nf, _ptr(a["cl"], C.c_int), _ptr(a["cr"], C.c_int),
                         _ptr(a["btype"], C.c_int), _ptr(a["pface"], C.c_int), _ptr(a["n"]),
                         _ptr(a["area"]), _ptr(a["fc"]), _ptr(a["vol"]), _ptr(a["cc"]),
-                        _ptr(a["pshift"]))
+                        _ptr(a["pshift"]), _ptr(a["dist"]))
 
     @property
     def ref(self):
@@ -116,6 +120,12 @@ def params(p: dict) -> OgParams:
     s.Minf = p.get("Minf", 1.0)
     s.etaT = p.get("etaT", 0.25)
     s.PLimPC = p.get("PLimPC", 10.0)
+    s.turb = {"none": 0, "sa": 1}[p.get("turb", "none")]
+    # Spa_All_param defaults, turbulence/SA/SA_param.h:25-34
+    s.sa_cb1, s.sa_k, s.sa_cv1, s.sa_cb2 = 0.1355, 0.41, 7.1, 0.622
+    s.sa_cw2, s.sa_cw3, s.sa_sigma = 0.3, 2.0, 2. / 3.
+    s.Prt = p.get("Prt", 0.9)
+    s.nut_bc = p.get("nut_bc", 5e-9)
     return s
 
 
@@ -140,15 +150,16 @@ class Oracle:
 
     def face_states(self, q):
         q = np.ascontiguousarray(q, np.float64)
-        qf = np.zeros((5, self.g.nf))
+        qf = np.zeros((6, self.g.nf))
         self.L.og_face_states(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf))
-        return qf
+        return np.ascontiguousarray(qf[:6 if self.p.turb else 5])
 
     def gradients(self, q, qf=None):
         q = np.ascontiguousarray(q, np.float64)
         if qf is None:
             qf = self.face_states(q)
-        grad = np.zeros((15, self.g.nc)); phi = np.zeros((5, self.g.nc))
+        nq = 6 if self.p.turb else 5
+        grad = np.zeros((3 * nq, self.g.nc)); phi = np.zeros((nq, self.g.nc))
         self.L.og_gradients(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf), _ptr(grad), _ptr(phi))
         return grad, phi
 
@@ -185,6 +196,22 @@ class Oracle:
                             B.ctypes.data_as(C.c_void_p), C.byref(nrm), Draw.ctypes.data_as(C.c_void_p))
         return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm, Draw=Draw)
 
+    def sa_assemble(self, q):
+        """nuTilde system: src, fjp, fjm (per face), D, B, tmu."""
+        q = np.ascontiguousarray(q, np.float64)
+        nc, nf = self.g.nc, self.g.nf
+        out = {k: np.zeros(n) for k, n in (("src", nc), ("fjp", nf), ("fjm", nf), ("D", nc), ("B", nc),
+                                            ("tmu", nc))}
+        self.L.og_sa_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(out["src"]), _ptr(out["fjp"]),
+                              _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"]))
+        return out
+
+    def sa_sgs(self, sys_, nsweeps):
+        X = np.zeros(self.g.nc)
+        self.L.og_sa_sgs(self.og.ref, int(nsweeps), _ptr(sys_["fjp"]), _ptr(sys_["fjm"]), _ptr(sys_["D"]),
+                         _ptr(sys_["B"]), _ptr(X))
+        return X
+
     def lsgs_order(self):
         order = np.zeros(self.g.nc, np.int32)
         n = self.L.og_lsgs_order(self.og.ref, _ptr(order, C.c_int))
@@ -216,6 +243,15 @@ class Oracle:
 
 
 # ---- point functions -------------------------------------------------------
+
+def sa_point(prm, T, pg, nut, vort, y, vol=1.0):
+    """(tmu, production - destruction, destruction Jacobian) of one cell."""
+    out = np.zeros(3)
+    p = params(dict(prm, turb="sa"))
+    lib().og_sa_point(C.byref(p), C.c_double(T), C.c_double(pg), C.c_double(nut), C.c_double(vort),
+                      C.c_double(y), C.c_double(vol), _ptr(out))
+    return out
+
 
 def hllc_flux(qL, qR, n, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4, eta=1.0, fj="f64"):
     f = np.zeros(5)

@@ -453,24 +453,29 @@ static inline int rcell(const og_grid *g, int f, real *x) {
   return g->cr[f];
 }
 
+/* variables per cell: T, u, v, w, pg (+ nuTilde) */
+static inline int nq_of(const og_params *p) { return p->turb ? 6 : 5; }
+
 /* Neighbour of cell c across face f for the gradient stencil: its position
  * and primitive values (cell, periodic ghost, or boundary-face state). */
-static void stencil_nb(const og_grid *g, const double *q, const double *qf, int c, int f,
+static void stencil_nb(const og_grid *g, const double *q, const double *qf, int c, int f, int nq,
                        real *x, real *v) {
   int nc = g->nc, nf = g->nf;
   int t = g->btype[f];
   if (t == OG_INTERIOR) {
     int o = (g->cl[f] == c) ? g->cr[f] : g->cl[f];
     ccenter(g, o, x);
-    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+    for (int k = 0; k < nq; ++k) v[k] = q[k * nc + o];
   } else if (t == OG_PERIODIC) {
     int o = rcell(g, f, x);
-    for (int k = 0; k < 5; ++k) v[k] = q[k * nc + o];
+    for (int k = 0; k < nq; ++k) v[k] = q[k * nc + o];
   } else {
     fcenter(g, f, x);
-    for (int k = 0; k < 5; ++k) v[k] = qf[k * nf + f];
+    for (int k = 0; k < nq; ++k) v[k] = qf[k * nf + f];
   }
 }
+
+static real tmu_cell(const og_grid *g, const og_params *p, const double *q, int c);
 
 /* ------------------------------------------------------------------ */
 /* low-Mach preconditioning parameter, src/fluxInviscid.loci:35-155     */
@@ -529,11 +534,21 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
     if (t == OG_INTERIOR) mref[r] = maxr(mref[r], p->PLimPC * dP / (rho[r] * a[r] * a[r]));
     if (p->viscous) {
       const real area = maxr(g->area[f], 1e-30);
-      const real mum = maxr(mu[l], mu[r]);
       const real rhom = 0.5 * (rho[l] + rho[r]);
-      const real nu = mum / rhom;
-      mref[l] = maxr(mref[l], nu / ((g->vol[l] / area) * a[l]));
-      if (t == OG_INTERIOR) mref[r] = maxr(mref[r], nu / ((g->vol[r] / area) * a[r]));
+      if (!p->turb) {
+        const real mum = maxr(mu[l], mu[r]);
+        const real nu = mum / rhom;
+        mref[l] = maxr(mref[l], nu / ((g->vol[l] / area) * a[l]));
+        if (t == OG_INTERIOR) mref[r] = maxr(mref[r], nu / ((g->vol[r] / area) * a[r]));
+      } else { /* :81-96; the cl side takes max(tmu_cl, tmu_cl) as written */
+        const real tl = tmu_cell(g, p, q, l), tr = tmu_cell(g, p, q, r);
+        const real nul = (maxr(mu[l], mu[r]) + maxr(tl, tl)) / rhom;
+        mref[l] = maxr(mref[l], nul / ((g->vol[l] / area) * a[l]));
+        if (t == OG_INTERIOR) {
+          const real nur = (maxr(mu[l], mu[r]) + maxr(tl, tr)) / rhom;
+          mref[r] = maxr(mref[r], nur / ((g->vol[r] / area) * a[r]));
+        }
+      }
     }
   }
   for (int c = 0; c < nc; ++c) {
@@ -607,6 +622,13 @@ void og_face_states(const og_grid *g, const og_params *p, const double *q, doubl
       pgb = pgi + 0.5 * rho * U * uit;
       Tb = Ti;
     }
+    if (p->turb) { /* nuTilde_f: SA_turb.loci:181-183 (wall), solverScalarTransport.loci:423-463 */
+      const real Xi = q[5 * nc + c];
+      real Xb = Xi;
+      if (t == OG_WALL) Xb = 0.0;
+      else if (t == OG_FARFIELD) Xb = (dot3(n, ub) - 0.0 >= -0.0) ? Xi : p->nut_bc;
+      qf[5 * nf + f] = Xb;
+    }
     qf[f] = Tb;
     qf[nf + f] = ub[0];
     qf[2 * nf + f] = ub[1];
@@ -676,38 +698,39 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
     lsq_inverse(g, &a, c, Ai);
     real xc[3];
     ccenter(g, c, xc);
-    real qc[5];
-    for (int k = 0; k < 5; ++k) qc[k] = q[k * nc + c];
-    real b[5][3];
-    real qmin[5], qmax[5];
-    for (int k = 0; k < 5; ++k) { b[k][0] = b[k][1] = b[k][2] = 0; qmin[k] = qmax[k] = qc[k]; }
+    const int nq = nq_of(p);
+    real qc[6];
+    for (int k = 0; k < nq; ++k) qc[k] = q[k * nc + c];
+    real b[6][3];
+    real qmin[6], qmax[6];
+    for (int k = 0; k < nq; ++k) { b[k][0] = b[k][1] = b[k][2] = 0; qmin[k] = qmax[k] = qc[k]; }
     int lists[3][2] = {{a.lo_off[c], a.lo_off[c + 1]}, {a.up_off[c], a.up_off[c + 1]}, {a.bd_off[c], a.bd_off[c + 1]}};
     const int *arr[3] = {a.lo, a.up, a.bd};
     for (int L = 0; L < 3; ++L)
       for (int j = lists[L][0]; j < lists[L][1]; ++j) {
         int f = arr[L][j];
-        real x[3], v[5];
-        stencil_nb(g, q, qf, c, f, x, v);
+        real x[3], v[6];
+        stencil_nb(g, q, qf, c, f, nq, x, v);
         real dx = x[0] - xc[0], dy = x[1] - xc[1], dz = x[2] - xc[2];
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < nq; ++k) {
           real dq = v[k] - qc[k];
           b[k][0] += dx * dq; b[k][1] += dy * dq; b[k][2] += dz * dq;
           qmin[k] = minr(qmin[k], v[k]);
           qmax[k] = maxr(qmax[k], v[k]);
         }
       }
-    real gr[5][3];
-    for (int k = 0; k < 5; ++k) {
+    real gr[6][3];
+    for (int k = 0; k < nq; ++k) {
       gr[k][0] = Ai[0] * b[k][0] + Ai[1] * b[k][1] + Ai[2] * b[k][2];
       gr[k][1] = Ai[1] * b[k][0] + Ai[3] * b[k][1] + Ai[4] * b[k][2];
       gr[k][2] = Ai[2] * b[k][0] + Ai[4] * b[k][1] + Ai[5] * b[k][2];
       for (int d = 0; d < 3; ++d) grad[(3 * k + d) * nc + c] = gr[k][d];
     }
     /* limiter */
-    real ph[5];
-    for (int k = 0; k < 5; ++k) ph[k] = 1.0;
+    real ph[6];
+    for (int k = 0; k < nq; ++k) ph[k] = 1.0;
     if (p->limiter == 2) {
-      for (int k = 0; k < 5; ++k) ph[k] = 0.0;
+      for (int k = 0; k < nq; ++k) ph[k] = 0.0;
     } else if (p->limiter == 1 || p->limiter == 3) {
       const real h3 = g->vol[c];
       const real eps2 = p->Kl * p->Kl * p->Kl * h3;
@@ -717,7 +740,7 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
           real xf[3];
           fcenter(g, f, xf);
           real dx = xf[0] - xc[0], dy = xf[1] - xc[1], dz = xf[2] - xc[2];
-          for (int k = 0; k < 5; ++k) {
+          for (int k = 0; k < nq; ++k) {
             real qdif = gr[k][0] * dx + gr[k][1] * dy + gr[k][2] * dz;
             real lim;
             if (p->limiter == 1) {
@@ -731,9 +754,9 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
             ph[k] = minr(ph[k], lim);
           }
         }
-      for (int k = 0; k < 5; ++k) ph[k] = maxr(0.0, minr(1.0, ph[k]));
+      for (int k = 0; k < nq; ++k) ph[k] = maxr(0.0, minr(1.0, ph[k]));
     }
-    for (int k = 0; k < 5; ++k) phi[k * nc + c] = ph[k];
+    for (int k = 0; k < nq; ++k) phi[k * nc + c] = ph[k];
   }
   free_adj(&a);
 }
@@ -786,6 +809,34 @@ static void face_grad(const og_grid *g, const double *q, const double *qf, const
   }
 }
 
+/* Spalart-Allmaras eddy viscosity of a cell: xcoeff, fv1, tmu
+ * (turbulence/SA/SA_turb.loci:93-107). */
+static real sa_fv1(const og_params *p, real x) {
+  return pow(x, 3.0) / (pow(x, 3.0) + pow(p->sa_cv1, 3.0));
+}
+static real tmu_cell(const og_grid *g, const og_params *p, const double *q, int c) {
+  const int nc = g->nc;
+  const real T = q[c], pg = q[4 * nc + c], nut = q[5 * nc + c];
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  const real mu = suth_mu(p, T);
+  const real x = rho * nut / mu;
+  return nut * sa_fv1(p, x) * rho;
+}
+/* tmu_f: volume-weighted interior average, the interior cell on boundary
+ * faces, zero on no-slip walls (src/fluxViscous.loci:192-201) */
+static real tmu_face(const og_grid *g, const og_params *p, const double *q, int f) {
+  const int t = g->btype[f];
+  const int c = g->cl[f];
+  if (t == OG_INTERIOR || t == OG_PERIODIC) {
+    real xr[3];
+    const int o = rcell(g, f, xr);
+    const real vl = g->vol[c], vr = g->vol[o];
+    return (tmu_cell(g, p, q, o) * vl + tmu_cell(g, p, q, c) * vr) / (vr + vl);
+  }
+  if (t == OG_WALL) return 0.0;
+  return tmu_cell(g, p, q, c);
+}
+
 /* Face viscous quantities: muTotal_f, kconductTotal, u_f and vflux[4]
  * (src/fluxViscous.loci:121-133,145-151,184-197,217-238,272-292). */
 static void viscous_face(const og_grid *g, const og_params *p, const double *q, const double *qf,
@@ -813,6 +864,11 @@ static void viscous_face(const og_grid *g, const og_params *p, const double *q, 
     for (int j = 0; j < 3; ++j) uf[j] = qf[(1 + j) * nf + f];
   }
   if (t == OG_SYMMETRY) { /* src/fluxViscous.loci:294-297 */
+    if (p->turb) {
+      const real tmf = tmu_face(g, p, q, f);
+      *k_f = *k_f + tmf * (p->Rt * p->gamma / (p->gamma - 1.0)) / p->Prt;
+      *mu_f = *mu_f + tmf;
+    }
     vflux[0] = vflux[1] = vflux[2] = vflux[3] = 0;
     return;
   }
@@ -820,6 +876,12 @@ static void viscous_face(const og_grid *g, const og_params *p, const double *q, 
   for (int i = 0; i < 3; ++i) face_grad(g, q, qf, grad, f, 1 + i, gu[i]);
   face_grad(g, q, qf, grad, f, 0, gT);
   real mu_total = *mu_f;
+  if (p->turb) { /* muTotal_f, kconductTotal: src/fluxViscous.loci:182-200,275-278 */
+    const real tmf = tmu_face(g, p, q, f);
+    mu_total = *mu_f + tmf;
+    *k_f = *k_f + tmf * (p->Rt * p->gamma / (p->gamma - 1.0)) / p->Prt;
+    *mu_f = mu_total;
+  }
   real dudx = gu[0][0], dvdy = gu[1][1], dwdz = gu[2][2];
   real dvdx = gu[1][0], dwdx = gu[2][0], dwdy = gu[2][1];
   real dudy = gu[0][1], dudz = gu[0][2], dvdz = gu[1][2];
@@ -1002,8 +1064,8 @@ static void residual_norms(const og_grid *g, const double *src, og_norms *r) {
 void og_residual(const og_grid *g, const og_params *p, const double *q, double *src,
                  double *iflux, double *vflux, og_norms *nrm) {
   int nc = g->nc, nf = g->nf;
-  double *qf = calloc((size_t)5 * nf, sizeof(double));
-  double *grad = malloc(sizeof(double) * 15 * nc), *phi = malloc(sizeof(double) * 5 * nc);
+  double *qf = calloc((size_t)6 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
   double *maxev = malloc(sizeof(double) * nf);
   og_face_states(g, p, q, qf);
   og_gradients(g, p, q, qf, grad, phi);
@@ -1195,8 +1257,8 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   real_fj *Draw = Draw_v;
   int nc = g->nc, nf = g->nf;
   real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
-  double *qf = calloc((size_t)5 * nf, sizeof(double));
-  double *grad = malloc(sizeof(double) * 15 * nc), *phi = malloc(sizeof(double) * 5 * nc);
+  double *qf = calloc((size_t)6 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
   double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
   double *maxev = malloc(sizeof(double) * nf);
   og_face_states(g, p, q, qf);
@@ -1383,7 +1445,17 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
   int *piv = malloc(sizeof(int) * 5 * (size_t)nc);
   double *X = malloc(sizeof(double) * 5 * (size_t)nc);
   real_fj *Draw = p->linear_solver == 1 ? malloc(sizeof(real_fj) * 25 * (size_t)nc) : NULL;
+  double *ssrc = NULL, *sfp = NULL, *sfm = NULL, *sD = NULL, *sB = NULL, *sX = NULL;
+  if (p->turb) {
+    ssrc = malloc(sizeof(double) * nc); sfp = malloc(sizeof(double) * nf); sfm = malloc(sizeof(double) * nf);
+    sD = malloc(sizeof(double) * nc); sB = malloc(sizeof(double) * nc); sX = malloc(sizeof(double) * nc);
+  }
   for (int s = 0; s < nsteps; ++s) {
+    /* the nuTilde system from the state at the start of the step */
+    if (p->turb) {
+      og_sa_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL);
+      og_sa_sgs(g, p->sgs_iter, sfp, sfm, sD, sB, sX);
+    }
     og_norms nrm;
     og_assemble2(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
     if (hist) {
@@ -1405,9 +1477,338 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       q[4 * nc + c] = q[4 * nc + c] + alpha * dp;
       for (int k = 1; k < 4; ++k) q[k * nc + c] = q[k * nc + c] + alpha * dq[k];
     }
+    if (p->turb) /* X_next, src/solverScalarTransport.loci:400-408 */
+      for (int c = 0; c < nc; ++c) {
+        const real Xc = q[5 * nc + c], dX = sX[c];
+        real tscale = 1.0;
+        tscale = minr(tscale, p->urelax * Xc / (fabs(dX) + 1e-15));
+        real Xn = Xc + tscale * dX;
+        q[5 * nc + c] = maxr(Xn, 1e-15);
+      }
   }
   free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X); free(Draw);
+  free(ssrc); free(sfp); free(sfm); free(sD); free(sB); free(sX);
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spalart-Allmaras nuTilde equation (baseline model, standard          */
+/* production), segregated as in the reference                         */
+/* ------------------------------------------------------------------ */
+
+/* scalar reconstruction lefts/rights(X) of variable k at face centre xf */
+static real recon_k(const og_grid *g, const double *q, const double *grad, const double *phi, int k, int c,
+                    const real *xcell, const real *xf) {
+  const int nc = g->nc;
+  real d[3] = {xf[0] - xcell[0], xf[1] - xcell[1], xf[2] - xcell[2]};
+  real gd = grad[(3 * k) * nc + c] * d[0] + grad[(3 * k + 1) * nc + c] * d[1] + grad[(3 * k + 2) * nc + c] * d[2];
+  return q[k * nc + c] + phi[k * nc + c] * gd;
+}
+
+/* Per-cell SA terms (turbulence/SA/SA_turb.loci:93-165, 258-315, 347-351):
+ * production - destruction into *srcv, the destruction Jacobian into *srcJ. */
+static void sa_core(const og_params *p, real T, real pg, real nut, real vortMag, real y, real vol, real *srcv,
+                    real *srcJ);
+static void sa_cell_terms(const og_grid *g, const og_params *p, const double *q, const double *grad, int c,
+                          real *srcv, real *srcJ) {
+  const int nc = g->nc;
+  /* vortMag from the cell velocity gradient: d u_i / d x_j at grad[3(1+i)+j]
+   * (src/solverTurbulence.loci:44-48,77-79) */
+#define GU(i, j) grad[(3 * (1 + (i)) + (j)) * nc + c]
+  const real wx = GU(2, 1) - GU(1, 2), wy = GU(0, 2) - GU(2, 0), wz = GU(1, 0) - GU(0, 1);
+#undef GU
+  const real vortMag = sqrt(wx * wx + wy * wy + wz * wz);
+  sa_core(p, q[c], q[4 * nc + c], q[5 * nc + c], vortMag, g->dist[c], g->vol[c], srcv, srcJ);
+}
+
+/* SA terms of one cell from its state, vorticity magnitude, wall distance
+ * and volume */
+static void sa_core(const og_params *p, real T, real pg, real nut, real vortMag, real y, real vol, real *srcv,
+                    real *srcJ) {
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  const real mu = suth_mu(p, T);
+  const real xcoeff = rho * nut / mu;
+  const real fv1 = sa_fv1(p, xcoeff);
+  const real fv2 = 1.0 - (xcoeff / (1.0 + xcoeff * fv1));
+  const real k = p->sa_k;
+  const real denom = k * k * y * y;
+  real S_tilda = 0.0;
+  S_tilda += vortMag;
+  S_tilda += nut * fv2 / denom;
+  const real S_t = maxr(S_tilda, 0.3 * vortMag);
+  const real rclip = 10;
+  const real r = (S_t < nut / (rclip * denom)) ? rclip : nut / (S_t * denom);
+  const real r2 = r * r, r6 = r2 * r2 * r2;
+  const real gc = r + p->sa_cw2 * (r6 - r);
+  const real cw36 = pow(p->sa_cw3, 6.0);
+  const real g2 = gc * gc, g6 = g2 * g2 * g2;
+  const real fw = gc * pow((1.0 + cw36) / (g6 + cw36), 1.0 / 6.0);
+  const real cw1 = (p->sa_cb1) / (k * k) + (1.0 + (p->sa_cb2)) / (p->sa_sigma);
+  real sv = 0.0;
+  sv += rho * nut * S_t * p->sa_cb1 * vol;
+  const real t = nut / y;
+  sv -= rho * fw * cw1 * t * t * vol;
+  /* destruction Jacobian */
+  real rp;
+  const real rclip2 = 9e6;
+  if (S_t < nut / (rclip2 * denom)) {
+    rp = 0;
+  } else {
+    const real chi = xcoeff, chiP = 1. / mu;
+    const real chi3 = chi * chi * chi, chi4 = chi3 * chi, chi6 = chi3 * chi3;
+    const real cv1 = p->sa_cv1, cv13 = cv1 * cv1 * cv1;
+    const real fv2p = chi * (4 * (chi3 * chiP) / (chi3 + cv13) - 3 * (chi6 * chiP) / pow(chi3 + cv13, 2.0)) /
+                          pow(chi4 / (chi3 + cv13) + 1, 2.0) -
+                      chiP / (chi4 / (chi3 + cv13) + 1);
+    const real Stp = (fv2 / rho + nut * fv2p) / (denom);
+    rp = 1. / (rho * denom * S_t) - nut * Stp / (denom * S_t * S_t);
+  }
+  const real gp = rp + p->sa_cw2 * (6. * pow(r, 5.0) * rp - rp);
+  const real term = pow((cw36 + 1.) / (cw36 + g6), 1. / 6.);
+  const real term2 = term * term, term5 = term2 * term2 * term;
+  const real rterm5 = 1. / term5, rsum = 1. / (g6 + cw36);
+  const real fwp = gp * term - gp * ((cw36 + 1) * g6) * rsum * rsum * rterm5;
+  *srcv = sv;
+  *srcJ = -(vol * cw1 * rho * nut * (rho * nut * fwp + 2. * fw) / (y * y));
+}
+
+/* rho and laminar mu of a face: volume-weighted interior averages, the face
+ * state on boundaries (src/fluxViscous.loci:121-133,210-215, solverEoS.loci:53) */
+static void face_rho_mu(const og_grid *g, const og_params *p, const double *q, const double *qf, int f,
+                        real *rho_f, real *mu_f) {
+  const int nc = g->nc, nf = g->nf, t = g->btype[f], c = g->cl[f];
+  if (t == OG_INTERIOR || t == OG_PERIODIC) {
+    real xr[3];
+    const int o = rcell(g, f, xr);
+    const real vl = g->vol[c], vr = g->vol[o];
+    const real rl = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
+    const real rr = (q[4 * nc + o] + p->Pambient) / (p->Rt * q[o]);
+    const real factor = 1. / (vl + vr);
+    *rho_f = factor * (vl * rr + vr * rl);
+    *mu_f = (vl * suth_mu(p, q[o]) + vr * suth_mu(p, q[c])) / (vl + vr);
+  } else {
+    *rho_f = (qf[4 * nf + f] + p->Pambient) / (p->Rt * qf[f]);
+    *mu_f = suth_mu(p, qf[f]);
+  }
+}
+
+/* One steady Newton step of the nuTilde equation:
+ *   convection mdot = iflux[0] upwind, floored at 0, with the continuity
+ *   stabilisation -X rhodot (src/solverScalarTransport.loci:50-135, 478-535);
+ *   SA sources and destruction Jacobian; baseline diffusion diff1 and the
+ *   cb2 term with their Jacobians (turbulence/SA/SA_turb.loci:354-482,
+ *   737-754); flux Jacobians into srcJ (solverScalarTransport.loci:666-678);
+ *   D = -srcJ + rho V / dtau, B = src (:681-690, 592-599).
+ * Join order per cell (Loci's is unpinned): the convective face terms
+ * (lower, upper, boundary faces), stabilisation, SA sources, diffusion face
+ * terms (lower, upper, boundary), the cb2 term. */
+void og_sa_point(const og_params *p, double T, double pg, double nut, double vortMag, double y, double vol,
+                 double *out3) {
+  real sv, sj;
+  sa_core(p, T, pg, nut, vortMag, y, vol, &sv, &sj);
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  const real x = rho * nut / suth_mu(p, T);
+  out3[0] = nut * sa_fv1(p, x) * rho;
+  out3[1] = sv;
+  out3[2] = sj;
+}
+
+void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
+                    double *fjm, double *D, double *B, double *tmu) {
+  const int nc = g->nc, nf = g->nf;
+  double *qf = calloc((size_t)6 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
+  double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
+  double *maxev = malloc(sizeof(double) * nf), *dtau = malloc(sizeof(double) * nc);
+  double *fsrc = malloc(sizeof(double) * 5 * nc);
+  og_face_states(g, p, q, qf);
+  og_gradients(g, p, q, qf, grad, phi);
+  og_fluxes(g, p, q, qf, grad, phi, iflux, vflux, maxev);
+  /* dtau of the fluid system (same cell time step) */
+  {
+    real_fj *a = malloc(sizeof(real_fj) * 25 * (size_t)nf), *b = malloc(sizeof(real_fj) * 25 * (size_t)nf);
+    real_fj *d = malloc(sizeof(real_fj) * 25 * (size_t)nc), *bb = malloc(sizeof(real_fj) * 5 * (size_t)nc);
+    int *pv = malloc(sizeof(int) * 5 * (size_t)nc);
+    og_assemble2(g, p, q, fsrc, dtau, a, b, d, pv, bb, NULL, NULL);
+    free(a); free(b); free(d); free(bb); free(pv);
+  }
+  og_adj a;
+  build_adj(g, &a);
+  const real sig = p->sa_sigma, cb2 = p->sa_cb2;
+  const real factorx = 2, factory = .5;
+  /* per-face quantities */
+  double *fconv = calloc(nf, sizeof(double)), *fdiff = calloc(nf, sizeof(double)), *fdiv = calloc(nf, sizeof(double));
+  double *sJL = calloc(nf, sizeof(double)), *sJU = calloc(nf, sizeof(double));
+  double *bJ = calloc(nf, sizeof(double)); /* boundary srcJ contribution to ci */
+  double *cJl = calloc(nf, sizeof(double)), *cJr = calloc(nf, sizeof(double)); /* cb2 diagonal parts */
+  for (int f = 0; f < nf; ++f) {
+    const int t = g->btype[f], c = g->cl[f];
+    real an[3], xf[3], xl[3];
+    fnormal(g, f, an);
+    fcenter(g, f, xf);
+    ccenter(g, c, xl);
+    const real area = g->area[f];
+    const real mdot = iflux[f];
+    real gfx[3];
+    face_grad(g, q, qf, grad, f, 5, gfx);
+    real rho_f, mu_f;
+    face_rho_mu(g, p, q, qf, f, &rho_f, &mu_f);
+    const real rhol = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
+    const real nul = q[5 * nc + c];
+    fjp[f] = fjm[f] = 0.0;
+    if (t == OG_INTERIOR || t == OG_PERIODIC) {
+      real xr[3];
+      const int o = rcell(g, f, xr);
+      const real XL = recon_k(g, q, grad, phi, 5, c, xl, xf), XR = recon_k(g, q, grad, phi, 5, o, xr, xf);
+      const real Xval = (mdot < 0.0) ? XR : XL;
+      fconv[f] = mdot * maxr(Xval, 0.0);
+      const real vl = g->vol[c], vr = g->vol[o];
+      const real nur = q[5 * nc + o];
+      const real vrvl = 1.0 / (vr + vl);
+      const real nuf = (vr * nul + vl * nur) * vrvl;
+      real d1 = dot3(gfx, an) * area;
+      fdiff[f] = (d1 * (mu_f / rho_f + (1. + cb2) * nuf)) / (sig);
+      fdiv[f] = dot3(gfx, an) * area;
+      /* Jacobians: convection, diffusion, cb2 */
+      fjp[f] += maxr(mdot, 0.0);
+      fjm[f] += minr(mdot, 0.0);
+      const real rhor = (q[4 * nc + o] + p->Pambient) / (p->Rt * q[o]);
+      const real rhom = .5 * (rhol + rhor);
+      const real C = (mu_f + (1. + cb2) * rhom * nuf) * area;
+      real dv[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+      const real dd = dot3(dv, an);
+      fjp[f] -= -C / (dd * sig);
+      fjm[f] += -C / (dd * sig);
+      const real C2 = cb2 * area;
+      const real d2 = sig * dd;
+      sJU[f] = -factorx * (C2 / d2) * rhol * nul;
+      sJL[f] = -factorx * (C2 / d2) * rhor * nur;
+      cJl[f] = factorx * factory * (C2 / d2) * rhol * nul;
+      cJr[f] = factorx * factory * (C2 / d2) * rhor * nur;
+      fjp[f] += sJL[f];
+      fjm[f] -= sJU[f];
+    } else {
+      const real XL = recon_k(g, q, grad, phi, 5, c, xl, xf);
+      const real Xf = qf[5 * nf + f];
+      const real Xval = (mdot < 0.0) ? Xf : XL;
+      fconv[f] = mdot * maxr(Xval, 0.0);
+      real d1 = dot3(gfx, an) * area;
+      fdiff[f] = (d1 * (mu_f / rho_f + (1. + cb2) * Xf)) / (sig);
+      fdiv[f] = dot3(gfx, an) * area;
+      real bj = 0.0;
+      bj -= maxr(mdot, 0.0);
+      if (t != OG_SYMMETRY) {
+        const real C = (mu_f + (1. + cb2) * rho_f * Xf) * area;
+        real dv[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+        const real dd = dot3(dv, an);
+        bj += -C / (dd * sig);
+        const real C2 = cb2 * area;
+        const real d2 = sig * dd;
+        bj += factorx * factory * (C2 / d2) * rhol * nul;
+      }
+      bJ[f] = bj;
+    }
+  }
+  for (int c = 0; c < nc; ++c) {
+    const real rho = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
+    const real X = q[5 * nc + c];
+    real sv = 0.0, sJ = 0.0, rhodot = 0.0;
+    /* convection and continuity */
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+      const int f = a.lo[j];
+      sv += fconv[f];
+      rhodot += iflux[f];
+    }
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+      const int f = a.up[j];
+      sv -= fconv[f];
+      rhodot -= iflux[f];
+    }
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) {
+      const int f = a.bd[j];
+      sv -= fconv[f];
+      rhodot -= iflux[f];
+    }
+    sv -= X * rhodot;
+    sJ += -rhodot;
+    /* SA sources */
+    real ssrc, sdJ;
+    sa_cell_terms(g, p, q, grad, c, &ssrc, &sdJ);
+    sv += ssrc;
+    sJ += sdJ;
+    /* diffusion and its divergence */
+    real div = 0.0;
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+      const int f = a.lo[j];
+      sv -= rho * fdiff[f];
+      div -= fdiv[f];
+    }
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+      const int f = a.up[j];
+      sv += rho * fdiff[f];
+      div += fdiv[f];
+    }
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) {
+      const int f = a.bd[j];
+      sv += rho * fdiff[f];
+      div += fdiv[f];
+    }
+    sv += -rho * X * div * cb2 / sig;
+    /* srcJ: diagonal cb2 parts, boundary terms, flux Jacobians (upper -fjp,
+     * lower +fjm), then the L/U cb2 parts taken back out of the diagonal */
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += cJr[a.lo[j]];
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sJ += cJl[a.up[j]];
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) sJ += bJ[a.bd[j]];
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sJ -= fjp[a.up[j]];
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += fjm[a.lo[j]];
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sJ += sJL[a.up[j]];
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += sJU[a.lo[j]];
+    src[c] = sv;
+    const real sd = rho * (g->vol[c] / dtau[c]);
+    D[c] = -sJ + sd;
+    const real rhs = 0.0 - sv;
+    B[c] = -rhs;
+    if (tmu) tmu[c] = tmu_cell(g, p, q, c);
+  }
+  /* L = -fjp, U = fjm are stored by the caller's convention: fjp/fjm */
+  free(fconv); free(fdiff); free(fdiv); free(sJL); free(sJU); free(bJ); free(cJl); free(cJr);
+  free_adj(&a);
+  free(qf); free(grad); free(phi); free(iflux); free(vflux); free(maxev); free(dtau); free(fsrc);
+}
+
+/* Scalar SGS, src/linearSolverSGS.loci:333-500: Dinv = 1/D, X = Dinv B, then
+ * symmetric sweeps t = -B + sum_upper X[cr] U + sum_lower X[cl] L,
+ * X = Dinv (-t), periodic ghosts refreshed before each half-sweep. */
+void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
+               const double *B, double *X) {
+  const int nc = g->nc, nf = g->nf;
+  og_adj a;
+  build_adj(g, &a);
+  double *Dinv = malloc(sizeof(double) * nc), *Xg = calloc(nf, sizeof(double));
+  for (int c = 0; c < nc; ++c) Dinv[c] = 1. / D[c];
+  for (int c = 0; c < nc; ++c) X[c] = Dinv[c] * B[c];
+  for (int it = 0; it < nsweeps; ++it)
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int f = 0; f < nf; ++f)
+        if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+      for (int ii = 0; ii < nc; ++ii) {
+        const int c = pass == 0 ? ii : nc - 1 - ii;
+        real t = -B[c];
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          const int f = a.up[j];
+          const real xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          t += xn * fjm[f];
+        }
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          const int f = a.lo[j];
+          t += X[g->cl[f]] * (-fjp[f]);
+        }
+        const real v = -t;
+        X[c] = Dinv[c] * v;
+      }
+    }
+  free(Dinv); free(Xg);
+  free_adj(&a);
 }
 
 /* ------------------------------------------------------------------ */

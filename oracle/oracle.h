@@ -12,7 +12,7 @@
  *   liboracle_f32.so : real_fj = float   (flowPsi's default, include/flowTypes.h:43-49)
  *
  * Data layout (all arrays structure-of-arrays, reference cell/face order):
- *   q    [5][nc]  primitive state (T, u, v, w, pg)
+ *   q    [5][nc]  primitive state (T, u, v, w, pg); [6][nc] with nuTilde when turb
  *   qf   [5][nf]  boundary-face state (T_f, u_f, pg_f); unused on interior faces
  *   grad [15][nc] gradient of q: grad[3*k+d] = d q_k / d x_d
  *   phi  [5][nc]  limiter per variable
@@ -39,6 +39,7 @@ typedef struct {
   const double *vol;      /* [nc] */
   const double *cc;       /* [3][nc] cell centre */
   const double *pshift;   /* [3][nf] periodic ghost centre = cc[cl[pface]] + pshift */
+  const double *dist;     /* [nc] wall distance dist_noslip (turbulent runs only) */
 } og_grid;
 
 typedef struct {
@@ -57,6 +58,10 @@ typedef struct {
   int lsgs_maxiter;
   double lsgs_reltol, lsgs_abstol, lsgs_relax;
   double Minf, etaT, PLimPC;  /* low-Mach preconditioning (Minf >= 0.5: eta = 1) */
+  int turb;               /* 0 laminar, 1 Spalart-Allmaras (q gains a 6th row nuTilde) */
+  double sa_cb1, sa_k, sa_cv1, sa_cb2, sa_cw2, sa_cw3, sa_sigma;
+  double Prt;             /* turbulent Prandtl number */
+  double nut_bc;          /* nuTilde of inflow faces */
 } og_params;
 
 typedef struct {
@@ -112,6 +117,17 @@ void og_assemble(const og_grid *g, const og_params *p, const double *q, double *
 void og_assemble2(const og_grid *g, const og_params *p, const double *q, double *src,
                   double *dtau, void *fjp, void *fjm, void *D, int *pivot, void *B,
                   og_norms *nrm, void *Draw);
+/* Spalart-Allmaras: the nuTilde equation of one steady Newton step.
+ * src[nc] (X_src), rhs norm, scalar L = -fjp, U = fjm per face ([nf], zero
+ * on boundary faces), D[nc] and B[nc]; tmu[nc] the eddy viscosity. */
+void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
+                    double *fjm, double *D, double *B, double *tmu);
+/* one cell: out3 = tmu, production - destruction, destruction Jacobian */
+void og_sa_point(const og_params *p, double T, double pg, double nut, double vortMag, double y, double vol,
+                 double *out3);
+/* scalar SGS on that system (Jacobi start + nsweeps symmetric sweeps) */
+void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
+               const double *B, double *X);
 /* line SGS: equation order and the relaxed solve (returns iterations run) */
 int  og_lsgs_order(const og_grid *g, int *order);
 int  og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const void *fjp,
