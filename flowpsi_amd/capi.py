@@ -35,7 +35,7 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
            "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats", "fpsi_set_wall_distance",
-           "fpsi_sa_get_system", "fpsi_sa_solve"]
+           "fpsi_sa_get_system", "fpsi_sa_solve", "fpsi_sa_lsgs_stats"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -250,6 +250,11 @@ class Solver:
         dX = np.zeros(self.grid.nc)
         self._chk(self.L.fpsi_sa_solve(self.h, int(nsweeps), _ptr(dX)), "fpsi_sa_solve")
         return dX
+
+    def sa_lsgs_stats(self) -> dict:
+        it = C.c_int(); w = C.c_double(); r = C.c_double()
+        self._chk(self.L.fpsi_sa_lsgs_stats(self.h, C.byref(it), C.byref(w), C.byref(r)), "fpsi_sa_lsgs_stats")
+        return dict(iters=it.value, w=w.value, resid=r.value)
 
     def residual(self, want_faces=True):
         nc, nf = self.grid.nc, self.grid.nf

@@ -65,6 +65,8 @@ struct fpsi_ctx {
   double *lsgs_host = nullptr;  // pinned mirror of DevState::lst
   int lsgs_iters = 0;
   double lsgs_w = 0, lsgs_resid = 0;
+  int sa_lsgs_iters = 0;
+  double sa_lsgs_w = 0, sa_lsgs_resid = 0;
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
   bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
   // Spalart-Allmaras: buffers allocated on first selection, wall distance set by the caller
@@ -868,6 +870,48 @@ static void enqueue_sa_sweeps(fpsi_ctx *c, int nsweeps) {
   }
 }
 
+// LSGSScalarSolve for nuTilde after the Jacobi start made by the assembly
+// (src/linearSolverLSGS.loci:1156-1404): like enqueue_lsgs, with the
+// scalar rows and each sweep's stopping test read back on the host.
+static int enqueue_sa_lsgs(fpsi_ctx *c, int maxiter) {
+  const int nl = (int)c->lsgs_lev_off.size() - 1;
+  const fpsi_params &p = c->prm;
+  launch_lsgs_init(c->s, p.lsgs_relax, c->stream);
+  c->sa_lsgs_iters = 0;
+  c->sa_lsgs_w = p.lsgs_relax;
+  c->sa_lsgs_resid = 0;
+  for (int it = 0; it < maxiter; ++it) {
+    launch_ghost1(c->g, c->s.sX, c->stream);
+    for (int l = 0; l < nl; ++l)
+      launch_sa_lsgs_level(c->g, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
+    for (int l = nl - 1; l >= 0; --l)
+      launch_sa_lsgs_level(c->g, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
+    launch_sa_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->sa_lsgs_w = c->lsgs_host[0];
+    c->sa_lsgs_resid = c->lsgs_host[2];
+    c->sa_lsgs_iters = (int)c->lsgs_host[3];
+    if (c->lsgs_host[4] != 0.0) break;
+  }
+  return 0;
+}
+
+static int enqueue_sa_solve(fpsi_ctx *c, int nsweeps) {
+  if (c->lsgs()) return enqueue_sa_lsgs(c, nsweeps);
+  enqueue_sa_sweeps(c, nsweeps);
+  return 0;
+}
+
+extern "C" int fpsi_sa_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *resid) {
+  if (!c) return FPSI_ERR_ARG;
+  if (iters) *iters = c->sa_lsgs_iters;
+  if (relax) *relax = c->sa_lsgs_w;
+  if (resid) *resid = c->sa_lsgs_resid;
+  return 0;
+}
+
 static int sa_alloc(fpsi_ctx *c) {
   if (c->sa_alloc) return 0;
   DevState &s = c->s;
@@ -876,7 +920,7 @@ static int sa_alloc(fpsi_ctx *c) {
   if ((rc = dalloc(c, &s.tmu, nc)) || (rc = dalloc(c, &s.saf, 10 * nf)) ||
       (rc = dalloc(c, &s.soff, (size_t)std::max(c->nnz, 1))) || (rc = dalloc(c, &s.ssrc, nc)) ||
       (rc = dalloc(c, &s.sdinv, nc)) || (rc = dalloc(c, &s.sB, nc)) ||
-      (rc = dalloc(c, &s.sX, nc + c->nper + 1)))
+      (rc = dalloc(c, &s.sX, nc + c->nper + 1)) || (rc = dalloc(c, &s.sxt, nc)))
     return rc;
   HIPCHK(c, hipMemset(s.sX, 0, (nc + c->nper + 1) * sizeof(double)));
   HIPCHK(c, hipMemset(s.soff, 0, (size_t)std::max(c->nnz, 1) * sizeof(double)));
@@ -919,7 +963,10 @@ extern "C" int fpsi_sa_solve(fpsi_ctx *c, int nsweeps, double *dX) {
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
   if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
-  enqueue_sa_sweeps(c, nsweeps);
+  if (int rc = enqueue_sa_solve(c, nsweeps)) {
+    hipStreamSynchronize(c->stream);
+    return rc;
+  }
   HIPCHK(c, hipGetLastError());
   if (dX) if (int rc = copy_cells_out(c, c->s.sX, 1, dX)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1125,7 +1172,10 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
     }
     if (c->kp.turb) {
       hipEvent_t e2 = prof_begin(c);
-      enqueue_sa_sweeps(c, c->prm.sgs_iter);
+      if (int rc = enqueue_sa_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
+        hipStreamSynchronize(c->stream);
+        return rc;
+      }
       prof_end(c, FPSI_K_SGS_SWEEP, e2);
     }
     hipEvent_t e = prof_begin(c);

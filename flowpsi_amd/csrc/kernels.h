@@ -26,6 +26,11 @@ void launch_copy_d(long n, const double *a, double *b, hipStream_t st);
 void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st);
+// scalar line SGS: one level of a sweep; relaxation + stopping test into
+// DevState::lst (shares the block solver's lst/lparts)
+void launch_sa_lsgs_level(const DevGrid &g, int backward, const int *cells, int lo, int hi, DevState &s,
+                          hipStream_t st);
+void launch_sa_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st);
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // weighted r.r, r.q, q.q of r = B - A X0, q = A D into out[0..2] (needs 3 * nblk(nc) parts)
 void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,

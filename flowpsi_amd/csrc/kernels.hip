@@ -1422,6 +1422,27 @@ __global__ void k_sa_sgs_level(DevGrid g, const int *__restrict__ cells, int lo,
   sX[c] = sdinv[c] * v;
 }
 
+// one level of a scalar line-SGS sweep (src/linearSolverLSGS.loci:1240-1317):
+// X_c = (B_c - lower couplings - upper couplings) / D_c, the row rotated to
+// lower-first; the forward sweep keeps the value it replaces in xt
+template <bool BACK>
+__global__ void k_sa_lsgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi,
+                                const double *__restrict__ soff, const double *__restrict__ sdinv,
+                                const double *__restrict__ sB, double *__restrict__ sX, double *__restrict__ sxt) {
+  const int k = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= hi) return;
+  const int c = cells ? cells[k] : k;
+  double r = sB[c];
+  const int m0 = g.m_off[c], len = g.m_off[c + 1] - m0, nup = g.m_nup[c];
+  for (int j = 0; j < len; ++j) {
+    int jr = j + nup;
+    if (jr >= len) jr -= len;
+    r -= sX[g.m_nbr[m0 + jr]] * soff[m0 + jr];
+  }
+  if constexpr (!BACK) sxt[c] = sX[c];
+  sX[c] = sdinv[c] * r;
+}
+
 // X_next = X + min(1, urelax X/(|dX| + 1e-15)) dX, floored at 1e-15
 // (src/solverScalarTransport.loci:400-408)
 __global__ void k_sa_update(DevGrid g, Prm p, const double *__restrict__ sX, double *__restrict__ q) {
@@ -1840,6 +1861,24 @@ void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevStat
   if (hi <= lo) return;
   hipLaunchKernelGGL(k_sa_sgs_level, dim3((hi - lo + TPB - 1) / TPB), dim3(TPB), 0, st, g, cells, lo, hi,
                      (const double *)s.soff, (const double *)s.sdinv, (const double *)s.sB, s.sX);
+}
+void launch_sa_lsgs_level(const DevGrid &g, int backward, const int *cells, int lo, int hi, DevState &s,
+                          hipStream_t st) {
+  if (hi <= lo) return;
+  const dim3 grid((hi - lo + TPB - 1) / TPB);
+  if (backward)
+    hipLaunchKernelGGL(k_sa_lsgs_level<true>, grid, dim3(TPB), 0, st, g, cells, lo, hi, (const double *)s.soff,
+                       (const double *)s.sdinv, (const double *)s.sB, s.sX, s.sxt);
+  else
+    hipLaunchKernelGGL(k_sa_lsgs_level<false>, grid, dim3(TPB), 0, st, g, cells, lo, hi, (const double *)s.soff,
+                       (const double *)s.sdinv, (const double *)s.sB, s.sX, s.sxt);
+}
+void launch_sa_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st) {
+  const int np = nblk(g.nc);
+  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)s.lst, s.sX,
+                     (const double *)s.sxt, s.lparts);
+  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
+                     reltol);
 }
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_sa_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.sX, s.q);

@@ -75,6 +75,7 @@ struct DevState {
   double *sdinv;  // [nc] 1 / D
   double *sB;     // [nc]
   double *sX;     // [nc + nper + 1] scalar solution with ghosts and the zero entry
+  double *sxt;    // [nc] scalar line SGS: each cell's value before the forward sweep
   // line SGS (allocated when that solver is selected)
   double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
   double *xt;     // [nc][5] X after the forward sweep

@@ -81,8 +81,13 @@ def renumber_cells(g: Grid, perm, q=None):
     vol[perm] = g.vol
     cc = np.empty_like(g.cc)
     cc[:, perm] = g.cc
+    meta = dict(g.meta)
+    if "dist" in meta:  # per-cell wall distance follows its cell
+        d = np.empty_like(meta["dist"])
+        d[perm] = meta["dist"]
+        meta["dist"] = d
     out = Grid(g.nc, g.nf, cl2, cr2, g.btype.copy(), g.pface.copy(), n, g.area.copy(), g.fc.copy(), vol, cc,
-               g.pshift.copy(), name=g.name + "_renumbered", meta=dict(g.meta))
+               g.pshift.copy(), name=g.name + "_renumbered", meta=meta)
     q2 = None
     if q is not None:
         q2 = np.empty_like(q)

@@ -189,10 +189,18 @@ int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
 
 /* nuTilde system of the assembled step (FPSI_TURB_SA): src[nc] (X_src),
  * fjp/fjm[nf] (zero on boundary faces; L = -fjp, U = fjm), D[nc], B[nc],
- * tmu[nc]; any output may be NULL.  fpsi_sa_solve runs nsweeps scalar SGS
- * sweeps from the Jacobi start and returns dX[nc]. */
+ * tmu[nc]; any output may be NULL (the X_* rules of
+ * src/solverScalarTransport.loci:140-615 for X = nuTilde,
+ * turbulence/SA/SA_turb.loci:755-756).  fpsi_sa_solve solves it from the
+ * Jacobi start and returns dX[nc]:
+ * FPSI_SOLVER_SGS: SGSScalarSolve (src/linearSolverSGS.loci:333-500),
+ *   nsweeps symmetric level-scheduled sweeps;
+ * FPSI_SOLVER_LSGS: LSGSScalarSolve (src/linearSolverLSGS.loci:1112-1404),
+ *   at most nsweeps relaxed sweeps in the breadth-first order with the
+ *   lsgs_* stopping tests; fpsi_sa_lsgs_stats reports its outcome. */
 int fpsi_sa_get_system(fpsi_ctx *ctx, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu);
 int fpsi_sa_solve(fpsi_ctx *ctx, int nsweeps, double *dX);
+int fpsi_sa_lsgs_stats(fpsi_ctx *ctx, int *iters, double *relax, double *resid);
 
 /* Outcome of the last line-SGS solve: sweeps run, final relaxation w and
  * update norm (the diagnostics of src/linearSolverLSGS.loci:1934-1940). */

@@ -212,6 +212,15 @@ class Oracle:
                          _ptr(sys_["B"]), _ptr(X))
         return X
 
+    def sa_lsgs(self, sys_, maxiter=10, abstol=1e-10, reltol=0.1, w=0.25):
+        X = np.zeros(self.g.nc)
+        wo = C.c_double(); ro = C.c_double()
+        self.L.og_sa_lsgs.restype = C.c_int
+        it = self.L.og_sa_lsgs(self.og.ref, int(maxiter), C.c_double(abstol), C.c_double(reltol), C.c_double(w),
+                               _ptr(sys_["fjp"]), _ptr(sys_["fjm"]), _ptr(sys_["D"]), _ptr(sys_["B"]), _ptr(X),
+                               C.byref(wo), C.byref(ro))
+        return X, dict(iters=it, w=wo.value, resid=ro.value)
+
     def lsgs_order(self):
         order = np.zeros(self.g.nc, np.int32)
         n = self.L.og_lsgs_order(self.og.ref, _ptr(order, C.c_int))

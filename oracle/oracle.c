@@ -1454,7 +1454,11 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
     /* the nuTilde system from the state at the start of the step */
     if (p->turb) {
       og_sa_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL);
-      og_sa_sgs(g, p->sgs_iter, sfp, sfm, sD, sB, sX);
+      if (p->linear_solver == 1) /* LSGSScalarSolve, src/solverScalarTransport.loci:633-637 */
+        og_sa_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, sfp, sfm, sD, sB, sX, NULL,
+                   NULL);
+      else /* SGSScalarSolve, :627-631 */
+        og_sa_sgs(g, p->sgs_iter, sfp, sfm, sD, sB, sX);
     }
     og_norms nrm;
     og_assemble2(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
@@ -1809,6 +1813,65 @@ void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *f
     }
   free(Dinv); free(Xg);
   free_adj(&a);
+}
+
+/* Scalar LSGS (src/linearSolverLSGS.loci:1112-1404) with one-cell lines in
+ * the breadth-first order of og_lsgs_order: inverse diagonal 1/D (:1139),
+ * line-Jacobi start X = B/D (:1165-1188), then per iteration the periodic
+ * copies, a forward sweep that keeps each cell's value before it in xt
+ * (:1240-1279), a backward sweep that does not (:1282-1317), the relaxation
+ * X = w X + (1-w) xt with the update norm, relaxation control and stopping
+ * tests (:1320-1346).  The row is B minus lower then upper couplings. */
+int og_sa_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const double *fjp,
+               const double *fjm, const double *D, const double *B, double *X, double *w_out,
+               double *resid_out) {
+  const int nc = g->nc, nf = g->nf;
+  og_adj a;
+  build_adj(g, &a);
+  int *order = malloc(sizeof(int) * nc);
+  og_lsgs_order(g, order);
+  double *Dinv = malloc(sizeof(double) * nc), *Xg = calloc(nf, sizeof(double)), *xt = malloc(sizeof(double) * nc);
+  for (int c = 0; c < nc; ++c) Dinv[c] = 1. / D[c];
+  for (int c = 0; c < nc; ++c) X[c] = Dinv[c] * B[c];
+  double w = w0, zero = 1.0, resid = 0.0;
+  int iter = 0;
+  for (iter = 0; iter < maxiter; ++iter) {
+    for (int f = 0; f < nf; ++f)
+      if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int ii = 0; ii < nc; ++ii) {
+        const int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
+        double r = B[c];
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          const int f = a.lo[j];
+          r -= X[g->cl[f]] * (-fjp[f]);
+        }
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          const int f = a.up[j];
+          const double xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          r -= xn * fjm[f];
+        }
+        if (pass == 0) xt[c] = X[c];
+        X[c] = Dinv[c] * r;
+      }
+    double residl = 0;
+    for (int c = 0; c < nc; ++c) {
+      X[c] = w * X[c] + (1. - w) * xt[c];
+      const double r = X[c] - xt[c];
+      residl += r * r;
+    }
+    resid = sqrt(residl);
+    if (iter == 0) zero = resid > 1e-16 ? resid : 1e-16;
+    const double rel = resid / zero;
+    if (rel > 1.0) w *= 1. / rel;
+    if (resid < abstol) { ++iter; break; }
+    if (rel < reltol) { ++iter; break; }
+  }
+  if (w_out) *w_out = w;
+  if (resid_out) *resid_out = resid;
+  free(order); free(Dinv); free(Xg); free(xt);
+  free_adj(&a);
+  return iter;
 }
 
 /* ------------------------------------------------------------------ */

@@ -128,6 +128,11 @@ void og_sa_point(const og_params *p, double T, double pg, double nut, double vor
 /* scalar SGS on that system (Jacobi start + nsweeps symmetric sweeps) */
 void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
                const double *B, double *X);
+/* scalar line SGS on that system (src/linearSolverLSGS.loci:1112-1404):
+ * returns iterations run, final relaxation and update norm */
+int  og_sa_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const double *fjp,
+                const double *fjm, const double *D, const double *B, double *X, double *w_out,
+                double *resid_out);
 /* line SGS: equation order and the relaxed solve (returns iterations run) */
 int  og_lsgs_order(const og_grid *g, int *order);
 int  og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const void *fjp,

@@ -12,7 +12,12 @@ src/solverScalarTransport.loci) by what the mathematics fixes (no GPU).
 * scalar SGS converges to the dense solve of the assembled nuTilde system;
 * nuTilde = 0 gives tmu = 0 and the fluid residual of the laminar path bit
   for bit (the eddy-viscosity coupling adds exactly nothing);
-* the projected wall distance of the channel equals h - |y|.
+* the projected wall distance of the channel equals h - |y|;
+* scalar line SGS (src/linearSolverLSGS.loci:1112-1404): it reaches the
+  dense solve; on a box one unrelaxed sweep is one scalar SGS sweep; and a
+  two-cell chain iterated by hand, where the relaxation blends with the
+  value before the forward sweep (the block solver blends with the value
+  before the backward sweep, and the chain tells the two apart).
 """
 import numpy as np
 import pytest
@@ -124,3 +129,72 @@ def test_zero_nutilde_is_laminar():
     a_lam = ol.assemble(q0[:5])
     for k in ("fjp", "fjm", "D", "B", "dtau"):
         assert np.array_equal(a_sa[k], a_lam[k])
+
+
+def _scalar_chain(a=2.0, c=2.0):
+    """Two cells joined by one face: row 0 is X0 + a X1 = 1, row 1 is c X0 + X1 = 0."""
+    x = np.linspace(0.0, 1.0, 3)
+    y = np.linspace(0.0, 0.5, 2)
+    Xn = np.stack(np.meshgrid(x, y, y, indexing="ij"))
+    g = grids.structured_hex(Xn, {s: grids.FARFIELD for s in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")})
+    fi = np.flatnonzero(g.btype == grids.INTERIOR)
+    assert len(fi) == 1 and g.cl[fi[0]] == 0 and g.cr[fi[0]] == 1
+    fjp = np.zeros(g.nf); fjm = np.zeros(g.nf)
+    fjm[fi[0]] = a        # U in row 0
+    fjp[fi[0]] = -c       # L = -fjp in row 1
+    return g, dict(fjp=fjp, fjm=fjm, D=np.ones(2), B=np.array([1.0, 0.0]))
+
+
+def test_scalar_lsgs_chain_by_hand():
+    """w0 = 1/4.  Start (1, 0).  Sweep 1: forward (1, -2), backward (5, -2),
+    blended with the start (1, 0) to (2, -1/2), norm sqrt(5)/2.  Sweep 2:
+    forward (2, -4), backward (9, -4), blended with (2, -1/2) to
+    (15/4, -11/8), norm 7 sqrt(5)/8 = 1.75 x the first, so w = 1/7."""
+    g, s = _scalar_chain()
+    o = oracle.Oracle(g, workloads.BASE)
+    X, info = o.sa_lsgs(s, maxiter=0)
+    assert np.array_equal(X, [1.0, 0.0])
+    X, info = o.sa_lsgs(s, maxiter=1)
+    assert np.array_equal(X, [2.0, -0.5])
+    assert info["w"] == 0.25 and info["resid"] == pytest.approx(np.sqrt(5) / 2, rel=1e-15)
+    X, info = o.sa_lsgs(s, maxiter=2)
+    assert np.array_equal(X, [3.75, -1.375])
+    assert info["w"] == pytest.approx(1 / 7, rel=1e-14) and info["iters"] == 2
+    # uncoupled cells: the first sweep changes nothing and the solve stops
+    g, s = _scalar_chain(0.0, 0.0)
+    X, info = o.sa_lsgs(s, maxiter=10)
+    assert info["iters"] == 1 and info["resid"] == 0.0 and np.array_equal(X, [1.0, 0.0])
+
+
+def test_scalar_lsgs_converges_to_dense_solve():
+    g, q, prm = workloads.channel_sa(nx=4, ny=6, nz=4, cflmax=2.0)
+    o = oracle.Oracle(g, prm)
+    s = o.sa_assemble(q)
+    A = np.diag(s["D"])
+    for f in range(g.nf):
+        if g.btype[f] == grids.INTERIOR:
+            A[g.cl[f], g.cr[f]] += s["fjm"][f]
+            A[g.cr[f], g.cl[f]] += -s["fjp"][f]
+        elif g.btype[f] == grids.PERIODIC:
+            A[g.cl[f], g.cl[g.pface[f]]] += s["fjm"][f]
+    x = np.linalg.solve(A, s["B"])
+    X, info = o.sa_lsgs(s, maxiter=400, abstol=0.0, reltol=0.0, w=1.0)
+    assert np.allclose(X, x, rtol=1e-9, atol=1e-9 * np.abs(x).max())
+
+
+def test_scalar_lsgs_unrelaxed_sweep_is_scalar_sgs_on_box():
+    """On a box the breadth-first order is a topological order of cl -> cr,
+    so one sweep with w = 1 equals one scalar SGS sweep (random
+    diagonally dominant system on the box's faces)."""
+    g, q, prm = workloads.box(5)
+    rng = np.random.default_rng(33)
+    it = g.btype == grids.INTERIOR
+    fjp = np.where(it, -rng.uniform(0.1, 1.0, g.nf), 0.0)
+    fjm = np.where(it, rng.uniform(-1.0, -0.1, g.nf), 0.0)
+    s = dict(fjp=fjp, fjm=fjm, D=rng.uniform(7.0, 9.0, g.nc), B=rng.standard_normal(g.nc))
+    o = oracle.Oracle(g, prm)
+    X, info = o.sa_lsgs(s, maxiter=1, w=1.0)
+    ref = o.sa_sgs(s, 1)
+    assert info["iters"] == 1
+    assert np.allclose(X, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    assert not np.allclose(X, o.sa_sgs(s, 0), rtol=1e-6)
