@@ -25,6 +25,8 @@ LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0, "lsgs": 1}
 SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2}
+TURB = {"none": 0, "sa": 1, "sst": 2}
+NSCALAR = {"none": 0, "sa": 1, "sst": 2}  # turbulence scalars after the 5 fluid variables
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
 # every entry point declared in include/fpsi.h
@@ -35,7 +37,8 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
            "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats", "fpsi_set_wall_distance",
-           "fpsi_sa_get_system", "fpsi_sa_solve", "fpsi_sa_lsgs_stats"]
+           "fpsi_sa_get_system", "fpsi_sa_solve", "fpsi_sa_lsgs_stats", "fpsi_turb_get_system",
+           "fpsi_turb_solve", "fpsi_turb_lsgs_stats"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -55,7 +58,7 @@ class Params(C.Structure):
                 ("sweep_mode", C.c_int), ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double),
                 ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double), ("Minf", C.c_double),
                 ("etaT", C.c_double), ("PLimPC", C.c_double), ("turb", C.c_int), ("Prt", C.c_double),
-                ("nut_bc", C.c_double)]
+                ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -147,9 +150,11 @@ def make_params(prm: dict) -> Params:
     s.Minf = prm.get("Minf", 1.0)
     s.etaT = prm.get("etaT", 0.25)
     s.PLimPC = prm.get("PLimPC", 10.0)
-    s.turb = {"none": 0, "sa": 1}[prm.get("turb", "none")]
+    s.turb = TURB[prm.get("turb", "none")]
     s.Prt = prm.get("Prt", 0.9)
     s.nut_bc = prm.get("nut_bc", 5e-9)
+    s.k_bc = prm.get("k_bc", 0.0)  # <= 0: the library's default from |u0|
+    s.w_bc = prm.get("w_bc", 0.0)
     return s
 
 
@@ -220,7 +225,7 @@ class Solver:
 
     @property
     def nvar(self):
-        return 6 if self.prm and self.prm.get("turb", "none") == "sa" else 5
+        return 5 + NSCALAR[(self.prm or {}).get("turb", "none")]
 
     def set_state(self, q):
         q = np.ascontiguousarray(q, np.float64)
@@ -254,6 +259,27 @@ class Solver:
     def sa_lsgs_stats(self) -> dict:
         it = C.c_int(); w = C.c_double(); r = C.c_double()
         self._chk(self.L.fpsi_sa_lsgs_stats(self.h, C.byref(it), C.byref(w), C.byref(r)), "fpsi_sa_lsgs_stats")
+        return dict(iters=it.value, w=w.value, resid=r.value)
+
+    def turb_get_system(self, which):
+        """Scalar system `which` (0 nuTilde or k, 1 omega): src, fjp, fjm, D, B, tmu."""
+        nc, nf = self.grid.nc, self.grid.nf
+        out = {k: np.zeros(n) for k, n in (("src", nc), ("fjp", nf), ("fjm", nf), ("D", nc), ("B", nc),
+                                            ("tmu", nc))}
+        self._chk(self.L.fpsi_turb_get_system(self.h, int(which), _ptr(out["src"]), _ptr(out["fjp"]),
+                                              _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]),
+                                              _ptr(out["tmu"])), "fpsi_turb_get_system")
+        return out
+
+    def turb_solve(self, which, nsweeps):
+        dX = np.zeros(self.grid.nc)
+        self._chk(self.L.fpsi_turb_solve(self.h, int(which), int(nsweeps), _ptr(dX)), "fpsi_turb_solve")
+        return dX
+
+    def turb_lsgs_stats(self, which) -> dict:
+        it = C.c_int(); w = C.c_double(); r = C.c_double()
+        self._chk(self.L.fpsi_turb_lsgs_stats(self.h, int(which), C.byref(it), C.byref(w), C.byref(r)),
+                  "fpsi_turb_lsgs_stats")
         return dict(iters=it.value, w=w.value, resid=r.value)
 
     def residual(self, want_faces=True):

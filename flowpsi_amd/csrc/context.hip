@@ -65,13 +65,14 @@ struct fpsi_ctx {
   double *lsgs_host = nullptr;  // pinned mirror of DevState::lst
   int lsgs_iters = 0;
   double lsgs_w = 0, lsgs_resid = 0;
-  int sa_lsgs_iters = 0;
-  double sa_lsgs_w = 0, sa_lsgs_resid = 0;
+  int sa_lsgs_iters[2] = {0, 0};  // per turbulence scalar
+  double sa_lsgs_w[2] = {0, 0}, sa_lsgs_resid[2] = {0, 0};
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
   bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
-  // Spalart-Allmaras: buffers allocated on first selection, wall distance set by the caller
+  // turbulence models: buffers allocated on first selection, wall distance set by the caller
   bool sa_alloc = false, have_dist = false;
-  int nvar() const { return prm.turb == FPSI_TURB_SA ? 6 : 5; }
+  int nscal() const { return prm.turb == FPSI_TURB_SST ? 2 : prm.turb == FPSI_TURB_SA ? 1 : 0; }
+  int nvar() const { return 5 + nscal(); }
   // tiled sweeps (none when the grid admits no acyclic tiling)
   TileGrid tg{};
   bool have_tiles = false;
@@ -584,9 +585,10 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   }
   DevState &s = c->s;
   const size_t nbb = c->nb;
-  // state, face states, gradients and limiters hold a 6th variable (nuTilde) for turbulent runs
-  if ((rc = dalloc(c, &s.q, 6 * (size_t)nc)) || (rc = dalloc(c, &s.qf, 6 * nbb)) ||
-      (rc = dalloc(c, &s.grad, 18 * (size_t)nc)) || (rc = dalloc(c, &s.phi, 6 * (size_t)nc)) ||
+  // state, face states, gradients and limiters hold up to two turbulence
+  // scalars after the five fluid variables
+  if ((rc = dalloc(c, &s.q, 7 * (size_t)nc)) || (rc = dalloc(c, &s.qf, 7 * nbb)) ||
+      (rc = dalloc(c, &s.grad, 21 * (size_t)nc)) || (rc = dalloc(c, &s.phi, 7 * (size_t)nc)) ||
       (rc = dalloc(c, &s.iflux, 5 * (size_t)nf)) || (rc = dalloc(c, &s.vflux, 4 * (size_t)nf)) ||
       (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
@@ -685,9 +687,10 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
   if (p->linear_solver != FPSI_SOLVER_SGS && p->linear_solver != FPSI_SOLVER_LSGS)
     return fail(c, FPSI_ERR_ARG, "unknown linear solver");
-  if (p->turb != FPSI_TURB_NONE && p->turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_ARG, "unknown turbulence model");
-  if (p->turb == FPSI_TURB_SA) {
-    if (!p->viscous) return fail(c, FPSI_ERR_ARG, "the Spalart-Allmaras model needs viscous = 1");
+  if (p->turb != FPSI_TURB_NONE && p->turb != FPSI_TURB_SA && p->turb != FPSI_TURB_SST)
+    return fail(c, FPSI_ERR_ARG, "unknown turbulence model");
+  if (p->turb != FPSI_TURB_NONE) {
+    if (!p->viscous) return fail(c, FPSI_ERR_ARG, "a turbulence model needs viscous = 1");
     if (int rc = sa_alloc(c)) return rc;
   }
   if (p->linear_solver == FPSI_SOLVER_LSGS) {
@@ -723,12 +726,18 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   // Spa_All_param defaults (turbulence/SA/SA_param.h:25-34), Cp
   // (src/solverEoS.loci:76-78), turbulentPrandtlNumber 0.9
   // (src/fluxViscous.loci:268-270), inflow nu_t 5e-9 (SA_turb.loci:167-174)
-  k.turb = p->turb == FPSI_TURB_SA ? 1 : 0;
+  k.turb = p->turb == FPSI_TURB_SST ? 2 : p->turb == FPSI_TURB_SA ? 1 : 0;
   k.sa_cb1 = 0.1355; k.sa_k = 0.41; k.sa_cv1 = 7.1; k.sa_cb2 = 0.622;
   k.sa_cw2 = 0.3; k.sa_cw3 = 2.0; k.sa_sigma = 2. / 3.;
   k.Cp = p->Rtilde * p->gamma / (p->gamma - 1.0);
   k.Prt = p->Prt > 0 ? p->Prt : 0.9;
   k.nut_bc = p->nut_bc > 0 ? p->nut_bc : 5e-9;
+  {  // sst_k_bc / sst_w_bc without options (turbulence/KOmega/sst_viscous.loci:63-118)
+    const double v = std::sqrt(p->u0[0] * p->u0[0] + p->u0[1] * p->u0[1] + p->u0[2] * p->u0[2]);
+    const double kd = std::max(1.5 * (1e-2 * v) * (1e-2 * v), 1e-3);
+    k.k_bc = p->k_bc > 0 ? p->k_bc : kd;
+    k.w_bc = p->w_bc > 0 ? p->w_bc : 0.09 * kd / (1e-3 * 1e-5);
+  }
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -779,8 +788,8 @@ extern "C" int fpsi_get_state(fpsi_ctx *c, double *q) {
 static int check_ready(fpsi_ctx *c) {
   if (!c->have_params) return fail(c, FPSI_ERR_STATE, "fpsi_set_params not called");
   if (!c->have_state) return fail(c, FPSI_ERR_STATE, "fpsi_set_state not called");
-  if (c->prm.turb == FPSI_TURB_SA && !c->have_dist)
-    return fail(c, FPSI_ERR_STATE, "the Spalart-Allmaras model needs fpsi_set_wall_distance");
+  if (c->prm.turb != FPSI_TURB_NONE && !c->have_dist)
+    return fail(c, FPSI_ERR_STATE, "the turbulence model needs fpsi_set_wall_distance");
   return 0;
 }
 
@@ -803,9 +812,9 @@ extern "C" int fpsi_set_wall_distance(fpsi_ctx *c, const double *dist) {
 
 static void enqueue_residual(fpsi_ctx *c) {
   hipEvent_t e = prof_begin(c);
-  launch_tmu(c->g, c->kp, c->s, c->stream);
   launch_bface(c->g, c->kp, c->s, c->stream);
   launch_grad(c->g, c->kp, c->s, c->stream);
+  launch_tmu(c->g, c->kp, c->s, c->stream);  // SST's eddy viscosity needs the velocity gradient
   launch_eta(c->g, c->kp, c->s, c->stream);
   launch_face_flux(c->g, c->kp, c->s, c->stream);
   launch_gather(c->g, c->kp, c->vtot, c->s, c->stream);
@@ -854,123 +863,160 @@ static void enqueue_assemble(fpsi_ctx *c) {
   prof_end(c, FPSI_K_FACE_JAC, e);
   e = prof_begin(c);
   launch_cell_assemble(c->g, c->kp, c->fp32, c->lsgs() ? 1 : 0, c->s, c->stream);
-  if (c->kp.turb) launch_sa_assemble(c->g, c->kp, c->s, c->stream);
+  if (c->kp.turb) launch_sa_assemble(c->g, c->kp, c->nnz, c->s, c->stream);
   prof_end(c, FPSI_K_ASSEMBLE, e);
 }
 
-// scalar SGS sweeps of the nuTilde system (src/linearSolverSGS.loci:333-500)
-static void enqueue_sa_sweeps(fpsi_ctx *c, int nsweeps) {
+// scalar SGS sweeps (src/linearSolverSGS.loci:333-500) of turbulence
+// scalars sc0 .. sc0 + ns - 1, all of them in each level's launch
+static void enqueue_sa_sweeps(fpsi_ctx *c, int sc0, int ns, int nsweeps) {
   const int nlev = (int)c->lev_off.size() - 1;
+  const int nall = c->nscal();
   for (int it = 0; it < nsweeps; ++it) {
-    launch_ghost1(c->g, c->s.sX, c->stream);
-    for (int l = 0; l < nlev; ++l) launch_sa_level(c->g, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
-    launch_ghost1(c->g, c->s.sX, c->stream);
+    launch_ghost1(c->g, nall, c->s.sX, c->stream);
+    for (int l = 0; l < nlev; ++l)
+      launch_sa_level(c->g, sc0, ns, c->nnz, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+    launch_ghost1(c->g, nall, c->s.sX, c->stream);
     for (int l = nlev - 1; l >= 0; --l)
-      launch_sa_level(c->g, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
+      launch_sa_level(c->g, sc0, ns, c->nnz, nullptr, c->lev_off[l], c->lev_off[l + 1], c->s, c->stream);
   }
 }
 
-// LSGSScalarSolve for nuTilde after the Jacobi start made by the assembly
-// (src/linearSolverLSGS.loci:1156-1404): like enqueue_lsgs, with the
-// scalar rows and each sweep's stopping test read back on the host.
-static int enqueue_sa_lsgs(fpsi_ctx *c, int maxiter) {
+// LSGSScalarSolve of turbulence scalar sc after the Jacobi start made by
+// the assembly (src/linearSolverLSGS.loci:1156-1404): like enqueue_lsgs,
+// with the scalar rows and each sweep's stopping test read back on the host.
+static int enqueue_sa_lsgs(fpsi_ctx *c, int sc, int maxiter) {
   const int nl = (int)c->lsgs_lev_off.size() - 1;
   const fpsi_params &p = c->prm;
   launch_lsgs_init(c->s, p.lsgs_relax, c->stream);
-  c->sa_lsgs_iters = 0;
-  c->sa_lsgs_w = p.lsgs_relax;
-  c->sa_lsgs_resid = 0;
+  c->sa_lsgs_iters[sc] = 0;
+  c->sa_lsgs_w[sc] = p.lsgs_relax;
+  c->sa_lsgs_resid[sc] = 0;
   for (int it = 0; it < maxiter; ++it) {
-    launch_ghost1(c->g, c->s.sX, c->stream);
+    launch_ghost1(c->g, c->nscal(), c->s.sX, c->stream);
     for (int l = 0; l < nl; ++l)
-      launch_sa_lsgs_level(c->g, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
+      launch_sa_lsgs_level(c->g, sc, c->nnz, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                           c->stream);
     for (int l = nl - 1; l >= 0; --l)
-      launch_sa_lsgs_level(c->g, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
-    launch_sa_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+      launch_sa_lsgs_level(c->g, sc, c->nnz, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                           c->stream);
+    launch_sa_lsgs_relax(c->g, sc, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->sa_lsgs_w = c->lsgs_host[0];
-    c->sa_lsgs_resid = c->lsgs_host[2];
-    c->sa_lsgs_iters = (int)c->lsgs_host[3];
+    c->sa_lsgs_w[sc] = c->lsgs_host[0];
+    c->sa_lsgs_resid[sc] = c->lsgs_host[2];
+    c->sa_lsgs_iters[sc] = (int)c->lsgs_host[3];
     if (c->lsgs_host[4] != 0.0) break;
   }
   return 0;
 }
 
-static int enqueue_sa_solve(fpsi_ctx *c, int nsweeps) {
-  if (c->lsgs()) return enqueue_sa_lsgs(c, nsweeps);
-  enqueue_sa_sweeps(c, nsweeps);
+// solve turbulence scalar `which` (-1: all of them)
+static int enqueue_sa_solve(fpsi_ctx *c, int which, int nsweeps) {
+  const int sc0 = which < 0 ? 0 : which, ns = which < 0 ? c->nscal() : 1;
+  if (c->lsgs()) {
+    for (int sc = sc0; sc < sc0 + ns; ++sc)
+      if (int rc = enqueue_sa_lsgs(c, sc, nsweeps)) return rc;
+    return 0;
+  }
+  enqueue_sa_sweeps(c, sc0, ns, nsweeps);
   return 0;
 }
 
-extern "C" int fpsi_sa_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *resid) {
+static int check_scalar(fpsi_ctx *c, int which) {
+  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
+  if (c->nscal() == 0) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
+  if (which < 0 || which >= c->nscal()) return fail(c, FPSI_ERR_ARG, "no such turbulence scalar");
+  return 0;
+}
+
+extern "C" int fpsi_turb_lsgs_stats(fpsi_ctx *c, int which, int *iters, double *relax, double *resid) {
   if (!c) return FPSI_ERR_ARG;
-  if (iters) *iters = c->sa_lsgs_iters;
-  if (relax) *relax = c->sa_lsgs_w;
-  if (resid) *resid = c->sa_lsgs_resid;
+  if (which < 0 || which > 1) return fail(c, FPSI_ERR_ARG, "no such turbulence scalar");
+  if (iters) *iters = c->sa_lsgs_iters[which];
+  if (relax) *relax = c->sa_lsgs_w[which];
+  if (resid) *resid = c->sa_lsgs_resid[which];
   return 0;
 }
+extern "C" int fpsi_sa_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *resid) {
+  return fpsi_turb_lsgs_stats(c, 0, iters, relax, resid);
+}
 
+// scalar arrays for two scalars (nuTilde uses the first), saf and scell
+// sized for SST
 static int sa_alloc(fpsi_ctx *c) {
   if (c->sa_alloc) return 0;
   DevState &s = c->s;
-  const size_t nc = c->nc, nf = c->nf;
+  const size_t nc = c->nc, nf = c->nf, nnz = (size_t)std::max(c->nnz, 1), xs = nc + c->nper + 1;
   int rc = 0;
-  if ((rc = dalloc(c, &s.tmu, nc)) || (rc = dalloc(c, &s.saf, 10 * nf)) ||
-      (rc = dalloc(c, &s.soff, (size_t)std::max(c->nnz, 1))) || (rc = dalloc(c, &s.ssrc, nc)) ||
-      (rc = dalloc(c, &s.sdinv, nc)) || (rc = dalloc(c, &s.sB, nc)) ||
-      (rc = dalloc(c, &s.sX, nc + c->nper + 1)) || (rc = dalloc(c, &s.sxt, nc)))
+  if ((rc = dalloc(c, &s.tmu, nc)) || (rc = dalloc(c, &s.saf, 12 * nf)) || (rc = dalloc(c, &s.scell, 8 * nc)) ||
+      (rc = dalloc(c, &s.soff, 2 * nnz)) || (rc = dalloc(c, &s.ssrc, 2 * nc)) ||
+      (rc = dalloc(c, &s.sdinv, 2 * nc)) || (rc = dalloc(c, &s.sB, 2 * nc)) || (rc = dalloc(c, &s.sX, 2 * xs)) ||
+      (rc = dalloc(c, &s.sxt, 2 * nc)))
     return rc;
-  HIPCHK(c, hipMemset(s.sX, 0, (nc + c->nper + 1) * sizeof(double)));
-  HIPCHK(c, hipMemset(s.soff, 0, (size_t)std::max(c->nnz, 1) * sizeof(double)));
+  HIPCHK(c, hipMemset(s.sX, 0, 2 * xs * sizeof(double)));
+  HIPCHK(c, hipMemset(s.soff, 0, 2 * nnz * sizeof(double)));
   c->sa_alloc = true;
   return 0;
 }
 
-extern "C" int fpsi_sa_get_system(fpsi_ctx *c, double *src, double *fjp, double *fjm, double *D, double *B,
-                                  double *tmu) {
+extern "C" int fpsi_turb_get_system(fpsi_ctx *c, int which, double *src, double *fjp, double *fjm, double *D,
+                                    double *B, double *tmu) {
   if (!c) return FPSI_ERR_ARG;
-  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
-  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
+  if (int rc = check_scalar(c, which)) return rc;
   const int nc = c->nc, nf = c->nf;
-  if (src) if (int rc = copy_cells_out(c, c->s.ssrc, 1, src)) return rc;
+  const size_t oc = (size_t)which * nc;
+  if (src) if (int rc = copy_cells_out(c, c->s.ssrc + oc, 1, src)) return rc;
   if (tmu) if (int rc = copy_cells_out(c, c->s.tmu, 1, tmu)) return rc;
   if (D || B) {
     std::vector<double> di(nc), b(nc);
-    HIPCHK(c, hipMemcpy(di.data(), c->s.sdinv, nc * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(b.data(), c->s.sB, nc * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(di.data(), c->s.sdinv + oc, nc * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), c->s.sB + oc, nc * sizeof(double), hipMemcpyDeviceToHost));
     for (int ni = 0; ni < nc; ++ni) {
       if (D) D[c->cperm[ni]] = 1. / di[ni];  // the kernel keeps 1/D
       if (B) B[c->cperm[ni]] = b[ni];
     }
   }
   if (fjp || fjm) {
-    std::vector<double> h(10 * (size_t)nf);
+    // saf rows of fjp / fjm: 3 / 4 for nuTilde, 4 / 5 (k) and 9 / 10 (omega) for SST
+    const int rp = c->nscal() == 2 ? 4 + 5 * which : 3;
+    std::vector<double> h(12 * (size_t)nf);
     HIPCHK(c, hipMemcpy(h.data(), c->s.saf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < nf; ++i) {
       const int f = c->fperm[i];
       const bool inner = i < c->nfi + c->nper;
-      if (fjp) fjp[f] = inner ? h[(size_t)3 * nf + i] : 0.0;
-      if (fjm) fjm[f] = inner ? h[(size_t)4 * nf + i] : 0.0;
+      if (fjp) fjp[f] = inner ? h[(size_t)rp * nf + i] : 0.0;
+      if (fjm) fjm[f] = inner ? h[(size_t)(rp + 1) * nf + i] : 0.0;
     }
   }
   return 0;
 }
-
-extern "C" int fpsi_sa_solve(fpsi_ctx *c, int nsweeps, double *dX) {
+extern "C" int fpsi_sa_get_system(fpsi_ctx *c, double *src, double *fjp, double *fjm, double *D, double *B,
+                                  double *tmu) {
   if (!c) return FPSI_ERR_ARG;
-  if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
-  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "no turbulence model selected");
+  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "the Spalart-Allmaras model is not selected");
+  return fpsi_turb_get_system(c, 0, src, fjp, fjm, D, B, tmu);
+}
+
+extern "C" int fpsi_turb_solve(fpsi_ctx *c, int which, int nsweeps, double *dX) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_scalar(c, which)) return rc;
   if (nsweeps < 0) return fail(c, FPSI_ERR_ARG, "nsweeps < 0");
-  if (int rc = enqueue_sa_solve(c, nsweeps)) {
+  if (int rc = enqueue_sa_solve(c, which, nsweeps)) {
     hipStreamSynchronize(c->stream);
     return rc;
   }
   HIPCHK(c, hipGetLastError());
-  if (dX) if (int rc = copy_cells_out(c, c->s.sX, 1, dX)) return rc;
+  const size_t xs = (size_t)c->nc + c->nper + 1;
+  if (dX) if (int rc = copy_cells_out(c, c->s.sX + which * xs, 1, dX)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+extern "C" int fpsi_sa_solve(fpsi_ctx *c, int nsweeps, double *dX) {
+  if (!c) return FPSI_ERR_ARG;
+  if (c->prm.turb != FPSI_TURB_SA) return fail(c, FPSI_ERR_STATE, "the Spalart-Allmaras model is not selected");
+  return fpsi_turb_solve(c, 0, nsweeps, dX);
 }
 
 extern "C" int fpsi_assemble(fpsi_ctx *c) {
@@ -1172,7 +1218,7 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
     }
     if (c->kp.turb) {
       hipEvent_t e2 = prof_begin(c);
-      if (int rc = enqueue_sa_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
+      if (int rc = enqueue_sa_solve(c, -1, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
         hipStreamSynchronize(c->stream);
         return rc;
       }

@@ -22,15 +22,19 @@ void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
                    hipStream_t st);
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st);
-// Spalart-Allmaras
+// turbulence models: eddy viscosity (after the gradients)
 void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
-void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
-void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st);
+// the turbulence scalar systems: nuTilde (turb 1) or k and omega (turb 2)
+void launch_sa_assemble(const DevGrid &g, const Prm &p, long nnz, DevState &s, hipStream_t st);
+// one SGS level of scalar systems sc0 .. sc0 + ns - 1 in one launch
+void launch_sa_level(const DevGrid &g, int sc0, int ns, long nnz, const int *cells, int lo, int hi, DevState &s,
+                     hipStream_t st);
 // scalar line SGS: one level of a sweep; relaxation + stopping test into
 // DevState::lst (shares the block solver's lst/lparts)
-void launch_sa_lsgs_level(const DevGrid &g, int backward, const int *cells, int lo, int hi, DevState &s,
+void launch_sa_lsgs_level(const DevGrid &g, int sc, long nnz, int backward, const int *cells, int lo, int hi,
+                          DevState &s, hipStream_t st);
+void launch_sa_lsgs_relax(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
                           hipStream_t st);
-void launch_sa_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st);
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // weighted r.r, r.q, q.q of r = B - A X0, q = A D into out[0..2] (needs 3 * nblk(nc) parts)
 void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
@@ -39,8 +43,8 @@ void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream
 void launch_axpy_d(long n, const double *a, double alpha, const double *d, double *x, hipStream_t st);
 // periodic ghost entries V[nc + i] = V[partner cell], 5-wide rows
 void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st);
-// the same for a scalar per cell (V holds nc + nper entries)
-void launch_ghost1(const DevGrid &g, double *V, hipStream_t st);
+// the same for ns scalars per cell (V holds ns x (nc + nper + 1) entries)
+void launch_ghost1(const DevGrid &g, int ns, double *V, hipStream_t st);
 // line SGS
 int lsgs_parts(const DevGrid &g);
 void launch_lsgs_init(DevState &s, double w0, hipStream_t st);

@@ -72,7 +72,25 @@ __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *
   qf[2 * nb + b] = o.u.y;
   qf[3 * nb + b] = o.u.z;
   qf[4 * nb + b] = o.pg;
-  if (p.turb) {  // nuTilde_f: turbulence/SA/SA_turb.loci:181-183, src/solverScalarTransport.loci:423-463
+  if (p.turb == 2) {
+    // k_f, w_f: inflow/outflow switch on u_f.n (turbulence/KOmega/sst_viscous.loci:131-144),
+    // Menter's wall values k = 0, omega = 60 mu/(rho_f beta1 d^2) with the
+    // cell's mu and wall distance (:236-243), symmetry copies the cell
+    const double ki = q[5 * nc + c], wi = q[6 * nc + c];
+    double kb = ki, wb = wi;
+    if (t == 2) {
+      const double rho_f = (o.pg + p.Pamb) / (p.Rt * o.T);
+      const double d = g.dist[c];
+      kb = 0.0;
+      wb = 10.0 * 6.0 * suth_mu(p, i.T) / (rho_f * 0.075 * pow(d, 2.0));
+    } else if (t == 1) {
+      kb = p.k_bc;
+      wb = p.w_bc;
+      if (dot(o.u, n) >= -0.0) { kb = ki; wb = wi; }
+    }
+    qf[5 * nb + b] = kb;
+    qf[6 * nb + b] = wb;
+  } else if (p.turb) {  // nuTilde_f: turbulence/SA/SA_turb.loci:181-183, src/solverScalarTransport.loci:423-463
     const double Xi = q[5 * nc + c];
     double Xb = Xi;
     if (t == 2) Xb = 0.0;
@@ -1004,10 +1022,11 @@ __global__ void k_ghost(DevGrid g, double *__restrict__ X) {
   for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + pi) * 5 + k] = X[(size_t)src * 5 + k];
 }
 
-__global__ void k_ghost1(DevGrid g, double *__restrict__ X) {
+__global__ void k_ghost1(DevGrid g, int ns, double *__restrict__ X) {
   const int pi = blockIdx.x * blockDim.x + threadIdx.x;
   if (pi >= g.nper) return;
-  X[g.nc + pi] = X[g.frc[g.nfi + pi]];
+  const size_t xs = (size_t)g.nc + g.nper + 1;
+  for (int sc = 0; sc < ns; ++sc) X[sc * xs + g.nc + pi] = X[sc * xs + g.frc[g.nfi + pi]];
 }
 
 // ---------------------------------------------------------------------------
@@ -1149,11 +1168,117 @@ __device__ __forceinline__ double sa_fv1(const Prm &p, double x) {
   return pow(x, 3.0) / (pow(x, 3.0) + pow(p.sa_cv1, 3.0));
 }
 
-// eddy viscosity tmu = nuTilde fv1 rho (:93-107)
-__global__ void k_tmu(DevGrid g, Prm p, const double *__restrict__ q, double *__restrict__ tmu) {
+// ---------------------------------------------------------------------------
+// Menter SST k-omega (turbulence/KOmega/sst.loci) with the default options:
+// turbulence_model SST, kprod_func SOmega, compressibilityCorrection wilcox,
+// dissipationCorrection none, smooth walls, Kprod_clip 1e100, tmu_clip 1e5.
+// sst1 (k-omega set) and sst2 (k-epsilon set), sst.loci:319-347; sigma_k of
+// the SST model 0.85 (:682-684).
+// ---------------------------------------------------------------------------
+constexpr double SST_SIGK1 = 0.85, SST_SIGW1 = 0.5, SST_BETA1 = 0.075, SST_BETAS = 0.09, SST_KAPPA = 0.41,
+                 SST_A1 = 0.31, SST_SIGK2 = 1.0, SST_SIGW2 = 0.856, SST_BETA2 = 0.0828;
+
+// strainRate_t, divu_t (sst.loci:65-104), vortMag_t (src/solverTurbulence.loci:45-49)
+// of gu[3i+j] = d u_i / d x_j
+__device__ __forceinline__ void sst_kinematics(const double *gu, double &S, double &divu, double &Om) {
+  const double dudx = gu[0], dudy = gu[1], dudz = gu[2];
+  const double dvdx = gu[3], dvdy = gu[4], dvdz = gu[5];
+  const double dwdx = gu[6], dwdy = gu[7], dwdz = gu[8];
+  divu = (dudx + dvdy + dwdz);
+  const double S2 = 2. * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + (dudy + dvdx) * (dudy + dvdx) +
+                    (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
+  S = sqrt(S2);
+  const double wx = dwdy - dvdz, wy = dudz - dwdx, wz = dvdx - dudy;
+  Om = sqrt(wx * wx + wy * wy + wz * wz);
+}
+// bF2 (:514-521) and kw_tmu with bF3 = 1 (:768-771)
+__device__ __forceinline__ double sst_F2(double k, double w, double y, double nu) {
+  const double arg2 = dmin(dmax(2.0 * sqrt(k) / (0.09 * w * y), 500.0 * nu / (w * y * y)), 1000.0);
+  return tanh(arg2 * arg2);
+}
+__device__ __forceinline__ double sst_tmu(double rho, double k, double w, double S, double F2) {
+  return dmin(rho * SST_A1 * k / dmax(1e-30, dmax(SST_A1 * w, S * F2 * 1.0)), 1e5);
+}
+__device__ __forceinline__ void cell_gu(const double *__restrict__ grad, int nc, int c, double *gu) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gu[3 * i + j] = grad[(3 * (1 + i) + j) * nc + c];
+}
+
+struct SstCell {
+  double F1, sigk, sigw, ksrc, wsrc, kJ, wJ;
+};
+// One cell: crossDiffusionTerm (:464-468), bF1 (:487-499), bF2, kw_tmu, the
+// blended coefficients (:702-711), the SOmega stress (:790-796), P_k
+// (:851-854), k and omega sources and destruction Jacobians (:870-904),
+// cross diffusion (:910-913), Wilcox compressibility (:283-287, 960-1006);
+// each source keeps the file's join order.
+__device__ SstCell sst_core(const Prm &p, double T, double pg, double k, double w, const double *gu, double y,
+                            double gkgw, double vol) {
+  SstCell o;
+  const double rho = (pg + p.Pamb) / (p.Rt * T);
+  const double mu = suth_mu(p, T);
+  const double nu = mu / rho;
+  double S, divu, Om;
+  sst_kinematics(gu, S, divu, Om);
+  const double crossterm = dmax(SST_SIGW2 * gkgw, 0.0);
+  const double CD = 2. * rho * crossterm / dmax(w, 1e-30);
+  const double CDkw = dmax(1.0e-10, CD);
+  const double ls = dmax(sqrt(k) / (0.09 * w * y), 500.0 * nu / (w * y * y));
+  const double arg1 = dmin(dmin(ls, 4. * rho * SST_SIGW2 * k / (CDkw * y * y)), 32.0);
+  const double arg1_2 = arg1 * arg1, arg1_4 = arg1_2 * arg1_2;
+  const double F1 = tanh(arg1_4);
+  const double F2 = sst_F2(k, w, y, nu);
+  const double tmu = sst_tmu(rho, k, w, S, F2);
+  const double beta = (F1 * SST_BETA1 + (1.0 - F1) * SST_BETA2);
+  const double gama1 = SST_BETA1 / SST_BETAS - SST_SIGW1 * pow(SST_KAPPA, 2.0) / sqrt(SST_BETAS);
+  const double gama2 = SST_BETA2 / SST_BETAS - SST_SIGW2 * pow(SST_KAPPA, 2.0) / sqrt(SST_BETAS);
+  const double gama = F1 * gama1 + (1.0 - F1) * gama2;
+  const double stress = dmax(tmu * (S * Om - 2. / 3. * divu * divu) - 2. / 3. * rho * k * divu, 0.0);
+  const double BetaS = SST_BETAS * 1.0;
+  const double Pk = dmin(stress, (1e100) * rho * BetaS * k * w);
+  const double a = sqrt(p.gamma * p.Rt * T);
+  const double Mt2 = 2.0 * k / (a * a);
+  const double mt0 = 0.25;
+  const double fmt = dmax(Mt2 - mt0 * mt0, 0.0);
+  const double BetaF = SST_BETAS * 1.0 * 1.5 * fmt;
+  const double nut = tmu / rho;
+  const double Dw = (gama / (nut + 1e-30)) * stress;
+  double ks = 0.0, ws = 0.0;
+  ks += Pk * vol;
+  ws += Dw * vol;
+  ks -= BetaS * rho * w * k * vol;
+  ws -= beta * rho * w * w * vol;
+  ws += (1.0 - F1) * CD * vol;
+  ks -= BetaF * rho * w * k * vol;
+  ws -= -BetaF * rho * w * w * vol * 1.0;
+  o.F1 = F1;
+  o.sigk = F1 * SST_SIGK1 + (1.0 - F1) * SST_SIGK2;
+  o.sigw = F1 * SST_SIGW1 + (1.0 - F1) * SST_SIGW2;
+  o.ksrc = ks;
+  o.wsrc = ws;
+  o.kJ = -SST_BETAS * 1.0 * rho * w * vol;
+  o.wJ = -2.0 * beta * rho * w * vol;
+  return o;
+}
+
+// eddy viscosity: SA tmu = nuTilde fv1 rho (:93-107); SST kw_tmu
+__global__ void k_tmu(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ grad,
+                      double *__restrict__ tmu) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
+  if (p.turb == 2) {
+    const double T = q[c], pg = q[4 * nc + c], k = q[5 * nc + c], w = q[6 * nc + c];
+    const double rho = (pg + p.Pamb) / (p.Rt * T);
+    const double nu = suth_mu(p, T) / rho;
+    double gu[9], S, divu, Om;
+    cell_gu(grad, nc, c, gu);
+    sst_kinematics(gu, S, divu, Om);
+    tmu[c] = sst_tmu(rho, k, w, S, sst_F2(k, w, g.dist[c], nu));
+    return;
+  }
   const double T = q[c], pg = q[4 * nc + c], nut = q[5 * nc + c];
   const double rho = (pg + p.Pamb) / (p.Rt * T);
   const double x = rho * nut / suth_mu(p, T);
@@ -1408,18 +1533,250 @@ __global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const 
   sX[c] = dinv * b;
 }
 
+// SST per-face pass 1, saf rows 0-1: area grad_f(k).n and its omega-weighted
+// form (w_f = max(0, mean of the reconstructions) on interior faces, the
+// face value on boundary faces), sst.loci:427-462
+__device__ __forceinline__ double recon_k(const double *__restrict__ q, const double *__restrict__ grad,
+                                          const double *__restrict__ phi, int nc, int k, int cc, const double *xc,
+                                          const double *xf) {
+  const double d0 = xf[0] - xc[0], d1 = xf[1] - xc[1], d2 = xf[2] - xc[2];
+  const double gd = grad[(3 * k) * nc + cc] * d0 + grad[(3 * k + 1) * nc + cc] * d1 + grad[(3 * k + 2) * nc + cc] * d2;
+  return q[k * nc + cc] + phi[k * nc + cc] * gd;
+}
+
+__global__ void k_sst_face_a(DevGrid g, const double *__restrict__ q, const double *__restrict__ qf,
+                             const double *__restrict__ grad, const double *__restrict__ phi,
+                             double *__restrict__ saf) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.nf) return;
+  const int nc = g.nc, nf = g.nf, nb = g.nb;
+  const int c = g.fcl[f];
+  const double an[3] = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double xf[3] = {g.fcx[f], g.fcy[f], g.fcz[f]};
+  const double xl[3] = {g.ccx[c], g.ccy[c], g.ccz[c]};
+  double gk[3];
+  if (f < g.bnd_begin) {
+    const int o = g.frc[f];
+    double xr[3] = {g.ccx[o], g.ccy[o], g.ccz[o]};
+    if (f >= g.nfi) {
+      const int pi = f - g.nfi;
+      xr[0] += g.psx[pi]; xr[1] += g.psy[pi]; xr[2] += g.psz[pi];
+    }
+    const double dvec[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+    const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+    face_grad_int(g, q, grad, c, o, 5, dvec, ddsq, gk);
+    const double gkn = g.farea[f] * (gk[0] * an[0] + gk[1] * an[1] + gk[2] * an[2]);
+    const double wL = recon_k(q, grad, phi, nc, 6, c, xl, xf), wR = recon_k(q, grad, phi, nc, 6, o, xr, xf);
+    const double w_f = dmax(0.0, 0.5 * (wL + wR));
+    saf[f] = gkn;
+    saf[(size_t)nf + f] = w_f * gkn;
+  } else {
+    const int b = f - g.bnd_begin;
+    const double dvec[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+    const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+    face_grad_bnd(g, q, grad, qf[5 * nb + b], c, 5, dvec, ddsq, gk);
+    const double gkn = g.farea[f] * (gk[0] * an[0] + gk[1] * an[1] + gk[2] * an[2]);
+    saf[f] = gkn;
+    saf[(size_t)nf + f] = qf[6 * nb + b] * gkn;
+  }
+}
+
+// SST per-cell pass 1: grad k . grad omega = (div(w grad k) - w lap k) / vol
+// gathered in the lower/upper/boundary order, then sst_core into
+// scell[8][nc]: F1, grad k.grad w, sigma_k, sigma_w, k and omega sources,
+// k and omega Jacobians
+__global__ void k_sst_cell_a(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ grad,
+                             const double *__restrict__ saf, double *__restrict__ scell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc, nf = g.nf;
+  double lap = 0.0, dwg = 0.0;
+  for (int j = g.g_off[c]; j < g.g_off[c + 1]; ++j) {
+    const int e = g.g_ent[j];
+    if (e >= 0) {
+      lap += -saf[e];
+      dwg += -saf[(size_t)nf + e];
+    } else {
+      lap += saf[~e];
+      dwg += saf[(size_t)nf + ~e];
+    }
+  }
+  const double w = q[6 * nc + c];
+  const double gkgw = (dwg - w * lap) / g.vol[c];
+  double gu[9];
+  cell_gu(grad, nc, c, gu);
+  const SstCell o = sst_core(p, q[c], q[4 * nc + c], q[5 * nc + c], w, gu, g.dist[c], gkgw, g.vol[c]);
+  scell[c] = o.F1;
+  scell[(size_t)nc + c] = gkgw;
+  scell[2 * (size_t)nc + c] = o.sigk;
+  scell[3 * (size_t)nc + c] = o.sigw;
+  scell[4 * (size_t)nc + c] = o.ksrc;
+  scell[5 * (size_t)nc + c] = o.wsrc;
+  scell[6 * (size_t)nc + c] = o.kJ;
+  scell[7 * (size_t)nc + c] = o.wJ;
+}
+
+// SST per-face pass 2, for k (saf rows 2-6) and omega (rows 7-11): upwind
+// convection of max(X, 0), diffusion X_nu_f grad_f(X).n area with X_nu_f =
+// mu_f + tmu_f sigma_f, fjp, fjm (also into the scalar rows soff), and the
+// boundary Jacobian (src/solverScalarTransport.loci:121-135, 548-567, 779-826)
+__global__ void k_sst_face_b(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+                             const double *__restrict__ grad, const double *__restrict__ phi,
+                             const double *__restrict__ iflux, const double *__restrict__ tmu,
+                             const double *__restrict__ scell, double *__restrict__ saf, double *__restrict__ soff,
+                             long nnz) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.nf) return;
+  const int nc = g.nc, nf = g.nf, nb = g.nb;
+  const int c = g.fcl[f];
+  const double an[3] = {g.fnx[f], g.fny[f], g.fnz[f]};
+  const double xf[3] = {g.fcx[f], g.fcy[f], g.fcz[f]};
+  const double xl[3] = {g.ccx[c], g.ccy[c], g.ccz[c]};
+  const double area = g.farea[f];
+  const double mdot = iflux[f];
+  const double tmf = tmu_face(g, tmu, f);
+  for (int sc = 0; sc < 2; ++sc) {
+    const int kv = 5 + sc;
+    const double sl = scell[(2 + sc) * (size_t)nc + c];
+    double v[5] = {0, 0, 0, 0, 0};
+    if (f < g.bnd_begin) {
+      const int o = g.frc[f];
+      double xr[3] = {g.ccx[o], g.ccy[o], g.ccz[o]};
+      if (f >= g.nfi) {
+        const int pi = f - g.nfi;
+        xr[0] += g.psx[pi]; xr[1] += g.psy[pi]; xr[2] += g.psz[pi];
+      }
+      const double XL = recon_k(q, grad, phi, nc, kv, c, xl, xf), XR = recon_k(q, grad, phi, nc, kv, o, xr, xf);
+      const double Xval = (mdot < 0.0) ? XR : XL;
+      v[0] = mdot * dmax(Xval, 0.0);
+      const double vl = g.vol[c], vr = g.vol[o];
+      const double sr = scell[(2 + sc) * (size_t)nc + o];
+      const double rvlvr = 1. / (vl + vr);
+      const double sig_f = (vr * sl + vl * sr) * rvlvr;
+      const double mu_f = (vl * suth_mu(p, q[o]) + vr * suth_mu(p, q[c])) / (vl + vr);
+      const double nu_f = (mu_f + tmf * sig_f);
+      const double dvec[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+      const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+      double gx[3];
+      face_grad_int(g, q, grad, c, o, kv, dvec, ddsq, gx);
+      v[1] = nu_f * (gx[0] * an[0] + gx[1] * an[1] + gx[2] * an[2]) * area;
+      const double C = nu_f * area;
+      const double d = dvec[0] * an[0] + dvec[1] * an[1] + dvec[2] * an[2];
+      double fjp = 0.0, fjm = 0.0;
+      fjp += dmax(mdot, 0.0);
+      fjm += dmin(mdot, 0.0);
+      fjp += C / d;
+      fjm += -C / d;
+      v[2] = fjp;
+      v[3] = fjm;
+      soff[(size_t)sc * nnz + g.slot_cl[f]] = fjm;
+      if (f < g.nfi) soff[(size_t)sc * nnz + g.slot_cr[f]] = -fjp;
+    } else {
+      const int b = f - g.bnd_begin;
+      const double XL = recon_k(q, grad, phi, nc, kv, c, xl, xf);
+      const double Xf = qf[kv * nb + b];
+      const double Xval = (mdot < 0.0) ? Xf : XL;
+      v[0] = mdot * dmax(Xval, 0.0);
+      const double mu_f = suth_mu(p, qf[b]);
+      const double nu_f = (mu_f + tmf * sl);
+      const double dvec[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+      const double ddsq = dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2];
+      double gx[3];
+      face_grad_bnd(g, q, grad, Xf, c, kv, dvec, ddsq, gx);
+      v[1] = nu_f * (gx[0] * an[0] + gx[1] * an[1] + gx[2] * an[2]) * area;
+      const double C = nu_f * area;
+      const double d = dvec[0] * an[0] + dvec[1] * an[1] + dvec[2] * an[2];
+      double bj = 0.0;
+      bj -= dmax(mdot, 0.0);
+      bj += -C / d;
+      v[4] = bj;
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) saf[(size_t)(2 + 5 * sc + r) * nf + f] = v[r];
+  }
+}
+
+// SST per-cell pass 2: the k and omega rows in a fixed order (convection
+// lower/upper/boundary, stabilisation -X rhodot, cell sources, diffusion
+// lower/upper/boundary; Jacobian: stabilisation, cell, boundary faces,
+// upper fjp, lower fjm), D = -srcJ + rho V/dtau, B = src, Jacobi start
+__global__ void k_sst_cell_b(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ dtau,
+                             const double *__restrict__ iflux, const double *__restrict__ saf,
+                             const double *__restrict__ scell, double *__restrict__ ssrc, double *__restrict__ sdinv,
+                             double *__restrict__ sB, double *__restrict__ sX) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc, nf = g.nf;
+  const size_t xs = (size_t)nc + g.nper + 1;
+  const double rho = (q[4 * nc + c] + p.Pamb) / (p.Rt * q[c]);
+  const int j0 = g.g_off[c], j1 = g.g_off[c + 1];
+  for (int sc = 0; sc < 2; ++sc) {
+    const double *F = saf + (size_t)(2 + 5 * sc) * nf;
+    const double X = q[(5 + sc) * nc + c];
+    double sv = 0.0, sJ = 0.0, rhodot = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e >= 0) {
+        sv += F[e];
+        rhodot += iflux[e];
+      } else {
+        sv -= F[~e];
+        rhodot -= iflux[~e];
+      }
+    }
+    sv -= X * rhodot;
+    sJ += -rhodot;
+    sv += scell[(4 + sc) * (size_t)nc + c];
+    sJ += scell[(6 + sc) * (size_t)nc + c];
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e >= 0) sv -= F[(size_t)nf + e];
+      else sv += F[(size_t)nf + ~e];
+    }
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e < 0 && ~e >= g.bnd_begin) sJ += F[(size_t)4 * nf + ~e];
+    }
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e < 0 && ~e < g.bnd_begin) sJ -= F[(size_t)2 * nf + ~e];
+    }
+    for (int j = j0; j < j1; ++j) {
+      const int e = g.g_ent[j];
+      if (e >= 0) sJ += F[(size_t)3 * nf + e];
+    }
+    ssrc[(size_t)sc * nc + c] = sv;
+    const double sd = rho * (g.vol[c] / dtau[c]);
+    const double D = -sJ + sd;
+    const double rhs = 0.0 - sv;
+    const double b = -rhs;
+    const double dinv = 1. / D;
+    sB[(size_t)sc * nc + c] = b;
+    sdinv[(size_t)sc * nc + c] = dinv;
+    sX[sc * xs + c] = dinv * b;
+  }
+}
+
 // one level of a scalar SGS half-sweep, one lane per cell (the row holds
 // U = fjm for upper faces, then L = -fjp for lower faces)
-__global__ void k_sa_sgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi,
+__global__ void k_sa_sgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi, int sc0, int ns, long nnz,
                                const double *__restrict__ soff, const double *__restrict__ sdinv,
                                const double *__restrict__ sB, double *__restrict__ sX) {
-  const int k = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= hi) return;
+  // lanes [0, w) scalar sc0, [w, 2w) scalar sc0 + 1: the systems are independent
+  const int w = hi - lo;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= w * ns) return;
+  const int sc = sc0 + t0 / w;
+  const int k = lo + (t0 - (sc - sc0) * w);
   const int c = cells ? cells[k] : k;
-  double t = -sB[c];
-  for (int j = g.m_off[c]; j < g.m_off[c + 1]; ++j) t += sX[g.m_nbr[j]] * soff[j];
+  const int nc = g.nc;
+  const size_t xs = (size_t)nc + g.nper + 1;
+  const double *off = soff + (size_t)sc * nnz;
+  double *X = sX + (size_t)sc * xs;
+  double t = -sB[(size_t)sc * nc + c];
+  for (int j = g.m_off[c]; j < g.m_off[c + 1]; ++j) t += X[g.m_nbr[j]] * off[j];
   const double v = -t;
-  sX[c] = sdinv[c] * v;
+  X[c] = sdinv[(size_t)sc * nc + c] * v;
 }
 
 // one level of a scalar line-SGS sweep (src/linearSolverLSGS.loci:1240-1317):
@@ -1445,14 +1802,17 @@ __global__ void k_sa_lsgs_level(DevGrid g, const int *__restrict__ cells, int lo
 
 // X_next = X + min(1, urelax X/(|dX| + 1e-15)) dX, floored at 1e-15
 // (src/solverScalarTransport.loci:400-408)
-__global__ void k_sa_update(DevGrid g, Prm p, const double *__restrict__ sX, double *__restrict__ q) {
+__global__ void k_sa_update(DevGrid g, Prm p, int ns, const double *__restrict__ sX, double *__restrict__ q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  const double Xc = q[5 * (size_t)g.nc + c], dX = sX[c];
-  double tscale = 1.0;
-  tscale = dmin(tscale, p.urelax * Xc / (fabs(dX) + 1e-15));
-  const double Xn = Xc + tscale * dX;
-  q[5 * (size_t)g.nc + c] = dmax(Xn, 1e-15);
+  const size_t xs = (size_t)g.nc + g.nper + 1;
+  for (int sc = 0; sc < ns; ++sc) {
+    const double Xc = q[(5 + sc) * (size_t)g.nc + c], dX = sX[sc * xs + c];
+    double tscale = 1.0;
+    tscale = dmin(tscale, p.urelax * Xc / (fabs(dX) + 1e-15));
+    const double Xn = Xc + tscale * dX;
+    q[(5 + sc) * (size_t)g.nc + c] = dmax(Xn, 1e-15);
+  }
 }
 
 // damped Newton update, src/solverNewton.loci:62-78
@@ -1480,7 +1840,8 @@ void launch_bface(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   if (g.nb) hipLaunchKernelGGL(k_bface, dim3(nblk(g.nb)), dim3(TPB), 0, st, g, p, s.q, s.qf);
 }
 void launch_grad(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  if (p.turb) hipLaunchKernelGGL(k_grad<6>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
+  if (p.turb == 2) hipLaunchKernelGGL(k_grad<7>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
+  else if (p.turb) hipLaunchKernelGGL(k_grad<6>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
   else hipLaunchKernelGGL(k_grad<5>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.qf, s.grad, s.phi);
 }
 void launch_eta(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
@@ -1842,14 +2203,30 @@ void launch_axpy_d(long n, const double *a, double alpha, const double *d, doubl
 void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, V);
 }
-void launch_ghost1(const DevGrid &g, double *V, hipStream_t st) {
-  if (g.nper) hipLaunchKernelGGL(k_ghost1, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, V);
+static inline int nscal(const Prm &p) { return p.turb == 2 ? 2 : p.turb ? 1 : 0; }
+void launch_ghost1(const DevGrid &g, int ns, double *V, hipStream_t st) {
+  if (g.nper && ns) hipLaunchKernelGGL(k_ghost1, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, ns, V);
 }
 
 void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  if (p.turb) hipLaunchKernelGGL(k_tmu, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q, s.tmu);
+  if (p.turb)
+    hipLaunchKernelGGL(k_tmu, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q, (const double *)s.grad,
+                       s.tmu);
 }
-void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+void launch_sa_assemble(const DevGrid &g, const Prm &p, long nnz, DevState &s, hipStream_t st) {
+  if (p.turb == 2) {
+    hipLaunchKernelGGL(k_sst_face_a, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, (const double *)s.q,
+                       (const double *)s.qf, (const double *)s.grad, (const double *)s.phi, s.saf);
+    hipLaunchKernelGGL(k_sst_cell_a, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q,
+                       (const double *)s.grad, (const double *)s.saf, s.scell);
+    hipLaunchKernelGGL(k_sst_face_b, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, (const double *)s.q,
+                       (const double *)s.qf, (const double *)s.grad, (const double *)s.phi, (const double *)s.iflux,
+                       (const double *)s.tmu, (const double *)s.scell, s.saf, s.soff, nnz);
+    hipLaunchKernelGGL(k_sst_cell_b, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q,
+                       (const double *)s.dtau, (const double *)s.iflux, (const double *)s.saf,
+                       (const double *)s.scell, s.ssrc, s.sdinv, s.sB, s.sX);
+    return;
+  }
   hipLaunchKernelGGL(k_sa_face, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, (const double *)s.q,
                      (const double *)s.qf, (const double *)s.grad, (const double *)s.phi, (const double *)s.iflux,
                      s.saf, s.soff);
@@ -1857,31 +2234,36 @@ void launch_sa_assemble(const DevGrid &g, const Prm &p, DevState &s, hipStream_t
                      (const double *)s.grad, (const double *)s.dtau, (const double *)s.iflux,
                      (const double *)s.saf, s.ssrc, s.sdinv, s.sB, s.sX);
 }
-void launch_sa_level(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st) {
-  if (hi <= lo) return;
-  hipLaunchKernelGGL(k_sa_sgs_level, dim3((hi - lo + TPB - 1) / TPB), dim3(TPB), 0, st, g, cells, lo, hi,
-                     (const double *)s.soff, (const double *)s.sdinv, (const double *)s.sB, s.sX);
+void launch_sa_level(const DevGrid &g, int sc0, int ns, long nnz, const int *cells, int lo, int hi, DevState &s,
+                     hipStream_t st) {
+  if (hi <= lo || ns <= 0) return;
+  const long lanes = (long)(hi - lo) * ns;
+  hipLaunchKernelGGL(k_sa_sgs_level, dim3((unsigned)((lanes + TPB - 1) / TPB)), dim3(TPB), 0, st, g, cells, lo, hi,
+                     sc0, ns, nnz, (const double *)s.soff, (const double *)s.sdinv, (const double *)s.sB, s.sX);
 }
-void launch_sa_lsgs_level(const DevGrid &g, int backward, const int *cells, int lo, int hi, DevState &s,
-                          hipStream_t st) {
+void launch_sa_lsgs_level(const DevGrid &g, int sc, long nnz, int backward, const int *cells, int lo, int hi,
+                          DevState &s, hipStream_t st) {
   if (hi <= lo) return;
   const dim3 grid((hi - lo + TPB - 1) / TPB);
+  const size_t nc = g.nc, xs = nc + g.nper + 1;
+  const double *off = s.soff + sc * nnz, *dinv = s.sdinv + sc * nc, *B = s.sB + sc * nc;
+  double *X = s.sX + sc * xs, *xt = s.sxt + sc * nc;
   if (backward)
-    hipLaunchKernelGGL(k_sa_lsgs_level<true>, grid, dim3(TPB), 0, st, g, cells, lo, hi, (const double *)s.soff,
-                       (const double *)s.sdinv, (const double *)s.sB, s.sX, s.sxt);
+    hipLaunchKernelGGL(k_sa_lsgs_level<true>, grid, dim3(TPB), 0, st, g, cells, lo, hi, off, dinv, B, X, xt);
   else
-    hipLaunchKernelGGL(k_sa_lsgs_level<false>, grid, dim3(TPB), 0, st, g, cells, lo, hi, (const double *)s.soff,
-                       (const double *)s.sdinv, (const double *)s.sB, s.sX, s.sxt);
+    hipLaunchKernelGGL(k_sa_lsgs_level<false>, grid, dim3(TPB), 0, st, g, cells, lo, hi, off, dinv, B, X, xt);
 }
-void launch_sa_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st) {
+void launch_sa_lsgs_relax(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
+                          hipStream_t st) {
   const int np = nblk(g.nc);
-  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)s.lst, s.sX,
-                     (const double *)s.sxt, s.lparts);
+  const size_t nc = g.nc, xs = nc + g.nper + 1;
+  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)s.lst, s.sX + sc * xs,
+                     (const double *)(s.sxt + sc * nc), s.lparts);
   hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
                      reltol);
 }
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_sa_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.sX, s.q);
+  hipLaunchKernelGGL(k_sa_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, nscal(p), (const double *)s.sX, s.q);
 }
 
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st) {

@@ -68,14 +68,17 @@ struct DevState {
   double *eta;    // [nc] Eta_p (read only when Prm::pre)
   double *eta_t;  // [nc] Eta_l / Eta_ll scratch
   // Spalart-Allmaras (allocated when selected): q holds nuTilde as its 6th row
+  // turbulence scalars (ns = 1 nuTilde, ns = 2 k and omega), scalar j at
+  // offset j x (its per-scalar size)
   double *tmu;    // [nc] eddy viscosity
-  double *saf;    // [10][nf] per-face terms of the nuTilde system
-  double *soff;   // [nnz] scalar row coefficients (U = fjm, L = -fjp)
-  double *ssrc;   // [nc] X_src
-  double *sdinv;  // [nc] 1 / D
-  double *sB;     // [nc]
-  double *sX;     // [nc + nper + 1] scalar solution with ghosts and the zero entry
-  double *sxt;    // [nc] scalar line SGS: each cell's value before the forward sweep
+  double *saf;    // [12][nf] per-face terms of the scalar systems
+  double *scell;  // [8][nc] SST per-cell terms: F1, grad k.grad w, sigma_k, sigma_w, k/w sources, k/w Jacobians
+  double *soff;   // [ns][nnz] scalar row coefficients (U = fjm, L = -fjp)
+  double *ssrc;   // [ns][nc] X_src
+  double *sdinv;  // [ns][nc] 1 / D
+  double *sB;     // [ns][nc]
+  double *sX;     // [ns][nc + nper + 1] scalar solutions with ghosts and the zero entry
+  double *sxt;    // [ns][nc] scalar line SGS: each cell's value before the forward sweep
   // line SGS (allocated when that solver is selected)
   double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
   double *xt;     // [nc][5] X after the forward sweep

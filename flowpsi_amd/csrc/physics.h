@@ -20,10 +20,12 @@ struct Prm {
   // low-Mach preconditioning: pre = 0 when Mlim2 >= 1 (eta = 1 everywhere)
   int pre;
   double Mlim2, PLimPC;
-  // Spalart-Allmaras (turb = 1): constants, Cp, turbulent Prandtl number,
-  // nuTilde of inflow faces
+  // turbulence: 0 none, 1 Spalart-Allmaras (constants, nuTilde of inflow
+  // faces), 2 Menter SST (k and omega of inflow faces); Cp and the
+  // turbulent Prandtl number of the eddy conductivity
   int turb;
   double sa_cb1, sa_k, sa_cv1, sa_cb2, sa_cw2, sa_cw3, sa_sigma, Cp, Prt, nut_bc;
+  double k_bc, w_bc;
 };
 
 struct V3 {

@@ -55,4 +55,24 @@ def channel_sa(n=32, seed=2, nx=None, ny=None, nz=None, nut_ratio=3.0, **over):
     return g, q, prm
 
 
+def channel_sst(n=32, seed=2, nx=None, ny=None, nz=None, tmu_ratio=10.0, **over):
+    """The channel with Menter's SST model: k = 1.5 (5 % of the peak speed)^2
+    and omega = k / (tmu_ratio nu) (each with seeded noise) plus the
+    near-wall 6 nu / (beta1 d^2), wall distance in g.meta["dist"]."""
+    import numpy as np_
+    g, q5, prm = channel(n, seed, nx, ny, nz)
+    rng = np_.random.default_rng(seed + 200)
+    rho = q5[4] / (grids.R_AIR * q5[0])
+    nu = grids.sutherland_mu(q5[0]) / rho
+    dist = grids.wall_distance(g)
+    umax = float(np_.abs(q5[1:4]).max())
+    k = 1.5 * (0.05 * umax) ** 2 * (1.0 + 0.2 * rng.uniform(-1, 1, g.nc))
+    w = k / (tmu_ratio * nu) * (1.0 + 0.1 * rng.uniform(-1, 1, g.nc)) + 6.0 * nu / (0.075 * dist ** 2)
+    q = np_.vstack([q5, k[None, :], w[None, :]])
+    g.meta["dist"] = dist
+    prm = dict(prm, turb="sst")
+    prm.update(over)
+    return g, q, prm
+
+
 WORKLOADS = {"box32": dict(fn=box, steps=50), "channel128": dict(fn=channel, steps=None)}

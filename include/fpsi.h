@@ -45,10 +45,12 @@ enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
 /* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, or line SGS
  * (src/linearSolverLSGS.loci) */
 enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
-/* turbulence model (turbulence/SA/SA_turb.loci): none, or Spalart-Allmaras
- * (baseline, standard production) solved segregated like the reference;
- * the state then carries nuTilde as its 6th row */
-enum { FPSI_TURB_NONE = 0, FPSI_TURB_SA = 1 };
+/* turbulence model, solved segregated like the reference: none,
+ * Spalart-Allmaras (turbulence/SA/SA_turb.loci; baseline, standard
+ * production; the state carries nuTilde as its 6th row), or Menter SST
+ * (turbulence/KOmega/sst.loci with its default options; the state carries
+ * k and omega as rows 6 and 7) */
+enum { FPSI_TURB_NONE = 0, FPSI_TURB_SA = 1, FPSI_TURB_SST = 2 };
 /* How a Gauss-Seidel half-sweep runs (same result bit for bit):
  * PER_LEVEL  one launch per dependency level (default);
  * PERSISTENT one cooperative launch, per-level completion counters;
@@ -109,6 +111,8 @@ typedef struct {
   int turb;             /* FPSI_TURB_* */
   double Prt;           /* turbulentPrandtlNumber (<= 0: 0.9) */
   double nut_bc;        /* nuTilde of inflow faces (<= 0: 5e-9, the BC default) */
+  double k_bc, w_bc;    /* SST: k and omega of inflow faces (<= 0: the BC defaults from |u0|,
+                           turbulence/KOmega/sst_viscous.loci:63-118) */
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -141,8 +145,8 @@ int fpsi_get_info(fpsi_ctx *ctx, fpsi_info *info);
 int fpsi_set_params(fpsi_ctx *ctx, const fpsi_params *params);
 
 /* temperature, u, gagePressure stores (include/flowPsi.lh:53-65):
- * q[5][nc] (T, u, v, w, pg), or q[6][nc] with nuTilde under FPSI_TURB_SA;
- * fpsi_set_params first. */
+ * q[5][nc] (T, u, v, w, pg); q[6][nc] with nuTilde under FPSI_TURB_SA;
+ * q[7][nc] with k and omega under FPSI_TURB_SST; fpsi_set_params first. */
 int fpsi_set_state(fpsi_ctx *ctx, const double *q);
 int fpsi_get_state(fpsi_ctx *ctx, double *q);
 
@@ -201,6 +205,17 @@ int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
 int fpsi_sa_get_system(fpsi_ctx *ctx, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu);
 int fpsi_sa_solve(fpsi_ctx *ctx, int nsweeps, double *dX);
 int fpsi_sa_lsgs_stats(fpsi_ctx *ctx, int *iters, double *relax, double *resid);
+
+/* The same for any turbulence scalar: which = 0 nuTilde (SA) or k (SST),
+ * 1 omega (SST).  Under SST the k and omega systems are the
+ * scalarTransportP(k | w, SST_BSL_KW) instances (turbulence/KOmega/
+ * sst.loci:856-860) with the sources, Jacobians and diffusion coefficients
+ * of sst.loci:425-1006 and the generic scalar rules of
+ * src/solverScalarTransport.loci. */
+int fpsi_turb_get_system(fpsi_ctx *ctx, int which, double *src, double *fjp, double *fjm, double *D, double *B,
+                         double *tmu);
+int fpsi_turb_solve(fpsi_ctx *ctx, int which, int nsweeps, double *dX);
+int fpsi_turb_lsgs_stats(fpsi_ctx *ctx, int which, int *iters, double *relax, double *resid);
 
 /* Outcome of the last line-SGS solve: sweeps run, final relaxation w and
  * update norm (the diagnostics of src/linearSolverLSGS.loci:1934-1940). */

@@ -35,7 +35,8 @@ class OgParams(C.Structure):
                 ("lsgs_relax", C.c_double), ("Minf", C.c_double), ("etaT", C.c_double),
                 ("PLimPC", C.c_double), ("turb", C.c_int), ("sa_cb1", C.c_double), ("sa_k", C.c_double),
                 ("sa_cv1", C.c_double), ("sa_cb2", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double),
-                ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double)]
+                ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double), ("k_bc", C.c_double),
+                ("w_bc", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -120,13 +121,25 @@ def params(p: dict) -> OgParams:
     s.Minf = p.get("Minf", 1.0)
     s.etaT = p.get("etaT", 0.25)
     s.PLimPC = p.get("PLimPC", 10.0)
-    s.turb = {"none": 0, "sa": 1}[p.get("turb", "none")]
+    s.turb = {"none": 0, "sa": 1, "sst": 2}[p.get("turb", "none")]
     # Spa_All_param defaults, turbulence/SA/SA_param.h:25-34
     s.sa_cb1, s.sa_k, s.sa_cv1, s.sa_cb2 = 0.1355, 0.41, 7.1, 0.622
     s.sa_cw2, s.sa_cw3, s.sa_sigma = 0.3, 2.0, 2. / 3.
     s.Prt = p.get("Prt", 0.9)
     s.nut_bc = p.get("nut_bc", 5e-9)
+    kb, wb = sst_bc_defaults(s.u0)
+    s.k_bc = p.get("k_bc", kb)
+    s.w_bc = p.get("w_bc", wb)
     return s
+
+
+def sst_bc_defaults(u0):
+    """k and omega of an inflow boundary without k/w options
+    (turbulence/KOmega/sst_viscous.loci:63-118): k = max(1.5 (0.01 |u|)^2,
+    1e-3), omega = 0.09 k / (1e-3 * 1e-5)."""
+    v = float(np.linalg.norm(np.asarray(list(u0), float)))
+    k = max(1.5 * (1e-2 * v) * (1e-2 * v), 1e-3)
+    return k, 0.09 * k / (1e-3 * 1e-5)
 
 
 class Oracle:
@@ -150,15 +163,19 @@ class Oracle:
 
     def face_states(self, q):
         q = np.ascontiguousarray(q, np.float64)
-        qf = np.zeros((6, self.g.nf))
+        qf = np.zeros((7, self.g.nf))
         self.L.og_face_states(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf))
-        return np.ascontiguousarray(qf[:6 if self.p.turb else 5])
+        return np.ascontiguousarray(qf[:self.nq])
+
+    @property
+    def nq(self):
+        return {0: 5, 1: 6, 2: 7}[self.p.turb]
 
     def gradients(self, q, qf=None):
         q = np.ascontiguousarray(q, np.float64)
         if qf is None:
             qf = self.face_states(q)
-        nq = 6 if self.p.turb else 5
+        nq = self.nq
         grad = np.zeros((3 * nq, self.g.nc)); phi = np.zeros((nq, self.g.nc))
         self.L.og_gradients(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qf), _ptr(grad), _ptr(phi))
         return grad, phi
@@ -205,6 +222,26 @@ class Oracle:
         self.L.og_sa_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(out["src"]), _ptr(out["fjp"]),
                               _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"]))
         return out
+
+    def sst_assemble(self, q):
+        """k and omega systems: src, D, B [2][nc]; fjp, fjm [2][nf]; tmu;
+        aux F1, F2, gkgw (grad k . grad omega), CD (cross diffusion)."""
+        q = np.ascontiguousarray(q, np.float64)
+        nc, nf = self.g.nc, self.g.nf
+        out = {k: np.zeros(n) for k, n in (("src", (2, nc)), ("fjp", (2, nf)), ("fjm", (2, nf)),
+                                            ("D", (2, nc)), ("B", (2, nc)), ("tmu", nc), ("aux", (4, nc)))}
+        self.L.og_sst_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(out["src"]), _ptr(out["fjp"]),
+                               _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"]),
+                               _ptr(out["aux"]))
+        for i, k in enumerate(("F1", "F2", "gkgw", "CD")):
+            out[k] = out["aux"][i]
+        return out
+
+    @staticmethod
+    def scalar(sys_, j):
+        """Scalar system j (0 = k, 1 = omega) of an SST system, in the layout
+        sa_sgs / sa_lsgs take."""
+        return {k: np.ascontiguousarray(sys_[k][j]) for k in ("src", "fjp", "fjm", "D", "B")}
 
     def sa_sgs(self, sys_, nsweeps):
         X = np.zeros(self.g.nc)
@@ -260,6 +297,17 @@ def sa_point(prm, T, pg, nut, vort, y, vol=1.0):
     lib().og_sa_point(C.byref(p), C.c_double(T), C.c_double(pg), C.c_double(nut), C.c_double(vort),
                       C.c_double(y), C.c_double(vol), _ptr(out))
     return out
+
+
+def sst_point(prm, T, pg, k, w, gu, y, gkgw, vol=1.0):
+    """dict F1, F2, tmu, ksrc, wsrc, kJ, wJ, Pk, CD of one SST cell
+    (gu[3i+j] = d u_i / d x_j)."""
+    out = np.zeros(9)
+    p = params(dict(prm, turb="sst"))
+    gu = np.ascontiguousarray(gu, np.float64).ravel()
+    lib().og_sst_point(C.byref(p), C.c_double(T), C.c_double(pg), C.c_double(k), C.c_double(w), _ptr(gu),
+                       C.c_double(y), C.c_double(gkgw), C.c_double(vol), _ptr(out))
+    return dict(zip(("F1", "F2", "tmu", "ksrc", "wsrc", "kJ", "wJ", "Pk", "CD"), out))
 
 
 def hllc_flux(qL, qR, n, area=1.0, pamb=0.0, Rt=287.0, gamma=1.4, eta=1.0, fj="f64"):

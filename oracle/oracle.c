@@ -453,8 +453,8 @@ static inline int rcell(const og_grid *g, int f, real *x) {
   return g->cr[f];
 }
 
-/* variables per cell: T, u, v, w, pg (+ nuTilde) */
-static inline int nq_of(const og_params *p) { return p->turb ? 6 : 5; }
+/* variables per cell: T, u, v, w, pg (+ nuTilde, or + k, omega) */
+static inline int nq_of(const og_params *p) { return p->turb == 2 ? 7 : p->turb ? 6 : 5; }
 
 /* Neighbour of cell c across face f for the gradient stencil: its position
  * and primitive values (cell, periodic ghost, or boundary-face state). */
@@ -475,7 +475,7 @@ static void stencil_nb(const og_grid *g, const double *q, const double *qf, int 
   }
 }
 
-static real tmu_cell(const og_grid *g, const og_params *p, const double *q, int c);
+static real tmu_cell(const og_grid *g, const og_params *p, const double *q, const double *grad, int c);
 
 /* ------------------------------------------------------------------ */
 /* low-Mach preconditioning parameter, src/fluxInviscid.loci:35-155     */
@@ -505,7 +505,7 @@ double og_eta_face(const og_params *p, double T, const double *U) {
  * src/solverPeriodic.loci:39-66), Eta_l (:106-110), then the maximum over
  * the face neighbours twice (Eta_ll :116-126, Eta_p :132-142).  Maxima are
  * exact, so the join order does not matter. */
-void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p) {
+static void eta_impl(const og_grid *g, const og_params *p, const double *q, const double *grad, double *eta_p) {
   int nc = g->nc, nf = g->nf;
   const real ml2 = mlim2(p);
   if (ml2 >= 1.0) { /* Eta_l = max(M2/(1-M2), 1) = 1 for M2 <= 1/2 */
@@ -541,7 +541,7 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
         mref[l] = maxr(mref[l], nu / ((g->vol[l] / area) * a[l]));
         if (t == OG_INTERIOR) mref[r] = maxr(mref[r], nu / ((g->vol[r] / area) * a[r]));
       } else { /* :81-96; the cl side takes max(tmu_cl, tmu_cl) as written */
-        const real tl = tmu_cell(g, p, q, l), tr = tmu_cell(g, p, q, r);
+        const real tl = tmu_cell(g, p, q, grad, l), tr = tmu_cell(g, p, q, grad, r);
         const real nul = (maxr(mu[l], mu[r]) + maxr(tl, tl)) / rhom;
         mref[l] = maxr(mref[l], nul / ((g->vol[l] / area) * a[l]));
         if (t == OG_INTERIOR) {
@@ -569,6 +569,25 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
     if (pass == 0) { src = ell; dst = eta_p; }
   }
   free(mref); free(el); free(ell); free(rho); free(a); free(mu);
+}
+
+void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf);
+void og_gradients(const og_grid *g, const og_params *p, const double *q, const double *qf, double *grad,
+                  double *phi);
+/* Eta_p of a state; under SST the eddy viscosity needs the velocity
+ * gradient, so the face states and gradients are made first */
+void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p) {
+  if (p->turb != 2) {
+    eta_impl(g, p, q, NULL, eta_p);
+    return;
+  }
+  const int nq = nq_of(p);
+  double *qf = calloc((size_t)nq * g->nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 3 * nq * (size_t)g->nc), *phi = malloc(sizeof(double) * nq * (size_t)g->nc);
+  og_face_states(g, p, q, qf);
+  og_gradients(g, p, q, qf, grad, phi);
+  eta_impl(g, p, q, grad, eta_p);
+  free(qf); free(grad); free(phi);
 }
 
 /* ------------------------------------------------------------------ */
@@ -622,7 +641,26 @@ void og_face_states(const og_grid *g, const og_params *p, const double *q, doubl
       pgb = pgi + 0.5 * rho * U * uit;
       Tb = Ti;
     }
-    if (p->turb) { /* nuTilde_f: SA_turb.loci:181-183 (wall), solverScalarTransport.loci:423-463 */
+    if (p->turb == 2) {
+      /* k_f, w_f: inflow/outflow switch on u_f.n (turbulence/KOmega/sst_viscous.loci:131-144),
+       * Menter's wall values k = 0, omega = 60 mu/(rho_f beta1 d^2) with the
+       * cell's mu and wall distance (:236-243), symmetry copies the cell
+       * (src/solverScalarTransport.loci:423-427) */
+      const real ki = q[5 * nc + c], wi = q[6 * nc + c];
+      real kb = ki, wb = wi;
+      if (t == OG_WALL) {
+        const real rho_f = (pgb + p->Pambient) / (p->Rt * Tb);
+        const real d = g->dist[c];
+        kb = 0.0;
+        wb = 10.0 * 6.0 * suth_mu_(p, Ti) / (rho_f * 0.075 * pow(d, 2.0));
+      } else if (t == OG_FARFIELD) {
+        kb = p->k_bc;
+        wb = p->w_bc;
+        if (dot3(ub, n) >= -0.0) { kb = ki; wb = wi; }
+      }
+      qf[5 * nf + f] = kb;
+      qf[6 * nf + f] = wb;
+    } else if (p->turb) { /* nuTilde_f: SA_turb.loci:181-183 (wall), solverScalarTransport.loci:423-463 */
       const real Xi = q[5 * nc + c];
       real Xb = Xi;
       if (t == OG_WALL) Xb = 0.0;
@@ -699,17 +737,17 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
     real xc[3];
     ccenter(g, c, xc);
     const int nq = nq_of(p);
-    real qc[6];
+    real qc[7];
     for (int k = 0; k < nq; ++k) qc[k] = q[k * nc + c];
-    real b[6][3];
-    real qmin[6], qmax[6];
+    real b[7][3];
+    real qmin[7], qmax[7];
     for (int k = 0; k < nq; ++k) { b[k][0] = b[k][1] = b[k][2] = 0; qmin[k] = qmax[k] = qc[k]; }
     int lists[3][2] = {{a.lo_off[c], a.lo_off[c + 1]}, {a.up_off[c], a.up_off[c + 1]}, {a.bd_off[c], a.bd_off[c + 1]}};
     const int *arr[3] = {a.lo, a.up, a.bd};
     for (int L = 0; L < 3; ++L)
       for (int j = lists[L][0]; j < lists[L][1]; ++j) {
         int f = arr[L][j];
-        real x[3], v[6];
+        real x[3], v[7];
         stencil_nb(g, q, qf, c, f, nq, x, v);
         real dx = x[0] - xc[0], dy = x[1] - xc[1], dz = x[2] - xc[2];
         for (int k = 0; k < nq; ++k) {
@@ -719,7 +757,7 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
           qmax[k] = maxr(qmax[k], v[k]);
         }
       }
-    real gr[6][3];
+    real gr[7][3];
     for (int k = 0; k < nq; ++k) {
       gr[k][0] = Ai[0] * b[k][0] + Ai[1] * b[k][1] + Ai[2] * b[k][2];
       gr[k][1] = Ai[1] * b[k][0] + Ai[3] * b[k][1] + Ai[4] * b[k][2];
@@ -727,7 +765,7 @@ void og_gradients(const og_grid *g, const og_params *p, const double *q, const d
       for (int d = 0; d < 3; ++d) grad[(3 * k + d) * nc + c] = gr[k][d];
     }
     /* limiter */
-    real ph[6];
+    real ph[7];
     for (int k = 0; k < nq; ++k) ph[k] = 1.0;
     if (p->limiter == 2) {
       for (int k = 0; k < nq; ++k) ph[k] = 0.0;
@@ -814,7 +852,9 @@ static void face_grad(const og_grid *g, const double *q, const double *qf, const
 static real sa_fv1(const og_params *p, real x) {
   return pow(x, 3.0) / (pow(x, 3.0) + pow(p->sa_cv1, 3.0));
 }
-static real tmu_cell(const og_grid *g, const og_params *p, const double *q, int c) {
+static real sst_tmu_cell(const og_grid *g, const og_params *p, const double *q, const double *grad, int c);
+static real tmu_cell(const og_grid *g, const og_params *p, const double *q, const double *grad, int c) {
+  if (p->turb == 2) return sst_tmu_cell(g, p, q, grad, c);
   const int nc = g->nc;
   const real T = q[c], pg = q[4 * nc + c], nut = q[5 * nc + c];
   const real rho = (pg + p->Pambient) / (p->Rt * T);
@@ -824,17 +864,17 @@ static real tmu_cell(const og_grid *g, const og_params *p, const double *q, int 
 }
 /* tmu_f: volume-weighted interior average, the interior cell on boundary
  * faces, zero on no-slip walls (src/fluxViscous.loci:192-201) */
-static real tmu_face(const og_grid *g, const og_params *p, const double *q, int f) {
+static real tmu_face(const og_grid *g, const og_params *p, const double *q, const double *grad, int f) {
   const int t = g->btype[f];
   const int c = g->cl[f];
   if (t == OG_INTERIOR || t == OG_PERIODIC) {
     real xr[3];
     const int o = rcell(g, f, xr);
     const real vl = g->vol[c], vr = g->vol[o];
-    return (tmu_cell(g, p, q, o) * vl + tmu_cell(g, p, q, c) * vr) / (vr + vl);
+    return (tmu_cell(g, p, q, grad, o) * vl + tmu_cell(g, p, q, grad, c) * vr) / (vr + vl);
   }
   if (t == OG_WALL) return 0.0;
-  return tmu_cell(g, p, q, c);
+  return tmu_cell(g, p, q, grad, c);
 }
 
 /* Face viscous quantities: muTotal_f, kconductTotal, u_f and vflux[4]
@@ -865,7 +905,7 @@ static void viscous_face(const og_grid *g, const og_params *p, const double *q, 
   }
   if (t == OG_SYMMETRY) { /* src/fluxViscous.loci:294-297 */
     if (p->turb) {
-      const real tmf = tmu_face(g, p, q, f);
+      const real tmf = tmu_face(g, p, q, grad, f);
       *k_f = *k_f + tmf * (p->Rt * p->gamma / (p->gamma - 1.0)) / p->Prt;
       *mu_f = *mu_f + tmf;
     }
@@ -877,7 +917,7 @@ static void viscous_face(const og_grid *g, const og_params *p, const double *q, 
   face_grad(g, q, qf, grad, f, 0, gT);
   real mu_total = *mu_f;
   if (p->turb) { /* muTotal_f, kconductTotal: src/fluxViscous.loci:182-200,275-278 */
-    const real tmf = tmu_face(g, p, q, f);
+    const real tmf = tmu_face(g, p, q, grad, f);
     mu_total = *mu_f + tmf;
     *k_f = *k_f + tmf * (p->Rt * p->gamma / (p->gamma - 1.0)) / p->Prt;
     *mu_f = mu_total;
@@ -909,7 +949,7 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
   int nc = g->nc, nf = g->nf;
   const real us_n = 0.0;
   real *eta_p = malloc(sizeof(real) * nc);
-  og_eta(g, p, q, eta_p);
+  eta_impl(g, p, q, grad, eta_p);
   OG_PAR
   for (int f = 0; f < nf; ++f) {
     int t = g->btype[f];
@@ -1064,8 +1104,8 @@ static void residual_norms(const og_grid *g, const double *src, og_norms *r) {
 void og_residual(const og_grid *g, const og_params *p, const double *q, double *src,
                  double *iflux, double *vflux, og_norms *nrm) {
   int nc = g->nc, nf = g->nf;
-  double *qf = calloc((size_t)6 * nf, sizeof(double));
-  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
+  double *qf = calloc((size_t)7 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
   double *maxev = malloc(sizeof(double) * nf);
   og_face_states(g, p, q, qf);
   og_gradients(g, p, q, qf, grad, phi);
@@ -1257,8 +1297,8 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   real_fj *Draw = Draw_v;
   int nc = g->nc, nf = g->nf;
   real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
-  double *qf = calloc((size_t)6 * nf, sizeof(double));
-  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
+  double *qf = calloc((size_t)7 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
   double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
   double *maxev = malloc(sizeof(double) * nf);
   og_face_states(g, p, q, qf);
@@ -1272,7 +1312,7 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   /* face Jacobians */
   const real us_n = 0.0;
   real *eta_p = malloc(sizeof(real) * nc);
-  og_eta(g, p, q, eta_p);
+  eta_impl(g, p, q, grad, eta_p);
   real_fj *bblk = calloc((size_t)nf * 25, sizeof(real_fj));
   OG_PAR
   for (int f = 0; f < nf; ++f) {
@@ -1446,19 +1486,26 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
   double *X = malloc(sizeof(double) * 5 * (size_t)nc);
   real_fj *Draw = p->linear_solver == 1 ? malloc(sizeof(real_fj) * 25 * (size_t)nc) : NULL;
   double *ssrc = NULL, *sfp = NULL, *sfm = NULL, *sD = NULL, *sB = NULL, *sX = NULL;
-  if (p->turb) {
-    ssrc = malloc(sizeof(double) * nc); sfp = malloc(sizeof(double) * nf); sfm = malloc(sizeof(double) * nf);
-    sD = malloc(sizeof(double) * nc); sB = malloc(sizeof(double) * nc); sX = malloc(sizeof(double) * nc);
+  const int ns = p->turb == 2 ? 2 : p->turb ? 1 : 0; /* turbulence scalars */
+  if (ns) {
+    ssrc = malloc(sizeof(double) * ns * nc); sfp = malloc(sizeof(double) * ns * nf);
+    sfm = malloc(sizeof(double) * ns * nf); sD = malloc(sizeof(double) * ns * nc);
+    sB = malloc(sizeof(double) * ns * nc); sX = malloc(sizeof(double) * ns * nc);
   }
   for (int s = 0; s < nsteps; ++s) {
-    /* the nuTilde system from the state at the start of the step */
-    if (p->turb) {
-      og_sa_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL);
-      if (p->linear_solver == 1) /* LSGSScalarSolve, src/solverScalarTransport.loci:633-637 */
-        og_sa_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, sfp, sfm, sD, sB, sX, NULL,
-                   NULL);
-      else /* SGSScalarSolve, :627-631 */
-        og_sa_sgs(g, p->sgs_iter, sfp, sfm, sD, sB, sX);
+    /* the scalar systems from the state at the start of the step, each
+     * solved on its own (scalarTransportP(nuTilde | k | w, ...)) */
+    if (ns) {
+      if (p->turb == 2) og_sst_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL, NULL);
+      else og_sa_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL);
+      for (int j = 0; j < ns; ++j) {
+        const size_t oc = (size_t)j * nc, of = (size_t)j * nf;
+        if (p->linear_solver == 1) /* LSGSScalarSolve, src/solverScalarTransport.loci:633-637 */
+          og_sa_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, sfp + of, sfm + of, sD + oc,
+                     sB + oc, sX + oc, NULL, NULL);
+        else /* SGSScalarSolve, :627-631 */
+          og_sa_sgs(g, p->sgs_iter, sfp + of, sfm + of, sD + oc, sB + oc, sX + oc);
+      }
     }
     og_norms nrm;
     og_assemble2(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
@@ -1481,13 +1528,13 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       q[4 * nc + c] = q[4 * nc + c] + alpha * dp;
       for (int k = 1; k < 4; ++k) q[k * nc + c] = q[k * nc + c] + alpha * dq[k];
     }
-    if (p->turb) /* X_next, src/solverScalarTransport.loci:400-408 */
+    for (int j = 0; j < ns; ++j) /* X_next, src/solverScalarTransport.loci:400-408 */
       for (int c = 0; c < nc; ++c) {
-        const real Xc = q[5 * nc + c], dX = sX[c];
+        const real Xc = q[(5 + j) * nc + c], dX = sX[(size_t)j * nc + c];
         real tscale = 1.0;
         tscale = minr(tscale, p->urelax * Xc / (fabs(dX) + 1e-15));
         real Xn = Xc + tscale * dX;
-        q[5 * nc + c] = maxr(Xn, 1e-15);
+        q[(5 + j) * nc + c] = maxr(Xn, 1e-15);
       }
   }
   free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X); free(Draw);
@@ -1620,8 +1667,8 @@ void og_sa_point(const og_params *p, double T, double pg, double nut, double vor
 void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
                     double *fjm, double *D, double *B, double *tmu) {
   const int nc = g->nc, nf = g->nf;
-  double *qf = calloc((size_t)6 * nf, sizeof(double));
-  double *grad = malloc(sizeof(double) * 18 * nc), *phi = malloc(sizeof(double) * 6 * nc);
+  double *qf = calloc((size_t)7 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
   double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
   double *maxev = malloc(sizeof(double) * nf), *dtau = malloc(sizeof(double) * nc);
   double *fsrc = malloc(sizeof(double) * 5 * nc);
@@ -1772,7 +1819,7 @@ void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, doubl
     D[c] = -sJ + sd;
     const real rhs = 0.0 - sv;
     B[c] = -rhs;
-    if (tmu) tmu[c] = tmu_cell(g, p, q, c);
+    if (tmu) tmu[c] = tmu_cell(g, p, q, grad, c);
   }
   /* L = -fjp, U = fjm are stored by the caller's convention: fjp/fjm */
   free(fconv); free(fdiff); free(fdiv); free(sJL); free(sJU); free(bJ); free(cJl); free(cJr);
@@ -1783,6 +1830,312 @@ void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, doubl
 /* Scalar SGS, src/linearSolverSGS.loci:333-500: Dinv = 1/D, X = Dinv B, then
  * symmetric sweeps t = -B + sum_upper X[cr] U + sum_lower X[cl] L,
  * X = Dinv (-t), periodic ghosts refreshed before each half-sweep. */
+/* ------------------------------------------------------------------ */
+/* Menter SST k-omega (turbulence/KOmega/sst.loci) with the default       */
+/* options: turbulence_model SST, kprod_func SOmega,                     */
+/* compressibilityCorrection wilcox, dissipationCorrection none, smooth  */
+/* walls, Kprod_clip 1e100, tmu_clip 1e5; k and omega segregated, each    */
+/* through the generic scalar transport of src/solverScalarTransport.loci */
+/* ------------------------------------------------------------------ */
+
+/* sst1 (k-omega set) and sst2 (k-epsilon set), sst.loci:319-347; sigma_k of
+ * the SST model 0.85 (:682-684) */
+#define SST_SIGK1 0.85
+#define SST_SIGW1 0.5
+#define SST_BETA1 0.075
+#define SST_BETAS 0.09
+#define SST_KAPPA 0.41
+#define SST_A1 0.31
+#define SST_SIGK2 1.0
+#define SST_SIGW2 0.856
+#define SST_BETA2 0.0828
+
+/* strainRate_t, divu_t (sst.loci:65-104) and vortMag_t
+ * (src/solverTurbulence.loci:45-49,77-79) of gu[3i+j] = d u_i / d x_j */
+static void sst_kinematics(const real *gu, real *S, real *divu, real *Om) {
+  const real dudx = gu[0], dudy = gu[1], dudz = gu[2];
+  const real dvdx = gu[3], dvdy = gu[4], dvdz = gu[5];
+  const real dwdx = gu[6], dwdy = gu[7], dwdz = gu[8];
+  *divu = (dudx + dvdy + dwdz);
+  const real S2 = 2. * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + (dudy + dvdx) * (dudy + dvdx) +
+                  (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
+  *S = sqrt(S2);
+  const real wx = dwdy - dvdz, wy = dudz - dwdx, wz = dvdx - dudy;
+  *Om = sqrt(wx * wx + wy * wy + wz * wz);
+}
+
+/* bF2 (sst.loci:514-521) and kw_tmu with bF3 = 1 (:768-771) */
+static real sst_F2(real k, real w, real y, real nu) {
+  const real arg2 = minr(maxr(2.0 * sqrt(k) / (0.09 * w * y), 500.0 * nu / (w * y * y)), 1000.0);
+  return tanh(arg2 * arg2);
+}
+static real sst_tmu(real rho, real k, real w, real S, real F2) {
+  return minr(rho * SST_A1 * k / maxr(1e-30, maxr(SST_A1 * w, S * F2 * 1.0)), 1e5);
+}
+
+static void cell_gu(const og_grid *g, const double *grad, int c, real *gu) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) gu[3 * i + j] = grad[(3 * (1 + i) + j) * g->nc + c];
+}
+
+static real sst_tmu_cell(const og_grid *g, const og_params *p, const double *q, const double *grad, int c) {
+  const int nc = g->nc;
+  const real T = q[c], pg = q[4 * nc + c], k = q[5 * nc + c], w = q[6 * nc + c];
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  const real nu = suth_mu(p, T) / rho;
+  real gu[9], S, divu, Om;
+  cell_gu(g, grad, c, gu);
+  sst_kinematics(gu, &S, &divu, &Om);
+  return sst_tmu(rho, k, w, S, sst_F2(k, w, g->dist[c], nu));
+}
+
+typedef struct {
+  real F1, F2, tmu, ksrc, wsrc, kJ, wJ, Pk, CD, sigk, sigw;
+} sst_cell_t;
+
+/* One cell: crossDiffusionTerm (:464-468) from grad k . grad omega, bF1
+ * (:487-499), bF2, kw_tmu, the blended coefficients (:702-711), the SOmega
+ * stress (:790-796), P_k (:851-854), the k and omega sources and
+ * destruction Jacobians (:870-904), cross diffusion (:910-913) and the
+ * Wilcox compressibility correction (:283-287, :960-1006).  Each source
+ * keeps the file's join order. */
+static sst_cell_t sst_core(const og_params *p, real T, real pg, real k, real w, const real *gu, real y, real gkgw,
+                           real vol) {
+  sst_cell_t o;
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  const real mu = suth_mu(p, T);
+  const real nu = mu / rho;
+  real S, divu, Om;
+  sst_kinematics(gu, &S, &divu, &Om);
+  const real crossterm = maxr(SST_SIGW2 * gkgw, 0.0);
+  const real CD = 2. * rho * crossterm / maxr(w, 1e-30);
+  const real CDkw = maxr(1.0e-10, CD);
+  const real ls = maxr(sqrt(k) / (0.09 * w * y), 500.0 * nu / (w * y * y));
+  const real arg1 = minr(minr(ls, 4. * rho * SST_SIGW2 * k / (CDkw * y * y)), 32.0);
+  const real arg1_2 = arg1 * arg1, arg1_4 = arg1_2 * arg1_2;
+  const real F1 = tanh(arg1_4);
+  const real F2 = sst_F2(k, w, y, nu);
+  const real tmu = sst_tmu(rho, k, w, S, F2);
+  const real sigk = F1 * SST_SIGK1 + (1.0 - F1) * SST_SIGK2;
+  const real sigw = F1 * SST_SIGW1 + (1.0 - F1) * SST_SIGW2;
+  const real beta = (F1 * SST_BETA1 + (1.0 - F1) * SST_BETA2);
+  const real gama1 = SST_BETA1 / SST_BETAS - SST_SIGW1 * pow(SST_KAPPA, 2.0) / sqrt(SST_BETAS);
+  const real gama2 = SST_BETA2 / SST_BETAS - SST_SIGW2 * pow(SST_KAPPA, 2.0) / sqrt(SST_BETAS);
+  const real gama = F1 * gama1 + (1.0 - F1) * gama2;
+  const real stress = maxr(tmu * (S * Om - 2. / 3. * divu * divu) - 2. / 3. * rho * k * divu, 0.0);
+  const real BetaS = SST_BETAS * 1.0;
+  const real Pk = minr(stress, (1e100) * rho * BetaS * k * w);
+  const real a = sqrt(p->gamma * p->Rt * T);
+  const real Mt2 = 2.0 * k / (a * a);
+  const real mt0 = 0.25;
+  const real fmt = maxr(Mt2 - mt0 * mt0, 0.0);
+  const real BetaF = SST_BETAS * 1.0 * 1.5 * fmt;
+  const real nut = tmu / rho;
+  const real Dw = (gama / (nut + 1e-30)) * stress;
+  real ks = 0.0, ws = 0.0;
+  ks += Pk * vol;
+  ws += Dw * vol;
+  ks -= BetaS * rho * w * k * vol;
+  ws -= beta * rho * w * w * vol;
+  ws += (1.0 - F1) * CD * vol;
+  ks -= BetaF * rho * w * k * vol;
+  ws -= -BetaF * rho * w * w * vol * 1.0;
+  o.F1 = F1; o.F2 = F2; o.tmu = tmu; o.ksrc = ks; o.wsrc = ws;
+  o.kJ = -SST_BETAS * 1.0 * rho * w * vol;
+  o.wJ = -2.0 * beta * rho * w * vol;
+  o.Pk = Pk; o.CD = CD; o.sigk = sigk; o.sigw = sigw;
+  return o;
+}
+
+void og_sst_point(const og_params *p, double T, double pg, double k, double w, const double *gu, double y,
+                  double gkgw, double vol, double *out) {
+  real g9[9];
+  for (int i = 0; i < 9; ++i) g9[i] = gu[i];
+  const sst_cell_t o = sst_core(p, T, pg, k, w, g9, y, gkgw, vol);
+  out[0] = o.F1; out[1] = o.F2; out[2] = o.tmu; out[3] = o.ksrc; out[4] = o.wsrc;
+  out[5] = o.kJ; out[6] = o.wJ; out[7] = o.Pk; out[8] = o.CD;
+}
+
+/* The k and omega systems of one steady Newton step.  Per scalar X the
+ * generic rules of src/solverScalarTransport.loci: upwind convection of
+ * max(X, 0) (:121-135, 493-503), the continuity stabilisation -X rhodot
+ * and its Jacobian (:66-103, 505-546), convective Jacobians (:548-567),
+ * diffusion X_nu_f grad_f(X).n area with its Jacobians C/d (:779-826),
+ * X_srcJ from upper fjp / lower fjm (:666-678), D = -srcJ + rho V/dtau
+ * (:680-690), B = src.  X_nu_f = mu_f + tmu_f sigma_f with sigma_f the
+ * volume-weighted blend of the two cells' sigma (sst.loci:725-737,
+ * 816-836).  grad k . grad omega = (div(w grad k) - w lap k)/vol with w_f =
+ * max(0, mean of the reconstructions) on interior faces and the face value
+ * on boundary faces (:427-462).
+ * Join order per cell (Loci's is unpinned): convection (lower, upper,
+ * boundary), stabilisation, cell sources, diffusion (lower, upper,
+ * boundary); Jacobian: stabilisation, cell, boundary faces, upper fjp,
+ * lower fjm.  Periodic faces act as interior faces. */
+void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
+                     double *fjm, double *D, double *B, double *tmu, double *aux) {
+  const int nc = g->nc, nf = g->nf;
+  double *qf = calloc((size_t)7 * nf, sizeof(double));
+  double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
+  double *iflux = malloc(sizeof(double) * 5 * nf), *vflux = malloc(sizeof(double) * 4 * nf);
+  double *maxev = malloc(sizeof(double) * nf), *dtau = malloc(sizeof(double) * nc);
+  double *fsrc = malloc(sizeof(double) * 5 * nc);
+  og_face_states(g, p, q, qf);
+  og_gradients(g, p, q, qf, grad, phi);
+  og_fluxes(g, p, q, qf, grad, phi, iflux, vflux, maxev);
+  {
+    real_fj *a = malloc(sizeof(real_fj) * 25 * (size_t)nf), *b = malloc(sizeof(real_fj) * 25 * (size_t)nf);
+    real_fj *d = malloc(sizeof(real_fj) * 25 * (size_t)nc), *bb = malloc(sizeof(real_fj) * 5 * (size_t)nc);
+    int *pv = malloc(sizeof(int) * 5 * (size_t)nc);
+    og_assemble2(g, p, q, fsrc, dtau, a, b, d, pv, bb, NULL, NULL);
+    free(a); free(b); free(d); free(bb); free(pv);
+  }
+  og_adj a;
+  build_adj(g, &a);
+  /* face pass 1: area grad_f(k).n and its omega-weighted form */
+  double *gkn = calloc(nf, sizeof(double)), *wgkn = calloc(nf, sizeof(double));
+  for (int f = 0; f < nf; ++f) {
+    const int t = g->btype[f], c = g->cl[f];
+    real an[3], gk[3];
+    fnormal(g, f, an);
+    face_grad(g, q, qf, grad, f, 5, gk);
+    const real gradkdotn = g->area[f] * dot3(gk, an);
+    gkn[f] = gradkdotn;
+    if (t == OG_INTERIOR || t == OG_PERIODIC) {
+      real xf[3], xl[3], xr[3];
+      fcenter(g, f, xf);
+      ccenter(g, c, xl);
+      const int o = rcell(g, f, xr);
+      const real wL = recon_k(g, q, grad, phi, 6, c, xl, xf), wR = recon_k(g, q, grad, phi, 6, o, xr, xf);
+      const real w_f = maxr(0.0, 0.5 * (wL + wR));
+      wgkn[f] = w_f * gradkdotn;
+    } else {
+      wgkn[f] = qf[6 * nf + f] * gradkdotn;
+    }
+  }
+  /* cell pass 1: cross diffusion, blending, eddy viscosity, sources */
+  sst_cell_t *cs = malloc(sizeof(sst_cell_t) * nc);
+  for (int c = 0; c < nc; ++c) {
+    real lap = 0.0, dwg = 0.0;
+    for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+      const int f = a.lo[j];
+      lap += -gkn[f];
+      dwg += -wgkn[f];
+    }
+    for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+      const int f = a.up[j];
+      lap += gkn[f];
+      dwg += wgkn[f];
+    }
+    for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) {
+      const int f = a.bd[j];
+      lap += gkn[f];
+      dwg += wgkn[f];
+    }
+    const real w = q[6 * nc + c];
+    const real gkgw = (dwg - w * lap) / g->vol[c];
+    real gu[9];
+    cell_gu(g, grad, c, gu);
+    cs[c] = sst_core(p, q[c], q[4 * nc + c], q[5 * nc + c], w, gu, g->dist[c], gkgw, g->vol[c]);
+    if (tmu) tmu[c] = cs[c].tmu;
+    if (aux) {
+      aux[c] = cs[c].F1; aux[nc + c] = cs[c].F2; aux[2 * nc + c] = gkgw; aux[3 * nc + c] = cs[c].CD;
+    }
+  }
+  /* per scalar: face terms, then the cell gather */
+  double *fconv = malloc(sizeof(double) * nf), *fdiff = malloc(sizeof(double) * nf), *bJ = calloc(nf, sizeof(double));
+  for (int sc = 0; sc < 2; ++sc) {
+    const int kv = 5 + sc;
+    double *Fp = fjp + (size_t)sc * nf, *Fm = fjm + (size_t)sc * nf;
+    for (int f = 0; f < nf; ++f) {
+      const int t = g->btype[f], c = g->cl[f];
+      real an[3], xf[3], xl[3], gx[3];
+      fnormal(g, f, an);
+      fcenter(g, f, xf);
+      ccenter(g, c, xl);
+      const real area = g->area[f];
+      const real mdot = iflux[f];
+      face_grad(g, q, qf, grad, f, kv, gx);
+      real rho_f, mu_f;
+      face_rho_mu(g, p, q, qf, f, &rho_f, &mu_f);
+      const real tmf = tmu_face(g, p, q, grad, f);
+      const real sl = sc == 0 ? cs[c].sigk : cs[c].sigw;
+      Fp[f] = Fm[f] = 0.0;
+      bJ[f] = 0.0;
+      if (t == OG_INTERIOR || t == OG_PERIODIC) {
+        real xr[3];
+        const int o = rcell(g, f, xr);
+        const real XL = recon_k(g, q, grad, phi, kv, c, xl, xf), XR = recon_k(g, q, grad, phi, kv, o, xr, xf);
+        const real Xval = (mdot < 0.0) ? XR : XL;
+        fconv[f] = mdot * maxr(Xval, 0.0);
+        const real vl = g->vol[c], vr = g->vol[o];
+        const real sr = sc == 0 ? cs[o].sigk : cs[o].sigw;
+        const real rvlvr = 1. / (vl + vr);
+        const real sig_f = (vr * sl + vl * sr) * rvlvr;
+        const real nu_f = (mu_f + tmf * sig_f);
+        fdiff[f] = nu_f * dot3(gx, an) * area;
+        const real C = nu_f * area;
+        real dv[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+        const real d = dot3(dv, an);
+        Fp[f] += maxr(mdot, 0.0);
+        Fm[f] += minr(mdot, 0.0);
+        Fp[f] += C / d;
+        Fm[f] += -C / d;
+      } else {
+        const real XL = recon_k(g, q, grad, phi, kv, c, xl, xf);
+        const real Xf = qf[kv * nf + f];
+        const real Xval = (mdot < 0.0) ? Xf : XL;
+        fconv[f] = mdot * maxr(Xval, 0.0);
+        const real nu_f = (mu_f + tmf * sl);
+        fdiff[f] = nu_f * dot3(gx, an) * area;
+        const real C = nu_f * area;
+        real dv[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+        const real d = dot3(dv, an);
+        bJ[f] -= maxr(mdot, 0.0);
+        bJ[f] += -C / d;
+      }
+    }
+    for (int c = 0; c < nc; ++c) {
+      const real rho = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
+      const real X = q[kv * nc + c];
+      real sv = 0.0, sJ = 0.0, rhodot = 0.0;
+      for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+        const int f = a.lo[j];
+        sv += fconv[f];
+        rhodot += iflux[f];
+      }
+      for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+        const int f = a.up[j];
+        sv -= fconv[f];
+        rhodot -= iflux[f];
+      }
+      for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) {
+        const int f = a.bd[j];
+        sv -= fconv[f];
+        rhodot -= iflux[f];
+      }
+      sv -= X * rhodot;
+      sJ += -rhodot;
+      sv += sc == 0 ? cs[c].ksrc : cs[c].wsrc;
+      sJ += sc == 0 ? cs[c].kJ : cs[c].wJ;
+      for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sv -= fdiff[a.lo[j]];
+      for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sv += fdiff[a.up[j]];
+      for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) sv += fdiff[a.bd[j]];
+      for (int j = a.bd_off[c]; j < a.bd_off[c + 1]; ++j) sJ += bJ[a.bd[j]];
+      for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sJ -= Fp[a.up[j]];
+      for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += Fm[a.lo[j]];
+      const size_t i = (size_t)sc * nc + c;
+      src[i] = sv;
+      const real sd = rho * (g->vol[c] / dtau[c]);
+      D[i] = -sJ + sd;
+      const real rhs = 0.0 - sv;
+      B[i] = -rhs;
+    }
+  }
+  free(gkn); free(wgkn); free(cs); free(fconv); free(fdiff); free(bJ);
+  free_adj(&a);
+  free(qf); free(grad); free(phi); free(iflux); free(vflux); free(maxev); free(dtau); free(fsrc);
+}
+
 void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
                const double *B, double *X) {
   const int nc = g->nc, nf = g->nf;

@@ -12,7 +12,8 @@
  *   liboracle_f32.so : real_fj = float   (flowPsi's default, include/flowTypes.h:43-49)
  *
  * Data layout (all arrays structure-of-arrays, reference cell/face order):
- *   q    [5][nc]  primitive state (T, u, v, w, pg); [6][nc] with nuTilde when turb
+ *   q    [5][nc]  primitive state (T, u, v, w, pg); [6][nc] with nuTilde under SA,
+ *                 [7][nc] with k and omega under SST
  *   qf   [5][nf]  boundary-face state (T_f, u_f, pg_f); unused on interior faces
  *   grad [15][nc] gradient of q: grad[3*k+d] = d q_k / d x_d
  *   phi  [5][nc]  limiter per variable
@@ -58,10 +59,12 @@ typedef struct {
   int lsgs_maxiter;
   double lsgs_reltol, lsgs_abstol, lsgs_relax;
   double Minf, etaT, PLimPC;  /* low-Mach preconditioning (Minf >= 0.5: eta = 1) */
-  int turb;               /* 0 laminar, 1 Spalart-Allmaras (q gains a 6th row nuTilde) */
+  int turb;               /* 0 laminar, 1 Spalart-Allmaras (q gains a 6th row nuTilde),
+                             2 Menter SST (q gains rows k and omega) */
   double sa_cb1, sa_k, sa_cv1, sa_cb2, sa_cw2, sa_cw3, sa_sigma;
   double Prt;             /* turbulent Prandtl number */
   double nut_bc;          /* nuTilde of inflow faces */
+  double k_bc, w_bc;      /* SST: k and omega of inflow faces */
 } og_params;
 
 typedef struct {
@@ -125,6 +128,17 @@ void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, doubl
 /* one cell: out3 = tmu, production - destruction, destruction Jacobian */
 void og_sa_point(const og_params *p, double T, double pg, double nut, double vortMag, double y, double vol,
                  double *out3);
+/* Menter SST: the k and omega equations of one steady Newton step, each in
+ * the layout of og_sa_assemble: src[2][nc], fjp/fjm[2][nf], D[2][nc],
+ * B[2][nc] (k first); tmu[nc]; aux[4][nc] (may be NULL): F1, F2,
+ * grad k . grad omega, the cross-diffusion term. */
+void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
+                     double *fjm, double *D, double *B, double *tmu, double *aux);
+/* one cell of SST from its state, velocity gradient gu[9] (d u_i/d x_j at
+ * 3i+j), wall distance, grad k . grad omega and volume: out[9] = F1, F2,
+ * tmu, k source, omega source, k Jacobian, omega Jacobian, P_k, CD */
+void og_sst_point(const og_params *p, double T, double pg, double k, double w, const double *gu, double y,
+                  double gkgw, double vol, double *out);
 /* scalar SGS on that system (Jacobi start + nsweeps symmetric sweeps) */
 void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
                const double *B, double *X);
