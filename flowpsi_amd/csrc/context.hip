@@ -30,7 +30,12 @@ struct fpsi_ctx {
   std::vector<int> lev_off; // level ranges in new numbering
   int nchunks = 0;          // sweep chunks (<= 12 cells, within one level)
   int sweep_waves = 0;      // co-resident waves of the cooperative sweep
-  const int *chunk_lo = nullptr, *chunk_lev = nullptr, *lev_size = nullptr;
+  // dataflow sweep: chunk ranges and, per chunk, the chunks holding its
+  // lower (forward) / upper (backward) neighbours; off when some chunk has
+  // more than 64 of either
+  const int *chunk_lo = nullptr, *fdep_off = nullptr, *fdep = nullptr, *bdep_off = nullptr, *bdep = nullptr;
+  bool have_flow = false;
+  int flow_epoch = 0;
   int max_width = 0;
   double vtot = 0;
   DevGrid g{};
@@ -550,21 +555,47 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     // two co-resident waves per CU at most: enough to keep a level in flight
     const int cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
-    c->sweep_waves = std::max(1, std::min(cap, 2 * ncu));
+    c->sweep_waves = std::max(1, std::min(cap, 4 * ncu));
   }
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
   g.nlev = nlev;
-  std::vector<int> chunk_lo, chunk_lev, lev_size(nlev);
-  for (int l = 0; l < nlev; ++l) {
-    lev_size[l] = c->lev_off[l + 1] - c->lev_off[l];
-    for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 12) {  // 12 cells per wave
-      chunk_lo.push_back(a);
-      chunk_lev.push_back(l);
-    }
-  }
+  std::vector<int> chunk_lo;
+  for (int l = 0; l < nlev; ++l)
+    for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 12) chunk_lo.push_back(a);  // 12 cells per wave
   c->nchunks = (int)chunk_lo.size();
   chunk_lo.push_back(nc);
+  std::vector<int> fdep_off(c->nchunks + 1, 0), fdep, bdep_off(c->nchunks + 1, 0), bdep;
+  {
+    std::vector<int> chunk_of(nc);
+    for (int ch = 0; ch < c->nchunks; ++ch)
+      for (int a = chunk_lo[ch]; a < chunk_lo[ch + 1]; ++a) chunk_of[a] = ch;
+    bool ok = true;
+    std::vector<int> lo_d, up_d;
+    for (int ch = 0; ch < c->nchunks; ++ch) {
+      lo_d.clear();
+      up_d.clear();
+      for (int a = chunk_lo[ch]; a < chunk_lo[ch + 1]; ++a)
+        for (int j = m_off[a]; j < m_off[a + 1]; ++j) {
+          const int o = m_nbr[j];
+          if (o >= nc) continue;  // periodic ghost: a lagged copy, no dependency
+          if (o < chunk_lo[ch]) lo_d.push_back(chunk_of[o]);
+          else if (o >= chunk_lo[ch + 1]) up_d.push_back(chunk_of[o]);
+        }
+      std::sort(lo_d.begin(), lo_d.end());
+      lo_d.erase(std::unique(lo_d.begin(), lo_d.end()), lo_d.end());
+      std::sort(up_d.begin(), up_d.end());
+      up_d.erase(std::unique(up_d.begin(), up_d.end()), up_d.end());
+      ok = ok && lo_d.size() <= 64 && up_d.size() <= 64;
+      fdep.insert(fdep.end(), lo_d.begin(), lo_d.end());
+      bdep.insert(bdep.end(), up_d.begin(), up_d.end());
+      fdep_off[ch + 1] = (int)fdep.size();
+      bdep_off[ch + 1] = (int)bdep.size();
+    }
+    c->have_flow = ok;
+    fdep.push_back(0);  // never empty
+    bdep.push_back(0);
+  }
   int rc = 0;
   if ((rc = upload(c, &g.fnx, fnx)) || (rc = upload(c, &g.fny, fny)) || (rc = upload(c, &g.fnz, fnz)) ||
       (rc = upload(c, &g.farea, farea)) || (rc = upload(c, &g.fcx, fcx)) || (rc = upload(c, &g.fcy, fcy)) ||
@@ -578,8 +609,9 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = upload(c, &g.slot_cr, slot_cr)) || (rc = upload(c, &g.old_id, old_id)) ||
       (rc = upload(c, &g.nbr_pad, nbr_pad)) || (rc = upload(c, &g.m_nup, m_nup)) ||
       (rc = upload(c, &g.lnbr_pad, lnbr_pad)) ||
-      (rc = upload(c, &c->chunk_lo, chunk_lo)) || (rc = upload(c, &c->chunk_lev, chunk_lev)) ||
-      (rc = upload(c, &c->lev_size, lev_size))) {
+      (rc = upload(c, &c->chunk_lo, chunk_lo)) || (rc = upload(c, &c->fdep_off, fdep_off)) ||
+      (rc = upload(c, &c->fdep, fdep)) || (rc = upload(c, &c->bdep_off, bdep_off)) ||
+      (rc = upload(c, &c->bdep, bdep))) {
     *out = c;
     return rc;
   }
@@ -594,7 +626,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
       (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper + 1))) ||
       (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
-      (rc = dalloc(c, &s.lev_cnt, (size_t)nlev * 32)) || (rc = dalloc(c, &s.eta, (size_t)nc)) ||
+      (rc = dalloc(c, &s.cflag, (size_t)std::max(c->nchunks, 1))) || (rc = dalloc(c, &s.serr, 1)) ||
+      (rc = dalloc(c, &s.eta, (size_t)nc)) ||
       (rc = dalloc(c, &s.eta_t, (size_t)nc))) {
     *out = c;
     return rc;
@@ -620,7 +653,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   HIPCHK(c, hipMemsetAsync(s.vflux, 0, 4 * (size_t)nf * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(c->Dg, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
-  HIPCHK(c, hipMemsetAsync(s.lev_cnt, 0, (size_t)nlev * 32 * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.cflag, 0, (size_t)std::max(c->nchunks, 1) * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.serr, 0, sizeof(int), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!lsgs_cells.empty()) {
     if ((rc = upload(c, &c->lsgs_cells, lsgs_cells))) { *out = c; return rc; }
@@ -699,6 +733,8 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     if (int rc = lsgs_alloc(c)) return rc;
   }
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
+  if (p->sweep_mode == FPSI_SWEEP_PERSISTENT && !c->have_flow)
+    return fail(c, FPSI_ERR_ARG, "no dataflow sweep on this grid (a sweep chunk has more than 64 dependencies)");
   if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL &&
       p->sweep_mode != FPSI_SWEEP_TILED)
     return fail(c, FPSI_ERR_ARG, "unknown sweep mode");
@@ -1094,15 +1130,27 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
     return 0;
   }
   if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
+    // a wait is bounded: far longer than any real one (~1e6 polls of a few
+    // hundred ns each), so only a broken schedule reaches it
+    const int spin_limit = 1 << 22;
     for (int it = 0; it < nsweeps; ++it)
       for (int dir = 0; dir < 2; ++dir) {
-        launch_sweep_prep(c->g, c->s, c->stream);
+        launch_ghost(c->g, c->s, c->stream);
         hipEvent_t e = prof_begin(c);
-        const hipError_t he = launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, c->chunk_lev,
-                                               c->lev_size, c->sweep_waves, c->s, c->stream);
+        const int epoch = ++c->flow_epoch;
+        const hipError_t he =
+            launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, dir ? c->bdep_off : c->fdep_off,
+                             dir ? c->bdep : c->fdep, c->s.cflag, epoch, spin_limit, c->s.serr, c->sweep_waves,
+                             c->s, c->stream);
         prof_end(c, FPSI_K_SGS_SWEEP, e);
         if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
       }
+    if (nsweeps > 0) {
+      int err = 0;
+      HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (err) return fail(c, FPSI_ERR_HIP, "dataflow sweep: a dependency wait exceeded its limit");
+    }
     return 0;
   }
   for (int it = 0; it < nsweeps; ++it) {

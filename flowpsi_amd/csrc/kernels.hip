@@ -945,71 +945,118 @@ __global__ void __launch_bounds__(64) k_sgs_level(DevGrid g, int lo, int hi, con
 }
 
 // One whole half-sweep in a single cooperative launch of W one-wave
-// workgroups, all co-resident.  Cells are cut into chunks of at most CPW cells
-// that never straddle a level, numbered in sweep order; wave w takes chunks
-// w, w+W, w+2W, ... in order.  A chunk of level L first waits until every cell
-// of the level it depends on (L-1 forward, L+1 backward) has been published,
-// so every dependency of flowPsi's sequential sweep is honoured.  The wave
-// holding the smallest unfinished chunk has finished all its earlier chunks
-// and all of that chunk's dependencies are smaller chunks, so it always
-// proceeds: every wait ends.  The loop has a uniform trip count and no
-// lane-specific branches.  X is written and read with device-coherent
-// accesses; a wave's stores complete before it publishes its cells with a
-// release atomic.  Counters sit 128 bytes apart.
-static constexpr int CNT_STRIDE = 32;
-
-template <typename F>
-__global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int backward, int nchunks, int nwaves,
-                                                  const int *__restrict__ chunk_lo,
-                                                  const int *__restrict__ chunk_lev,
-                                                  const int *__restrict__ lev_size,
-                                                  int *__restrict__ lev_cnt, const F *__restrict__ off,
+// workgroups, all co-resident, as a dataflow over chunks.  Cells are cut
+// into chunks of at most CPW cells that never straddle a level, numbered in
+// sweep order; wave w takes chunks w, w+W, w+2W, ... in order.  A chunk
+// waits only for the chunks holding its cells' dependencies (lower
+// neighbours forward, upper neighbours backward; periodic ghosts are lagged
+// copies made before the launch), each flagged with this launch's epoch.
+// The wave holding the smallest unfinished chunk has finished its earlier
+// chunks and all of that chunk's dependencies are smaller chunks, so it
+// always proceeds: every wait ends.  A wait that exceeds `spin_limit` polls
+// sets *err and returns (the host reports it), so a broken schedule cannot
+// hang the device.
+//
+// Loads that cannot change before the chunk's dependencies are met are
+// issued before the wait: indices, blocks, factors, B, and X of the
+// neighbours on the side the sweep has not reached (upper forward, lower
+// backward).  Forward, the upper terms come first in flowPsi's summation
+// order, so they are also accumulated before the wait; the order of every
+// sum is that of k_sgs_level, so the result is bitwise the same.  X is read
+// and written with device-coherent accesses; a wave's stores complete
+// (vmcnt = 0) before lane 0 publishes the chunk's flag.
+template <typename F, bool BACK>
+__global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
+                                                  const int *__restrict__ dep_off, const int *__restrict__ dep,
+                                                  int *__restrict__ flag, int epoch, int spin_limit,
+                                                  int *__restrict__ err, const F *__restrict__ off,
                                                   const F *__restrict__ lu, const int *__restrict__ piv,
-                                                  const F *__restrict__ B, double *__restrict__ X, int never) {
+                                                  const F *__restrict__ B, double *__restrict__ X) {
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
-  const int nlev = g.nlev;
+  const int gbase = 5 * (q < CPW ? q : 0);
   for (int t = blockIdx.x; t < nchunks; t += nwaves) {
-    const int ch = backward ? nchunks - 1 - t : t;
+    const int ch = BACK ? nchunks - 1 - t : t;
     const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
-    const int L = chunk_lev[ch];
-    const int dep = backward ? L + 1 : L - 1;
     const int c = lo + q;
     const bool act = q < CPW && c < hi;
+    const int cc = act ? c : lo;
+    double tsum = -(double)B[(size_t)cc * 5 + i];
+    const int pk = piv[cc];
+    const int m0 = g.m_off[cc];
+    const int len = g.m_off[cc + 1] - m0;
+    const int nup = g.m_nup[cc];
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)cc * MAXR + j];
+    F mrow[MAXR][5], a[25];
+    double xv[MAXR][5];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+    }
+#pragma unroll
+    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu[(size_t)cc * 25 + i2];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool pre = BACK ? (j >= nup) : (j < nup);
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) xv[j][jj] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + jj) : 0.0;
+    }
+    if (!BACK) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+          if (j < nup) tsum += mrow[j][jj] * xv[j][jj];
+    }
+    // wait for the chunks this one depends on
+    const int d0 = dep_off[ch], nd = dep_off[ch + 1] - d0;
+    const int dl = lane < nd ? dep[d0 + lane] : -1;
+    int spins = 0;
+    while (true) {
+      const int v = dl >= 0 ? __hip_atomic_load(flag + dl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
+      if (__all(v >= epoch)) break;
+      if (++spins > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool post = BACK ? (j < nup) : (j >= nup);
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (post) xv[j][jj] = ldx<true>(X + (size_t)nb[j] * 5 + jj);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (BACK || j >= nup) tsum += mrow[j][jj] * xv[j][jj];
+    if (act)
+      for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+        const F *M = off + (size_t)(m0 + j) * 25;
+        const double *xn = X + (size_t)g.m_nbr[m0 + j] * 5;
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj) tsum += M[i + 5 * jj] * ldx<true>(xn + jj);
+      }
+    double v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
     if (act) {
-      // pull the row's blocks toward the caches while the level is pending:
-      // touch one word per 128-byte line; the sum is kept alive by a store
-      // that never happens (never == 0)
-      const int m0 = g.m_off[c];
-      const char *p0 = (const char *)(off + (size_t)m0 * 25);
-      const int nbytes = (g.m_off[c + 1] - m0) * 25 * (int)sizeof(F);
-      float touch = 0.f;
-      for (int o = i * 128; o < nbytes; o += 5 * 128) touch += *(const float *)(p0 + o);
-      if (never) X[0] = (double)touch;
+      double x[5];
+      lusolve5<F>(a, pk, v, x);
+      const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+      stx<true>(X + (size_t)c * 5 + i, xi);
     }
-    if (dep >= 0 && dep < nlev) {
-      const int need = lev_size[dep];
-      const int *cnt = lev_cnt + (size_t)dep * CNT_STRIDE;
-      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
-        __builtin_amdgcn_s_sleep(2);
-    }
-    sgs_cell5<F, true>(g, c, i, 5 * (q < CPW ? q : 0), act, off, lu, piv, B, X);
     // every store of the wave has completed (vmcnt = 0) before the publish
     __builtin_amdgcn_s_waitcnt(0x0F70);
-    __hip_atomic_fetch_add(lev_cnt + (size_t)L * CNT_STRIDE, (act && i == 0) ? 1 : 0, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Before each half-sweep: zero the level counters and refresh the
-// periodic ghost copies (src/linearSolverSGS.loci:124-127 and :215-219).
-__global__ void k_sweep_prep(DevGrid g, int *__restrict__ lev_cnt, double *__restrict__ X) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < g.nlev) lev_cnt[(size_t)i * CNT_STRIDE] = 0;
-  if (i < g.nper) {
-    const int src = g.frc[g.nfi + i];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) X[(size_t)(g.nc + i) * 5 + k] = X[(size_t)src * 5 + k];
+    if (lane == 0) __hip_atomic_store(flag + ch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1897,31 +1944,26 @@ void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, h
                        (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, s.X);
 }
 int sgs_sweep_waves(int fp32, int ncu) {
-  int per_cu = 0;
-  hipError_t e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgs_sweep<float>, 64, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgs_sweep<double>, 64, 0);
-  if (e != hipSuccess || per_cu <= 0) return 0;
+  int per_cu = 0, a = 0, b = 0;
+  hipError_t e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_sgs_sweep<float, false>, 64, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_sgs_sweep<double, false>, 64, 0);
+  if (e != hipSuccess) return 0;
+  e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sgs_sweep<float, true>, 64, 0)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sgs_sweep<double, true>, 64, 0);
+  if (e != hipSuccess) return 0;
+  per_cu = std::min(a, b);
+  if (per_cu <= 0) return 0;
   return per_cu * ncu;
 }
-
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
-                            const int *chunk_lev, const int *lev_size, int nwaves, DevState &s, hipStream_t st) {
+                            const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
+                            int nwaves, DevState &s, hipStream_t st) {
   int nw = std::max(1, std::min(nwaves, nchunks));
-  int never = 0;
-  if (fp32) {
-    const float *off = (const float *)s.off, *lu = (const float *)s.lu, *B = (const float *)s.B;
-    void *args[] = {(void *)&g, &backward, &nchunks, &nw, &chunk_lo, &chunk_lev, &lev_size, &s.lev_cnt,
-                    &off, &lu, &s.piv, &B, &s.X, &never};
-    return hipLaunchCooperativeKernel((const void *)k_sgs_sweep<float>, dim3(nw), dim3(64), args, 0, st);
-  }
-  const double *off = (const double *)s.off, *lu = (const double *)s.lu, *B = (const double *)s.B;
-  void *args[] = {(void *)&g, &backward, &nchunks, &nw, &chunk_lo, &chunk_lev, &lev_size, &s.lev_cnt,
-                  &off, &lu, &s.piv, &B, &s.X, &never};
-  return hipLaunchCooperativeKernel((const void *)k_sgs_sweep<double>, dim3(nw), dim3(64), args, 0, st);
-}
-void launch_sweep_prep(const DevGrid &g, DevState &s, hipStream_t st) {
-  const int n = std::max(g.nlev, g.nper);
-  hipLaunchKernelGGL(k_sweep_prep, dim3(nblk(std::max(n, 1))), dim3(TPB), 0, st, g, s.lev_cnt, s.X);
+  void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &dep_off, &dep, &flag, &epoch, &spin_limit, &err,
+                  (void *)&s.off, (void *)&s.lu, (void *)&s.piv, (void *)&s.B, (void *)&s.X};
+  const void *fn = fp32 ? (backward ? (const void *)k_sgs_sweep<float, true> : (const void *)k_sgs_sweep<float, false>)
+                        : (backward ? (const void *)k_sgs_sweep<double, true> : (const void *)k_sgs_sweep<double, false>);
+  return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
 }
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);

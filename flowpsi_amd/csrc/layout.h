@@ -64,7 +64,8 @@ struct DevState {
   double *X;      // [nc + nper + 1][5], last entry stays zero
   double *npart;  // norm partials
   double *nout;   // final norms
-  int *lev_cnt;   // [nlev * 32] cells published per level (persistent sweep)
+  int *cflag;     // [nchunks] epoch each sweep chunk last finished (dataflow sweep)
+  int *serr;      // [1] set when a dataflow wait exceeded its spin limit
   double *eta;    // [nc] Eta_p (read only when Prm::pre)
   double *eta_t;  // [nc] Eta_l / Eta_ll scratch
   // Spalart-Allmaras (allocated when selected): q holds nuTilde as its 6th row

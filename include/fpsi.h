@@ -53,7 +53,9 @@ enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
 enum { FPSI_TURB_NONE = 0, FPSI_TURB_SA = 1, FPSI_TURB_SST = 2 };
 /* How a Gauss-Seidel half-sweep runs (same result bit for bit):
  * PER_LEVEL  one launch per dependency level (default);
- * PERSISTENT one cooperative launch, per-level completion counters;
+ * PERSISTENT one cooperative launch per half-sweep run as a dataflow: each
+ *            chunk of <= 12 cells waits only for the chunks holding its
+ *            dependencies (per-chunk flags);
  * TILED      one cooperative launch over tiles of <= 32 cells whose levels run
  *            in LDS (needs a grid whose cell boxes form an acyclic tile graph;
  *            fpsi_info.sweep_tiles > 0). */

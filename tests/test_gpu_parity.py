@@ -195,7 +195,7 @@ def test_bitwise_reproducible():
 
 
 def test_sweep_modes_agree():
-    """One launch per level, the cooperative per-level kernel and the tiled
+    """One launch per level, the cooperative dataflow kernel and the tiled
     kernel give the same bits."""
     g, q, prm = workloads.channel(nx=12, ny=16, nz=10)
     outs = []
@@ -210,16 +210,17 @@ def test_sweep_modes_agree():
 
 
 @pytest.mark.parametrize("mk,prec", [("box40", "fp64"), ("channel64", "fp64"), ("channel64", "fp32")])
-def test_tiled_sweep_bitwise_at_size(mk, prec):
-    """More tiles than co-resident workgroups (each workgroup walks many tiles):
-    the tiled sweep equals the level-per-launch sweep bit for bit."""
+def test_cooperative_sweeps_bitwise_at_size(mk, prec):
+    """More tiles / chunks than co-resident workgroups (each workgroup walks
+    many): the tiled and the dataflow sweeps equal the level-per-launch sweep
+    bit for bit."""
     if mk == "box40":
         g, q, prm = workloads.box(40)
     else:
         g, q, prm = workloads.channel(64)
     prm = dict(prm, precision=prec)
     outs = []
-    for mode in ("per_level", "tiled"):
+    for mode in ("per_level", "tiled", "persistent"):
         s = Solver(g, dict(prm, sweep=mode))
         info = s.info()
         s.set_state(q)
@@ -227,3 +228,4 @@ def test_tiled_sweep_bitwise_at_size(mk, prec):
         outs.append(s.linear_solve(3))
     print(f"{mk} {prec}: {info['sweep_tiles']} tiles, {info['nlevels']} levels")
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
