@@ -81,7 +81,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="channel128", choices=["channel128", "box32"])
+    ap.add_argument("--workload", default="channel128", choices=["channel128", "channel128_sa", "channel128_sst", "box32"],
+                    help="channel128 is the metric's workload; the turbulent channels (SA, SST) are extra lines")
     ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--sweep", default="persistent", choices=["per_level", "tiled", "persistent"])
@@ -106,6 +107,12 @@ def main():
         n = a.n or 128
         g, q, prm = workloads.channel(n)
         wname = f"laminar NS channel {n}^3 ({g.nc} cells), HLLC+viscous, SGS x{prm['sgs_iter']}"
+    elif a.workload in ("channel128_sa", "channel128_sst"):
+        n = a.n or 128
+        turb = a.workload.split("_")[1]
+        g, q, prm = (workloads.channel_sa if turb == "sa" else workloads.channel_sst)(n)
+        wname = (f"{turb.upper()} NS channel {n}^3 ({g.nc} cells), HLLC+viscous, segregated "
+                 f"{'nuTilde' if turb == 'sa' else 'k and omega'}, SGS x{prm['sgs_iter']}")
     else:
         n = a.n or 32
         g, q, prm = workloads.box(n)

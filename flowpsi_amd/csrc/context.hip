@@ -1116,7 +1116,9 @@ extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *
   return 0;
 }
 
-static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
+// ns > 0 (dataflow mode only): the turbulence scalar systems are swept in
+// the same launches as the fluid system
+static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0) {
   const int nlev = (int)c->lev_off.size() - 1;
   if (c->tiled()) {
     for (int it = 0; it < nsweeps; ++it)
@@ -1136,12 +1138,13 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps) {
     for (int it = 0; it < nsweeps; ++it)
       for (int dir = 0; dir < 2; ++dir) {
         launch_ghost(c->g, c->s, c->stream);
+        if (ns) launch_ghost1(c->g, c->nscal(), c->s.sX, c->stream);
         hipEvent_t e = prof_begin(c);
         const int epoch = ++c->flow_epoch;
         const hipError_t he =
             launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, dir ? c->bdep_off : c->fdep_off,
                              dir ? c->bdep : c->fdep, c->s.cflag, epoch, spin_limit, c->s.serr, c->sweep_waves,
-                             c->s, c->stream);
+                             ns, c->nnz, c->s, c->stream);
         prof_end(c, FPSI_K_SGS_SWEEP, e);
         if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
       }
@@ -1260,11 +1263,14 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (int rc = enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
+    // turbulence scalars ride in the fluid's dataflow sweep when both use SGS
+    const bool fuse = c->kp.turb && !c->lsgs() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
+    if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
+                      : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
       hipStreamSynchronize(c->stream);
       return rc;
     }
-    if (c->kp.turb) {
+    if (c->kp.turb && !fuse) {
       hipEvent_t e2 = prof_begin(c);
       if (int rc = enqueue_sa_solve(c, -1, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
         hipStreamSynchronize(c->stream);

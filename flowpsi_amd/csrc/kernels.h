@@ -13,11 +13,12 @@ void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hip
 void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
 void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st);
 void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st);
-// dataflow half-sweep in one cooperative launch (co-resident waves: sgs_sweep_waves)
+// dataflow half-sweep in one cooperative launch (co-resident waves: sgs_sweep_waves);
+// ns > 0 sweeps the turbulence scalar systems in the same launch
 int sgs_sweep_waves(int fp32, int ncu);
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
-                            int nwaves, DevState &s, hipStream_t st);
+                            int nwaves, int ns, long nnz, DevState &s, hipStream_t st);
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,

@@ -965,13 +965,27 @@ __global__ void __launch_bounds__(64) k_sgs_level(DevGrid g, int lo, int hi, con
 // sum is that of k_sgs_level, so the result is bitwise the same.  X is read
 // and written with device-coherent accesses; a wave's stores complete
 // (vmcnt = 0) before lane 0 publishes the chunk's flag.
+//
+// With ns > 0 the same launch also sweeps the ns turbulence scalar systems
+// (segregated nuTilde, or k and omega): lane sc < ns of each cell's group
+// updates scalar sc of that cell with the arithmetic of k_sa_sgs_level.  The
+// scalar rows share the fluid's row structure, so the chunk dependencies
+// cover them; this is the 6x6 / 7x7 block sweep with every fluid-scalar
+// coupling zero, equal bit for bit to the separate sweeps.
+struct ScalarRows {
+  int ns;
+  long nnz;
+  const double *off, *dinv, *B;
+  double *X;  // [ns][nc + nper + 1]
+};
+
 template <typename F, bool BACK>
 __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                   const int *__restrict__ dep_off, const int *__restrict__ dep,
                                                   int *__restrict__ flag, int epoch, int spin_limit,
                                                   int *__restrict__ err, const F *__restrict__ off,
                                                   const F *__restrict__ lu, const int *__restrict__ piv,
-                                                  const F *__restrict__ B, double *__restrict__ X) {
+                                                  const F *__restrict__ B, double *__restrict__ X, ScalarRows sr) {
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
   const int gbase = 5 * (q < CPW ? q : 0);
@@ -1005,6 +1019,27 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       const bool pre = BACK ? (j >= nup) : (j < nup);
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj) xv[j][jj] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + jj) : 0.0;
+    }
+    // turbulence scalar sc = i of this cell (lanes i < ns)
+    const bool sact = act && i < sr.ns;
+    const size_t xs = (size_t)g.nc + g.nper + 1;
+    const int sc = sact ? i : 0;
+    const double *soff = sr.ns ? sr.off + (size_t)sc * sr.nnz + m0 : nullptr;
+    double *sx = sr.ns ? sr.X + sc * xs : nullptr;
+    double ssum = 0.0, sco[MAXR], sxv[MAXR];
+    if (sact) {
+      ssum = -sr.B[(size_t)sc * g.nc + cc];
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        sco[j] = j < len ? soff[j] : 0.0;
+        const bool pre = BACK ? (j >= nup) : (j < nup);
+        sxv[j] = (pre && j < len) ? ldx<true>(sx + nb[j]) : 0.0;
+      }
+      if (!BACK) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nup && j < len) ssum += sxv[j] * sco[j];
+      }
     }
     if (!BACK) {
 #pragma unroll
@@ -1045,6 +1080,19 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
         for (int jj = 0; jj < 5; ++jj) tsum += M[i + 5 * jj] * ldx<true>(xn + jj);
       }
+    if (sact) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = BACK ? (j < nup) : (j >= nup);
+        if (post && j < len) sxv[j] = ldx<true>(sx + nb[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if ((BACK || j >= nup) && j < len) ssum += sxv[j] * sco[j];
+      for (int j = MAXR; j < len; ++j) ssum += ldx<true>(sx + g.m_nbr[m0 + j]) * soff[j];
+      const double sv = -ssum;
+      stx<true>(sx + c, sr.dinv[(size_t)sc * g.nc + c] * sv);
+    }
     double v[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
@@ -1957,10 +2005,12 @@ int sgs_sweep_waves(int fp32, int ncu) {
 }
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
-                            int nwaves, DevState &s, hipStream_t st) {
+                            int nwaves, int ns, long nnz, DevState &s, hipStream_t st) {
   int nw = std::max(1, std::min(nwaves, nchunks));
+  ScalarRows sr{ns, nnz, s.soff, s.sdinv, s.sB, s.sX};
+  if (!ns) sr = ScalarRows{0, 0, nullptr, nullptr, nullptr, nullptr};
   void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &dep_off, &dep, &flag, &epoch, &spin_limit, &err,
-                  (void *)&s.off, (void *)&s.lu, (void *)&s.piv, (void *)&s.B, (void *)&s.X};
+                  (void *)&s.off, (void *)&s.lu, (void *)&s.piv, (void *)&s.B, (void *)&s.X, (void *)&sr};
   const void *fn = fp32 ? (backward ? (const void *)k_sgs_sweep<float, true> : (const void *)k_sgs_sweep<float, false>)
                         : (backward ? (const void *)k_sgs_sweep<double, true> : (const void *)k_sgs_sweep<double, false>);
   return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);

@@ -131,3 +131,20 @@ def test_zero_k_is_laminar():
     syl = sl.get_system()
     for k in ("fjp", "fjm", "D", "B", "dtau"):
         assert np.array_equal(sy[k], syl[k])
+
+
+def test_scalar_rows_fused_in_dataflow_sweep(case):
+    """The dataflow sweep carries the turbulence scalar rows in the fluid's
+    launches (the block sweep with all fluid-scalar couplings zero): the
+    state after Newton steps equals that of the separate fluid and scalar
+    sweeps bit for bit, FP64 and FP32 blocks."""
+    name, g, q, prm = case
+    for prec in ("fp64", "fp32"):
+        outs = []
+        for mode in ("per_level", "persistent"):
+            s = Solver(g, dict(prm, precision=prec, sweep=mode))
+            s.set_state(q)
+            s.newton(4)
+            outs.append(s.get_state())
+        print(f"SST fused vs separate sweeps, {prec}: max diff {np.abs(outs[0] - outs[1]).max():.1e}")
+        assert np.array_equal(outs[0], outs[1])
