@@ -76,3 +76,74 @@ def channel_sst(n=32, seed=2, nx=None, ny=None, nz=None, tmu_ratio=10.0, **over)
 
 
 WORKLOADS = {"box32": dict(fn=box, steps=50), "channel128": dict(fn=channel, steps=None)}
+
+
+def _freestream(mach, re_per_m=None, T=300.0, p=101325.0):
+    """(pressure, speed) of a freestream at `mach`; with `re_per_m` the
+    pressure is set so that rho U / mu = re_per_m (Sutherland mu at T)."""
+    import numpy as np_
+    a = np_.sqrt(grids.GAMMA * grids.R_AIR * T)
+    U = mach * a
+    if re_per_m is not None:
+        rho = re_per_m * grids.sutherland_mu(T) / U
+        p = rho * grids.R_AIR * T
+    return p, U
+
+
+def c3_tets(n=88, seed=3, **over):
+    """Unit cube of n^3 hexes cut into 6 jittered tetrahedra each (interior
+    nodes moved by U(-0.1, 0.1) h), viscous wall at y = 0, farfield elsewhere;
+    Spalart-Allmaras, M = 0.5 along x, Re = 1e6 per metre, nuTilde = 3 nu;
+    FP32 Jacobians (the mixed-precision path)."""
+    import numpy as np_
+    g = grids.tet_cube(n, 0.1, seed)
+    g.meta["dist"] = grids.wall_distance(g)
+    p, U = _freestream(0.5, 1e6)
+    q5 = grids.freestream_state(g, 0.5, p=p, seed=seed)
+    rho = q5[4] / (grids.R_AIR * q5[0])
+    nu = grids.sutherland_mu(q5[0]) / rho
+    q = np_.vstack([q5, (3.0 * nu)[None, :]])
+    prm = dict(BASE, flux="hllc", viscous=1, u0=(U, 0.0, 0.0), pg0=p, cflmax=10.0, sgs_iter=10, turb="sa",
+               precision="fp32")
+    prm.update(over)
+    return g, q, prm
+
+
+def c4_hybrid(nx=8, nz=8, n_prism=8, n_hex=8, seed=4, **over):
+    """Hex core (interior nodes jittered by 10 %) over prism layers at a wall
+    (first height 1e-4, growth 1.2); Spalart-Allmaras, M = 0.3 along x,
+    Re = 1e6 per metre, nuTilde = 3 nu."""
+    import numpy as np_
+    g = grids.prism_hex_box(nx, nz, n_prism, n_hex, first=1e-4, growth=1.2, jitter=0.1, seed=seed)
+    g.meta["dist"] = grids.wall_distance(g)
+    p, U = _freestream(0.3, 1e6)
+    q5 = grids.freestream_state(g, 0.3, p=p, seed=seed)
+    rho = q5[4] / (grids.R_AIR * q5[0])
+    nu = grids.sutherland_mu(q5[0]) / rho
+    q = np_.vstack([q5, (3.0 * nu)[None, :]])
+    prm = dict(BASE, flux="hllc", viscous=1, u0=(U, 0.0, 0.0), pg0=p, cflmax=10.0, sgs_iter=10, turb="sa")
+    prm.update(over)
+    return g, q, prm
+
+
+def c5_bl(nx=16, ny=60, nz=16, aspect=1e3, seed=5, **over):
+    """Boundary-layer hex grid (wall at y = 0, first cell aspect ratio
+    `aspect`, geometric growth to y = 1); Menter SST, M = 0.6 along x,
+    Re = 5e6 per metre; k and omega from the inflow defaults with seeded
+    noise plus the near-wall 6 nu / (beta1 d^2); line SGS."""
+    import numpy as np_
+    g = grids.bl_box(nx, ny, nz, aspect)
+    g.meta["dist"] = grids.wall_distance(g)
+    p, U = _freestream(0.6, 5e6)
+    q5 = grids.freestream_state(g, 0.6, p=p, seed=seed)
+    rho = q5[4] / (grids.R_AIR * q5[0])
+    nu = grids.sutherland_mu(q5[0]) / rho
+    rng = np_.random.default_rng(seed + 300)
+    kb = max(1.5 * (1e-2 * U) ** 2, 1e-3)
+    k = kb * (1.0 + 0.1 * rng.uniform(-1, 1, g.nc))
+    w = 0.09 * kb / 1e-8 * (1.0 + 0.1 * rng.uniform(-1, 1, g.nc)) + 6.0 * nu / (0.075 * g.meta["dist"] ** 2)
+    q = np_.vstack([q5, k[None, :], w[None, :]])
+    prm = dict(BASE, flux="hllc", viscous=1, u0=(U, 0.0, 0.0), pg0=p, cflmax=10.0, sgs_iter=10, turb="sst",
+               solver="lsgs")
+    prm.update(over)
+    return g, q, prm
