@@ -1474,7 +1474,11 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
     fpsi_lin_stats st{};
     if (int rc = solve_tol(c, rtol, max_sweeps, use_guess, &st)) return rc;
     if (lin) lin[s] = st;
+    // turbulence scalars: the reference's fixed-sweep scalar solve
+    if (c->kp.turb)
+      if (int rc = enqueue_sa_solve(c, -1, c->prm.sgs_iter)) return rc;
     launch_update(c->g, c->kp, c->s, c->stream);
+    if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -89,3 +89,35 @@ def test_sweep_modes_agree(case):
         s.newton(3)
         outs.append(s.get_state())
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_adaptive_newton_with_turbulence_and_learned_guess():
+    """The hybrid-grid SA workload (prism layers, 7-faced cells) through the
+    tolerance-driven Newton step: with rtol = 0 and the plain start it is
+    the fixed-sweep step bit for bit (fluid and nuTilde); with a trained
+    network and a 1e-3 linear tolerance the guess is never worse than the
+    plain start and nuTilde stays positive."""
+    g, q, prm = workloads.c4_hybrid(6, 5, 6, 5)
+    nsw = prm["sgs_iter"]
+    a = Solver(g, prm)
+    a.set_state(q)
+    a.newton(3)
+    b = Solver(g, prm)
+    b.set_state(q)
+    b.newton_adaptive(3, 0.0, nsw, use_guess=False)
+    assert np.array_equal(a.get_state(), b.get_state())
+    s = Solver(g, prm)
+    s.set_state(q)
+    s.mlp_init(seed=5)
+    for _ in range(3):
+        s.assemble()
+        s.mlp_harvest(30)
+        s.update()
+    s.mlp_train(200, lr=2e-3)
+    _, lin = s.newton_adaptive(3, 1e-3, 60, use_guess=True)
+    print(f"hybrid SA, learned guess: sweeps {[l['sweeps'] for l in lin]}, kept {[l['used_guess'] for l in lin]}")
+    for l in lin:
+        assert l["rguess"] <= l["r0"] * (1 + 1e-12)
+        assert l["rfinal"] <= 1e-3 * l["r0"] or l["sweeps"] == 60
+    qs = s.get_state()
+    assert np.all(np.isfinite(qs)) and np.all(qs[5] >= 1e-15)
