@@ -23,7 +23,7 @@ _ip = C.POINTER(C.c_int)
 FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
-SOLVER = {"sgs": 0, "lsgs": 1}
+SOLVER = {"sgs": 0, "lsgs": 1, "fsgs": 2}
 SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2}
 TURB = {"none": 0, "sa": 1, "sst": 2}
 NSCALAR = {"none": 0, "sa": 1, "sst": 2}  # turbulence scalars after the 5 fluid variables

@@ -74,6 +74,9 @@ struct fpsi_ctx {
   double sa_lsgs_w[2] = {0, 0}, sa_lsgs_resid[2] = {0, 0};
   size_t fj_bytes() const { return fp32 ? 4 : 8; }
   bool lsgs() const { return prm.linear_solver == FPSI_SOLVER_LSGS; }
+  bool fsgs() const { return prm.linear_solver == FPSI_SOLVER_FSGS; }
+  // line SGS and FSGS share the breadth-first order and the unfactored D
+  bool line_order() const { return prm.linear_solver != FPSI_SOLVER_SGS; }
   // turbulence models: buffers allocated on first selection, wall distance set by the caller
   bool sa_alloc = false, have_dist = false;
   int nscal() const { return prm.turb == FPSI_TURB_SST ? 2 : prm.turb == FPSI_TURB_SA ? 1 : 0; }
@@ -719,7 +722,8 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->flux != FPSI_FLUX_HLLC && p->flux != FPSI_FLUX_ROE) return fail(c, FPSI_ERR_ARG, "unknown flux");
   if (p->limiter < 0 || p->limiter > 3) return fail(c, FPSI_ERR_ARG, "unknown limiter");
   if (p->precision != FPSI_PREC_FP64 && p->precision != FPSI_PREC_FP32) return fail(c, FPSI_ERR_ARG, "unknown precision");
-  if (p->linear_solver != FPSI_SOLVER_SGS && p->linear_solver != FPSI_SOLVER_LSGS)
+  if (p->linear_solver != FPSI_SOLVER_SGS && p->linear_solver != FPSI_SOLVER_LSGS &&
+      p->linear_solver != FPSI_SOLVER_FSGS)
     return fail(c, FPSI_ERR_ARG, "unknown linear solver");
   if (p->turb != FPSI_TURB_NONE && p->turb != FPSI_TURB_SA && p->turb != FPSI_TURB_SST)
     return fail(c, FPSI_ERR_ARG, "unknown turbulence model");
@@ -731,6 +735,14 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     if (p->lsgs_maxiter < 0) return fail(c, FPSI_ERR_ARG, "LSGSMaxIter < 0");
     if (!(p->lsgs_relax > 0.0)) return fail(c, FPSI_ERR_ARG, "LSGSRelaxation must be > 0");
     if (int rc = lsgs_alloc(c)) return rc;
+  }
+  if (p->linear_solver == FPSI_SOLVER_FSGS) {
+    if (int rc = lsgs_alloc(c)) return rc;  // the line-SGS assembly writes lu64
+    if (!c->s.luf) {
+      double *luf = nullptr;
+      if (int rc = dalloc(c, &luf, 25 * (size_t)c->nc)) return rc;
+      c->s.luf = luf;
+    }
   }
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
   if (p->sweep_mode == FPSI_SWEEP_PERSISTENT && !c->have_flow)
@@ -898,7 +910,8 @@ static void enqueue_assemble(fpsi_ctx *c) {
   launch_face_jac(c->g, c->kp, c->fp32, c->s, c->stream);
   prof_end(c, FPSI_K_FACE_JAC, e);
   e = prof_begin(c);
-  launch_cell_assemble(c->g, c->kp, c->fp32, c->lsgs() ? 1 : 0, c->s, c->stream);
+  launch_cell_assemble(c->g, c->kp, c->fp32, c->line_order() ? 1 : 0, c->s, c->stream);
+  if (c->fsgs()) launch_fsgs_factor(c->g, c->fp32, c->s, c->stream);
   if (c->kp.turb) launch_sa_assemble(c->g, c->kp, c->nnz, c->s, c->stream);
   prof_end(c, FPSI_K_ASSEMBLE, e);
 }
@@ -949,8 +962,29 @@ static int enqueue_sa_lsgs(fpsi_ctx *c, int sc, int maxiter) {
 }
 
 // solve turbulence scalar `which` (-1: all of them)
+// FSGSScalarSolve (src/linearSolverFSGS.loci:1039-1200) of scalar sc: X
+// from zero, nsweeps symmetric sweeps in the breadth-first order
+static void enqueue_sa_fsgs(fpsi_ctx *c, int sc, int nsweeps) {
+  const int nl = (int)c->lsgs_lev_off.size() - 1;
+  const long xs = (long)c->nc + c->nper + 1;
+  launch_zero_d(c->nc + (long)c->nper, c->s.sX + sc * xs, c->stream);
+  for (int it = 0; it < nsweeps; ++it) {
+    launch_ghost1(c->g, c->nscal(), c->s.sX, c->stream);
+    for (int l = 0; l < nl; ++l)
+      launch_sa_lsgs_level(c->g, sc, c->nnz, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                           c->stream);
+    for (int l = nl - 1; l >= 0; --l)
+      launch_sa_lsgs_level(c->g, sc, c->nnz, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                           c->stream);
+  }
+}
+
 static int enqueue_sa_solve(fpsi_ctx *c, int which, int nsweeps) {
   const int sc0 = which < 0 ? 0 : which, ns = which < 0 ? c->nscal() : 1;
+  if (c->fsgs()) {
+    for (int sc = sc0; sc < sc0 + ns; ++sc) enqueue_sa_fsgs(c, sc, nsweeps);
+    return 0;
+  }
   if (c->lsgs()) {
     for (int sc = sc0; sc < sc0 + ns; ++sc)
       if (int rc = enqueue_sa_lsgs(c, sc, nsweeps)) return rc;
@@ -1207,6 +1241,26 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
   return 0;
 }
 
+// FSGSBlockedSSolve (src/linearSolverFSGS.loci:1598-1708): X from zero, then
+// nsweeps symmetric sweeps in the breadth-first line order with the
+// unpivoted Jacobian-precision factors, periodic ghosts refreshed before each.
+static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
+  const int nl = (int)c->lsgs_lev_off.size() - 1;
+  launch_zero_d(5 * ((long)c->nc + c->nper + 1), c->s.X, c->stream);
+  for (int it = 0; it < nsweeps; ++it) {
+    launch_ghost(c->g, c->s, c->stream);
+    hipEvent_t e = prof_begin(c);
+    for (int l = 0; l < nl; ++l)
+      launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
+    e = prof_begin(c);
+    for (int l = nl - 1; l >= 0; --l)
+      launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
+    prof_end(c, FPSI_K_SGS_SWEEP, e);
+  }
+  return 0;
+}
+
 extern "C" int fpsi_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *resid) {
   if (!c) return FPSI_ERR_ARG;
   if (iters) *iters = c->lsgs_iters;
@@ -1216,11 +1270,12 @@ extern "C" int fpsi_lsgs_stats(fpsi_ctx *c, int *iters, double *relax, double *r
 }
 
 static int enqueue_solve(fpsi_ctx *c, int nsweeps) {
+  if (c->fsgs()) return enqueue_fsgs(c, nsweeps);
   return c->lsgs() ? enqueue_lsgs(c, nsweeps) : enqueue_sweeps(c, nsweeps);
 }
 
 static int need_sgs(fpsi_ctx *c) {
-  if (c->lsgs()) return fail(c, FPSI_ERR_STATE, "this call needs linear_solver = FPSI_SOLVER_SGS");
+  if (c->line_order()) return fail(c, FPSI_ERR_STATE, "this call needs linear_solver = FPSI_SOLVER_SGS");
   return 0;
 }
 
@@ -1264,7 +1319,7 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
     enqueue_assemble(c);
     HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     // turbulence scalars ride in the fluid's dataflow sweep when both use SGS
-    const bool fuse = c->kp.turb && !c->lsgs() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
+    const bool fuse = c->kp.turb && !c->line_order() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
     if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
                       : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
       hipStreamSynchronize(c->stream);

@@ -47,6 +47,11 @@ void launch_axpy_d(long n, const double *a, double alpha, const double *d, doubl
 void launch_ghost_of(const DevGrid &g, double *V, hipStream_t st);
 // the same for ns scalars per cell (V holds ns x (nc + nper + 1) entries)
 void launch_ghost1(const DevGrid &g, int ns, double *V, hipStream_t st);
+// FSGS: unpivoted factors of the unfactored D in lu (Jacobian precision) into
+// luf, one level of a sweep in the line order, and a zero fill
+void launch_fsgs_factor(const DevGrid &g, int fp32, DevState &s, hipStream_t st);
+void launch_fsgs_level(const DevGrid &g, int fp32, const int *cells, int lo, int hi, DevState &s, hipStream_t st);
+void launch_zero_d(long n, double *x, hipStream_t st);
 // line SGS
 int lsgs_parts(const DevGrid &g);
 void launch_lsgs_init(DevState &s, double w0, hipStream_t st);

@@ -1198,6 +1198,96 @@ __global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restr
   }
 }
 
+// FSGS (src/linearSolverFSGS.loci:1297-1356): the diagonal factored without
+// pivots in the Jacobian precision F (from the unfactored fluid_D the LSGS
+// assembly keeps in lu), and one level of a sweep in the line order with
+// those factors; rows read lower faces first, as in line SGS
+template <typename F>
+__global__ void k_fsgs_factor(int nc, const F *__restrict__ D, F *__restrict__ luf) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  F a[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = D[(size_t)c * 25 + i];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      const F l = a[i + 5 * k] / a[k + 5 * k];
+      a[i + 5 * k] = l;
+#pragma unroll
+      for (int j = k + 1; j < 5; ++j) a[i + 5 * j] = a[i + 5 * j] - l * a[k + 5 * j];
+    }
+#pragma unroll
+  for (int i = 0; i < 25; ++i) luf[(size_t)c * 25 + i] = a[i];
+}
+
+template <typename F>
+__global__ void __launch_bounds__(64) k_fsgs_level(DevGrid g, const int *__restrict__ cells, int lo, int hi,
+                                                   const F *__restrict__ off, const F *__restrict__ luf,
+                                                   const F *__restrict__ B, double *__restrict__ X) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int k = lo + blockIdx.x * CPW + q;
+  const bool act = q < CPW && k < hi;
+  const int gbase = 5 * (q < CPW ? q : 0);
+  double t = 0.0;
+  F a[25];
+  int c = 0;
+  if (act) {
+    c = cells ? cells[k] : k;
+    t = -(double)B[(size_t)c * 5 + i];
+    const int m0 = g.m_off[c];
+    const int len = g.m_off[c + 1] - m0;
+    const int nup = g.m_nup[c];
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)c * MAXR + j];
+    F mrow[MAXR][5];
+    double xv[MAXR][5];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) {
+        mrow[j][jj] = M[i + 5 * jj];
+        xv[j][jj] = X[(size_t)nb[j] * 5 + jj];
+      }
+    }
+#pragma unroll
+    for (int i2 = 0; i2 < 25; ++i2) a[i2] = luf[(size_t)c * 25 + i2];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
+    for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const F *M = off + (size_t)(m0 + jr) * 25;
+      const double *xn = X + (size_t)g.m_nbr[m0 + jr] * 5;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += M[i + 5 * jj] * xn[jj];
+    }
+  }
+  double v[5];
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) v[kk] = -__shfl(t, gbase + kk, 64);
+  if (act) {
+    double x[5];
+    lusolve5<F>(a, PIV_ID, v, x);
+    const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+    X[(size_t)c * 5 + i] = xi;
+  }
+}
+
+__global__ void k_zero_d(long n, double *__restrict__ x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
 // st: w, zero_resid, resid, iterations run, done
 __global__ void k_lsgs_init(double *__restrict__ st, double w0) {
   if (threadIdx.x == 0) {
@@ -2261,6 +2351,27 @@ void launch_lsgs_level(const DevGrid &g, int fp32, int backward, const int *cell
     if (backward) lsgs_level_t<double, true>(g, cells, lo, hi, s, st);
     else lsgs_level_t<double, false>(g, cells, lo, hi, s, st);
   }
+}
+void launch_fsgs_factor(const DevGrid &g, int fp32, DevState &s, hipStream_t st) {
+  if (fp32)
+    hipLaunchKernelGGL(k_fsgs_factor<float>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g.nc, (const float *)s.lu,
+                       (float *)s.luf);
+  else
+    hipLaunchKernelGGL(k_fsgs_factor<double>, dim3(nblk(g.nc)), dim3(TPB), 0, st, g.nc, (const double *)s.lu,
+                       (double *)s.luf);
+}
+void launch_fsgs_level(const DevGrid &g, int fp32, const int *cells, int lo, int hi, DevState &s, hipStream_t st) {
+  if (hi <= lo) return;
+  const int nb = (hi - lo + CPW - 1) / CPW;
+  if (fp32)
+    hipLaunchKernelGGL(k_fsgs_level<float>, dim3(nb), dim3(64), 0, st, g, cells, lo, hi, (const float *)s.off,
+                       (const float *)s.luf, (const float *)s.B, s.X);
+  else
+    hipLaunchKernelGGL(k_fsgs_level<double>, dim3(nb), dim3(64), 0, st, g, cells, lo, hi, (const double *)s.off,
+                       (const double *)s.luf, (const double *)s.B, s.X);
+}
+void launch_zero_d(long n, double *x, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, x);
 }
 void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st) {
   const int n = g.nc * 5, np = lsgs_parts(g);

@@ -80,6 +80,7 @@ struct DevState {
   double *sB;     // [ns][nc]
   double *sX;     // [ns][nc + nper + 1] scalar solutions with ghosts and the zero entry
   double *sxt;    // [ns][nc] scalar line SGS: each cell's value before the forward sweep
+  void *luf;      // [nc][25] FSGS: diagonal factored without pivots in the Jacobian precision
   // line SGS (allocated when that solver is selected)
   double *lu64;   // [nc][25] unpivoted LU of fluid_D in FP64
   double *xt;     // [nc][5] X after the forward sweep

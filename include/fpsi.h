@@ -42,9 +42,9 @@ enum { FPSI_BC_INTERIOR = 0, FPSI_BC_FARFIELD = 1, FPSI_BC_WALL = 2,
 enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
 enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
 enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
-/* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, or line SGS
- * (src/linearSolverLSGS.loci) */
-enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1 };
+/* fluidLinearSolver (src/solverNewton.loci:229-259): block SGS, line SGS
+ * (src/linearSolverLSGS.loci), or FSGS (src/linearSolverFSGS.loci) */
+enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1, FPSI_SOLVER_FSGS = 2 };
 /* turbulence model, solved segregated like the reference: none,
  * Spalart-Allmaras (turbulence/SA/SA_turb.loci; baseline, standard
  * production; the state carries nuTilde as its 6th row), or Menter SST
@@ -190,7 +190,10 @@ int fpsi_get_system(fpsi_ctx *ctx, void *fjp, void *fjm, void *D, int *pivot, vo
  *   then nsweeps symmetric level-scheduled sweeps.
  * FPSI_SOLVER_LSGS: LSGSBlockedSSolve (src/linearSolverLSGS.loci:1692-1954),
  *   line-Jacobi start then at most nsweeps relaxed symmetric sweeps in the
- *   breadth-first line order, stopping on lsgs_abstol / lsgs_reltol. */
+ *   breadth-first line order, stopping on lsgs_abstol / lsgs_reltol.
+ * FPSI_SOLVER_FSGS: FSGSBlockedSSolve (src/linearSolverFSGS.loci:1598-1708),
+ *   X from zero, nsweeps symmetric sweeps in the same breadth-first order
+ *   with the diagonal factored without pivots in the Jacobian precision. */
 int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
 
 /* nuTilde system of the assembled step (FPSI_TURB_SA): src[nc] (X_src),
@@ -203,7 +206,9 @@ int fpsi_linear_solve(fpsi_ctx *ctx, int nsweeps, double *dq);
  *   nsweeps symmetric level-scheduled sweeps;
  * FPSI_SOLVER_LSGS: LSGSScalarSolve (src/linearSolverLSGS.loci:1112-1404),
  *   at most nsweeps relaxed sweeps in the breadth-first order with the
- *   lsgs_* stopping tests; fpsi_sa_lsgs_stats reports its outcome. */
+ *   lsgs_* stopping tests; fpsi_sa_lsgs_stats reports its outcome;
+ * FPSI_SOLVER_FSGS: FSGSScalarSolve (src/linearSolverFSGS.loci:1039-1200),
+ *   X from zero, nsweeps symmetric sweeps in that order. */
 int fpsi_sa_get_system(fpsi_ctx *ctx, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu);
 int fpsi_sa_solve(fpsi_ctx *ctx, int nsweeps, double *dX);
 int fpsi_sa_lsgs_stats(fpsi_ctx *ctx, int *iters, double *relax, double *resid);

@@ -113,7 +113,7 @@ def params(p: dict) -> OgParams:
     s.urelax = p.get("urelax", 1e30)
     s.sgs_iter = p.get("sgs_iter", 5)
     s.minPg = p.get("minPg", -s.Pambient)
-    s.linear_solver = {"sgs": 0, "lsgs": 1}[p.get("solver", "sgs")]
+    s.linear_solver = {"sgs": 0, "lsgs": 1, "fsgs": 2}[p.get("solver", "sgs")]
     s.lsgs_maxiter = p.get("lsgs_maxiter", 10)
     s.lsgs_reltol = p.get("lsgs_reltol", 0.1)
     s.lsgs_abstol = p.get("lsgs_abstol", 1e-10)
@@ -273,6 +273,20 @@ class Oracle:
                             sys["Draw"].ctypes.data_as(C.c_void_p), sys["B"].ctypes.data_as(C.c_void_p),
                             _ptr(X), C.byref(wo), C.byref(ro))
         return X, dict(iters=it, w=wo.value, resid=ro.value)
+
+    def fsgs(self, sys, nsweeps):
+        """FSGS from zero: nsweeps symmetric sweeps in the line order."""
+        X = np.zeros((self.g.nc, 5))
+        self.L.og_fsgs(self.og.ref, int(nsweeps), sys["fjp"].ctypes.data_as(C.c_void_p),
+                       sys["fjm"].ctypes.data_as(C.c_void_p), sys["Draw"].ctypes.data_as(C.c_void_p),
+                       sys["B"].ctypes.data_as(C.c_void_p), _ptr(X))
+        return X
+
+    def sa_fsgs(self, sys_, nsweeps):
+        X = np.zeros(self.g.nc)
+        self.L.og_sa_fsgs(self.og.ref, int(nsweeps), _ptr(sys_["fjp"]), _ptr(sys_["fjm"]), _ptr(sys_["D"]),
+                          _ptr(sys_["B"]), _ptr(X))
+        return X
 
     def sgs(self, sys, nsweeps):
         X = np.zeros((self.g.nc, 5))

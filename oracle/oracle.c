@@ -1484,7 +1484,7 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
   real_fj *D = malloc(sizeof(real_fj) * 25 * (size_t)nc), *B = malloc(sizeof(real_fj) * 5 * (size_t)nc);
   int *piv = malloc(sizeof(int) * 5 * (size_t)nc);
   double *X = malloc(sizeof(double) * 5 * (size_t)nc);
-  real_fj *Draw = p->linear_solver == 1 ? malloc(sizeof(real_fj) * 25 * (size_t)nc) : NULL;
+  real_fj *Draw = p->linear_solver != 0 ? malloc(sizeof(real_fj) * 25 * (size_t)nc) : NULL;
   double *ssrc = NULL, *sfp = NULL, *sfm = NULL, *sD = NULL, *sB = NULL, *sX = NULL;
   const int ns = p->turb == 2 ? 2 : p->turb ? 1 : 0; /* turbulence scalars */
   if (ns) {
@@ -1503,6 +1503,8 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
         if (p->linear_solver == 1) /* LSGSScalarSolve, src/solverScalarTransport.loci:633-637 */
           og_sa_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, sfp + of, sfm + of, sD + oc,
                      sB + oc, sX + oc, NULL, NULL);
+        else if (p->linear_solver == 2) /* FSGSScalarSolve, :639-643 */
+          og_sa_fsgs(g, p->sgs_iter, sfp + of, sfm + of, sD + oc, sB + oc, sX + oc);
         else /* SGSScalarSolve, :627-631 */
           og_sa_sgs(g, p->sgs_iter, sfp + of, sfm + of, sD + oc, sB + oc, sX + oc);
       }
@@ -1515,6 +1517,8 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
     }
     if (p->linear_solver == 1)  /* fluidLinearSolver lsgs, src/solverNewton.loci:256-259 */
       og_lsgs(g, p->lsgs_maxiter, p->lsgs_abstol, p->lsgs_reltol, p->lsgs_relax, fjp, fjm, Draw, B, X, NULL, NULL);
+    else if (p->linear_solver == 2)  /* fluidLinearSolver fsgs */
+      og_fsgs(g, p->sgs_iter, fjp, fjm, Draw, B, X);
     else
       og_sgs(g, p->sgs_iter, fjp, fjm, D, piv, B, X);
     OG_PAR
@@ -2227,6 +2231,43 @@ int og_sa_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, doub
   return iter;
 }
 
+/* FSGSScalarSolve (src/linearSolverFSGS.loci:998-1200) with one-cell lines:
+ * 1/D (:1019-1031), X starting at zero (:1049), nsweeps symmetric sweeps in
+ * the breadth-first order with periodic copies (:1066-1106), X = (B - lower
+ * - upper couplings)/D; no relaxation, no stopping test. */
+void og_sa_fsgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
+                const double *B, double *X) {
+  const int nc = g->nc, nf = g->nf;
+  og_adj a;
+  build_adj(g, &a);
+  int *order = malloc(sizeof(int) * nc);
+  og_lsgs_order(g, order);
+  double *Dinv = malloc(sizeof(double) * nc), *Xg = calloc(nf, sizeof(double));
+  for (int c = 0; c < nc; ++c) Dinv[c] = 1. / D[c];
+  for (int c = 0; c < nc; ++c) X[c] = 0.0;
+  for (int it = 0; it < nsweeps; ++it) {
+    for (int f = 0; f < nf; ++f)
+      if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int ii = 0; ii < nc; ++ii) {
+        const int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
+        double r = B[c];
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          const int f = a.lo[j];
+          r -= X[g->cl[f]] * (-fjp[f]);
+        }
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          const int f = a.up[j];
+          const double xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          r -= xn * fjm[f];
+        }
+        X[c] = Dinv[c] * r;
+      }
+  }
+  free(order); free(Dinv); free(Xg);
+  free_adj(&a);
+}
+
 /* ------------------------------------------------------------------ */
 /* line SGS (src/linearSolverLSGS.loci)                                */
 /* ------------------------------------------------------------------ */
@@ -2300,6 +2341,30 @@ static void lu_nopiv_solve(const real *a, const real *b, real *x, int nb) {
     x[i] = tmp / a[i + nb * i];
   }
 }
+/* the same factorisation and solve with the factors in real_fj (FSGS keeps
+ * them in the Jacobian precision, src/linearSolverFSGS.loci:1299-1308 and
+ * solve_lu_N<N, real_fj, real, real> :73-113) */
+static void lu_nopiv_fj(real_fj *a, int nb) {
+  for (int k = 0; k < nb; ++k)
+    for (int i = k + 1; i < nb; ++i) {
+      real_fj l = a[i + nb * k] / a[k + nb * k];
+      a[i + nb * k] = l;
+      for (int j = k + 1; j < nb; ++j) a[i + nb * j] = a[i + nb * j] - l * a[k + nb * j];
+    }
+}
+static void lu_nopiv_solve_fj(const real_fj *a, const real *b, real *x, int nb) {
+  for (int i = 0; i < nb; ++i) {
+    real tmp = b[i];
+    for (int j = 0; j < i; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp;
+  }
+  for (int i = nb - 1; i >= 0; --i) {
+    real tmp = x[i];
+    for (int j = i + 1; j < nb; ++j) tmp -= a[i + nb * j] * x[j];
+    x[i] = tmp / a[i + nb * i];
+  }
+}
+
 /* t[i] += sum_j M(i,j) v[j] for a double block */
 static inline void dotprod_accum_d(const real *M, const real *v, real *t) {
   for (int j = 0; j < NB; ++j) {
@@ -2383,4 +2448,53 @@ int og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double 
   free(order); free(Dl); free(Xg); free(xt);
   free_adj(&a);
   return iter;
+}
+
+/* FSGSBlockedSSolve (src/linearSolverFSGS.loci:1228-1708): the equation order
+ * of the LSGS topology (:800-845 builds the same one-cell lines in the same
+ * breadth-first order), D factored without pivots in real_fj
+ * (matrixHelpersNoPivotN::factorDiagonal :1299-1308), X starting at zero
+ * (:1608), then nsweeps symmetric sweeps (FSGSMaxIter = gauss_seidel_iter,
+ * :413-440) with periodic copies before each (:1620-1660); per cell
+ * t = -B + sum of lower then upper couplings, X = D^-1 (-t) (sweep :1310-1356).
+ * No relaxation and no stopping test. */
+void og_fsgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v, const void *Draw_v,
+             const void *B_v, double *X) {
+  int nc = g->nc, nf = g->nf;
+  const real_fj *fjp = fjp_v, *fjm = fjm_v, *Draw = Draw_v, *B = B_v;
+  og_adj a;
+  build_adj(g, &a);
+  int *order = malloc(sizeof(int) * nc);
+  og_lsgs_order(g, order);
+  real_fj *Dl = malloc(sizeof(real_fj) * 25 * (size_t)nc);
+  for (size_t i = 0; i < 25 * (size_t)nc; ++i) Dl[i] = Draw[i];
+  for (int c = 0; c < nc; ++c) lu_nopiv_fj(Dl + 25 * (size_t)c, 5);
+  double *Xg = calloc((size_t)5 * nf, sizeof(double));
+  for (size_t i = 0; i < 5 * (size_t)nc; ++i) X[i] = 0.0;
+  for (int it = 0; it < nsweeps; ++it) {
+    for (int f = 0; f < nf; ++f)
+      if (g->btype[f] == OG_PERIODIC)
+        for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int ii = 0; ii < nc; ++ii) {
+        int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
+        real t[5], v[5];
+        for (int k = 0; k < 5; ++k) t[k] = -(real)B[5 * (size_t)c + k];
+        for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
+          int f = a.lo[j];
+          real_fj L[25];
+          for (int i = 0; i < 25; ++i) L[i] = -fjp[25 * (size_t)f + i];
+          dotprod_accum(L, X + 5 * (size_t)g->cl[f], t);
+        }
+        for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
+          int f = a.up[j];
+          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          dotprod_accum(fjm + 25 * (size_t)f, xn, t);
+        }
+        for (int k = 0; k < 5; ++k) v[k] = -t[k];
+        lu_nopiv_solve_fj(Dl + 25 * (size_t)c, v, X + 5 * (size_t)c, 5);
+      }
+  }
+  free(order); free(Dl); free(Xg);
+  free_adj(&a);
 }

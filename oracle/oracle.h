@@ -55,7 +55,7 @@ typedef struct {
   double cflmax, dtmax, urelax;
   int sgs_iter;
   double minPg;           /* floor of the reconstructed gage pressure */
-  int linear_solver;      /* 0 = block SGS, 1 = line SGS */
+  int linear_solver;      /* 0 = block SGS, 1 = line SGS, 2 = FSGS */
   int lsgs_maxiter;
   double lsgs_reltol, lsgs_abstol, lsgs_relax;
   double Minf, etaT, PLimPC;  /* low-Mach preconditioning (Minf >= 0.5: eta = 1) */
@@ -152,6 +152,12 @@ int  og_lsgs_order(const og_grid *g, int *order);
 int  og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double w0, const void *fjp,
              const void *fjm, const void *Draw, const void *B, double *X, double *w_out,
              double *resid_out);
+/* FSGS: X from zero, nsweeps symmetric sweeps in the line-SGS order with the
+ * diagonal factored without pivots in real_fj (block; scalar with 1/D) */
+void og_fsgs(const og_grid *g, int nsweeps, const void *fjp, const void *fjm, const void *Draw, const void *B,
+             double *X);
+void og_sa_fsgs(const og_grid *g, int nsweeps, const double *fjp, const double *fjm, const double *D,
+                const double *B, double *X);
 /* block SGS (Jacobi start + sweeps in cell-index order) on an assembled system */
 void og_sgs(const og_grid *g, int nsweeps, const void *fjp, const void *fjm, const void *D,
             const int *pivot, const void *B, double *X);
