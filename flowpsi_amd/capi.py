@@ -38,7 +38,8 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
            "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats", "fpsi_set_wall_distance",
            "fpsi_sa_get_system", "fpsi_sa_solve", "fpsi_sa_lsgs_stats", "fpsi_turb_get_system",
-           "fpsi_turb_solve", "fpsi_turb_lsgs_stats"]
+           "fpsi_turb_solve", "fpsi_turb_lsgs_stats", "fpsi_set_time_levels", "fpsi_get_time_levels",
+           "fpsi_unsteady_step"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -58,7 +59,8 @@ class Params(C.Structure):
                 ("sweep_mode", C.c_int), ("lsgs_maxiter", C.c_int), ("lsgs_reltol", C.c_double),
                 ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double), ("Minf", C.c_double),
                 ("etaT", C.c_double), ("PLimPC", C.c_double), ("turb", C.c_int), ("Prt", C.c_double),
-                ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double)]
+                ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double),
+                ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -155,6 +157,11 @@ def make_params(prm: dict) -> Params:
     s.nut_bc = prm.get("nut_bc", 5e-9)
     s.k_bc = prm.get("k_bc", 0.0)  # <= 0: the library's default from |u0|
     s.w_bc = prm.get("w_bc", 0.0)
+    # time-accurate BDF2: unsteady=True, time step dtmax, newton_iter
+    # iterations per step (<= 0: 6), pseudo_cflmax (<= 0: 10)
+    s.time_mode = 1 if prm.get("unsteady", False) else 0
+    s.newton_iter = int(prm.get("newton_iter", 6))
+    s.pseudo_cflmax = float(prm.get("pseudo_cflmax", 10.0))
     return s
 
 
@@ -331,6 +338,31 @@ class Solver:
         hist = (Norms * max(nsteps, 1))()
         self._chk(self.L.fpsi_newton_step(self.h, int(nsteps), hist), "fpsi_newton_step")
         return [hist[i].as_dict() for i in range(nsteps)]
+
+    # -- time-accurate BDF2
+    def unsteady(self, nsteps):
+        """nsteps BDF2 time steps; the norms of every Newton iteration as a
+        list of nsteps lists of newton_iter dicts."""
+        ni = int(self.prm.get("newton_iter", 6))
+        ni = ni if ni > 0 else 6
+        hist = (Norms * max(nsteps * ni, 1))()
+        self._chk(self.L.fpsi_unsteady_step(self.h, int(nsteps), hist), "fpsi_unsteady_step")
+        return [[hist[s * ni + i].as_dict() for i in range(ni)] for s in range(nsteps)]
+
+    def set_time_levels(self, qn=None, qnm1=None):
+        arrs = []
+        for a in (qn, qnm1):
+            if a is not None:
+                a = np.ascontiguousarray(a, np.float64)
+                assert a.shape == (self.nvar, self.grid.nc)
+            arrs.append(a)
+        self._chk(self.L.fpsi_set_time_levels(self.h, *(None if a is None else _ptr(a) for a in arrs)),
+                  "fpsi_set_time_levels")
+
+    def get_time_levels(self):
+        qn, qnm1 = np.zeros((self.nvar, self.grid.nc)), np.zeros((self.nvar, self.grid.nc))
+        self._chk(self.L.fpsi_get_time_levels(self.h, _ptr(qn), _ptr(qnm1)), "fpsi_get_time_levels")
+        return qn, qnm1
 
     # -- linear tolerance and learned guess
     def linear_residual(self):

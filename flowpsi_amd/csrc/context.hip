@@ -90,6 +90,12 @@ struct fpsi_ctx {
   bool tiled() const {
     return have_tiles && prm.sweep_mode == FPSI_SWEEP_TILED;
   }
+  // time-accurate BDF2: buffers allocated on first selection; have_prev when
+  // DevState::qm1 holds q{n-1}, in_step when DevState::qn holds the q{n} of
+  // a time step in progress
+  bool bdf2() const { return prm.time_mode == FPSI_TIME_BDF2; }
+  bool bdf2_alloc = false, have_prev = false, in_step = false;
+  int newton_iter = 6;
 };
 
 static int fail(fpsi_ctx *c, int code, const std::string &m) {
@@ -753,7 +759,21 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->sweep_mode == FPSI_SWEEP_TILED && !c->have_tiles)
     return fail(c, FPSI_ERR_ARG, "no tiled sweep on this grid: " + c->tile_msg);
   if (!(p->gamma > 1.0) || !(p->Rtilde > 0.0)) return fail(c, FPSI_ERR_ARG, "bad gas constants");
+  if (p->time_mode != FPSI_TIME_STEADY && p->time_mode != FPSI_TIME_BDF2)
+    return fail(c, FPSI_ERR_ARG, "unknown time mode");
+  if (p->time_mode == FPSI_TIME_BDF2) {
+    if (!(p->dtmax > 0.0) || !std::isfinite(p->dtmax))
+      return fail(c, FPSI_ERR_ARG, "BDF2 needs a finite time step dtmax > 0");
+    if (!c->bdf2_alloc) {
+      const size_t n = 7 * (size_t)c->nc;  // room for every turbulence model's rows
+      if (int rc = dalloc(c, &c->s.qn, n)) return rc;
+      if (int rc = dalloc(c, &c->s.qm1, n)) return rc;
+      if (int rc = dalloc(c, &c->s.rhs, 5 * (size_t)c->nc)) return rc;
+      c->bdf2_alloc = true;
+    }
+  }
   c->prm = *p;
+  c->newton_iter = p->newton_iter > 0 ? p->newton_iter : 6;
   Prm &k = c->kp;
   k.gamma = p->gamma; k.Rt = p->Rtilde; k.Pamb = p->Pambient;
   k.flux = p->flux; k.limiter = p->limiter; k.Kl = p->Kl; k.viscous = p->viscous;
@@ -786,6 +806,8 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     k.k_bc = p->k_bc > 0 ? p->k_bc : kd;
     k.w_bc = p->w_bc > 0 ? p->w_bc : 0.09 * kd / (1e-3 * 1e-5);
   }
+  k.unsteady = p->time_mode == FPSI_TIME_BDF2 ? 1 : 0;
+  k.pseudo_cflmax = p->pseudo_cflmax > 0 ? p->pseudo_cflmax : 10.0;
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -804,6 +826,7 @@ extern "C" int fpsi_set_state(fpsi_ctx *c, const double *q) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
   c->assembled = false;
+  c->have_prev = c->in_step = false;  // q{n-1} = q until set otherwise
   return 0;
 }
 
@@ -911,6 +934,7 @@ static void enqueue_assemble(fpsi_ctx *c) {
   prof_end(c, FPSI_K_FACE_JAC, e);
   e = prof_begin(c);
   launch_cell_assemble(c->g, c->kp, c->fp32, c->line_order() ? 1 : 0, c->s, c->stream);
+  if (c->kp.unsteady) launch_rhs_norms(c->g, c->vtot, c->s, c->stream);
   if (c->fsgs()) launch_fsgs_factor(c->g, c->fp32, c->s, c->stream);
   if (c->kp.turb) launch_sa_assemble(c->g, c->kp, c->nnz, c->s, c->stream);
   prof_end(c, FPSI_K_ASSEMBLE, e);
@@ -1089,9 +1113,19 @@ extern "C" int fpsi_sa_solve(fpsi_ctx *c, int nsweeps, double *dX) {
   return fpsi_turb_solve(c, 0, nsweeps, dX);
 }
 
+// BDF2 time levels for the systems of the step in progress: q{n-1} = q
+// unless set, q{n} = q at the step's start
+static void enqueue_levels(fpsi_ctx *c) {
+  const long n = (long)c->nvar() * c->nc;
+  if (!c->have_prev) launch_copy_d(n, c->s.q, c->s.qm1, c->stream);
+  if (!c->in_step) launch_copy_d(n, c->s.q, c->s.qn, c->stream);
+  c->have_prev = c->in_step = true;
+}
+
 extern "C" int fpsi_assemble(fpsi_ctx *c) {
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
+  if (c->bdf2()) enqueue_levels(c);
   enqueue_assemble(c);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1308,48 +1342,116 @@ extern "C" int fpsi_update(fpsi_ctx *c) {
   return 0;
 }
 
-extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
-  if (!c) return FPSI_ERR_ARG;
-  if (int rc = check_ready(c)) return rc;
-  if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
-  std::vector<double> h(12 * (size_t)std::max(nsteps, 1));
+// one Newton iteration after the norms of its residual went to dh: linear
+// solves of the fluid and the turbulence scalars, then the updates
+static int enqueue_iteration(fpsi_ctx *c, double *dh) {
+  enqueue_assemble(c);
+  HIPCHK(c, hipMemcpyAsync(dh, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  // turbulence scalars ride in the fluid's dataflow sweep when both use SGS
+  const bool fuse = c->kp.turb && !c->line_order() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
+  if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
+                    : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter))
+    return rc;
+  if (c->kp.turb && !fuse) {
+    hipEvent_t e2 = prof_begin(c);
+    if (int rc = enqueue_sa_solve(c, -1, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) return rc;
+    prof_end(c, FPSI_K_SGS_SWEEP, e2);
+  }
+  hipEvent_t e = prof_begin(c);
+  launch_update(c->g, c->kp, c->s, c->stream);
+  if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
+  prof_end(c, FPSI_K_UPDATE, e);
+  return 0;
+}
+
+// n Newton iterations (per time step when levels: then q{n,0} = q{n} and the
+// levels shift after the step's `levels` iterations); hist[n] nullable
+static int run_iterations(fpsi_ctx *c, int n, int levels, fpsi_norms *hist) {
+  std::vector<double> h(12 * (size_t)std::max(n, 1));
   double *dh = nullptr;
-  if (nsteps > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
-  for (int s = 0; s < nsteps; ++s) {
-    enqueue_assemble(c);
-    HIPCHK(c, hipMemcpyAsync(dh + 12 * (size_t)s, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    // turbulence scalars ride in the fluid's dataflow sweep when both use SGS
-    const bool fuse = c->kp.turb && !c->line_order() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
-    if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
-                      : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
+  if (n > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
+  const long nv = (long)c->nvar() * c->nc;
+  for (int s = 0; s < n; ++s) {
+    if (levels && s % levels == 0) {  // q{n,0} = q{n}; q{n-1} = q unless set
+      c->in_step = false;
+      enqueue_levels(c);
+    }
+    if (int rc = enqueue_iteration(c, dh + 12 * (size_t)s)) {
       hipStreamSynchronize(c->stream);
+      hipFree(dh);
       return rc;
     }
-    if (c->kp.turb && !fuse) {
-      hipEvent_t e2 = prof_begin(c);
-      if (int rc = enqueue_sa_solve(c, -1, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
-        hipStreamSynchronize(c->stream);
-        return rc;
-      }
-      prof_end(c, FPSI_K_SGS_SWEEP, e2);
+    if (levels && (s + 1) % levels == 0) {  // q{n+1} = the last iterate, then q{n-1} = q{n}
+      launch_copy_d(nv, c->s.qn, c->s.qm1, c->stream);
+      c->in_step = false;
     }
-    hipEvent_t e = prof_begin(c);
-    launch_update(c->g, c->kp, c->s, c->stream);
-    if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
-    prof_end(c, FPSI_K_UPDATE, e);
   }
   HIPCHK(c, hipGetLastError());
-  if (nsteps > 0) {
+  if (n > 0) {
     HIPCHK(c, hipMemcpyAsync(h.data(), dh, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipFreeAsync(dh, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->assembled = false;
-  for (int s = 0; s < nsteps; ++s) {
+  for (int s = 0; s < n; ++s) {
     if (hist) norms_from(&h[12 * (size_t)s], &hist[s]);
     const double tot = h[12 * s] + h[12 * s + 1] + h[12 * s + 2];
-    if (tot != tot) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
+    if (tot != tot) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton iteration " + std::to_string(s));
   }
+  return 0;
+}
+
+extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
+  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time_mode is BDF2: use fpsi_unsteady_step");
+  return run_iterations(c, nsteps, 0, hist);
+}
+
+extern "C" int fpsi_unsteady_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
+  if (!c->bdf2()) return fail(c, FPSI_ERR_STATE, "fpsi_unsteady_step needs time_mode = FPSI_TIME_BDF2");
+  return run_iterations(c, nsteps * c->newton_iter, c->newton_iter, hist);
+}
+
+static int check_levels(fpsi_ctx *c) {
+  if (!c->have_params || !c->bdf2()) return fail(c, FPSI_ERR_STATE, "time levels need time_mode = FPSI_TIME_BDF2");
+  if (!c->have_state) return fail(c, FPSI_ERR_STATE, "fpsi_set_state not called");
+  return 0;
+}
+
+extern "C" int fpsi_set_time_levels(fpsi_ctx *c, const double *qn, const double *qnm1) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_levels(c)) return rc;
+  const int nc = c->nc, nv = c->nvar();
+  std::vector<double> h((size_t)nv * nc);
+  for (int lev = 0; lev < 2; ++lev) {
+    const double *src = lev == 0 ? qn : qnm1;
+    double *dst = lev == 0 ? c->s.qn : c->s.qm1;
+    if (src) {
+      for (int k = 0; k < nv; ++k)
+        for (int ni = 0; ni < nc; ++ni) h[(size_t)k * nc + ni] = src[(size_t)k * nc + c->cperm[ni]];
+      HIPCHK(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else if (lev == 0 || !c->have_prev) {
+      launch_copy_d((long)nv * nc, c->s.q, dst, c->stream);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_prev = c->in_step = true;
+  c->assembled = false;
+  return 0;
+}
+
+extern "C" int fpsi_get_time_levels(fpsi_ctx *c, double *qn, double *qnm1) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_levels(c)) return rc;
+  if (qn) if (int rc = copy_cells_out(c, c->in_step ? c->s.qn : c->s.q, c->nvar(), qn)) return rc;
+  if (qnm1) if (int rc = copy_cells_out(c, c->have_prev ? c->s.qm1 : c->s.q, c->nvar(), qnm1)) return rc;
   return 0;
 }
 
@@ -1518,6 +1620,7 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
   if (int rc = need_sgs(c)) return rc;
+  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time_mode is BDF2: use fpsi_unsteady_step");
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     double o[12];

@@ -10,7 +10,11 @@ void launch_eta(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 int norm_parts(const DevGrid &g);
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st);
+// norms of the BDF2 right-hand side in DevState::rhs into DevState::nout
+void launch_rhs_norms(const DevGrid &g, double vtot, DevState &s, hipStream_t st);
 void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipStream_t st);
+// fluid_D, its factors, B and the Jacobi start; with Prm::unsteady the BDF2
+// system of time levels DevState::qn, qm1 (rhs also into DevState::rhs)
 void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st);
 void launch_sgs_level(const DevGrid &g, int fp32, int lo, int hi, DevState &s, hipStream_t st);
 // dataflow half-sweep in one cooperative launch (co-resident waves: sgs_sweep_waves);

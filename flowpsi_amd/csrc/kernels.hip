@@ -520,6 +520,33 @@ __device__ void block_norms(NormAcc a, double *__restrict__ part) {
   }
 }
 
+// norm terms of one cell from its residual s (src/solverResidual.loci:73-111)
+__device__ __forceinline__ void cell_norms(const DevGrid &g, double vtot, int c, const double *s, NormAcc &a) {
+  const double vol = g.vol[c];
+  const double rvol = 1. / vol;
+  const double weight = vol / vtot;
+  const int id = g.old_id[c];
+  const double r0 = rvol * s[0];
+  if (fabs(0.0) < fabs(r0)) { a.rmax = r0; a.rid = id; a.rcell = c; }
+  double rt = 0;
+  rt += r0 * r0;
+  rt *= weight;
+  double mt = 0, mmax = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double v = rvol * s[1 + i];
+    if (fabs(mmax) < fabs(v)) mmax = v;
+    mt += v * v;
+  }
+  mt *= weight / 3.0;
+  const double ve = rvol * s[4];
+  a.rt = rt;
+  a.mt = mt;
+  a.et = weight * ve * ve;
+  a.emax = ve; a.eid = id; a.ecell = c;
+  a.mmax = mmax; a.mid = id;
+}
+
 __global__ void k_gather_src(DevGrid g, int viscous, double vtot, const double *__restrict__ iflux,
                              const double *__restrict__ vflux, double *__restrict__ src,
                              double *__restrict__ part) {
@@ -550,30 +577,21 @@ __global__ void k_gather_src(DevGrid g, int viscous, double vtot, const double *
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) src[k * nc + c] = s[k];
-    // norm terms of this cell
-    const double vol = g.vol[c];
-    const double rvol = 1. / vol;
-    const double weight = vol / vtot;
-    const int id = g.old_id[c];
-    const double r0 = rvol * s[0];
-    if (fabs(0.0) < fabs(r0)) { a.rmax = r0; a.rid = id; a.rcell = c; }
-    double rt = 0;
-    rt += r0 * r0;
-    rt *= weight;
-    double mt = 0, mmax = 0;
+    cell_norms(g, vtot, c, s, a);
+  }
+  block_norms(a, part);
+}
+
+// residual norm partials of a right-hand side held as [5][nc] (the BDF2 rhs,
+// src/solverResidual.loci:113-150)
+__global__ void k_rhs_norms(DevGrid g, double vtot, const double *__restrict__ rhs, double *__restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  NormAcc a = {0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1};
+  if (c < g.nc) {
+    double s[5];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double v = rvol * s[1 + i];
-      if (fabs(mmax) < fabs(v)) mmax = v;
-      mt += v * v;
-    }
-    mt *= weight / 3.0;
-    const double ve = rvol * s[4];
-    a.rt = rt;
-    a.mt = mt;
-    a.et = weight * ve * ve;
-    a.emax = ve; a.eid = id; a.ecell = c;
-    a.mmax = mmax; a.mid = id;
+    for (int k = 0; k < 5; ++k) s[k] = rhs[(size_t)k * g.nc + c];
+    cell_norms(g, vtot, c, s, a);
   }
   block_norms(a, part);
 }
@@ -733,6 +751,16 @@ __global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const
   }
 }
 
+// density, momentum and total energy per volume of a primitive state
+__device__ __forceinline__ void cons5(const Prm &p, double T, double pg, const V3 &u, double *Q) {
+  const double rho = (pg + p.Pamb) / (p.Rt * T);
+  Q[0] = rho;
+  Q[1] = rho * u.x;
+  Q[2] = rho * u.y;
+  Q[3] = rho * u.z;
+  Q[4] = rho * (p.Rt * T / (p.gamma - 1.) + .5 * dot(u, u));
+}
+
 template <typename F, bool LS>
 __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const double *__restrict__ src, const double *__restrict__ maxev,
@@ -740,7 +768,8 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const F *__restrict__ bblk, double *__restrict__ dtau_out,
                                 F *__restrict__ lu, int *__restrict__ piv, F *__restrict__ Bout,
                                 double *__restrict__ X, double *__restrict__ lu64,
-                                const double *__restrict__ etap) {
+                                const double *__restrict__ etap, const double *__restrict__ qn,
+                                const double *__restrict__ qm1, double *__restrict__ rhs) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -769,16 +798,16 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   const double qq = 0.5 * dot(U, U);
   const double fT = rT * qq;
   const double fP = (1. / (gm - 1) + rP * qq);
-  const double reta = 1. / (p.pre ? etap[c] : 1.0);  // src/solverLinearSystem.loci:195
+  const double reta0 = 1. / (p.pre ? etap[c] : 1.0);  // src/solverLinearSystem.loci:195
   const double a2 = gm * Rt * T;
-  const double pterm = (1. - reta) * (gm - 1.) / a2;
-  const double cterm = fP * rT * reta - fT * rP * reta;
+  const double pterm0 = (1. - reta0) * (gm - 1.) / a2;
+  const double cterm = fP * rT * reta0 - fT * rP * reta0;
   const double q2 = 2. * qq;
   double s[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) s[k] = src[k * nc + c];
   const double dsu = s[1] * U.x + s[2] * U.y + s[3] * U.z;
-  const double dtemp = rvol * (s[0] * (fP * rT * reta + fT * pterm) + (dsu - s[4] - q2 * s[0]) * (rT * (pterm + rP * reta))) / (cterm * rT);
+  const double dtemp = rvol * (s[0] * (fP * rT * reta0 + fT * pterm0) + (dsu - s[4] - q2 * s[0]) * (rT * (pterm0 + rP * reta0))) / (cterm * rT);
   dt = dmin(dt, p.urelax * T / (fabs(dtemp) + 1e-30));
   const double dp = rvol * (rT * (q2 * s[0] - dsu + s[4]) - fT * s[0]);
   dt = dmin(dt, p.urelax * pp / (fabs(dp) + 1e-30));
@@ -814,7 +843,15 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   F D[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) D[i] = -sJ[i];
-  const double coef = vol / dt;
+  double coef = vol / dt;
+  double reta = reta0, pterm = pterm0;
+  if (p.unsteady) {  // BDF2 fluid_D, src/solverLinearSystem.loci:384-420
+    coef = 1.5 * vol / dt;
+    const double pseudo_dt = p.pseudo_cflmax / cflpdt;
+    const double w = 1. / (1. + pseudo_dt / dt);
+    reta = 1. / (1. + w * ((p.pre ? etap[c] : 1.0) - 1.));
+    pterm = (1. - reta) * (gm - 1.) / a2;
+  }
   acc(D[0 + 5 * 0], coef * rT);
   acc(D[1 + 5 * 0], coef * rT * U.x);
   acc(D[2 + 5 * 0], coef * rT * U.y);
@@ -831,6 +868,20 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   acc(D[4 + 5 * 2], coef * rho * U.y);
   acc(D[3 + 5 * 3], coef * rho);
   acc(D[4 + 5 * 3], coef * rho * U.z);
+  if (p.unsteady) {  // BDF2 rhs, src/solverTimestep.loci:400-428
+    double Qp[5], Qn[5], Qm[5];
+    cons5(p, T, pg, U, Qp);
+    cons5(p, qn[c], qn[4 * nc + c], V3{qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]}, Qn);
+    cons5(p, qm1[c], qm1[4 * nc + c], V3{qm1[nc + c], qm1[2 * nc + c], qm1[3 * nc + c]}, Qm);
+    const double rdtvol = vol / p.dtmax;
+    const double w = 0.5;
+    const double w1 = (1.0 + w) * rdtvol, w2 = w * rdtvol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      s[k] = (s[k] - w1 * (Qp[k] - Qn[k]) + w2 * (Qn[k] - Qm[k]));
+      rhs[(size_t)k * nc + c] = s[k];
+    }
+  }
   double b[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -1631,10 +1682,37 @@ __global__ void k_sa_face(DevGrid g, Prm p, const double *__restrict__ q, const 
 // continuity over lower, upper, boundary faces; stabilisation; SA sources;
 // diffusion and its divergence; the cb2 term), srcJ, D = -srcJ + rho V/dtau,
 // B = src, 1/D and the Jacobi start X = B/D (src/linearSolverSGS.loci:343-352).
+// D and B of a scalar row from its source and Jacobian sums: steady
+// (src/solverScalarTransport.loci:584-590, 680-690) or BDF2 with the density
+// of time levels n and n - 1 (:602-616, :702-716)
+__device__ __forceinline__ void scalar_time(const Prm &p, double vol, double dtau, double rho, double X,
+                                            const double *__restrict__ qn, const double *__restrict__ qm1, int nc,
+                                            int c, int kv, double sv, double sJ, double &D, double &b) {
+  if (!p.unsteady) {
+    const double sd = rho * (vol / dtau);
+    D = -sJ + sd;
+    const double rhs = 0.0 - sv;
+    b = -rhs;
+    return;
+  }
+  const double rn = (qn[4 * nc + c] + p.Pamb) / (p.Rt * qn[c]);
+  const double rnm1 = (qm1[4 * nc + c] + p.Pamb) / (p.Rt * qm1[c]);
+  const double Xn = qn[(size_t)kv * nc + c], Xm = qm1[(size_t)kv * nc + c];
+  const double factor = vol / p.dtmax;
+  const double w = 0.5;
+  const double w1 = (1. + w) * factor, w2 = -w * factor;
+  const double rhs = w1 * (X - Xn) * rn + w2 * (Xn * rn - Xm * rnm1) - w2 * X * (rn - rnm1) - sv;
+  b = -rhs;
+  const double fd = vol / dtau;
+  const double sd = (1. + w) * fd * rn + w * fd * (rn - rnm1);
+  D = -sJ + sd;
+}
+
 __global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ grad,
                           const double *__restrict__ dtau, const double *__restrict__ iflux,
                           const double *__restrict__ saf, double *__restrict__ ssrc, double *__restrict__ sdinv,
-                          double *__restrict__ sB, double *__restrict__ sX) {
+                          double *__restrict__ sB, double *__restrict__ sX, const double *__restrict__ qn,
+                          const double *__restrict__ qm1) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc, nf = g.nf;
@@ -1655,7 +1733,7 @@ __global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const 
     }
   }
   sv -= X * rhodot;
-  sJ += -rhodot;
+  sJ += p.unsteady ? -1.5 * rhodot : -rhodot;  // :526-546
 #define GU(i, jj) grad[(3 * (1 + (i)) + (jj)) * nc + c]
   const double wx = GU(2, 1) - GU(1, 2), wy = GU(0, 2) - GU(2, 0), wz = GU(1, 0) - GU(0, 1);
 #undef GU
@@ -1708,10 +1786,8 @@ __global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const 
     if (e >= 0) sJ += saf[(size_t)6 * nf + e];
   }
   ssrc[c] = sv;
-  const double sd = rho * (g.vol[c] / dtau[c]);
-  const double D = -sJ + sd;
-  const double rhs = 0.0 - sv;
-  const double b = -rhs;
+  double D, b;
+  scalar_time(p, g.vol[c], dtau[c], rho, X, qn, qm1, nc, c, 5, sv, sJ, D, b);
   const double dinv = 1. / D;
   sB[c] = b;
   sdinv[c] = dinv;
@@ -1888,7 +1964,8 @@ __global__ void k_sst_face_b(DevGrid g, Prm p, const double *__restrict__ q, con
 __global__ void k_sst_cell_b(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ dtau,
                              const double *__restrict__ iflux, const double *__restrict__ saf,
                              const double *__restrict__ scell, double *__restrict__ ssrc, double *__restrict__ sdinv,
-                             double *__restrict__ sB, double *__restrict__ sX) {
+                             double *__restrict__ sB, double *__restrict__ sX, const double *__restrict__ qn,
+                             const double *__restrict__ qm1) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc, nf = g.nf;
@@ -1910,7 +1987,7 @@ __global__ void k_sst_cell_b(DevGrid g, Prm p, const double *__restrict__ q, con
       }
     }
     sv -= X * rhodot;
-    sJ += -rhodot;
+    sJ += p.unsteady ? -1.5 * rhodot : -rhodot;
     sv += scell[(4 + sc) * (size_t)nc + c];
     sJ += scell[(6 + sc) * (size_t)nc + c];
     for (int j = j0; j < j1; ++j) {
@@ -1931,10 +2008,8 @@ __global__ void k_sst_cell_b(DevGrid g, Prm p, const double *__restrict__ q, con
       if (e >= 0) sJ += F[(size_t)3 * nf + e];
     }
     ssrc[(size_t)sc * nc + c] = sv;
-    const double sd = rho * (g.vol[c] / dtau[c]);
-    const double D = -sJ + sd;
-    const double rhs = 0.0 - sv;
-    const double b = -rhs;
+    double D, b;
+    scalar_time(p, g.vol[c], dtau[c], rho, X, qn, qm1, nc, c, 5 + sc, sv, sJ, D, b);
     const double dinv = 1. / D;
     sB[(size_t)sc * nc + c] = b;
     sdinv[(size_t)sc * nc + c] = dinv;
@@ -2000,8 +2075,9 @@ __global__ void k_sa_update(DevGrid g, Prm p, int ns, const double *__restrict__
   }
 }
 
-// damped Newton update, src/solverNewton.loci:62-78
-__global__ void k_update(DevGrid g, double Pamb, const double *__restrict__ X, double *__restrict__ q) {
+// damped Newton update, src/solverNewton.loci:62-78 (steady) and :160-180
+// (BDF2: at most 75 % of the current pressure and temperature)
+__global__ void k_update(DevGrid g, double Pamb, int bdf2, const double *__restrict__ X, double *__restrict__ q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -2009,8 +2085,15 @@ __global__ void k_update(DevGrid g, double Pamb, const double *__restrict__ X, d
   const double dT = dq[0], dp = dq[4];
   double alpha = 1.0;
   const double T = q[c], pg = q[4 * nc + c];
-  alpha = dmin(alpha, 0.9 * T / (fabs(dT) + 1e-30));
-  alpha = dmin(alpha, 0.9 * (pg + Pamb) / (fabs(dp) + 1e-30));
+  if (bdf2) {
+    const double relax = 0.75;
+    const double pnit = pg + Pamb;
+    alpha = dmin(alpha, relax * pnit / dmax(fabs(dp), 1e-30));
+    alpha = dmin(alpha, relax * T / dmax(fabs(dT), 1e-30));
+  } else {
+    alpha = dmin(alpha, 0.9 * T / (fabs(dT) + 1e-30));
+    alpha = dmin(alpha, 0.9 * (pg + Pamb) / (fabs(dp) + 1e-30));
+  }
   q[c] = T + alpha * dT;
   q[4 * nc + c] = pg + alpha * dp;
   q[nc + c] = q[nc + c] + alpha * dq[1];
@@ -2040,6 +2123,10 @@ void launch_face_flux(const DevGrid &g, const Prm &p, DevState &s, hipStream_t s
                      (const double *)s.eta, (const double *)s.tmu, s.iflux, s.vflux, s.maxev);
 }
 int norm_parts(const DevGrid &g) { return nblk(g.nc); }
+void launch_rhs_norms(const DevGrid &g, double vtot, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_rhs_norms, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, vtot, (const double *)s.rhs, s.npart);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(TPB), 0, st, g, nblk(g.nc), s.npart, s.nout);
+}
 void launch_gather(const DevGrid &g, const Prm &p, double vtot, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_src, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.viscous, vtot, s.iflux,
                      s.vflux, s.src, s.npart);
@@ -2059,7 +2146,8 @@ template <typename F, bool LS>
 static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL((k_cell_assemble<F, LS>), dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src, s.maxev,
                      (const F *)s.off, (const F *)s.pblk, (const F *)s.bblk, s.dtau, (F *)s.lu, s.piv,
-                     (F *)s.B, s.X, s.lu64, (const double *)s.eta);
+                     (F *)s.B, s.X, s.lu64, (const double *)s.eta, (const double *)s.qn, (const double *)s.qm1,
+                     s.rhs);
 }
 void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st) {
   if (fp32) {
@@ -2109,7 +2197,7 @@ void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);
 }
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.Pamb, s.X, s.q);
+  hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.Pamb, p.unsteady, s.X, s.q);
 }
 
 
@@ -2427,7 +2515,8 @@ void launch_sa_assemble(const DevGrid &g, const Prm &p, long nnz, DevState &s, h
                        (const double *)s.tmu, (const double *)s.scell, s.saf, s.soff, nnz);
     hipLaunchKernelGGL(k_sst_cell_b, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q,
                        (const double *)s.dtau, (const double *)s.iflux, (const double *)s.saf,
-                       (const double *)s.scell, s.ssrc, s.sdinv, s.sB, s.sX);
+                       (const double *)s.scell, s.ssrc, s.sdinv, s.sB, s.sX, (const double *)s.qn,
+                       (const double *)s.qm1);
     return;
   }
   hipLaunchKernelGGL(k_sa_face, dim3(nblk(g.nf)), dim3(TPB), 0, st, g, p, (const double *)s.q,
@@ -2435,7 +2524,8 @@ void launch_sa_assemble(const DevGrid &g, const Prm &p, long nnz, DevState &s, h
                      s.saf, s.soff);
   hipLaunchKernelGGL(k_sa_cell, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, (const double *)s.q,
                      (const double *)s.grad, (const double *)s.dtau, (const double *)s.iflux,
-                     (const double *)s.saf, s.ssrc, s.sdinv, s.sB, s.sX);
+                     (const double *)s.saf, s.ssrc, s.sdinv, s.sB, s.sX, (const double *)s.qn,
+                     (const double *)s.qm1);
 }
 void launch_sa_level(const DevGrid &g, int sc0, int ns, long nnz, const int *cells, int lo, int hi, DevState &s,
                      hipStream_t st) {

@@ -86,6 +86,10 @@ struct DevState {
   double *xt;     // [nc][5] X after the forward sweep
   double *lparts; // update-norm partials
   double *lst;    // [8] w, zero_resid, resid, iterations, done
+  // time-accurate BDF2 (allocated when selected)
+  double *qn;     // [nvar][nc] state at time level n (start of the time step)
+  double *qm1;    // [nvar][nc] state at time level n - 1
+  double *rhs;    // [5][nc] BDF2 right-hand side (its norms are the residual norms)
 };
 
 }  // namespace fpsi

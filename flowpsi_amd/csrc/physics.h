@@ -26,6 +26,10 @@ struct Prm {
   int turb;
   double sa_cb1, sa_k, sa_cv1, sa_cb2, sa_cw2, sa_cw3, sa_sigma, Cp, Prt, nut_bc;
   double k_bc, w_bc;
+  // time-accurate BDF2 (timeStepMode unsteady): time step dtmax, pseudo-time
+  // CFL of the preconditioning blend
+  int unsteady;
+  double pseudo_cflmax;
 };
 
 struct V3 {

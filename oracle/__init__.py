@@ -36,7 +36,8 @@ class OgParams(C.Structure):
                 ("PLimPC", C.c_double), ("turb", C.c_int), ("sa_cb1", C.c_double), ("sa_k", C.c_double),
                 ("sa_cv1", C.c_double), ("sa_cb2", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double),
                 ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double), ("k_bc", C.c_double),
-                ("w_bc", C.c_double)]
+                ("w_bc", C.c_double), ("unsteady", C.c_int), ("newton_iter", C.c_int),
+                ("pseudo_cflmax", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -130,6 +131,12 @@ def params(p: dict) -> OgParams:
     kb, wb = sst_bc_defaults(s.u0)
     s.k_bc = p.get("k_bc", kb)
     s.w_bc = p.get("w_bc", wb)
+    # timeStepMode unsteady with the BDF2 scheme (src/solverTimestep.loci),
+    # newtonMaxIter (src/solverNewton.loci:197-199), pseudo_cflmax
+    # (src/solverLinearSystem.loci:380-382)
+    s.unsteady = int(bool(p.get("unsteady", False)))
+    s.newton_iter = int(p.get("newton_iter", 6))
+    s.pseudo_cflmax = float(p.get("pseudo_cflmax", 10.0))
     return s
 
 
@@ -199,38 +206,51 @@ class Oracle:
                            _ptr(vflux), C.byref(nrm))
         return src, iflux, vflux, nrm
 
-    def assemble(self, q):
+    def _levels(self, qn, qm1):
+        if qn is None:
+            return None, None
+        return (np.ascontiguousarray(qn, np.float64), np.ascontiguousarray(qm1, np.float64))
+
+    def assemble(self, q, qn=None, qm1=None):
+        """The fluid system at q; with the time levels qn, qm1 (unsteady
+        parameters) the BDF2 system of a Newton iteration."""
         q = np.ascontiguousarray(q, np.float64)
+        qn, qm1 = self._levels(qn, qm1)
         nc, nf = self.g.nc, self.g.nf
         src = np.zeros((5, nc)); dtau = np.zeros(nc)
         fjp = np.zeros((nf, 25), self.fj); fjm = np.zeros((nf, 25), self.fj)
         D = np.zeros((nc, 25), self.fj); piv = np.zeros((nc, 5), np.int32)
         B = np.zeros((nc, 5), self.fj); Draw = np.zeros((nc, 25), self.fj)
         nrm = OgNorms()
-        self.L.og_assemble2(self.og.ref, C.byref(self.p), _ptr(q), _ptr(src), _ptr(dtau),
-                            fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
+        self.L.og_assemble_t(self.og.ref, C.byref(self.p), _ptr(q),
+                             None if qn is None else _ptr(qn), None if qm1 is None else _ptr(qm1),
+                             _ptr(src), _ptr(dtau), fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
                             D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
                             B.ctypes.data_as(C.c_void_p), C.byref(nrm), Draw.ctypes.data_as(C.c_void_p))
         return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm, Draw=Draw)
 
-    def sa_assemble(self, q):
+    def sa_assemble(self, q, qn=None, qm1=None):
         """nuTilde system: src, fjp, fjm (per face), D, B, tmu."""
         q = np.ascontiguousarray(q, np.float64)
+        qn, qm1 = self._levels(qn, qm1)
         nc, nf = self.g.nc, self.g.nf
         out = {k: np.zeros(n) for k, n in (("src", nc), ("fjp", nf), ("fjm", nf), ("D", nc), ("B", nc),
                                             ("tmu", nc))}
-        self.L.og_sa_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(out["src"]), _ptr(out["fjp"]),
+        self.L.og_sa_assemble_t(self.og.ref, C.byref(self.p), _ptr(q), None if qn is None else _ptr(qn),
+                                None if qm1 is None else _ptr(qm1), _ptr(out["src"]), _ptr(out["fjp"]),
                               _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"]))
         return out
 
-    def sst_assemble(self, q):
+    def sst_assemble(self, q, qn=None, qm1=None):
         """k and omega systems: src, D, B [2][nc]; fjp, fjm [2][nf]; tmu;
         aux F1, F2, gkgw (grad k . grad omega), CD (cross diffusion)."""
         q = np.ascontiguousarray(q, np.float64)
+        qn, qm1 = self._levels(qn, qm1)
         nc, nf = self.g.nc, self.g.nf
         out = {k: np.zeros(n) for k, n in (("src", (2, nc)), ("fjp", (2, nf)), ("fjm", (2, nf)),
                                             ("D", (2, nc)), ("B", (2, nc)), ("tmu", nc), ("aux", (4, nc)))}
-        self.L.og_sst_assemble(self.og.ref, C.byref(self.p), _ptr(q), _ptr(out["src"]), _ptr(out["fjp"]),
+        self.L.og_sst_assemble_t(self.og.ref, C.byref(self.p), _ptr(q), None if qn is None else _ptr(qn),
+                                 None if qm1 is None else _ptr(qm1), _ptr(out["src"]), _ptr(out["fjp"]),
                                _ptr(out["fjm"]), _ptr(out["D"]), _ptr(out["B"]), _ptr(out["tmu"]),
                                _ptr(out["aux"]))
         for i, k in enumerate(("F1", "F2", "gkgw", "CD")):
@@ -300,6 +320,18 @@ class Oracle:
         hist = np.zeros((nsteps, 6))
         self.L.og_newton(self.og.ref, C.byref(self.p), _ptr(q), nsteps, _ptr(hist))
         return q, hist
+
+    def unsteady(self, q, qm1, nsteps):
+        """nsteps BDF2 time steps from level n (q) and n-1 (qm1): returns the
+        new (q, qm1) and hist [nsteps, newton_iter, 6]."""
+        assert self.p.unsteady, "unsteady parameters required"
+        q = np.array(q, np.float64, copy=True, order="C")
+        qm1 = np.array(qm1, np.float64, copy=True, order="C")
+        hist = np.zeros((nsteps, self.p.newton_iter, 6))
+        self.L.og_unsteady.restype = C.c_int
+        rc = self.L.og_unsteady(self.og.ref, C.byref(self.p), _ptr(q), _ptr(qm1), nsteps, _ptr(hist))
+        assert rc == 0
+        return q, qm1, hist
 
 
 # ---- point functions -------------------------------------------------------

@@ -1294,6 +1294,21 @@ void og_assemble(const og_grid *g, const og_params *p, const double *q, double *
 void og_assemble2(const og_grid *g, const og_params *p, const double *q, double *src,
                   double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
                   og_norms *nrm, void *Draw_v) {
+  og_assemble_t(g, p, q, NULL, NULL, src, dtau_out, fjp_v, fjm_v, D_v, pivot, B_v, nrm, Draw_v);
+}
+
+/* density, momentum and total energy per volume of a primitive state */
+static void cons_of(const og_params *p, real T, real pg, const real *u, real *Q) {
+  const real rho = (pg + p->Pambient) / (p->Rt * T);
+  Q[0] = rho;
+  for (int d = 0; d < 3; ++d) Q[1 + d] = rho * u[d];
+  Q[4] = rho * (p->Rt * T / (p->gamma - 1.) + .5 * dot3(u, u));
+}
+
+void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
+                   double *src, double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
+                   og_norms *nrm, void *Draw_v) {
+  const int bdf2 = p->unsteady && qn && qm1;
   real_fj *Draw = Draw_v;
   int nc = g->nc, nf = g->nf;
   real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
@@ -1307,7 +1322,8 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
   og_adj a;
   build_adj(g, &a);
   gather_src(g, p, &a, iflux, vflux, src);
-  if (nrm) residual_norms(g, src, nrm);
+  double *rhs = bdf2 ? malloc(sizeof(double) * 5 * (size_t)nc) : NULL; /* BDF2: norms of rhs */
+  if (nrm && !bdf2) residual_norms(g, src, nrm);
 
   /* face Jacobians */
   const real us_n = 0.0;
@@ -1404,6 +1420,16 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
     real_fj *Dc = D + 25 * (size_t)c;
     for (int i = 0; i < 25; ++i) Dc[i] = -sJ[i];
     real coef = g->vol[c] / dt;
+    real rta = reta, ptm = pterm;
+    if (bdf2) { /* BDF2 fluid_D, src/solverLinearSystem.loci:384-420 */
+      coef = 1.5 * g->vol[c] / dt;
+      const real pseudo_dt = p->pseudo_cflmax / cflpdt;
+      const real w = 1. / (1. + pseudo_dt / dt);
+      rta = 1. / (1. + w * (eta_p[c] - 1.));
+      ptm = (1. - rta) * (gm - 1.) / a2;
+    }
+    {
+    const real reta = rta, pterm = ptm;
     addfj(&M_(Dc, 0, 0), coef * rT);
     addfj(&M_(Dc, 1, 0), coef * rT * U[0]);
     addfj(&M_(Dc, 2, 0), coef * rT * U[1]);
@@ -1420,10 +1446,26 @@ void og_assemble2(const og_grid *g, const og_params *p, const double *q, double 
     addfj(&M_(Dc, 4, 2), coef * rho * U[1]);
     addfj(&M_(Dc, 3, 3), coef * rho);
     addfj(&M_(Dc, 4, 3), coef * rho * U[2]);
+    }
+    if (bdf2) { /* BDF2 rhs, src/solverTimestep.loci:400-428 (w = 1/2) */
+      real un[3] = {qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]};
+      real um[3] = {qm1[nc + c], qm1[2 * nc + c], qm1[3 * nc + c]};
+      real Qp[5], Qn[5], Qm[5];
+      cons_of(p, T, pg, U, Qp);
+      cons_of(p, qn[c], qn[4 * nc + c], un, Qn);
+      cons_of(p, qm1[c], qm1[4 * nc + c], um, Qm);
+      const real rdtvol = g->vol[c] / p->dtmax;
+      const real w = 0.5;
+      const real w1 = (1.0 + w) * rdtvol, w2 = w * rdtvol;
+      for (int k = 0; k < 5; ++k) s[k] = (s[k] - w1 * (Qp[k] - Qn[k]) + w2 * (Qn[k] - Qm[k]));
+      for (int k = 0; k < 5; ++k) rhs[k * (size_t)nc + c] = s[k];
+    }
     for (int k = 0; k < 5; ++k) B[5 * (size_t)c + k] = (real_fj)s[k];
     if (Draw) memcpy(Draw + 25 * (size_t)c, Dc, 25 * sizeof(real_fj));
     lu_decompose(Dc, pivot + 5 * (size_t)c, 5);
   }
+  if (nrm && bdf2) residual_norms(g, rhs, nrm); /* src/solverResidual.loci:113-150 */
+  free(rhs);
   free(bblk);
   free(eta_p);
   free_adj(&a);
@@ -1477,7 +1519,11 @@ void og_sgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v,
 }
 
 /* Steady Newton steps, src/solverNewton.loci:62-78 (damped update). */
-int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, double *hist) {
+/* nsteps Newton iterations at q; with time levels qn, qm1 (BDF2) the
+ * systems carry the time terms and the update limit is the BDF2 one */
+static int newton_run(const og_grid *g, const og_params *p, double *q, const double *qn, const double *qm1,
+                      int nsteps, double *hist) {
+  const int bdf2 = p->unsteady && qn && qm1;
   int nc = g->nc, nf = g->nf;
   double *src = malloc(sizeof(double) * 5 * nc);
   real_fj *fjp = malloc(sizeof(real_fj) * 25 * (size_t)nf), *fjm = malloc(sizeof(real_fj) * 25 * (size_t)nf);
@@ -1496,8 +1542,8 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
     /* the scalar systems from the state at the start of the step, each
      * solved on its own (scalarTransportP(nuTilde | k | w, ...)) */
     if (ns) {
-      if (p->turb == 2) og_sst_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL, NULL);
-      else og_sa_assemble(g, p, q, ssrc, sfp, sfm, sD, sB, NULL);
+      if (p->turb == 2) og_sst_assemble_t(g, p, q, qn, qm1, ssrc, sfp, sfm, sD, sB, NULL, NULL);
+      else og_sa_assemble_t(g, p, q, qn, qm1, ssrc, sfp, sfm, sD, sB, NULL);
       for (int j = 0; j < ns; ++j) {
         const size_t oc = (size_t)j * nc, of = (size_t)j * nf;
         if (p->linear_solver == 1) /* LSGSScalarSolve, src/solverScalarTransport.loci:633-637 */
@@ -1510,7 +1556,7 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       }
     }
     og_norms nrm;
-    og_assemble2(g, p, q, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
+    og_assemble_t(g, p, q, qn, qm1, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
     if (hist) {
       hist[6 * s + 0] = nrm.rtrms; hist[6 * s + 1] = nrm.etrms; hist[6 * s + 2] = nrm.mtrms;
       hist[6 * s + 3] = nrm.rtmax; hist[6 * s + 4] = nrm.etmax; hist[6 * s + 5] = nrm.mtmax;
@@ -1526,8 +1572,15 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
       const real *dq = X + 5 * (size_t)c;
       const real dT = dq[0], dp = dq[4];
       real alpha = 1.0;
-      alpha = minr(alpha, 0.9 * q[c] / (fabs(dT) + 1e-30));
-      alpha = minr(alpha, 0.9 * (q[4 * nc + c] + p->Pambient) / (fabs(dp) + 1e-30));
+      if (bdf2) { /* src/solverNewton.loci:160-180 */
+        const real relax = 0.75;
+        const real pnit = q[4 * nc + c] + p->Pambient;
+        alpha = minr(alpha, relax * pnit / maxr(fabs(dp), 1e-30));
+        alpha = minr(alpha, relax * q[c] / maxr(fabs(dT), 1e-30));
+      } else {
+        alpha = minr(alpha, 0.9 * q[c] / (fabs(dT) + 1e-30));
+        alpha = minr(alpha, 0.9 * (q[4 * nc + c] + p->Pambient) / (fabs(dp) + 1e-30));
+      }
       q[c] = q[c] + alpha * dT;
       q[4 * nc + c] = q[4 * nc + c] + alpha * dp;
       for (int k = 1; k < 4; ++k) q[k * nc + c] = q[k * nc + c] + alpha * dq[k];
@@ -1543,6 +1596,26 @@ int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, doubl
   }
   free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X); free(Draw);
   free(ssrc); free(sfp); free(sfm); free(sD); free(sB); free(sX);
+  return 0;
+}
+
+int og_newton(const og_grid *g, const og_params *p, double *q, int nsteps, double *hist) {
+  return newton_run(g, p, q, NULL, NULL, nsteps, hist);
+}
+
+/* BDF2 dual time stepping: q{n,0} = q{n} (src/solverNewton.loci), newton_iter
+ * iterations, q{n+1} = the last iterate (:182-189), then the levels shift */
+int og_unsteady(const og_grid *g, const og_params *p, double *q, double *qm1, int nsteps, double *hist) {
+  if (!p->unsteady || p->newton_iter < 1) return -1;
+  const int nq = 5 + (p->turb == 2 ? 2 : p->turb ? 1 : 0);
+  const size_t n = (size_t)nq * g->nc;
+  double *qn = malloc(sizeof(double) * n);
+  for (int s = 0; s < nsteps; ++s) {
+    memcpy(qn, q, sizeof(double) * n);
+    newton_run(g, p, q, qn, qm1, p->newton_iter, hist ? hist + 6 * (size_t)s * p->newton_iter : NULL);
+    memcpy(qm1, qn, sizeof(double) * n);
+  }
+  free(qn);
   return 0;
 }
 
@@ -1670,6 +1743,39 @@ void og_sa_point(const og_params *p, double T, double pg, double nut, double vor
 
 void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
                     double *fjm, double *D, double *B, double *tmu) {
+  og_sa_assemble_t(g, p, q, NULL, NULL, src, fjp, fjm, D, B, tmu);
+}
+
+/* scalar time terms (src/solverScalarTransport.loci): steady X_rhs/X_D
+ * (:584-590, :680-690), BDF2 (:602-612, :702-716) with the stabilising
+ * Jacobian -1.5 rhodot (:534-539) */
+static void scalar_time(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
+                        int c, int kv, real dtau, real sv, real sJ, double *Dout, double *Bout) {
+  const int nc = g->nc;
+  const real rho = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
+  if (!(p->unsteady && qn && qm1)) {
+    const real sd = rho * (g->vol[c] / dtau);
+    *Dout = -sJ + sd;
+    const real rhs = 0.0 - sv;
+    *Bout = -rhs;
+    return;
+  }
+  const real rn = (qn[4 * nc + c] + p->Pambient) / (p->Rt * qn[c]);
+  const real rnm1 = (qm1[4 * nc + c] + p->Pambient) / (p->Rt * qm1[c]);
+  const real X = q[kv * nc + c], Xn = qn[kv * nc + c], Xm = qm1[kv * nc + c];
+  const real factor = g->vol[c] / p->dtmax;
+  const real w = 0.5;
+  const real w1 = (1. + w) * factor, w2 = -w * factor;
+  const real rhs = w1 * (X - Xn) * rn + w2 * (Xn * rn - Xm * rnm1) - w2 * X * (rn - rnm1) - sv;
+  *Bout = -rhs;
+  const real fd = g->vol[c] / dtau;
+  const real sd = (1. + w) * fd * rn + w * fd * (rn - rnm1);
+  *Dout = -sJ + sd;
+}
+
+void og_sa_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
+                      double *src, double *fjp, double *fjm, double *D, double *B, double *tmu) {
+  const int bdf2 = p->unsteady && qn && qm1;
   const int nc = g->nc, nf = g->nf;
   double *qf = calloc((size_t)7 * nf, sizeof(double));
   double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
@@ -1785,7 +1891,7 @@ void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, doubl
       rhodot -= iflux[f];
     }
     sv -= X * rhodot;
-    sJ += -rhodot;
+    sJ += bdf2 ? -1.5 * rhodot : -rhodot;
     /* SA sources */
     real ssrc, sdJ;
     sa_cell_terms(g, p, q, grad, c, &ssrc, &sdJ);
@@ -1819,10 +1925,7 @@ void og_sa_assemble(const og_grid *g, const og_params *p, const double *q, doubl
     for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) sJ += sJL[a.up[j]];
     for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += sJU[a.lo[j]];
     src[c] = sv;
-    const real sd = rho * (g->vol[c] / dtau[c]);
-    D[c] = -sJ + sd;
-    const real rhs = 0.0 - sv;
-    B[c] = -rhs;
+    scalar_time(g, p, q, qn, qm1, c, 5, dtau[c], sv, sJ, &D[c], &B[c]);
     if (tmu) tmu[c] = tmu_cell(g, p, q, grad, c);
   }
   /* L = -fjp, U = fjm are stored by the caller's convention: fjp/fjm */
@@ -1977,6 +2080,13 @@ void og_sst_point(const og_params *p, double T, double pg, double k, double w, c
  * lower fjm.  Periodic faces act as interior faces. */
 void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, double *src, double *fjp,
                      double *fjm, double *D, double *B, double *tmu, double *aux) {
+  og_sst_assemble_t(g, p, q, NULL, NULL, src, fjp, fjm, D, B, tmu, aux);
+}
+
+void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn,
+                       const double *qm1, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu,
+                       double *aux) {
+  const int bdf2 = p->unsteady && qn && qm1;
   const int nc = g->nc, nf = g->nf;
   double *qf = calloc((size_t)7 * nf, sizeof(double));
   double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
@@ -2099,7 +2209,6 @@ void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, doub
       }
     }
     for (int c = 0; c < nc; ++c) {
-      const real rho = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
       const real X = q[kv * nc + c];
       real sv = 0.0, sJ = 0.0, rhodot = 0.0;
       for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
@@ -2118,7 +2227,7 @@ void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, doub
         rhodot -= iflux[f];
       }
       sv -= X * rhodot;
-      sJ += -rhodot;
+      sJ += bdf2 ? -1.5 * rhodot : -rhodot;
       sv += sc == 0 ? cs[c].ksrc : cs[c].wsrc;
       sJ += sc == 0 ? cs[c].kJ : cs[c].wJ;
       for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sv -= fdiff[a.lo[j]];
@@ -2129,10 +2238,7 @@ void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, doub
       for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sJ += Fm[a.lo[j]];
       const size_t i = (size_t)sc * nc + c;
       src[i] = sv;
-      const real sd = rho * (g->vol[c] / dtau[c]);
-      D[i] = -sJ + sd;
-      const real rhs = 0.0 - sv;
-      B[i] = -rhs;
+      scalar_time(g, p, q, qn, qm1, c, kv, dtau[c], sv, sJ, &D[i], &B[i]);
     }
   }
   free(gkn); free(wgkn); free(cs); free(fconv); free(fdiff); free(bJ);

@@ -1,0 +1,145 @@
+"""Parity of the time-accurate BDF2 path (time_mode = FPSI_TIME_BDF2, through
+the C ABI) against the oracle: the fluid system of a Newton iteration with
+three time levels, the turbulence scalar systems with their time terms,
+whole time steps of dual time stepping (newton_iter iterations per step)
+with laminar, Spalart-Allmaras and SST physics under the SGS, line-SGS and
+FSGS solvers, and the time-level bookkeeping of the context.
+Tolerances: 1e-12 relative on face and cell quantities, 1e-10 on the state
+after N steps (1e-11 / 1e-10 under a turbulence model: pow, tanh and sqrt
+may round differently on the device)."""
+import numpy as np
+import pytest
+
+import oracle
+from flowpsi_amd import workloads
+from flowpsi_amd.capi import FpsiError, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, float) - np.asarray(b, float)).max() / (np.abs(b).max() + 1e-300))
+
+
+def levels(q, seed, amp=2e-3):
+    rng = np.random.default_rng(seed)
+    qn = q * (1.0 + amp * rng.standard_normal(q.shape))
+    qm1 = q * (1.0 + amp * rng.standard_normal(q.shape))
+    return qn, qm1
+
+
+def _cases():
+    return {
+        "box_roe_lowmach": lambda: workloads.box(6, Minf=0.1, dtmax=2e-5),
+        "channel_visc": lambda: workloads.channel(nx=6, ny=10, nz=4, dtmax=5e-5),
+    }
+
+
+@pytest.fixture(scope="module", params=list(_cases()))
+def case(request):
+    g, q, prm = _cases()[request.param]()
+    prm = dict(prm, unsteady=True, newton_iter=3)
+    print(f"\n[bdf2 case {request.param}] nc={g.nc} nf={g.nf}", flush=True)
+    return request.param, g, q, prm
+
+
+def test_fluid_system(case):
+    name, g, q, prm = case
+    qn, qm1 = levels(q[:5], 1)
+    ref = oracle.Oracle(g, prm).assemble(q, qn, qm1)
+    s = Solver(g, prm)
+    s.set_state(q)
+    s.set_time_levels(qn, qm1)
+    s.assemble()
+    sy = s.get_system()
+    for key in ("fjp", "fjm"):
+        assert rel(sy[key], ref[key]) < 1e-12, key
+    errD = float((np.abs(sy["D"] - ref["D"]) / np.abs(ref["D"]).max(axis=1, keepdims=True)).max())
+    eB = float((np.abs(sy["B"] - ref["B"]) / np.abs(ref["B"]).max(axis=0, keepdims=True)).max())
+    print(f"LU(D) err {errD:.2e} B err {eB:.2e}")
+    assert errD < 1e-11 and eB < 1e-12
+    assert np.array_equal(sy["piv"], ref["piv"])
+    assert rel(sy["dtau"], ref["dtau"]) < 1e-12
+    gq, gm = s.get_time_levels()
+    assert np.array_equal(gq, qn) and np.array_equal(gm, qm1)
+
+
+def test_time_steps(case):
+    name, g, q, prm = case
+    o = oracle.Oracle(g, prm)
+    s = Solver(g, prm)
+    s.set_state(q)
+    hist = s.unsteady(3)
+    q_r, qm1_r, h_r = o.unsteady(q, q, 3)
+    q_g = s.get_state()
+    _, qm1_g = s.get_time_levels()
+    err = rel(q_g, q_r)
+    tot = np.array([[h["rtrms"] + h["etrms"] + h["mtrms"] for h in st] for st in hist])
+    tot_r = h_r[..., 0] + h_r[..., 1] + h_r[..., 2]
+    print(f"state err after 3 steps x 3 iterations {err:.2e}; inner drops "
+          f"{np.sqrt(tot[:, -1] / tot[:, 0])}")
+    assert err < 1e-10
+    assert rel(qm1_g, qm1_r) < 1e-10
+    assert np.allclose(tot, tot_r, rtol=1e-9)
+    # continue from the context's levels, and from levels set by the caller
+    q_r2, qm1_r2, _ = o.unsteady(q_r, qm1_r, 2)
+    s.unsteady(2)
+    assert rel(s.get_state(), q_r2) < 1e-10
+    s.set_state(q)
+    s.set_time_levels(None, qm1_r)
+    q_r3, _, _ = o.unsteady(q, qm1_r, 2)
+    s.unsteady(2)
+    assert rel(s.get_state(), q_r3) < 1e-10
+
+
+@pytest.mark.parametrize("turb,solver,sweep", [("sa", "sgs", "persistent"), ("sa", "lsgs", "per_level"),
+                                               ("sst", "sgs", "persistent"), ("sst", "fsgs", "per_level"),
+                                               ("sst", "sgs", "per_level")])
+def test_turbulent_time_steps(turb, solver, sweep):
+    fn = workloads.channel_sa if turb == "sa" else workloads.channel_sst
+    g, q, prm = fn(nx=5, ny=8, nz=4)
+    prm = dict(prm, unsteady=True, newton_iter=2, dtmax=5e-5, solver=solver, sweep=sweep)
+    print(f"\n[bdf2 {turb} {solver} {sweep}] nc={g.nc}", flush=True)
+    o = oracle.Oracle(g, prm)
+    qn, qm1 = levels(q, 5)
+    # the scalar systems of one iteration with three distinct levels
+    ref = o.sa_assemble(q, qn, qm1) if turb == "sa" else o.sst_assemble(q, qn, qm1)
+    s = Solver(g, prm)
+    s.set_wall_distance(g.meta["dist"])
+    s.set_state(q)
+    s.set_time_levels(qn, qm1)
+    s.assemble()
+    for j in range(1 if turb == "sa" else 2):
+        sy = s.turb_get_system(j)
+        r = ref if turb == "sa" else oracle.Oracle.scalar(ref, j)
+        e = [rel(sy[k], r[k]) for k in ("src", "D", "B")]
+        print(f"scalar {j} src/D/B errors {e}")
+        assert max(e) < 1e-11
+    # whole time steps
+    q_r, qm1_r, h_r = o.unsteady(q, qm1, 2)
+    s.set_state(q)
+    s.set_time_levels(None, qm1)
+    hist = s.unsteady(2)
+    err = rel(s.get_state(), q_r)
+    tot = np.array([[h["rtrms"] + h["etrms"] + h["mtrms"] for h in st] for st in hist])
+    tot_r = h_r[..., 0] + h_r[..., 1] + h_r[..., 2]
+    print(f"state err after 2 steps x 2 iterations {err:.2e}")
+    assert err < 1e-10
+    assert np.allclose(tot, tot_r, rtol=1e-9)
+
+
+def test_mode_checks():
+    g, q, prm = workloads.box(4)
+    s = Solver(g, dict(prm, unsteady=True, dtmax=1e-4))
+    s.set_state(q)
+    with pytest.raises(FpsiError):
+        s.newton(1)
+    st = Solver(g, prm)
+    st.set_state(q)
+    with pytest.raises(FpsiError):
+        st.unsteady(1)
+    with pytest.raises(FpsiError):
+        Solver(g, dict(prm, unsteady=True, dtmax=0.0))
+    # before any step the levels are the state itself (constant start)
+    qn, qm1 = s.get_time_levels()
+    assert np.array_equal(qn, q) and np.array_equal(qm1, q)
