@@ -86,6 +86,11 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--sweep", default="persistent", choices=["per_level", "tiled", "persistent"])
+    ap.add_argument("--time-scheme", default="steady", choices=["steady", "bdf2", "kec"],
+                    help="bdf2/kec: time-accurate dual time stepping; a step is one time step of "
+                         "--newton-iter Newton iterations and the value counts Newton iterations")
+    ap.add_argument("--newton-iter", type=int, default=3)
+    ap.add_argument("--dt", type=float, default=2e-6, help="physical time step of bdf2/kec (s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--calib-copy", action="store_true",
@@ -118,27 +123,33 @@ def main():
         g, q, prm = workloads.box(n)
         wname = f"Euler box {n}^3 ({g.nc} cells), Roe, SGS x{prm['sgs_iter']}"
     prm = dict(prm, precision=a.precision, sweep=a.sweep)
+    unsteady = a.time_scheme != "steady"
+    iters = a.newton_iter if unsteady else 1
+    if unsteady:
+        prm = dict(prm, unsteady=True, time_scheme=a.time_scheme, newton_iter=a.newton_iter, dtmax=a.dt)
+        wname += f", {a.time_scheme.upper()} dt={a.dt:g}s x{a.newton_iter} Newton iterations"
     log(f"grid {g.name}: nc={g.nc} nf={g.nf} built in {time.time() - t0:.1f}s")
     s = Solver(g, prm)
     info = s.info()
     log(f"context: {info}")
     s.set_state(q)
+    step = (lambda n: [h for st in s.unsteady(n) for h in st]) if unsteady else s.newton
     t1 = time.time()
-    s.newton(a.warmup)
+    step(a.warmup)
     s.synchronize()
     log(f"warmup {a.warmup} steps {time.time() - t1:.2f}s")
 
     s.profile(True)
     s.synchronize()
     t0 = time.perf_counter()
-    hist = s.newton(a.steps)
+    hist = step(a.steps)
     s.synchronize()
     t1 = time.perf_counter()
     prof = s.profile_read()
     s.profile(False)
     dt = t1 - t0
     ms_per_step = 1e3 * dt / a.steps
-    value = g.nc * a.steps / dt
+    value = g.nc * a.steps * iters / dt
     log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step; groups " +
         ", ".join(f"{k}={v['ms'] / max(v['launches'], 1):.3f}ms x{v['launches']}" for k, v in prof.items()))
 
@@ -171,7 +182,7 @@ def main():
     sweeps = prm["sgs_iter"] * 2
     step_bytes = (prof["residual"]["bytes"] + prof["face_jac"]["bytes"] + prof["assemble"]["bytes"] +
                   sweeps * prof["sgs_sweep"]["bytes"] + prof["update"]["bytes"])
-    step_gbs = step_bytes * a.steps / dt / 1e9
+    step_gbs = step_bytes * a.steps * iters / dt / 1e9
 
     if a.calib_copy:
         calib_copy(1 << 30)
@@ -179,11 +190,11 @@ def main():
 
     tot = [h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist]
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and not unsteady:
         cpu = cpu_baseline(g, q, prm, a.cpu_sample_steps)
 
     conv = None
-    if not a.no_converge:
+    if not a.no_converge and not unsteady:
         from flowpsi_amd import converge
         gb, qb, pb = workloads.box(32)
         log("time to 1e-8 drop on the 32^3 Euler box: fp64 fixed / fp32 fixed / fp32 tol / fp32+NN")
@@ -208,7 +219,8 @@ def main():
         "config": {"workload": wname, "cells": g.nc, "faces": g.nf, "levels": info["nlevels"],
                    "precision": a.precision, "sweep": "tiled" if tiled else a.sweep,
                    "tiles": info["sweep_tiles"], "sgs_sweeps": prm["sgs_iter"],
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "time_scheme": a.time_scheme,
+                   "newton_iterations_per_step": iters},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "bytes_per_launch": bytes_launch},

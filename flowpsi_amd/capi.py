@@ -26,6 +26,7 @@ PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0, "lsgs": 1, "fsgs": 2}
 SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2}
 TURB = {"none": 0, "sa": 1, "sst": 2}
+TIME_SCHEME = {"bdf2": 1, "kec": 2}
 NSCALAR = {"none": 0, "sa": 1, "sst": 2}  # turbulence scalars after the 5 fluid variables
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
@@ -39,7 +40,7 @@ EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "f
            "fpsi_mlp_eval", "fpsi_mlp_get_sample", "fpsi_lsgs_stats", "fpsi_set_wall_distance",
            "fpsi_sa_get_system", "fpsi_sa_solve", "fpsi_sa_lsgs_stats", "fpsi_turb_get_system",
            "fpsi_turb_solve", "fpsi_turb_lsgs_stats", "fpsi_set_time_levels", "fpsi_get_time_levels",
-           "fpsi_unsteady_step"]
+           "fpsi_unsteady_step", "fpsi_finish_time_step"]
 KGROUPS = ["residual", "face_jac", "assemble", "sgs_sweep", "update"]
 
 
@@ -60,7 +61,8 @@ class Params(C.Structure):
                 ("lsgs_abstol", C.c_double), ("lsgs_relax", C.c_double), ("Minf", C.c_double),
                 ("etaT", C.c_double), ("PLimPC", C.c_double), ("turb", C.c_int), ("Prt", C.c_double),
                 ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double),
-                ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double)]
+                ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double),
+                ("pbias_control", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -157,11 +159,13 @@ def make_params(prm: dict) -> Params:
     s.nut_bc = prm.get("nut_bc", 5e-9)
     s.k_bc = prm.get("k_bc", 0.0)  # <= 0: the library's default from |u0|
     s.w_bc = prm.get("w_bc", 0.0)
-    # time-accurate BDF2: unsteady=True, time step dtmax, newton_iter
-    # iterations per step (<= 0: 6), pseudo_cflmax (<= 0: 10)
-    s.time_mode = 1 if prm.get("unsteady", False) else 0
+    # time-accurate: unsteady=True, time_scheme "bdf2" (default) or "kec",
+    # time step dtmax, newton_iter iterations per step (<= 0: 6),
+    # pseudo_cflmax (<= 0: 10), pbias_control (KEC's timePressureBiasControl)
+    s.time_mode = TIME_SCHEME[prm.get("time_scheme", "bdf2")] if prm.get("unsteady", False) else 0
     s.newton_iter = int(prm.get("newton_iter", 6))
     s.pseudo_cflmax = float(prm.get("pseudo_cflmax", 10.0))
+    s.pbias_control = float(prm.get("pbias_control", 0.0))
     return s
 
 
@@ -341,13 +345,16 @@ class Solver:
 
     # -- time-accurate BDF2
     def unsteady(self, nsteps):
-        """nsteps BDF2 time steps; the norms of every Newton iteration as a
-        list of nsteps lists of newton_iter dicts."""
+        """nsteps time steps (BDF2 or KEC); the norms of every Newton
+        iteration as a list of nsteps lists of newton_iter dicts."""
         ni = int(self.prm.get("newton_iter", 6))
         ni = ni if ni > 0 else 6
         hist = (Norms * max(nsteps * ni, 1))()
         self._chk(self.L.fpsi_unsteady_step(self.h, int(nsteps), hist), "fpsi_unsteady_step")
         return [[hist[s * ni + i].as_dict() for i in range(ni)] for s in range(nsteps)]
+
+    def finish_time_step(self):
+        self._chk(self.L.fpsi_finish_time_step(self.h), "fpsi_finish_time_step")
 
     def set_time_levels(self, qn=None, qnm1=None):
         arrs = []

@@ -93,7 +93,7 @@ struct fpsi_ctx {
   // time-accurate BDF2: buffers allocated on first selection; have_prev when
   // DevState::qm1 holds q{n-1}, in_step when DevState::qn holds the q{n} of
   // a time step in progress
-  bool bdf2() const { return prm.time_mode == FPSI_TIME_BDF2; }
+  bool bdf2() const { return prm.time_mode == FPSI_TIME_BDF2 || prm.time_mode == FPSI_TIME_KEC; }
   bool bdf2_alloc = false, have_prev = false, in_step = false;
   int newton_iter = 6;
 };
@@ -759,16 +759,17 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   if (p->sweep_mode == FPSI_SWEEP_TILED && !c->have_tiles)
     return fail(c, FPSI_ERR_ARG, "no tiled sweep on this grid: " + c->tile_msg);
   if (!(p->gamma > 1.0) || !(p->Rtilde > 0.0)) return fail(c, FPSI_ERR_ARG, "bad gas constants");
-  if (p->time_mode != FPSI_TIME_STEADY && p->time_mode != FPSI_TIME_BDF2)
+  if (p->time_mode != FPSI_TIME_STEADY && p->time_mode != FPSI_TIME_BDF2 && p->time_mode != FPSI_TIME_KEC)
     return fail(c, FPSI_ERR_ARG, "unknown time mode");
-  if (p->time_mode == FPSI_TIME_BDF2) {
+  if (p->time_mode != FPSI_TIME_STEADY) {
     if (!(p->dtmax > 0.0) || !std::isfinite(p->dtmax))
-      return fail(c, FPSI_ERR_ARG, "BDF2 needs a finite time step dtmax > 0");
+      return fail(c, FPSI_ERR_ARG, "a time-accurate scheme needs a finite time step dtmax > 0");
     if (!c->bdf2_alloc) {
       const size_t n = 7 * (size_t)c->nc;  // room for every turbulence model's rows
       if (int rc = dalloc(c, &c->s.qn, n)) return rc;
       if (int rc = dalloc(c, &c->s.qm1, n)) return rc;
       if (int rc = dalloc(c, &c->s.rhs, 5 * (size_t)c->nc)) return rc;
+      if (int rc = dalloc(c, &c->s.pbias, (size_t)c->nc)) return rc;
       c->bdf2_alloc = true;
     }
   }
@@ -806,8 +807,9 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     k.k_bc = p->k_bc > 0 ? p->k_bc : kd;
     k.w_bc = p->w_bc > 0 ? p->w_bc : 0.09 * kd / (1e-3 * 1e-5);
   }
-  k.unsteady = p->time_mode == FPSI_TIME_BDF2 ? 1 : 0;
+  k.unsteady = p->time_mode == FPSI_TIME_KEC ? 2 : p->time_mode == FPSI_TIME_BDF2 ? 1 : 0;
   k.pseudo_cflmax = p->pseudo_cflmax > 0 ? p->pseudo_cflmax : 10.0;
+  k.pbias_ctl = p->pbias_control;
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -1364,13 +1366,20 @@ static int enqueue_iteration(fpsi_ctx *c, double *dh) {
   return 0;
 }
 
+// end of a time step: q{n+1} = the last iterate or (KEC) its reconstruction,
+// then q{n-1} = q{n}
+static void enqueue_finish_step(fpsi_ctx *c) {
+  if (c->kp.unsteady == 2) launch_kec_final(c->g, c->kp, c->s, c->stream);
+  launch_copy_d((long)c->nvar() * c->nc, c->s.qn, c->s.qm1, c->stream);
+  c->in_step = false;
+}
+
 // n Newton iterations (per time step when levels: then q{n,0} = q{n} and the
 // levels shift after the step's `levels` iterations); hist[n] nullable
 static int run_iterations(fpsi_ctx *c, int n, int levels, fpsi_norms *hist) {
   std::vector<double> h(12 * (size_t)std::max(n, 1));
   double *dh = nullptr;
   if (n > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
-  const long nv = (long)c->nvar() * c->nc;
   for (int s = 0; s < n; ++s) {
     if (levels && s % levels == 0) {  // q{n,0} = q{n}; q{n-1} = q unless set
       c->in_step = false;
@@ -1381,10 +1390,7 @@ static int run_iterations(fpsi_ctx *c, int n, int levels, fpsi_norms *hist) {
       hipFree(dh);
       return rc;
     }
-    if (levels && (s + 1) % levels == 0) {  // q{n+1} = the last iterate, then q{n-1} = q{n}
-      launch_copy_d(nv, c->s.qn, c->s.qm1, c->stream);
-      c->in_step = false;
-    }
+    if (levels && (s + 1) % levels == 0) enqueue_finish_step(c);
   }
   HIPCHK(c, hipGetLastError());
   if (n > 0) {
@@ -1405,7 +1411,7 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
   if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
-  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time_mode is BDF2: use fpsi_unsteady_step");
+  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
   return run_iterations(c, nsteps, 0, hist);
 }
 
@@ -1413,13 +1419,24 @@ extern "C" int fpsi_unsteady_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
   if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
-  if (!c->bdf2()) return fail(c, FPSI_ERR_STATE, "fpsi_unsteady_step needs time_mode = FPSI_TIME_BDF2");
+  if (!c->bdf2()) return fail(c, FPSI_ERR_STATE, "fpsi_unsteady_step needs a time-accurate time_mode");
   return run_iterations(c, nsteps * c->newton_iter, c->newton_iter, hist);
 }
 
 static int check_levels(fpsi_ctx *c) {
-  if (!c->have_params || !c->bdf2()) return fail(c, FPSI_ERR_STATE, "time levels need time_mode = FPSI_TIME_BDF2");
+  if (!c->have_params || !c->bdf2()) return fail(c, FPSI_ERR_STATE, "time levels need a time-accurate time_mode");
   if (!c->have_state) return fail(c, FPSI_ERR_STATE, "fpsi_set_state not called");
+  return 0;
+}
+
+extern "C" int fpsi_finish_time_step(fpsi_ctx *c) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_levels(c)) return rc;
+  if (!c->in_step) return fail(c, FPSI_ERR_STATE, "no time step in progress (fpsi_assemble starts one)");
+  enqueue_finish_step(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;
   return 0;
 }
 
@@ -1620,7 +1637,7 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
   if (int rc = need_sgs(c)) return rc;
-  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time_mode is BDF2: use fpsi_unsteady_step");
+  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
     double o[12];

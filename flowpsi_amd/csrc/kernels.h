@@ -25,6 +25,8 @@ hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunk
                             int nwaves, int ns, long nnz, DevState &s, hipStream_t st);
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
+// KEC: the time step's level n+1 from its last Newton iterate
+void launch_kec_final(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
                    hipStream_t st);
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st);

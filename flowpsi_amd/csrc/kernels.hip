@@ -761,7 +761,36 @@ __device__ __forceinline__ void cons5(const Prm &p, double T, double pg, const V
   Q[4] = rho * (p.Rt * T / (p.gamma - 1.) + .5 * dot(u, u));
 }
 
-template <typename F, bool LS>
+// KEC level n+1 from the iterate q* (src/solverTimestep.loci:366-385,
+// src/solverNewton.loci:127-157): harmonic-mean temperature, biased pressure,
+// density-weighted velocity; out = {T, u, v, w, pg, rho}
+__device__ __forceinline__ void kec_np1(const Prm &p, double Ts, double pgs, const V3 &us, double Tn, double pgn,
+                                        const V3 &un, double peps, double *out) {
+  const double Tnp1 = Tn * Ts / (2. * Tn - Ts);
+  const double pgnp1 = (2. * pgs - (1. - peps) * pgn) / (1 + peps);
+  const double rhon = (pgn + p.Pamb) / (p.Rt * Tn);
+  const double rhonp = dmax((pgnp1 + p.Pamb) / (p.Rt * Tnp1), 1e-15);
+  const double srhonp = sqrt(rhonp), srhon = sqrt(rhon);
+  const double w1 = (srhonp + srhon) / srhonp, w2 = srhon / srhonp;
+  out[0] = Tnp1;
+  out[1] = w1 * us.x - w2 * un.x;
+  out[2] = w1 * us.y - w2 * un.y;
+  out[3] = w1 * us.z - w2 * un.z;
+  out[4] = pgnp1;
+  out[5] = rhonp;
+}
+
+// KEC scalar level n+1 (and X_np1) from the iterate,
+// src/solverScalarTransport.loci:367-395
+__device__ __forceinline__ double kec_xnp1(double X, double Xn) {
+  const double Xstar = dmax(X, 1e-15);
+  double dXstar = dmax((Xstar - Xn) / Xstar, -1.95);
+  if (dXstar < 0.0) dXstar += 0.25 * dXstar * dXstar;
+  return Xstar * (1. + dXstar);
+}
+
+// TS: time scheme, 0 steady, 1 BDF2, 2 KEC
+template <typename F, bool LS, int TS>
 __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const double *__restrict__ src, const double *__restrict__ maxev,
                                 const F *__restrict__ off, const F *__restrict__ pblk,
@@ -769,7 +798,8 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 F *__restrict__ lu, int *__restrict__ piv, F *__restrict__ Bout,
                                 double *__restrict__ X, double *__restrict__ lu64,
                                 const double *__restrict__ etap, const double *__restrict__ qn,
-                                const double *__restrict__ qm1, double *__restrict__ rhs) {
+                                const double *__restrict__ qm1, double *__restrict__ rhs,
+                                double *__restrict__ pbias) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -845,13 +875,77 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   for (int i = 0; i < 25; ++i) D[i] = -sJ[i];
   double coef = vol / dt;
   double reta = reta0, pterm = pterm0;
-  if (p.unsteady) {  // BDF2 fluid_D, src/solverLinearSystem.loci:384-420
+  if constexpr (TS == 1) {  // BDF2 fluid_D, src/solverLinearSystem.loci:384-420
     coef = 1.5 * vol / dt;
     const double pseudo_dt = p.pseudo_cflmax / cflpdt;
     const double w = 1. / (1. + pseudo_dt / dt);
     reta = 1. / (1. + w * ((p.pre ? etap[c] : 1.0) - 1.));
     pterm = (1. - reta) * (gm - 1.) / a2;
   }
+  if constexpr (TS == 2) {
+    // KEC: localPBias (src/solverTimestep.loci:346-351), fluid_D = -srcJ +
+    // (V/dtau) dQ/dq dq/dq* (src/solverLinearSystem.loci:266-378), rhs
+    // (src/solverTimestep.loci:366-396)
+    const double cfl = cflpdt * p.dtmax;
+    const double factor = dmin(1.0, dmax(1e-40, 0.1 * (cfl - 1.)));
+    const double pb = dmax(p.pbias_ctl, pow(factor, 0.7));
+    pbias[c] = pb;
+    double dQ[5][5], dqs[5][5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) dQ[i][j] = dqs[i][j] = 0.0;
+    dQ[0][0] = rT; dQ[1][0] = rT * U.x; dQ[2][0] = rT * U.y; dQ[3][0] = rT * U.z; dQ[4][0] = fT;
+    dQ[0][4] = rP; dQ[1][4] = rP * U.x; dQ[2][4] = rP * U.y; dQ[3][4] = rP * U.z; dQ[4][4] = fP;
+    dQ[1][1] = rho; dQ[4][1] = rho * U.x;
+    dQ[2][2] = rho; dQ[4][2] = rho * U.y;
+    dQ[3][3] = rho; dQ[4][3] = rho * U.z;
+    const double Tn = qn[c], pgn = qn[4 * nc + c];
+    const V3 un = {qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]};
+    double np1[6];
+    kec_np1(p, T, pg, U, Tn, pgn, un, pb, np1);
+    const double rhon = (pgn + p.Pamb) / (Rt * Tn);
+    const double Pcoef = 2.0 / (1 + pb);
+    const double srhonp1 = sqrt(np1[5]), srhon = sqrt(rhon);
+    const double ucoef = (srhonp1 + srhon) / srhonp1;
+    const double tfactor = 1. / (2 * Tn - T);
+    const double dTdTs = 2. * Tn * Tn * tfactor * tfactor;
+    const double drdt = -dTdTs * rho / T;
+    const double drdp = Pcoef * rho / (pg + p.Pamb);
+    dqs[0][0] = dTdTs;
+    dqs[1][1] = ucoef; dqs[2][2] = ucoef; dqs[3][3] = ucoef;
+    dqs[4][4] = Pcoef;
+    const double ucoef2 = 0.5 * srhon / (srhonp1 * srhonp1 * srhonp1);
+    const double du[3] = {un.x - U.x, un.y - U.y, un.z - U.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      dqs[1 + d][0] += ucoef2 * drdt * du[d];
+      dqs[1 + d][4] += ucoef2 * drdp * du[d];
+    }
+    const double kcoef = vol / dt;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) dQ[i][j] *= kcoef;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc(D[i + 5 * j], dQ[i][k] * dqs[k][j]);
+    const double e0n = p.Rt * Tn / (p.gamma - 1.) + .5 * dot(un, un);
+    const V3 unp1 = {np1[1], np1[2], np1[3]};
+    const double e0np = p.Rt * np1[0] / (p.gamma - 1.) + .5 * dot(unp1, unp1);
+    const double rdtvol = vol / p.dtmax;
+    const double rhonp = np1[5];
+    s[0] = s[0] - (rhonp - rhon) * rdtvol;
+    s[1] = s[1] - (rhonp * np1[1] - rhon * un.x) * rdtvol;
+    s[2] = s[2] - (rhonp * np1[2] - rhon * un.y) * rdtvol;
+    s[3] = s[3] - (rhonp * np1[3] - rhon * un.z) * rdtvol;
+    s[4] = s[4] - (rhonp * e0np - rhon * e0n) * rdtvol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rhs[(size_t)k * nc + c] = s[k];
+  } else {
   acc(D[0 + 5 * 0], coef * rT);
   acc(D[1 + 5 * 0], coef * rT * U.x);
   acc(D[2 + 5 * 0], coef * rT * U.y);
@@ -868,7 +962,8 @@ __global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
   acc(D[4 + 5 * 2], coef * rho * U.y);
   acc(D[3 + 5 * 3], coef * rho);
   acc(D[4 + 5 * 3], coef * rho * U.z);
-  if (p.unsteady) {  // BDF2 rhs, src/solverTimestep.loci:400-428
+  }
+  if constexpr (TS == 1) {  // BDF2 rhs, src/solverTimestep.loci:400-428
     double Qp[5], Qn[5], Qm[5];
     cons5(p, T, pg, U, Qp);
     cons5(p, qn[c], qn[4 * nc + c], V3{qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]}, Qn);
@@ -1688,6 +1783,17 @@ __global__ void k_sa_face(DevGrid g, Prm p, const double *__restrict__ q, const 
 __device__ __forceinline__ void scalar_time(const Prm &p, double vol, double dtau, double rho, double X,
                                             const double *__restrict__ qn, const double *__restrict__ qm1, int nc,
                                             int c, int kv, double sv, double sJ, double &D, double &b) {
+  if (p.unsteady == 2) {  // KEC, :592-600 and :691-700
+    const double rn = (qn[4 * nc + c] + p.Pamb) / (p.Rt * qn[c]);
+    const double factor = vol / p.dtmax;
+    const double Xn = qn[(size_t)kv * nc + c];
+    const double Xnp1 = kec_xnp1(X, Xn);
+    const double rhs = factor * ((Xnp1 - Xn) * rn) - sv;
+    b = -rhs;
+    const double sd = (2. * rn) * (vol / dtau);
+    D = -sJ + sd;
+    return;
+  }
   if (!p.unsteady) {
     const double sd = rho * (vol / dtau);
     D = -sJ + sd;
@@ -1732,8 +1838,9 @@ __global__ void k_sa_cell(DevGrid g, Prm p, const double *__restrict__ q, const 
       rhodot -= iflux[f];
     }
   }
-  sv -= X * rhodot;
-  sJ += p.unsteady ? -1.5 * rhodot : -rhodot;  // :526-546
+  // stabilisation, src/solverScalarTransport.loci:508-546
+  sv -= (p.unsteady == 2 ? kec_xnp1(X, qn[5 * nc + c]) : X) * rhodot;
+  sJ += p.unsteady == 2 ? -2. * rhodot : p.unsteady ? -1.5 * rhodot : -rhodot;
 #define GU(i, jj) grad[(3 * (1 + (i)) + (jj)) * nc + c]
   const double wx = GU(2, 1) - GU(1, 2), wy = GU(0, 2) - GU(2, 0), wz = GU(1, 0) - GU(0, 1);
 #undef GU
@@ -1986,8 +2093,8 @@ __global__ void k_sst_cell_b(DevGrid g, Prm p, const double *__restrict__ q, con
         rhodot -= iflux[~e];
       }
     }
-    sv -= X * rhodot;
-    sJ += p.unsteady ? -1.5 * rhodot : -rhodot;
+    sv -= (p.unsteady == 2 ? kec_xnp1(X, qn[(5 + sc) * nc + c]) : X) * rhodot;
+    sJ += p.unsteady == 2 ? -2. * rhodot : p.unsteady ? -1.5 * rhodot : -rhodot;
     sv += scell[(4 + sc) * (size_t)nc + c];
     sJ += scell[(6 + sc) * (size_t)nc + c];
     for (int j = j0; j < j1; ++j) {
@@ -2075,9 +2182,12 @@ __global__ void k_sa_update(DevGrid g, Prm p, int ns, const double *__restrict__
   }
 }
 
-// damped Newton update, src/solverNewton.loci:62-78 (steady) and :160-180
-// (BDF2: at most 75 % of the current pressure and temperature)
-__global__ void k_update(DevGrid g, double Pamb, int bdf2, const double *__restrict__ X, double *__restrict__ q) {
+// damped Newton update, src/solverNewton.loci:62-78 (steady), :160-180
+// (BDF2: at most 75 % of the current pressure and temperature) and :93-125
+// (KEC: limits from the level n+1 reconstruction)
+__global__ void k_update(DevGrid g, Prm p, const double *__restrict__ X, double *__restrict__ q,
+                         const double *__restrict__ qn, const double *__restrict__ pbias) {
+  const double Pamb = p.Pamb;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int nc = g.nc;
@@ -2085,11 +2195,20 @@ __global__ void k_update(DevGrid g, double Pamb, int bdf2, const double *__restr
   const double dT = dq[0], dp = dq[4];
   double alpha = 1.0;
   const double T = q[c], pg = q[4 * nc + c];
-  if (bdf2) {
+  if (p.unsteady == 1) {
     const double relax = 0.75;
     const double pnit = pg + Pamb;
     alpha = dmin(alpha, relax * pnit / dmax(fabs(dp), 1e-30));
     alpha = dmin(alpha, relax * T / dmax(fabs(dT), 1e-30));
+  } else if (p.unsteady == 2) {
+    const double relax = 0.8;
+    const double Tn = qn[c];
+    const double dTlim = (relax * (2 * Tn - T)) / 2.0;
+    alpha = dmin(alpha, dTlim / dmax(fabs(dT), 1e-30));
+    const double peps = pbias[c];
+    const double pgnp1 = (2. * pg - (1. - peps) * qn[4 * nc + c]) / (1 + peps);
+    const double dplim = (1 + peps) * relax * (pgnp1 + Pamb) / 2.0;
+    alpha = dmin(alpha, dplim / dmax(fabs(dp), 1e-30));
   } else {
     alpha = dmin(alpha, 0.9 * T / (fabs(dT) + 1e-30));
     alpha = dmin(alpha, 0.9 * (pg + Pamb) / (fabs(dp) + 1e-30));
@@ -2142,12 +2261,18 @@ void launch_face_jac(const DevGrid &g, const Prm &p, int fp32, DevState &s, hipS
                        s.phi, s.vflux, (const double *)s.eta, (const double *)s.tmu, (double *)s.off,
                        (double *)s.pblk, (double *)s.bblk);
 }
-template <typename F, bool LS>
-static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  hipLaunchKernelGGL((k_cell_assemble<F, LS>), dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src, s.maxev,
+template <typename F, bool LS, int TS>
+static void cell_assemble_ts(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  hipLaunchKernelGGL((k_cell_assemble<F, LS, TS>), dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.q, s.src, s.maxev,
                      (const F *)s.off, (const F *)s.pblk, (const F *)s.bblk, s.dtau, (F *)s.lu, s.piv,
                      (F *)s.B, s.X, s.lu64, (const double *)s.eta, (const double *)s.qn, (const double *)s.qm1,
-                     s.rhs);
+                     s.rhs, s.pbias);
+}
+template <typename F, bool LS>
+static void cell_assemble_t(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  if (p.unsteady == 2) cell_assemble_ts<F, LS, 2>(g, p, s, st);
+  else if (p.unsteady == 1) cell_assemble_ts<F, LS, 1>(g, p, s, st);
+  else cell_assemble_ts<F, LS, 0>(g, p, s, st);
 }
 void launch_cell_assemble(const DevGrid &g, const Prm &p, int fp32, int lsgs, DevState &s, hipStream_t st) {
   if (fp32) {
@@ -2197,7 +2322,30 @@ void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);
 }
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p.Pamb, p.unsteady, s.X, s.q);
+  hipLaunchKernelGGL(k_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, s.X, s.q, (const double *)s.qn,
+                     (const double *)s.pbias);
+}
+
+// KEC: q{n+1} from the last iterate with its localPBias
+// (src/solverNewton.loci:127-157), scalars (src/solverScalarTransport.loci:367-377)
+__global__ void k_kec_final(DevGrid g, Prm p, int ns, const double *__restrict__ qn,
+                            const double *__restrict__ pbias, double *__restrict__ q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int nc = g.nc;
+  const V3 us = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+  const V3 un = {qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]};
+  double np1[6];
+  kec_np1(p, q[c], q[4 * nc + c], us, qn[c], qn[4 * nc + c], un, pbias[c], np1);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[(size_t)k * nc + c] = np1[k];
+  for (int j = 0; j < ns; ++j)
+    q[(size_t)(5 + j) * nc + c] = kec_xnp1(q[(size_t)(5 + j) * nc + c], qn[(size_t)(5 + j) * nc + c]);
+}
+void launch_kec_final(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
+  const int ns = p.turb == 2 ? 2 : p.turb ? 1 : 0;
+  hipLaunchKernelGGL(k_kec_final, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, ns, (const double *)s.qn,
+                     (const double *)s.pbias, s.q);
 }
 
 

@@ -89,7 +89,8 @@ struct DevState {
   // time-accurate BDF2 (allocated when selected)
   double *qn;     // [nvar][nc] state at time level n (start of the time step)
   double *qm1;    // [nvar][nc] state at time level n - 1
-  double *rhs;    // [5][nc] BDF2 right-hand side (its norms are the residual norms)
+  double *rhs;    // [5][nc] BDF2 / KEC right-hand side (its norms are the residual norms)
+  double *pbias;  // [nc] KEC localPBias of the current iteration
 };
 
 }  // namespace fpsi

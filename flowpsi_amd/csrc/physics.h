@@ -28,8 +28,9 @@ struct Prm {
   double k_bc, w_bc;
   // time-accurate BDF2 (timeStepMode unsteady): time step dtmax, pseudo-time
   // CFL of the preconditioning blend
-  int unsteady;
+  int unsteady;           // 0 steady, 1 BDF2, 2 KEC
   double pseudo_cflmax;
+  double pbias_ctl;       // timePressureBiasControl (KEC)
 };
 
 struct V3 {

@@ -51,8 +51,8 @@ enum { FPSI_SOLVER_SGS = 0, FPSI_SOLVER_LSGS = 1, FPSI_SOLVER_FSGS = 2 };
  * (turbulence/KOmega/sst.loci with its default options; the state carries
  * k and omega as rows 6 and 7) */
 enum { FPSI_TURB_NONE = 0, FPSI_TURB_SA = 1, FPSI_TURB_SST = 2 };
-/* timeStepMode steady, or unsteady with timeStepScheme BDF2 */
-enum { FPSI_TIME_STEADY = 0, FPSI_TIME_BDF2 = 1 };
+/* timeStepMode steady, or unsteady with timeStepScheme BDF2 or KEC */
+enum { FPSI_TIME_STEADY = 0, FPSI_TIME_BDF2 = 1, FPSI_TIME_KEC = 2 };
 /* How a Gauss-Seidel half-sweep runs (same result bit for bit):
  * PER_LEVEL  one launch per dependency level (default);
  * PERSISTENT one cooperative launch per half-sweep run as a dataflow: each
@@ -117,14 +117,16 @@ typedef struct {
   double nut_bc;        /* nuTilde of inflow faces (<= 0: 5e-9, the BC default) */
   double k_bc, w_bc;    /* SST: k and omega of inflow faces (<= 0: the BC defaults from |u0|,
                            turbulence/KOmega/sst_viscous.loci:63-118) */
-  /* timeStepMode (src/solverTimestep.loci:112-140): FPSI_TIME_STEADY, or
-   * FPSI_TIME_BDF2 (unsteady, timeStepScheme BDF2) with the time step dtmax,
-   * newtonMaxIter Newton iterations per time step (src/solverNewton.loci:197-199,
-   * <= 0: 6) and pseudo_cflmax of the preconditioning blend
-   * (src/solverLinearSystem.loci:380-382, <= 0: 10) */
+  /* timeStepMode (src/solverTimestep.loci:85-140): FPSI_TIME_STEADY, or
+   * unsteady with timeStepScheme BDF2 (FPSI_TIME_BDF2) or KEC (FPSI_TIME_KEC)
+   * with the time step dtmax, newtonMaxIter Newton iterations per time step
+   * (src/solverNewton.loci:197-199, <= 0: 6), pseudo_cflmax of BDF2's
+   * preconditioning blend (src/solverLinearSystem.loci:380-382, <= 0: 10)
+   * and KEC's timePressureBiasControl (src/solverTimestep.loci:340-344) */
   int time_mode;
   int newton_iter;
   double pseudo_cflmax;
+  double pbias_control;
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -247,14 +249,18 @@ int fpsi_update(fpsi_ctx *ctx);
  * residual evaluated at the start of each step. */
 int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
 
-/* Time-accurate BDF2 (FPSI_TIME_BDF2).  The context holds the state q at time
- * level n and the level before it, q{n-1}; fpsi_set_state makes q{n-1} = q
- * (the reference's start "constant in time", src/solverTimestep.loci:156-170).
- * Within a time step, the systems fpsi_assemble builds are those of a Newton
- * iteration with time levels q{n} and q{n-1} (rhs src/solverTimestep.loci:400-428,
- * fluid_D src/solverLinearSystem.loci:384-434, scalar rows
- * src/solverScalarTransport.loci:534-539, 602-616, 702-716), and their norms
- * are those of the BDF2 right-hand side (src/solverResidual.loci:113-150).
+/* Time-accurate steps (FPSI_TIME_BDF2, FPSI_TIME_KEC).  The context holds the
+ * state q at time level n and the level before it, q{n-1} (read by BDF2
+ * only); fpsi_set_state makes q{n-1} = q (the reference's start "constant in
+ * time", src/solverTimestep.loci:156-170).  Within a time step, the systems
+ * fpsi_assemble builds are those of a Newton iteration with time levels q{n}
+ * and q{n-1}: BDF2 rhs src/solverTimestep.loci:400-428, fluid_D
+ * src/solverLinearSystem.loci:384-434, scalar rows
+ * src/solverScalarTransport.loci:534-539, 602-616, 702-716; KEC (the iterate
+ * is the mid-step state q*) localPBias and rhs src/solverTimestep.loci:346-396,
+ * fluid_D src/solverLinearSystem.loci:226-378, scalar rows
+ * src/solverScalarTransport.loci:388-395, 514-539, 592-600, 691-700.  The
+ * norms are those of the right-hand side (src/solverResidual.loci:113-150).
  *
  * fpsi_set_time_levels sets q{n} and q{n-1} (a restart's icfile levels,
  * src/solverIC.loci:562-570, 612-660); qn NULL takes the current state.
@@ -262,13 +268,18 @@ int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
 int fpsi_set_time_levels(fpsi_ctx *ctx, const double *qn, const double *qnm1);
 int fpsi_get_time_levels(fpsi_ctx *ctx, double *qn, double *qnm1);
 
-/* nsteps BDF2 time steps, device resident: q{n,0} = q{n}; newton_iter
+/* nsteps time steps, device resident: q{n,0} = q{n}; newton_iter
  * iterations of residual + time terms, assembly, linear solve (as
- * fpsi_newton_step), update (fluid and turbulence scalars); q{n+1} = the last
- * iterate (src/solverNewton.loci:182-189); then q{n-1} = q{n}.
- * hist[nsteps * newton_iter] (nullable): the norms of each iteration's
- * right-hand side. */
+ * fpsi_newton_step), update (fluid and turbulence scalars, src/solverNewton.loci:93-125
+ * and :160-180); then fpsi_finish_time_step.  hist[nsteps * newton_iter]
+ * (nullable): the norms of each iteration's right-hand side. */
 int fpsi_unsteady_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
+
+/* Ends a time step driven by fpsi_assemble / fpsi_linear_solve /
+ * fpsi_update: q{n+1} = the last iterate (BDF2, src/solverNewton.loci:182-189)
+ * or its KEC reconstruction (:127-157, scalars
+ * src/solverScalarTransport.loci:367-377), then q{n-1} = q{n}. */
+int fpsi_finish_time_step(fpsi_ctx *ctx);
 
 /* Kernel groups timed with HIP events on the context's stream while
  * profiling is on, and their algorithmic bytes per launch (compulsory

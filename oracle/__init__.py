@@ -37,7 +37,7 @@ class OgParams(C.Structure):
                 ("sa_cv1", C.c_double), ("sa_cb2", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double),
                 ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double), ("k_bc", C.c_double),
                 ("w_bc", C.c_double), ("unsteady", C.c_int), ("newton_iter", C.c_int),
-                ("pseudo_cflmax", C.c_double)]
+                ("pseudo_cflmax", C.c_double), ("pbias_control", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -134,9 +134,11 @@ def params(p: dict) -> OgParams:
     # timeStepMode unsteady with the BDF2 scheme (src/solverTimestep.loci),
     # newtonMaxIter (src/solverNewton.loci:197-199), pseudo_cflmax
     # (src/solverLinearSystem.loci:380-382)
-    s.unsteady = int(bool(p.get("unsteady", False)))
+    # timeStepScheme BDF2 (1) or KEC (2)
+    s.unsteady = {"bdf2": 1, "kec": 2}[p.get("time_scheme", "bdf2")] if p.get("unsteady", False) else 0
     s.newton_iter = int(p.get("newton_iter", 6))
     s.pseudo_cflmax = float(p.get("pseudo_cflmax", 10.0))
+    s.pbias_control = float(p.get("pbias_control", 0.0))  # timePressureBiasControl
     return s
 
 
@@ -209,11 +211,13 @@ class Oracle:
     def _levels(self, qn, qm1):
         if qn is None:
             return None, None
-        return (np.ascontiguousarray(qn, np.float64), np.ascontiguousarray(qm1, np.float64))
+        return (np.ascontiguousarray(qn, np.float64),
+                None if qm1 is None else np.ascontiguousarray(qm1, np.float64))
 
     def assemble(self, q, qn=None, qm1=None):
         """The fluid system at q; with the time levels qn, qm1 (unsteady
-        parameters) the BDF2 system of a Newton iteration."""
+        parameters) the BDF2 or KEC system of a Newton iteration (KEC reads
+        only qn; pbias holds its localPBias)."""
         q = np.ascontiguousarray(q, np.float64)
         qn, qm1 = self._levels(qn, qm1)
         nc, nf = self.g.nc, self.g.nf
@@ -221,13 +225,15 @@ class Oracle:
         fjp = np.zeros((nf, 25), self.fj); fjm = np.zeros((nf, 25), self.fj)
         D = np.zeros((nc, 25), self.fj); piv = np.zeros((nc, 5), np.int32)
         B = np.zeros((nc, 5), self.fj); Draw = np.zeros((nc, 25), self.fj)
+        pbias = np.zeros(nc)
         nrm = OgNorms()
         self.L.og_assemble_t(self.og.ref, C.byref(self.p), _ptr(q),
                              None if qn is None else _ptr(qn), None if qm1 is None else _ptr(qm1),
                              _ptr(src), _ptr(dtau), fjp.ctypes.data_as(C.c_void_p), fjm.ctypes.data_as(C.c_void_p),
-                            D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
-                            B.ctypes.data_as(C.c_void_p), C.byref(nrm), Draw.ctypes.data_as(C.c_void_p))
-        return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm, Draw=Draw)
+                             D.ctypes.data_as(C.c_void_p), _ptr(piv, C.c_int),
+                             B.ctypes.data_as(C.c_void_p), C.byref(nrm), Draw.ctypes.data_as(C.c_void_p),
+                             _ptr(pbias))
+        return dict(src=src, dtau=dtau, fjp=fjp, fjm=fjm, D=D, piv=piv, B=B, norms=nrm, Draw=Draw, pbias=pbias)
 
     def sa_assemble(self, q, qn=None, qm1=None):
         """nuTilde system: src, fjp, fjm (per face), D, B, tmu."""

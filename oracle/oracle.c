@@ -1294,7 +1294,40 @@ void og_assemble(const og_grid *g, const og_params *p, const double *q, double *
 void og_assemble2(const og_grid *g, const og_params *p, const double *q, double *src,
                   double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
                   og_norms *nrm, void *Draw_v) {
-  og_assemble_t(g, p, q, NULL, NULL, src, dtau_out, fjp_v, fjm_v, D_v, pivot, B_v, nrm, Draw_v);
+  og_assemble_t(g, p, q, NULL, NULL, src, dtau_out, fjp_v, fjm_v, D_v, pivot, B_v, nrm, Draw_v, NULL);
+}
+
+/* KEC's level n+1 from the iterate q* (src/solverTimestep.loci:366-385,
+ * src/solverNewton.loci:127-157): harmonic-mean temperature, biased
+ * pressure, density-weighted velocity; out = {T, u, v, w, pg, rho} */
+static void kec_np1(const og_params *p, real Ts, real pgs, const real *us, real Tn, real pgn, const real *un,
+                    real peps, real *out) {
+  const real Tnp1 = Tn * Ts / (2. * Tn - Ts);
+  const real pgnp1 = (2. * pgs - (1. - peps) * pgn) / (1 + peps);
+  const real rhon = (pgn + p->Pambient) / (p->Rt * Tn);
+  const real rhonp = maxr((pgnp1 + p->Pambient) / (p->Rt * Tnp1), 1e-15);
+  const real srhonp = sqrt(rhonp), srhon = sqrt(rhon);
+  const real w1 = (srhonp + srhon) / srhonp, w2 = srhon / srhonp;
+  out[0] = Tnp1;
+  for (int d = 0; d < 3; ++d) out[1 + d] = w1 * us[d] - w2 * un[d];
+  out[4] = pgnp1;
+  out[5] = rhonp;
+}
+
+/* localPBias, src/solverTimestep.loci:346-351 */
+static real kec_pbias(const og_params *p, real cflpdt) {
+  const real cfl = cflpdt * p->dtmax;
+  const real factor = minr(1.0, maxr(1e-40, 0.1 * (cfl - 1.)));
+  return maxr(p->pbias_control, pow(factor, 0.7));
+}
+
+/* KEC scalar level n+1 (and X_np1) from the iterate,
+ * src/solverScalarTransport.loci:367-395 */
+static real kec_xnp1(real X, real Xn) {
+  const real Xstar = maxr(X, 1e-15);
+  real dXstar = maxr((Xstar - Xn) / Xstar, -1.95);
+  if (dXstar < 0.0) dXstar += 0.25 * dXstar * dXstar;
+  return Xstar * (1. + dXstar);
 }
 
 /* density, momentum and total energy per volume of a primitive state */
@@ -1307,8 +1340,8 @@ static void cons_of(const og_params *p, real T, real pg, const real *u, real *Q)
 
 void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
                    double *src, double *dtau_out, void *fjp_v, void *fjm_v, void *D_v, int *pivot, void *B_v,
-                   og_norms *nrm, void *Draw_v) {
-  const int bdf2 = p->unsteady && qn && qm1;
+                   og_norms *nrm, void *Draw_v, double *pbias_out) {
+  const int bdf2 = p->unsteady == 1 && qn && qm1, kec = p->unsteady == 2 && qn;
   real_fj *Draw = Draw_v;
   int nc = g->nc, nf = g->nf;
   real_fj *fjp = fjp_v, *fjm = fjm_v, *D = D_v, *B = B_v;
@@ -1322,8 +1355,8 @@ void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const 
   og_adj a;
   build_adj(g, &a);
   gather_src(g, p, &a, iflux, vflux, src);
-  double *rhs = bdf2 ? malloc(sizeof(double) * 5 * (size_t)nc) : NULL; /* BDF2: norms of rhs */
-  if (nrm && !bdf2) residual_norms(g, src, nrm);
+  double *rhs = bdf2 || kec ? malloc(sizeof(double) * 5 * (size_t)nc) : NULL; /* norms of the rhs */
+  if (nrm && !rhs) residual_norms(g, src, nrm);
 
   /* face Jacobians */
   const real us_n = 0.0;
@@ -1428,7 +1461,53 @@ void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const 
       rta = 1. / (1. + w * (eta_p[c] - 1.));
       ptm = (1. - rta) * (gm - 1.) / a2;
     }
-    {
+    const real pb = kec ? kec_pbias(p, cflpdt) : 0.0;
+    if (kec && pbias_out) pbias_out[c] = pb;
+    if (kec) { /* KEC fluid_D: (V/dtau) dQ/dq dq/dq*, src/solverLinearSystem.loci:266-378 */
+      real dQ[5][5], dqs[5][5];
+      memset(dQ, 0, sizeof(dQ));
+      memset(dqs, 0, sizeof(dqs));
+      dQ[0][0] = rT; dQ[1][0] = rT * U[0]; dQ[2][0] = rT * U[1]; dQ[3][0] = rT * U[2]; dQ[4][0] = fT;
+      dQ[0][4] = rP; dQ[1][4] = rP * U[0]; dQ[2][4] = rP * U[1]; dQ[3][4] = rP * U[2]; dQ[4][4] = fP;
+      dQ[1][1] = rho; dQ[4][1] = rho * U[0];
+      dQ[2][2] = rho; dQ[4][2] = rho * U[1];
+      dQ[3][3] = rho; dQ[4][3] = rho * U[2];
+      const real Tn = qn[c], pgn = qn[4 * nc + c];
+      const real un[3] = {qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]};
+      real np1[6];
+      kec_np1(p, T, pg, U, Tn, pgn, un, pb, np1);
+      const real rhon = (pgn + Pamb) / (Rt * Tn);
+      const real Pcoef = 2.0 / (1 + pb);
+      const real srhonp1 = sqrt(np1[5]), srhon = sqrt(rhon);
+      const real ucoef = (srhonp1 + srhon) / srhonp1;
+      const real tfactor = 1. / (2 * Tn - T);
+      const real dTdTs = 2. * Tn * Tn * tfactor * tfactor;
+      const real drdt = -dTdTs * rho / T;
+      const real drdp = Pcoef * rho / (pg + Pamb);
+      dqs[0][0] = dTdTs;
+      dqs[1][1] = ucoef; dqs[2][2] = ucoef; dqs[3][3] = ucoef;
+      dqs[4][4] = Pcoef;
+      const real ucoef2 = 0.5 * srhon / (srhonp1 * srhonp1 * srhonp1);
+      for (int d = 0; d < 3; ++d) {
+        dqs[1 + d][0] += ucoef2 * drdt * (un[d] - U[d]);
+        dqs[1 + d][4] += ucoef2 * drdp * (un[d] - U[d]);
+      }
+      const real kcoef = g->vol[c] / dt;
+      for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) dQ[i][j] *= kcoef;
+      for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j)
+          for (int k = 0; k < 5; ++k) addfj(&M_(Dc, i, j), dQ[i][k] * dqs[k][j]);
+      /* KEC rhs, src/solverTimestep.loci:366-396 */
+      const real e0n = p->Rt * Tn / (p->gamma - 1.) + .5 * dot3(un, un);
+      const real e0np = p->Rt * np1[0] / (p->gamma - 1.) + .5 * dot3(np1 + 1, np1 + 1);
+      const real rdtvol = g->vol[c] / p->dtmax;
+      const real rhonp = np1[5];
+      s[0] = s[0] - (rhonp - rhon) * rdtvol;
+      for (int d = 0; d < 3; ++d) s[1 + d] = s[1 + d] - (rhonp * np1[1 + d] - rhon * un[d]) * rdtvol;
+      s[4] = s[4] - (rhonp * e0np - rhon * e0n) * rdtvol;
+      for (int k = 0; k < 5; ++k) rhs[k * (size_t)nc + c] = s[k];
+    } else {
     const real reta = rta, pterm = ptm;
     addfj(&M_(Dc, 0, 0), coef * rT);
     addfj(&M_(Dc, 1, 0), coef * rT * U[0]);
@@ -1464,7 +1543,7 @@ void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const 
     if (Draw) memcpy(Draw + 25 * (size_t)c, Dc, 25 * sizeof(real_fj));
     lu_decompose(Dc, pivot + 5 * (size_t)c, 5);
   }
-  if (nrm && bdf2) residual_norms(g, rhs, nrm); /* src/solverResidual.loci:113-150 */
+  if (nrm && rhs) residual_norms(g, rhs, nrm); /* src/solverResidual.loci:113-150 */
   free(rhs);
   free(bblk);
   free(eta_p);
@@ -1523,7 +1602,8 @@ void og_sgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v,
  * systems carry the time terms and the update limit is the BDF2 one */
 static int newton_run(const og_grid *g, const og_params *p, double *q, const double *qn, const double *qm1,
                       int nsteps, double *hist) {
-  const int bdf2 = p->unsteady && qn && qm1;
+  const int bdf2 = p->unsteady == 1 && qn && qm1, kec = p->unsteady == 2 && qn;
+  double *pbias = kec ? malloc(sizeof(double) * g->nc) : NULL;
   int nc = g->nc, nf = g->nf;
   double *src = malloc(sizeof(double) * 5 * nc);
   real_fj *fjp = malloc(sizeof(real_fj) * 25 * (size_t)nf), *fjm = malloc(sizeof(real_fj) * 25 * (size_t)nf);
@@ -1556,7 +1636,7 @@ static int newton_run(const og_grid *g, const og_params *p, double *q, const dou
       }
     }
     og_norms nrm;
-    og_assemble_t(g, p, q, qn, qm1, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw);
+    og_assemble_t(g, p, q, qn, qm1, src, NULL, fjp, fjm, D, piv, B, &nrm, Draw, pbias);
     if (hist) {
       hist[6 * s + 0] = nrm.rtrms; hist[6 * s + 1] = nrm.etrms; hist[6 * s + 2] = nrm.mtrms;
       hist[6 * s + 3] = nrm.rtmax; hist[6 * s + 4] = nrm.etmax; hist[6 * s + 5] = nrm.mtmax;
@@ -1577,6 +1657,15 @@ static int newton_run(const og_grid *g, const og_params *p, double *q, const dou
         const real pnit = q[4 * nc + c] + p->Pambient;
         alpha = minr(alpha, relax * pnit / maxr(fabs(dp), 1e-30));
         alpha = minr(alpha, relax * q[c] / maxr(fabs(dT), 1e-30));
+      } else if (kec) { /* :93-125 */
+        const real relax = 0.8;
+        const real Tstar = q[c], Tn = qn[c];
+        const real dTlim = (relax * (2 * Tn - Tstar)) / 2.0;
+        alpha = minr(alpha, dTlim / maxr(fabs(dT), 1e-30));
+        const real pgstar = q[4 * nc + c], peps = pbias[c];
+        const real pgnp1 = (2. * pgstar - (1. - peps) * qn[4 * nc + c]) / (1 + peps);
+        const real dplim = (1 + peps) * relax * (pgnp1 + p->Pambient) / 2.0;
+        alpha = minr(alpha, dplim / maxr(fabs(dp), 1e-30));
       } else {
         alpha = minr(alpha, 0.9 * q[c] / (fabs(dT) + 1e-30));
         alpha = minr(alpha, 0.9 * (q[4 * nc + c] + p->Pambient) / (fabs(dp) + 1e-30));
@@ -1594,6 +1683,19 @@ static int newton_run(const og_grid *g, const og_params *p, double *q, const dou
         q[(5 + j) * nc + c] = maxr(Xn, 1e-15);
       }
   }
+  if (kec && nsteps > 0) { /* q{n+1} from the last iterate (src/solverNewton.loci:127-157 with
+                            * localPBias_next = the last iteration's localPBias; scalars
+                            * src/solverScalarTransport.loci:367-377) */
+    for (int c = 0; c < nc; ++c) {
+      const real us[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+      const real un[3] = {qn[nc + c], qn[2 * nc + c], qn[3 * nc + c]};
+      real np1[6];
+      kec_np1(p, q[c], q[4 * nc + c], us, qn[c], qn[4 * nc + c], un, pbias[c], np1);
+      for (int k = 0; k < 5; ++k) q[k * nc + c] = np1[k];
+      for (int j = 0; j < ns; ++j) q[(5 + j) * nc + c] = kec_xnp1(q[(5 + j) * nc + c], qn[(5 + j) * nc + c]);
+    }
+  }
+  free(pbias);
   free(src); free(fjp); free(fjm); free(D); free(B); free(piv); free(X); free(Draw);
   free(ssrc); free(sfp); free(sfm); free(sD); free(sB); free(sX);
   return 0;
@@ -1753,7 +1855,17 @@ static void scalar_time(const og_grid *g, const og_params *p, const double *q, c
                         int c, int kv, real dtau, real sv, real sJ, double *Dout, double *Bout) {
   const int nc = g->nc;
   const real rho = (q[4 * nc + c] + p->Pambient) / (p->Rt * q[c]);
-  if (!(p->unsteady && qn && qm1)) {
+  if (p->unsteady == 2 && qn) { /* KEC, :592-600 and :691-700 */
+    const real rn = (qn[4 * nc + c] + p->Pambient) / (p->Rt * qn[c]);
+    const real factor = g->vol[c] / p->dtmax;
+    const real Xnp1 = kec_xnp1(q[kv * nc + c], qn[kv * nc + c]);
+    const real rhs = factor * ((Xnp1 - qn[kv * nc + c]) * rn) - sv;
+    *Bout = -rhs;
+    const real sd = (2. * rn) * (g->vol[c] / dtau);
+    *Dout = -sJ + sd;
+    return;
+  }
+  if (!(p->unsteady == 1 && qn && qm1)) {
     const real sd = rho * (g->vol[c] / dtau);
     *Dout = -sJ + sd;
     const real rhs = 0.0 - sv;
@@ -1775,7 +1887,7 @@ static void scalar_time(const og_grid *g, const og_params *p, const double *q, c
 
 void og_sa_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
                       double *src, double *fjp, double *fjm, double *D, double *B, double *tmu) {
-  const int bdf2 = p->unsteady && qn && qm1;
+  const int bdf2 = p->unsteady == 1 && qn && qm1, kec = p->unsteady == 2 && qn;
   const int nc = g->nc, nf = g->nf;
   double *qf = calloc((size_t)7 * nf, sizeof(double));
   double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
@@ -1890,8 +2002,9 @@ void og_sa_assemble_t(const og_grid *g, const og_params *p, const double *q, con
       sv -= fconv[f];
       rhodot -= iflux[f];
     }
-    sv -= X * rhodot;
-    sJ += bdf2 ? -1.5 * rhodot : -rhodot;
+    /* stabilisation (src/solverScalarTransport.loci:508-546) */
+    sv -= (kec ? kec_xnp1(X, qn[5 * nc + c]) : X) * rhodot;
+    sJ += kec ? -2. * rhodot : bdf2 ? -1.5 * rhodot : -rhodot;
     /* SA sources */
     real ssrc, sdJ;
     sa_cell_terms(g, p, q, grad, c, &ssrc, &sdJ);
@@ -2086,7 +2199,7 @@ void og_sst_assemble(const og_grid *g, const og_params *p, const double *q, doub
 void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn,
                        const double *qm1, double *src, double *fjp, double *fjm, double *D, double *B, double *tmu,
                        double *aux) {
-  const int bdf2 = p->unsteady && qn && qm1;
+  const int bdf2 = p->unsteady == 1 && qn && qm1, kec = p->unsteady == 2 && qn;
   const int nc = g->nc, nf = g->nf;
   double *qf = calloc((size_t)7 * nf, sizeof(double));
   double *grad = malloc(sizeof(double) * 21 * nc), *phi = malloc(sizeof(double) * 7 * nc);
@@ -2226,8 +2339,8 @@ void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, co
         sv -= fconv[f];
         rhodot -= iflux[f];
       }
-      sv -= X * rhodot;
-      sJ += bdf2 ? -1.5 * rhodot : -rhodot;
+      sv -= (kec ? kec_xnp1(X, qn[kv * nc + c]) : X) * rhodot;
+      sJ += kec ? -2. * rhodot : bdf2 ? -1.5 * rhodot : -rhodot;
       sv += sc == 0 ? cs[c].ksrc : cs[c].wsrc;
       sJ += sc == 0 ? cs[c].kJ : cs[c].wJ;
       for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) sv -= fdiff[a.lo[j]];

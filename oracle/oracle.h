@@ -65,9 +65,10 @@ typedef struct {
   double Prt;             /* turbulent Prandtl number */
   double nut_bc;          /* nuTilde of inflow faces */
   double k_bc, w_bc;      /* SST: k and omega of inflow faces */
-  int unsteady;           /* 0 steady, 1 time-accurate BDF2 (time step dtmax) */
+  int unsteady;           /* 0 steady; time-accurate with time step dtmax: 1 BDF2, 2 KEC */
   int newton_iter;        /* newtonMaxIter: Newton iterations per time step */
   double pseudo_cflmax;   /* pseudo-time CFL of the BDF2 preconditioning blend */
+  double pbias_control;   /* timePressureBiasControl (KEC) */
 } og_params;
 
 typedef struct {
@@ -164,17 +165,21 @@ void og_sa_fsgs(const og_grid *g, int nsweeps, const double *fjp, const double *
 /* block SGS (Jacobi start + sweeps in cell-index order) on an assembled system */
 void og_sgs(const og_grid *g, int nsweeps, const void *fjp, const void *fjm, const void *D,
             const int *pivot, const void *B, double *X);
-/* Time-accurate BDF2 steps (src/solverTimestep.loci:400-428 rhs,
+/* Time-accurate steps.  BDF2 (src/solverTimestep.loci:400-428 rhs,
  * src/solverLinearSystem.loci:384-434 fluid_D, src/solverNewton.loci:160-198
- * update, src/solverScalarTransport.loci:602-716 scalars): q holds the state
- * at level n on entry and n + nsteps on exit, qm1 the level before it (both
- * [nq][nc], updated); newton_iter Newton iterations per step;
+ * update, src/solverScalarTransport.loci:602-716 scalars) or KEC
+ * (src/solverTimestep.loci:341-398, src/solverLinearSystem.loci:226-378,
+ * src/solverNewton.loci:93-157, src/solverScalarTransport.loci:367-395,
+ * 514-539, 592-600, 691-700): q holds the state at level n on entry and
+ * n + nsteps on exit, qm1 the level before it (both [nq][nc], updated; KEC
+ * does not read it); newton_iter Newton iterations per step;
  * hist[6*(step*newton_iter + it) ..] as og_newton. */
 int  og_unsteady(const og_grid *g, const og_params *p, double *q, double *qm1, int nsteps, double *hist);
-/* the BDF2 systems of one Newton iteration at q with time levels qn, qm1 */
+/* the time-accurate systems of one Newton iteration at q with time levels
+ * qn, qm1 (KEC: qm1 unused); pbias (nullable) receives KEC's localPBias */
 void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
                    double *src, double *dtau, void *fjp, void *fjm, void *D, int *pivot, void *B,
-                   og_norms *nrm, void *Draw);
+                   og_norms *nrm, void *Draw, double *pbias);
 void og_sa_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn, const double *qm1,
                       double *src, double *fjp, double *fjm, double *D, double *B, double *tmu);
 void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, const double *qn,

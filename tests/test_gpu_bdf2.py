@@ -1,9 +1,10 @@
-"""Parity of the time-accurate BDF2 path (time_mode = FPSI_TIME_BDF2, through
-the C ABI) against the oracle: the fluid system of a Newton iteration with
-three time levels, the turbulence scalar systems with their time terms,
-whole time steps of dual time stepping (newton_iter iterations per step)
-with laminar, Spalart-Allmaras and SST physics under the SGS, line-SGS and
-FSGS solvers, and the time-level bookkeeping of the context.
+"""Parity of the time-accurate paths (time_mode = FPSI_TIME_BDF2 and
+FPSI_TIME_KEC, through the C ABI) against the oracle: the fluid system of a
+Newton iteration with distinct time levels, the turbulence scalar systems
+with their time terms, whole time steps of dual time stepping (newton_iter
+iterations per step, KEC's level n+1 reconstruction) with laminar,
+Spalart-Allmaras and SST physics under the SGS, line-SGS and FSGS solvers,
+a time step driven call by call, and the time-level bookkeeping.
 Tolerances: 1e-12 relative on face and cell quantities, 1e-10 on the state
 after N steps (1e-11 / 1e-10 under a turbulence model: pow, tanh and sqrt
 may round differently on the device)."""
@@ -43,8 +44,10 @@ def case(request):
     return request.param, g, q, prm
 
 
-def test_fluid_system(case):
+@pytest.mark.parametrize("scheme", ["bdf2", "kec"])
+def test_fluid_system(case, scheme):
     name, g, q, prm = case
+    prm = dict(prm, time_scheme=scheme)
     qn, qm1 = levels(q[:5], 1)
     ref = oracle.Oracle(g, prm).assemble(q, qn, qm1)
     s = Solver(g, prm)
@@ -64,8 +67,10 @@ def test_fluid_system(case):
     assert np.array_equal(gq, qn) and np.array_equal(gm, qm1)
 
 
-def test_time_steps(case):
+@pytest.mark.parametrize("scheme", ["bdf2", "kec"])
+def test_time_steps(case, scheme):
     name, g, q, prm = case
+    prm = dict(prm, time_scheme=scheme)
     o = oracle.Oracle(g, prm)
     s = Solver(g, prm)
     s.set_state(q)
@@ -92,14 +97,15 @@ def test_time_steps(case):
     assert rel(s.get_state(), q_r3) < 1e-10
 
 
-@pytest.mark.parametrize("turb,solver,sweep", [("sa", "sgs", "persistent"), ("sa", "lsgs", "per_level"),
-                                               ("sst", "sgs", "persistent"), ("sst", "fsgs", "per_level"),
-                                               ("sst", "sgs", "per_level")])
-def test_turbulent_time_steps(turb, solver, sweep):
+@pytest.mark.parametrize("scheme,turb,solver,sweep", [
+    ("bdf2", "sa", "sgs", "persistent"), ("bdf2", "sa", "lsgs", "per_level"), ("bdf2", "sst", "sgs", "persistent"),
+    ("bdf2", "sst", "fsgs", "per_level"), ("bdf2", "sst", "sgs", "per_level"), ("kec", "sa", "sgs", "persistent"),
+    ("kec", "sst", "lsgs", "per_level"), ("kec", "sst", "sgs", "persistent")])
+def test_turbulent_time_steps(scheme, turb, solver, sweep):
     fn = workloads.channel_sa if turb == "sa" else workloads.channel_sst
     g, q, prm = fn(nx=5, ny=8, nz=4)
-    prm = dict(prm, unsteady=True, newton_iter=2, dtmax=5e-5, solver=solver, sweep=sweep)
-    print(f"\n[bdf2 {turb} {solver} {sweep}] nc={g.nc}", flush=True)
+    prm = dict(prm, unsteady=True, time_scheme=scheme, newton_iter=2, dtmax=5e-5, solver=solver, sweep=sweep)
+    print(f"\n[{scheme} {turb} {solver} {sweep}] nc={g.nc}", flush=True)
     o = oracle.Oracle(g, prm)
     qn, qm1 = levels(q, 5)
     # the scalar systems of one iteration with three distinct levels
@@ -126,6 +132,28 @@ def test_turbulent_time_steps(turb, solver, sweep):
     print(f"state err after 2 steps x 2 iterations {err:.2e}")
     assert err < 1e-10
     assert np.allclose(tot, tot_r, rtol=1e-9)
+
+
+@pytest.mark.parametrize("scheme", ["bdf2", "kec"])
+def test_step_driven_call_by_call(scheme):
+    g, q, prm = workloads.channel(nx=5, ny=8, nz=4, dtmax=5e-5)
+    prm = dict(prm, unsteady=True, time_scheme=scheme, newton_iter=3)
+    o = oracle.Oracle(g, prm)
+    _, qm1 = levels(q, 11)
+    q_r, qm1_r, _ = o.unsteady(q, qm1, 1)
+    s = Solver(g, prm)
+    s.set_state(q)
+    s.set_time_levels(None, qm1)
+    for _ in range(3):
+        s.assemble()
+        s.linear_solve(prm["sgs_iter"])
+        s.update()
+    s.finish_time_step()
+    assert rel(s.get_state(), q_r) < 1e-10
+    qn, qm = s.get_time_levels()
+    assert np.array_equal(qn, s.get_state()) and rel(qm, qm1_r) < 1e-10
+    with pytest.raises(FpsiError):
+        s.finish_time_step()  # no step in progress
 
 
 def test_mode_checks():

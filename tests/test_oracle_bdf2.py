@@ -65,10 +65,10 @@ def blocks(D):
     return np.asarray(D, float).reshape(-1, 5, 5).transpose(0, 2, 1)
 
 
-def levels(q, seed):
+def levels(q, seed, amp=2e-3):
     rng = np.random.default_rng(seed)
-    qn = q * (1.0 + 2e-3 * rng.standard_normal(q.shape))
-    qm1 = q * (1.0 + 2e-3 * rng.standard_normal(q.shape))
+    qn = q * (1.0 + amp * rng.standard_normal(q.shape))
+    qm1 = q * (1.0 + amp * rng.standard_normal(q.shape))
     return qn, qm1
 
 
