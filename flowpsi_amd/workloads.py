@@ -147,3 +147,47 @@ def c5_bl(nx=16, ny=60, nz=16, aspect=1e3, seed=5, **over):
                solver="lsgs")
     prm.update(over)
     return g, q, prm
+
+
+# ---- the reference's own grids (quickTest/*.vog, kept under tests/golden/vog) ----
+
+def vog_path(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden", "vog",
+                        name + ".vog")
+
+
+def vog_case(name, seed=6, **over):
+    """A Newton-step workload on one of the reference's VOG grids, with the
+    boundary types this path implements ('reflecting' walls of the
+    reference's cases become symmetry planes, inflow/outflow conditions
+    farfield): testGrid (inviscid, farfield box), nozzle (inviscid,
+    axisymmetric-slice nozzle), periodic (laminar, periodic channel with a
+    blockage wall), shock_tube (inviscid; the diaphragm state p = 1e6 / 1e5
+    Pa at x = 0 of the reference's case)."""
+    import numpy as np_
+    from . import vog
+    F, W, S, P = grids.FARFIELD, grids.WALL, grids.SYMMETRY, grids.PERIODIC
+    spec = {
+        "testGrid": (dict(BC_1=F, BC_2=F, BC_3=F, BC_4=F, BC_5=F, BC_6=F), None, 0, 0.3),
+        "nozzle": (dict(axis=S, **{"in": F}, nozzle=S, out=F, symmetry1=S, symmetry2=S), None, 0, 0.3),
+        "periodic": (dict(blockage=W, bottom=P, top=P, **{"in": F}, out=F, BC_8=S, BC_9=S), {"top": "bottom"}, 1,
+                     0.1),
+        "shock_tube": (dict(BC_1=S, BC_2=S, BC_3=S, BC_4=S, BC_5=S, BC_6=S), None, 0, 0.0),
+    }[name]
+    bcs, per, visc, mach = spec
+    g = vog.load(vog_path(name), bcs, per, name)
+    g.check()
+    if name == "shock_tube":
+        q = np_.zeros((5, g.nc))
+        q[0] = 300.0
+        q[4] = np_.where(g.cc[0] < 0.0, 1e6, 1e5)
+        u0 = (0.0, 0.0, 0.0)
+    else:
+        q = grids.freestream_state(g, mach, seed=seed)
+        u0 = (mach * A300, 0.0, 0.0)
+    if visc:
+        g.meta["dist"] = grids.wall_distance(g)
+    prm = dict(BASE, flux="hllc" if visc else "roe", viscous=visc, u0=u0, cflmax=10.0, sgs_iter=5)
+    prm.update(over)
+    return g, q, prm
