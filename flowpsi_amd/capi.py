@@ -77,7 +77,7 @@ class Norms(C.Structure):
 class Info(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("max_level_width", C.c_int), ("nnz_blocks", C.c_int),
                 ("device_bytes", C.c_double), ("lsgs_nlevels", C.c_int), ("lsgs_contiguous", C.c_int),
-                ("sweep_tiles", C.c_int)]
+                ("sweep_tiles", C.c_int), ("lsgs_dataflow", C.c_int)]
 
 
 class LinStats(C.Structure):
@@ -226,7 +226,8 @@ class Solver:
         self._chk(self.L.fpsi_get_info(self.h, C.byref(i)), "fpsi_get_info")
         return dict(nlevels=i.nlevels, max_level_width=i.max_level_width, nnz_blocks=i.nnz_blocks,
                     device_bytes=i.device_bytes, lsgs_nlevels=i.lsgs_nlevels,
-                    lsgs_contiguous=bool(i.lsgs_contiguous), sweep_tiles=i.sweep_tiles)
+                    lsgs_contiguous=bool(i.lsgs_contiguous), sweep_tiles=i.sweep_tiles,
+                    lsgs_dataflow=i.lsgs_dataflow)
 
     # -- rules
     def set_params(self, prm: dict):

@@ -36,6 +36,10 @@ struct fpsi_ctx {
   const int *chunk_lo = nullptr, *fdep_off = nullptr, *fdep = nullptr, *bdep_off = nullptr, *bdep = nullptr;
   bool have_flow = false;
   int flow_epoch = 0;
+  // line SGS runs as the same dataflow when its breadth-first levels are the
+  // SGS levels (every cell in the same level, so the same dependency graph)
+  bool lsgs_flow = false;
+  int lsgs_waves = 0;
   int max_width = 0;
   double vtot = 0;
   DevGrid g{};
@@ -503,6 +507,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     bool ident = true;
     for (int k = 0; k < nc && ident; ++k) ident = lsgs_cells[k] == k;
     if (ident) lsgs_cells.clear();
+    c->lsgs_flow = ident && c->lsgs_lev_off == c->lev_off;  // and have_flow, below
   }
 
   // ---- tiles of the tiled sweep
@@ -565,6 +570,8 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     // two co-resident waves per CU at most: enough to keep a level in flight
     const int cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
     c->sweep_waves = std::max(1, std::min(cap, 4 * ncu));
+    const int lcap = lsgs_sweep_waves(1, ncu) > 0 ? std::min(lsgs_sweep_waves(0, ncu), lsgs_sweep_waves(1, ncu)) : 0;
+    c->lsgs_waves = std::max(1, std::min(lcap, 4 * ncu));
   }
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
@@ -602,6 +609,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       bdep_off[ch + 1] = (int)bdep.size();
     }
     c->have_flow = ok;
+    c->lsgs_flow = c->lsgs_flow && ok;
     fdep.push_back(0);  // never empty
     bdep.push_back(0);
   }
@@ -717,6 +725,7 @@ extern "C" int fpsi_get_info(fpsi_ctx *c, fpsi_info *info) {
   info->device_bytes = (double)c->bytes;
   info->lsgs_nlevels = (int)c->lsgs_lev_off.size() - 1;
   info->lsgs_contiguous = c->lsgs_cells == nullptr;
+  info->lsgs_dataflow = c->lsgs_flow ? 1 : 0;
   info->sweep_tiles = c->have_tiles ? c->tg.ntiles : 0;
   return 0;
 }
@@ -1253,18 +1262,32 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
   c->lsgs_iters = 0;
   c->lsgs_w = p.lsgs_relax;
   c->lsgs_resid = 0;
+  // one cooperative dataflow launch per half-sweep where the levels allow
+  // (bitwise equal to the level launches), else one launch per level
+  const bool flow = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT && c->lsgs_flow;
+  const int spin_limit = 1 << 22;
   for (int it = 0; it < maxiter; ++it) {
     launch_ghost(c->g, c->s, c->stream);
-    hipEvent_t e = prof_begin(c);
-    for (int l = 0; l < nl; ++l)
-      launch_lsgs_level(c->g, c->fp32, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
-                        c->stream);
-    prof_end(c, FPSI_K_SGS_SWEEP, e);
-    e = prof_begin(c);
-    for (int l = nl - 1; l >= 0; --l)
-      launch_lsgs_level(c->g, c->fp32, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
-                        c->stream);
-    prof_end(c, FPSI_K_SGS_SWEEP, e);
+    for (int dir = 0; dir < 2; ++dir) {
+      hipEvent_t e = prof_begin(c);
+      if (flow) {
+        const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
+                                                dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
+                                                ++c->flow_epoch, spin_limit, c->s.serr, c->lsgs_waves, c->s,
+                                                c->stream);
+        if (he != hipSuccess)
+          return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
+      } else if (dir == 0) {
+        for (int l = 0; l < nl; ++l)
+          launch_lsgs_level(c->g, c->fp32, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                            c->stream);
+      } else {
+        for (int l = nl - 1; l >= 0; --l)
+          launch_lsgs_level(c->g, c->fp32, 1, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                            c->stream);
+      }
+      prof_end(c, FPSI_K_SGS_SWEEP, e);
+    }
     launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1274,6 +1297,12 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
     c->lsgs_iters = (int)c->lsgs_host[3];
     if (c->lsgs_host[4] != 0.0) break;
   }
+  if (flow) {
+    int err = 0;
+    HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (err) return fail(c, FPSI_ERR_HIP, "dataflow line sweep: a dependency wait exceeded its limit");
+  }
   return 0;
 }
 
@@ -1282,17 +1311,35 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
 // unpivoted Jacobian-precision factors, periodic ghosts refreshed before each.
 static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
   const int nl = (int)c->lsgs_lev_off.size() - 1;
+  const bool flow = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT && c->lsgs_flow;
   launch_zero_d(5 * ((long)c->nc + c->nper + 1), c->s.X, c->stream);
   for (int it = 0; it < nsweeps; ++it) {
     launch_ghost(c->g, c->s, c->stream);
-    hipEvent_t e = prof_begin(c);
-    for (int l = 0; l < nl; ++l)
-      launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
-    prof_end(c, FPSI_K_SGS_SWEEP, e);
-    e = prof_begin(c);
-    for (int l = nl - 1; l >= 0; --l)
-      launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s, c->stream);
-    prof_end(c, FPSI_K_SGS_SWEEP, e);
+    for (int dir = 0; dir < 2; ++dir) {
+      hipEvent_t e = prof_begin(c);
+      if (flow) {
+        const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 1, dir, c->nchunks, c->chunk_lo,
+                                                dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
+                                                ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, c->s, c->stream);
+        if (he != hipSuccess)
+          return fail(c, FPSI_ERR_HIP, std::string("cooperative FSGS sweep launch: ") + hipGetErrorString(he));
+      } else if (dir == 0) {
+        for (int l = 0; l < nl; ++l)
+          launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                            c->stream);
+      } else {
+        for (int l = nl - 1; l >= 0; --l)
+          launch_fsgs_level(c->g, c->fp32, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
+                            c->stream);
+      }
+      prof_end(c, FPSI_K_SGS_SWEEP, e);
+    }
+  }
+  if (flow && nsweeps > 0) {
+    int err = 0;
+    HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (err) return fail(c, FPSI_ERR_HIP, "dataflow FSGS sweep: a dependency wait exceeded its limit");
   }
   return 0;
 }

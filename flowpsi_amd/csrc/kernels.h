@@ -23,6 +23,13 @@ int sgs_sweep_waves(int fp32, int ncu);
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
                             int nwaves, int ns, long nnz, DevState &s, hipStream_t st);
+// line SGS (fsgs = 0) or FSGS (fsgs = 1) half-sweep as one cooperative
+// dataflow launch over the SGS chunks (valid when the breadth-first levels
+// are the SGS levels)
+int lsgs_sweep_waves(int fp32, int ncu);
+hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
+                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
+                             int nwaves, DevState &s, hipStream_t st);
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // KEC: the time step's level n+1 from its last Newton iterate

@@ -1254,6 +1254,120 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
   }
 }
 
+// Line-SGS / FSGS half-sweep as one cooperative dataflow launch: the
+// chunks, dependency lists and flags of k_sgs_sweep (valid when the
+// breadth-first levels are the SGS levels), the per-cell update of
+// k_lsgs_level (rows rotated lower-first, FP64 unpivoted diagonal, X kept in
+// xt before the backward sweep overwrites it) or of k_fsgs_level (DG = F:
+// the unpivoted factors in the Jacobian precision, no xt).  The summation
+// order per component is the level kernels', so the result is bitwise the
+// same: forward, the lower terms come first and are summed after the wait;
+// backward, the lower terms (final since the forward sweep) are summed
+// before it.
+template <typename F, typename DG, bool BACK>
+__global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
+                                                   const int *__restrict__ dep_off, const int *__restrict__ dep,
+                                                   int *__restrict__ flag, int epoch, int spin_limit,
+                                                   int *__restrict__ err, const F *__restrict__ off,
+                                                   const DG *__restrict__ diag, const F *__restrict__ B,
+                                                   double *__restrict__ X, double *__restrict__ xt) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int gbase = 5 * (q < CPW ? q : 0);
+  for (int t = blockIdx.x; t < nchunks; t += nwaves) {
+    const int ch = BACK ? nchunks - 1 - t : t;
+    const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
+    const int c = lo + q;
+    const bool act = q < CPW && c < hi;
+    const int cc = act ? c : lo;
+    double tsum = -(double)B[(size_t)cc * 5 + i];
+    const int m0 = g.m_off[cc];
+    const int len = g.m_off[cc + 1] - m0;
+    const int nup = g.m_nup[cc];
+    const int nlow = len - nup;  // rotated rows: lower entries j < nlow, then upper
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)cc * MAXR + j];
+    F mrow[MAXR][5];
+    DG a[25];
+    double xv[MAXR][5];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+    }
+#pragma unroll
+    for (int i2 = 0; i2 < 25; ++i2) a[i2] = diag[(size_t)cc * 25 + i2];
+    // values this half-sweep does not write: upper (forward) / lower (backward), and pads
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) xv[j][jj] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + jj) : 0.0;
+    }
+    double xold = 0.0;
+    if constexpr (BACK) {
+      if (xt) xold = ldx<true>(X + (size_t)cc * 5 + i);
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+          if (j < nlow) tsum += mrow[j][jj] * xv[j][jj];
+    }
+    const int d0 = dep_off[ch], nd = dep_off[ch + 1] - d0;
+    const int dl = lane < nd ? dep[d0 + lane] : -1;
+    int spins = 0;
+    while (true) {
+      const int v = dl >= 0 ? __hip_atomic_load(flag + dl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
+      if (__all(v >= epoch)) break;
+      if (++spins > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (post) xv[j][jj] = ldx<true>(X + (size_t)nb[j] * 5 + jj);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (!BACK || j >= nlow) tsum += mrow[j][jj] * xv[j][jj];
+    if (act)
+      for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        const F *M = off + (size_t)(m0 + jr) * 25;
+        const double *xn = X + (size_t)g.m_nbr[m0 + jr] * 5;
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj) tsum += M[i + 5 * jj] * ldx<true>(xn + jj);
+      }
+    double v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
+    if (act) {
+      if constexpr (BACK) {
+        if (xt) xt[(size_t)c * 5 + i] = xold;
+      }
+      double x[5];
+      lusolve5<DG>(a, PIV_ID, v, x);
+      const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+      stx<true>(X + (size_t)c * 5 + i, xi);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (lane == 0) __hip_atomic_store(flag + ch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // periodic ghost copies, src/linearSolverSGS.loci:124-127 and :215-219
 __global__ void k_ghost(DevGrid g, double *__restrict__ X) {
   const int pi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2305,6 +2419,36 @@ int sgs_sweep_waves(int fp32, int ncu) {
   per_cu = std::min(a, b);
   if (per_cu <= 0) return 0;
   return per_cu * ncu;
+}
+template <typename F, typename DG>
+static int lsgs_waves_t(int ncu) {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_lsgs_sweep<F, DG, false>, 64, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_lsgs_sweep<F, DG, true>, 64, 0) != hipSuccess) return 0;
+  const int per_cu = std::min(a, b);
+  return per_cu > 0 ? per_cu * ncu : 0;
+}
+int lsgs_sweep_waves(int fp32, int ncu) {
+  const int a = fp32 ? lsgs_waves_t<float, double>(ncu) : lsgs_waves_t<double, double>(ncu);
+  const int b = fp32 ? lsgs_waves_t<float, float>(ncu) : a;
+  return std::min(a, b);
+}
+hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
+                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
+                             int nwaves, DevState &s, hipStream_t st) {
+  int nw = std::max(1, std::min(nwaves, nchunks));
+  const void *diag = fsgs ? (const void *)s.luf : (const void *)s.lu64;
+  double *xt = fsgs ? nullptr : s.xt;
+  void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &dep_off, &dep, &flag, &epoch, &spin_limit, &err,
+                  (void *)&s.off, (void *)&diag, (void *)&s.B, (void *)&s.X, (void *)&xt};
+  const void *fn;
+  if (fp32) {
+    if (fsgs) fn = backward ? (const void *)k_lsgs_sweep<float, float, true> : (const void *)k_lsgs_sweep<float, float, false>;
+    else fn = backward ? (const void *)k_lsgs_sweep<float, double, true> : (const void *)k_lsgs_sweep<float, double, false>;
+  } else {
+    fn = backward ? (const void *)k_lsgs_sweep<double, double, true> : (const void *)k_lsgs_sweep<double, double, false>;
+  }
+  return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
 }
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,

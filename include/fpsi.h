@@ -144,6 +144,8 @@ typedef struct {
   int lsgs_nlevels;     /* dependency levels of the line-SGS order */
   int lsgs_contiguous;  /* 1 when every line-SGS level is a contiguous cell range */
   int sweep_tiles;      /* tiles of the tiled sweep (0: the grid admits none) */
+  int lsgs_dataflow;    /* 1 when line SGS / FSGS run as the dataflow sweep under
+                           FPSI_SWEEP_PERSISTENT (their levels are the SGS levels) */
 } fpsi_info;
 
 /* Build the context: renumbers cells into level order, builds the cell->face
