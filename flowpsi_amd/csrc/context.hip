@@ -571,7 +571,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     const int cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
     c->sweep_waves = std::max(1, std::min(cap, 4 * ncu));
     const int lcap = lsgs_sweep_waves(1, ncu) > 0 ? std::min(lsgs_sweep_waves(0, ncu), lsgs_sweep_waves(1, ncu)) : 0;
-    c->lsgs_waves = std::max(1, std::min(lcap, 4 * ncu));
+    c->lsgs_waves = std::max(1, lcap);  // wide levels: every co-resident wave
   }
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
@@ -710,9 +710,10 @@ static int lsgs_alloc(fpsi_ctx *c) {
   DevState &s = c->s;
   int rc = 0;
   if ((rc = dalloc(c, &s.lu64, 25 * (size_t)c->nc)) || (rc = dalloc(c, &s.xt, 5 * (size_t)c->nc)) ||
-      (rc = dalloc(c, &s.lparts, (size_t)lsgs_parts(c->g))) || (rc = dalloc(c, &s.lst, 8)))
+      (rc = dalloc(c, &s.lparts, 3 * (size_t)lsgs_parts(c->g))) || (rc = dalloc(c, &s.lst, 24)))
     return rc;
-  HIPCHK(c, hipHostMalloc((void **)&c->lsgs_host, 8 * sizeof(double), hipHostMallocDefault));
+  // slot 0 the fluid, slots 1-2 the turbulence scalars (their loops may run together)
+  HIPCHK(c, hipHostMalloc((void **)&c->lsgs_host, 24 * sizeof(double), hipHostMallocDefault));
   c->lsgs_alloc = true;
   return 0;
 }
@@ -1273,8 +1274,8 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
       if (flow) {
         const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
                                                 dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, spin_limit, c->s.serr, c->lsgs_waves, c->s,
-                                                c->stream);
+                                                ++c->flow_epoch, spin_limit, c->s.serr, c->lsgs_waves, 1, 0, 0,
+                                                c->nnz, c->s, c->stream);
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
       } else if (dir == 0) {
@@ -1306,6 +1307,67 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
   return 0;
 }
 
+// Line SGS of the fluid and of the turbulence scalars in the same dataflow
+// launches.  Each system keeps its own loop exactly as when solved alone
+// (ghost refresh, forward, backward, relaxation and stopping test with its
+// own state slot), so the bits are those of the separate solves; a system
+// leaves the launches once its test stops it.
+static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
+  const fpsi_params &p = c->prm;
+  const int ns = c->nscal();
+  for (int slot = 0; slot <= ns; ++slot) launch_lsgs_init(c->s, p.lsgs_relax, c->stream, slot);
+  c->lsgs_iters = 0;
+  c->lsgs_w = p.lsgs_relax;
+  c->lsgs_resid = 0;
+  for (int sc = 0; sc < ns; ++sc) {
+    c->sa_lsgs_iters[sc] = 0;
+    c->sa_lsgs_w[sc] = p.lsgs_relax;
+    c->sa_lsgs_resid[sc] = 0;
+  }
+  bool fluid = true;
+  int mask = (1 << ns) - 1;
+  for (int it = 0; it < maxiter && (fluid || mask); ++it) {
+    if (fluid) launch_ghost(c->g, c->s, c->stream);
+    if (mask) launch_ghost1(c->g, ns, c->s.sX, c->stream);
+    for (int dir = 0; dir < 2; ++dir) {
+      hipEvent_t e = prof_begin(c);
+      const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
+                                              dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
+                                              ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, fluid ? 1 : 0, ns,
+                                              mask, c->nnz, c->s, c->stream);
+      prof_end(c, FPSI_K_SGS_SWEEP, e);
+      if (he != hipSuccess)
+        return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
+    }
+    if (fluid) launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    for (int sc = 0; sc < ns; ++sc)
+      if (mask >> sc & 1) launch_sa_lsgs_relax(c->g, sc, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream, 1 + sc);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 8 * (1 + ns) * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (fluid) {
+      c->lsgs_w = c->lsgs_host[0];
+      c->lsgs_resid = c->lsgs_host[2];
+      c->lsgs_iters = (int)c->lsgs_host[3];
+      if (c->lsgs_host[4] != 0.0) fluid = false;
+    }
+    for (int sc = 0; sc < ns; ++sc)
+      if (mask >> sc & 1) {
+        const double *h = c->lsgs_host + 8 * (1 + sc);
+        c->sa_lsgs_w[sc] = h[0];
+        c->sa_lsgs_resid[sc] = h[2];
+        c->sa_lsgs_iters[sc] = (int)h[3];
+        if (h[4] != 0.0) mask &= ~(1 << sc);
+      }
+  }
+  int err = 0;
+  HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (err) return fail(c, FPSI_ERR_HIP, "dataflow line sweep: a dependency wait exceeded its limit");
+  return 0;
+}
+
 // FSGSBlockedSSolve (src/linearSolverFSGS.loci:1598-1708): X from zero, then
 // nsweeps symmetric sweeps in the breadth-first line order with the
 // unpivoted Jacobian-precision factors, periodic ghosts refreshed before each.
@@ -1320,7 +1382,8 @@ static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
       if (flow) {
         const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 1, dir, c->nchunks, c->chunk_lo,
                                                 dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, c->s, c->stream);
+                                                ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, 1, 0, 0, c->nnz,
+                                                c->s, c->stream);
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative FSGS sweep launch: ") + hipGetErrorString(he));
       } else if (dir == 0) {
@@ -1396,12 +1459,18 @@ extern "C" int fpsi_update(fpsi_ctx *c) {
 static int enqueue_iteration(fpsi_ctx *c, double *dh) {
   enqueue_assemble(c);
   HIPCHK(c, hipMemcpyAsync(dh, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  // turbulence scalars ride in the fluid's dataflow sweep when both use SGS
-  const bool fuse = c->kp.turb && !c->line_order() && c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
-  if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
-                    : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter))
+  // turbulence scalars ride in the fluid's dataflow sweep when both use SGS,
+  // or line SGS on levels the dataflow covers
+  const bool persistent = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
+  const bool fuse = c->kp.turb && !c->line_order() && persistent;
+  const bool lfuse = c->kp.turb && c->lsgs() && persistent && c->lsgs_flow;
+  if (lfuse) {
+    if (int rc = enqueue_lsgs_fused(c, c->prm.lsgs_maxiter)) return rc;
+  } else if (int rc = fuse ? enqueue_sweeps(c, c->prm.sgs_iter, c->nscal())
+                           : enqueue_solve(c, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) {
     return rc;
-  if (c->kp.turb && !fuse) {
+  }
+  if (c->kp.turb && !fuse && !lfuse) {
     hipEvent_t e2 = prof_begin(c);
     if (int rc = enqueue_sa_solve(c, -1, c->lsgs() ? c->prm.lsgs_maxiter : c->prm.sgs_iter)) return rc;
     prof_end(c, FPSI_K_SGS_SWEEP, e2);

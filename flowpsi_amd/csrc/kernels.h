@@ -27,9 +27,11 @@ hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunk
 // dataflow launch over the SGS chunks (valid when the breadth-first levels
 // are the SGS levels)
 int lsgs_sweep_waves(int fp32, int ncu);
+// fluid = 0 skips the block rows; bit sc of smask sweeps turbulence scalar sc
+// (line SGS rows, xt kept before the forward sweep) in the same launch
 hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
                              const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
-                             int nwaves, DevState &s, hipStream_t st);
+                             int nwaves, int fluid, int ns, int smask, long nnz, DevState &s, hipStream_t st);
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // KEC: the time step's level n+1 from its last Newton iterate
@@ -49,7 +51,7 @@ void launch_sa_level(const DevGrid &g, int sc0, int ns, long nnz, const int *cel
 void launch_sa_lsgs_level(const DevGrid &g, int sc, long nnz, int backward, const int *cells, int lo, int hi,
                           DevState &s, hipStream_t st);
 void launch_sa_lsgs_relax(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
-                          hipStream_t st);
+                          hipStream_t st, int slot = 0);
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // weighted r.r, r.q, q.q of r = B - A X0, q = A D into out[0..2] (needs 3 * nblk(nc) parts)
 void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
@@ -67,7 +69,7 @@ void launch_fsgs_level(const DevGrid &g, int fp32, const int *cells, int lo, int
 void launch_zero_d(long n, double *x, hipStream_t st);
 // line SGS
 int lsgs_parts(const DevGrid &g);
-void launch_lsgs_init(DevState &s, double w0, hipStream_t st);
+void launch_lsgs_init(DevState &s, double w0, hipStream_t st, int slot = 0);
 void launch_lsgs_level(const DevGrid &g, int fp32, int backward, const int *cells, int lo, int hi, DevState &s,
                        hipStream_t st);
 void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st);

@@ -1125,6 +1125,15 @@ struct ScalarRows {
   double *X;  // [ns][nc + nper + 1]
 };
 
+// turbulence scalar rows of a line-SGS half-sweep: scalar sc is swept when
+// bit sc of mask is set; xt keeps each cell's value before the forward sweep
+struct LineScalars {
+  int ns, mask;
+  long nnz;
+  const double *off, *dinv, *B;
+  double *X, *xt;
+};
+
 template <typename F, bool BACK>
 __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                   const int *__restrict__ dep_off, const int *__restrict__ dep,
@@ -1264,13 +1273,19 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 // same: forward, the lower terms come first and are summed after the wait;
 // backward, the lower terms (final since the forward sweep) are summed
 // before it.
+// With line-scalar rows (ls.mask != 0) lane i < ns of each cell also sweeps
+// turbulence scalar i as k_sa_lsgs_level does (X = (B - sum off X_nb) / D,
+// rows rotated lower-first), in the same launch; fluid = 0 leaves the block
+// rows alone.  Every system keeps its own per-cell order, so the result is
+// bitwise that of separate sweeps.
 template <typename F, typename DG, bool BACK>
 __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                    const int *__restrict__ dep_off, const int *__restrict__ dep,
                                                    int *__restrict__ flag, int epoch, int spin_limit,
                                                    int *__restrict__ err, const F *__restrict__ off,
                                                    const DG *__restrict__ diag, const F *__restrict__ B,
-                                                   double *__restrict__ X, double *__restrict__ xt) {
+                                                   double *__restrict__ X, double *__restrict__ xt, int fluid,
+                                                   LineScalars ls) {
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
   const int gbase = 5 * (q < CPW ? q : 0);
@@ -1279,6 +1294,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
     const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
     const int c = lo + q;
     const bool act = q < CPW && c < hi;
+    const bool fact = act && fluid;
     const int cc = act ? c : lo;
     double tsum = -(double)B[(size_t)cc * 5 + i];
     const int m0 = g.m_off[cc];
@@ -1318,6 +1334,30 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         for (int jj = 0; jj < 5; ++jj)
           if (j < nlow) tsum += mrow[j][jj] * xv[j][jj];
     }
+    // turbulence scalar sc = i of this cell
+    const bool sact = act && i < ls.ns && ((ls.mask >> i) & 1);
+    const int sc = sact ? i : 0;
+    const size_t xs = (size_t)g.nc + g.nper + 1;
+    const double *soff = ls.mask ? ls.off + (size_t)sc * ls.nnz + m0 : nullptr;
+    double *sx = ls.mask ? ls.X + sc * xs : nullptr;
+    double sres = 0.0, sold = 0.0, sco[MAXR], sxv[MAXR];
+    if (sact) {
+      sres = ls.B[(size_t)sc * g.nc + cc];
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        sco[j] = j < len ? soff[jr] : 0.0;
+        const bool pre = j < len && (BACK ? (j < nlow) : (j >= nlow));
+        sxv[j] = pre ? ldx<true>(sx + nb[j]) : 0.0;
+      }
+      if (!BACK) sold = ldx<true>(sx + cc);
+      if (BACK) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nlow) sres -= sxv[j] * sco[j];
+      }
+    }
     const int d0 = dep_off[ch], nd = dep_off[ch + 1] - d0;
     const int dl = lane < nd ? dep[d0 + lane] : -1;
     int spins = 0;
@@ -1342,7 +1382,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj)
         if (!BACK || j >= nlow) tsum += mrow[j][jj] * xv[j][jj];
-    if (act)
+    if (fact)
       for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
         int jr = j + nup;
         if (jr >= len) jr -= len;
@@ -1351,10 +1391,27 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
         for (int jj = 0; jj < 5; ++jj) tsum += M[i + 5 * jj] * ldx<true>(xn + jj);
       }
+    if (sact) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = j < len && (BACK ? (j >= nlow) : (j < nlow));
+        if (post) sxv[j] = ldx<true>(sx + nb[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (j < len && (!BACK || j >= nlow)) sres -= sxv[j] * sco[j];
+      for (int j = MAXR; j < len; ++j) {
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        sres -= ldx<true>(sx + g.m_nbr[m0 + jr]) * soff[jr];
+      }
+      if (!BACK) ls.xt[(size_t)sc * g.nc + c] = sold;
+      stx<true>(sx + c, ls.dinv[(size_t)sc * g.nc + c] * sres);
+    }
     double v[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
-    if (act) {
+    if (fact) {
       if constexpr (BACK) {
         if (xt) xt[(size_t)c * 5 + i] = xold;
       }
@@ -2435,12 +2492,14 @@ int lsgs_sweep_waves(int fp32, int ncu) {
 }
 hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
                              const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
-                             int nwaves, DevState &s, hipStream_t st) {
+                             int nwaves, int fluid, int ns, int smask, long nnz, DevState &s, hipStream_t st) {
   int nw = std::max(1, std::min(nwaves, nchunks));
   const void *diag = fsgs ? (const void *)s.luf : (const void *)s.lu64;
   double *xt = fsgs ? nullptr : s.xt;
+  LineScalars ls{ns, smask, nnz, s.soff, s.sdinv, s.sB, s.sX, s.sxt};
+  if (!smask) ls = LineScalars{0, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
   void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &dep_off, &dep, &flag, &epoch, &spin_limit, &err,
-                  (void *)&s.off, (void *)&diag, (void *)&s.B, (void *)&s.X, (void *)&xt};
+                  (void *)&s.off, (void *)&diag, (void *)&s.B, (void *)&s.X, (void *)&xt, &fluid, (void *)&ls};
   const void *fn;
   if (fp32) {
     if (fsgs) fn = backward ? (const void *)k_lsgs_sweep<float, float, true> : (const void *)k_lsgs_sweep<float, float, false>;
@@ -2712,8 +2771,8 @@ void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, doubl
 }
 
 int lsgs_parts(const DevGrid &g) { return (int)(((long)g.nc * 5 + TPB - 1) / TPB); }
-void launch_lsgs_init(DevState &s, double w0, hipStream_t st) {
-  hipLaunchKernelGGL(k_lsgs_init, dim3(1), dim3(64), 0, st, s.lst, w0);
+void launch_lsgs_init(DevState &s, double w0, hipStream_t st, int slot) {
+  hipLaunchKernelGGL(k_lsgs_init, dim3(1), dim3(64), 0, st, s.lst + 8 * slot, w0);
 }
 template <typename F, bool BACK>
 static void lsgs_level_t(const DevGrid &g, const int *cells, int lo, int hi, DevState &s, hipStream_t st) {
@@ -2760,6 +2819,8 @@ void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol,
   hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
                      reltol);
 }
+// slot > 0: a turbulence scalar's own relaxation state and partial sums
+// (lst + 8 slot, lparts + slot x lsgs_parts), so its loop can run beside the fluid's
 
 void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
                         double *bsum, double *out, hipStream_t st) {
@@ -2839,13 +2900,13 @@ void launch_sa_lsgs_level(const DevGrid &g, int sc, long nnz, int backward, cons
     hipLaunchKernelGGL(k_sa_lsgs_level<false>, grid, dim3(TPB), 0, st, g, cells, lo, hi, off, dinv, B, X, xt);
 }
 void launch_sa_lsgs_relax(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
-                          hipStream_t st) {
+                          hipStream_t st, int slot) {
   const int np = nblk(g.nc);
   const size_t nc = g.nc, xs = nc + g.nper + 1;
-  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)s.lst, s.sX + sc * xs,
-                     (const double *)(s.sxt + sc * nc), s.lparts);
-  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
-                     reltol);
+  double *lst = s.lst + 8 * slot, *parts = s.lparts + (size_t)slot * lsgs_parts(g);
+  hipLaunchKernelGGL(k_lsgs_relax, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)lst, s.sX + sc * xs,
+                     (const double *)(s.sxt + sc * nc), parts);
+  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)parts, lst, iter, abstol, reltol);
 }
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st) {
   hipLaunchKernelGGL(k_sa_update, dim3(nblk(g.nc)), dim3(TPB), 0, st, g, p, nscal(p), (const double *)s.sX, s.q);

@@ -2,9 +2,10 @@
 (sweep = persistent) against one launch per level: the same bits for the
 solution, the line-SGS iteration count, relaxation and update norm, and
 the state after Newton steps, in FP64 and FP32 blocks, on grids whose
-breadth-first levels are the SGS levels (where the dataflow runs) and on a
-tetrahedral grid where they are not (where the persistent mode falls back
-to the level launches)."""
+breadth-first levels are the SGS levels (where the dataflow runs; with a
+turbulence model the scalars' line-SGS loops run in the same launches) and
+on a tetrahedral grid where they are not (where the persistent mode falls
+back to the level launches)."""
 import numpy as np
 import pytest
 
@@ -20,6 +21,7 @@ def _cases():
         "bl_sst": lambda: workloads.c5_bl(nx=12, ny=30, nz=10),
         "box40": lambda: workloads.box(40),
         "tets": lambda: workloads.c3_tets(5, precision="fp64"),
+        "channel_sa": lambda: workloads.channel_sa(nx=10, ny=16, nz=6),
     }
 
 
@@ -39,12 +41,15 @@ def test_dataflow_line_sweeps_bitwise(name, prec, solver):
         dq = s.linear_solve(prm["lsgs_maxiter"] if solver == "lsgs" else prm["sgs_iter"])
         st = s.lsgs_stats() if solver == "lsgs" else None
         hist = s.newton(2)
-        out[sweep] = (dq, st, s.get_state(), hist, info)
+        # turbulence scalars: their line-SGS loops of the last Newton step
+        tst = [s.turb_lsgs_stats(j) for j in range(s.nvar - 5)] if solver == "lsgs" else None
+        out[sweep] = (dq, st, s.get_state(), hist, info, tst)
         s.close()
     if name != "tets":
         assert out["persistent"][4]["lsgs_dataflow"] == 1
-    dq0, st0, q0, h0, _ = out["per_level"]
-    dq1, st1, q1, h1, _ = out["persistent"]
+    dq0, st0, q0, h0, _, t0 = out["per_level"]
+    dq1, st1, q1, h1, _, t1 = out["persistent"]
+    assert t0 == t1
     assert np.array_equal(dq0, dq1)
     assert st0 == st1
     assert np.array_equal(q0, q1)
