@@ -81,11 +81,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="channel128", choices=["channel128", "channel128_sa", "channel128_sst", "box32"],
-                    help="channel128 is the metric's workload; the turbulent channels (SA, SST) are extra lines")
+    ap.add_argument("--workload", default="channel128",
+                    choices=["channel128", "channel128_sa", "channel128_sst", "box32", "c3", "c4", "c5"],
+                    help="channel128 is the metric's workload; the turbulent channels (SA, SST) and the "
+                         "full-size tetrahedral (c3, 4.1M cells), hybrid (c4, 8M) and boundary-layer "
+                         "(c5, 20M, line SGS) grids are extra lines")
     ap.add_argument("--n", type=int, default=None, help="grid cells per axis (default 128 / 32)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--sweep", default="persistent", choices=["per_level", "tiled", "persistent"])
+    ap.add_argument("--sweep", default=None, choices=["per_level", "tiled", "persistent"],
+                    help="default persistent (the dataflow); per_level for c3, whose six dependency levels "
+                         "of ~690k cells each fill the GPU in one launch per level")
     ap.add_argument("--time-scheme", default="steady", choices=["steady", "bdf2", "kec"],
                     help="bdf2/kec: time-accurate dual time stepping; a step is one time step of "
                          "--newton-iter Newton iterations and the value counts Newton iterations")
@@ -118,10 +123,27 @@ def main():
         g, q, prm = (workloads.channel_sa if turb == "sa" else workloads.channel_sst)(n)
         wname = (f"{turb.upper()} NS channel {n}^3 ({g.nc} cells), HLLC+viscous, segregated "
                  f"{'nuTilde' if turb == 'sa' else 'k and omega'}, SGS x{prm['sgs_iter']}")
+    elif a.workload == "c3":
+        n = a.n or 88
+        g, q, prm = workloads.c3_tets(n, precision=a.precision)
+        wname = (f"SA on {g.nc} jittered tetrahedra ({n}^3 hexes x 6), M 0.5, Re 1e6/m, HLLC+viscous, "
+                 f"SGS x{prm['sgs_iter']}")
+    elif a.workload == "c4":
+        n = a.n or 200
+        g, q, prm = workloads.c4_hybrid(nx=n, nz=n, n_prism=30, n_hex=140)
+        wname = (f"SA on a {g.nc}-cell prism-layer + jittered-hex grid ({n}x{n} columns, 30 prism and 140 "
+                 f"hex layers), M 0.3, HLLC+viscous, SGS x{prm['sgs_iter']}")
+    elif a.workload == "c5":
+        n = a.n or 577
+        g, q, prm = workloads.c5_bl(nx=n, ny=60, nz=n)
+        wname = (f"SST on a {g.nc}-cell boundary-layer grid ({n}x60x{n}, first-cell aspect 1e3), M 0.6, "
+                 f"Re 5e6/m, line SGS (max {prm.get('lsgs_maxiter', 10)} sweeps, turbulence scalars fused)")
     else:
         n = a.n or 32
         g, q, prm = workloads.box(n)
         wname = f"Euler box {n}^3 ({g.nc} cells), Roe, SGS x{prm['sgs_iter']}"
+    if a.sweep is None:
+        a.sweep = "per_level" if a.workload == "c3" else "persistent"
     prm = dict(prm, precision=a.precision, sweep=a.sweep)
     unsteady = a.time_scheme != "steady"
     iters = a.newton_iter if unsteady else 1
@@ -161,7 +183,15 @@ def main():
     bytes_launch = d["bytes"]
     kname = dom
     tiled = a.sweep == "tiled"
-    if dom == "sgs_sweep":
+    line = prm.get("solver", "sgs") == "lsgs"
+    if dom == "sgs_sweep" and line:
+        if a.sweep == "persistent" and info.get("lsgs_dataflow"):
+            kname = "k_lsgs_sweep"         # one cooperative launch per half-sweep
+        else:
+            avg_s /= info["lsgs_nlevels"]
+            bytes_launch /= info["lsgs_nlevels"]
+            kname = "k_lsgs_level"
+    elif dom == "sgs_sweep":
         if tiled:
             kname = "k_sgs_tiles"          # one cooperative launch per half-sweep
         elif a.sweep == "persistent":
@@ -179,7 +209,8 @@ def main():
         if t.get("precision") == a.precision and key in t["kernels"]:
             traffic = t["kernels"][key]["bytes"]
     # whole-step algorithmic bytes (per Newton step) for the step-level roofline
-    sweeps = prm["sgs_iter"] * 2
+    # half-sweeps per Newton iteration as run (line SGS stops by its own test)
+    sweeps = prof["sgs_sweep"]["launches"] / (a.steps * iters)
     step_bytes = (prof["residual"]["bytes"] + prof["face_jac"]["bytes"] + prof["assemble"]["bytes"] +
                   sweeps * prof["sgs_sweep"]["bytes"] + prof["update"]["bytes"])
     step_gbs = step_bytes * a.steps * iters / dt / 1e9
@@ -190,11 +221,11 @@ def main():
 
     tot = [h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist]
     cpu = None
-    if not a.no_cpu_baseline and not unsteady:
+    if not a.no_cpu_baseline and not unsteady and a.workload.startswith("channel"):
         cpu = cpu_baseline(g, q, prm, a.cpu_sample_steps)
 
     conv = None
-    if not a.no_converge and not unsteady:
+    if not a.no_converge and not unsteady and a.workload == "channel128":
         from flowpsi_amd import converge
         gb, qb, pb = workloads.box(32)
         log("time to 1e-8 drop on the 32^3 Euler box: fp64 fixed / fp32 fixed / fp32 tol / fp32+NN")
