@@ -1640,7 +1640,18 @@ __global__ void k_lsgs_check(int nparts, const double *__restrict__ part, double
                              double abstol, double reltol) {
   __shared__ double sh[TPB];
   double a = 0.0;
-  for (int j = threadIdx.x; j < nparts; j += TPB) a += part[j];
+  {
+    // the same in-order sum per thread, with eight loads in flight
+    int j = threadIdx.x;
+    for (; j + 7 * TPB < nparts; j += 8 * TPB) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[j + u * TPB];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a += v[u];
+    }
+    for (; j < nparts; j += TPB) a += part[j];
+  }
   sh[threadIdx.x] = a;
   __syncthreads();
   for (int s = TPB / 2; s > 0; s >>= 1) {
