@@ -129,7 +129,7 @@ __device__ __forceinline__ void stencil(const DevGrid &g, const double *__restri
 // textbook unweighted least squares, Venkatakrishnan / Barth limiters)
 // ---------------------------------------------------------------------------
 template <int NQ>
-__global__ void k_grad(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+__global__ void __launch_bounds__(TPB) k_grad(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                        double *__restrict__ grad, double *__restrict__ phi) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
@@ -390,7 +390,7 @@ __global__ void k_eta_max(DevGrid g, const double *__restrict__ src, double *__r
 // src/bcViscousWall.loci:216-229, src/bcImpermeable.loci:66-73), vflux and
 // maxev_f (src/solverSetup.loci:83-105)
 // ---------------------------------------------------------------------------
-__global__ void k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+__global__ void __launch_bounds__(TPB) k_face_flux(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                             const double *__restrict__ grad, const double *__restrict__ phi,
                             const double *__restrict__ etap, const double *__restrict__ tmu,
                             double *__restrict__ iflux, double *__restrict__ vflux, double *__restrict__ maxev) {
@@ -641,7 +641,7 @@ __device__ __forceinline__ void store_blk(F *__restrict__ dst, const F *blk, boo
 }
 
 template <typename F>
-__global__ void k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+__global__ void __launch_bounds__(TPB) k_face_jac(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                            const double *__restrict__ grad, const double *__restrict__ phi,
                            const double *__restrict__ vflux, const double *__restrict__ etap,
                            const double *__restrict__ tmu, F *__restrict__ off, F *__restrict__ pblk,
@@ -791,7 +791,7 @@ __device__ __forceinline__ double kec_xnp1(double X, double Xn) {
 
 // TS: time scheme, 0 steady, 1 BDF2, 2 KEC
 template <typename F, bool LS, int TS>
-__global__ void k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
+__global__ void __launch_bounds__(TPB) k_cell_assemble(DevGrid g, Prm p, const double *__restrict__ q,
                                 const double *__restrict__ src, const double *__restrict__ maxev,
                                 const F *__restrict__ off, const F *__restrict__ pblk,
                                 const F *__restrict__ bblk, double *__restrict__ dtau_out,
@@ -2170,7 +2170,7 @@ __global__ void k_sst_cell_a(DevGrid g, Prm p, const double *__restrict__ q, con
 // convection of max(X, 0), diffusion X_nu_f grad_f(X).n area with X_nu_f =
 // mu_f + tmu_f sigma_f, fjp, fjm (also into the scalar rows soff), and the
 // boundary Jacobian (src/solverScalarTransport.loci:121-135, 548-567, 779-826)
-__global__ void k_sst_face_b(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
+__global__ void __launch_bounds__(TPB) k_sst_face_b(DevGrid g, Prm p, const double *__restrict__ q, const double *__restrict__ qf,
                              const double *__restrict__ grad, const double *__restrict__ phi,
                              const double *__restrict__ iflux, const double *__restrict__ tmu,
                              const double *__restrict__ scell, double *__restrict__ saf, double *__restrict__ soff,
