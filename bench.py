@@ -227,8 +227,8 @@ def main():
     conv = None
     if not a.no_converge and not unsteady and a.workload == "channel128":
         from flowpsi_amd import converge
-        gb, qb, pb = workloads.box(32)
-        log("time to 1e-8 drop on the 32^3 Euler box: fp64 fixed / fp32 fixed / fp32 tol / fp32+NN")
+        gb, qb, pb = workloads.box(32, sweep="persistent")
+        log("time to 1e-8 drop on the 32^3 Euler box (dataflow sweeps): fp64 fixed / fp32 fixed / fp32 tol / fp32+NN")
         conv = converge.compare(gb, qb, pb)
         conv["workload"] = "Euler box 32^3 (32768 cells), Roe, CFL 10, SGS x5 (fixed paths)"
         log(f"converge: " + ", ".join(f"{k}={v.get('time_to_target_s')}" for k, v in conv.items()
