@@ -563,13 +563,13 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
 
   // ---- device upload
   HIPCHK(c, hipStreamCreate(&c->stream));
+  int sweep_ncu = 1, sweep_cap = 0;
   {
     int dev = 0, ncu = 0;
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    // two co-resident waves per CU at most: enough to keep a level in flight
-    const int cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
-    c->sweep_waves = std::max(1, std::min(cap, 4 * ncu));
+    sweep_ncu = ncu;
+    sweep_cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
     const int lcap = lsgs_sweep_waves(1, ncu) > 0 ? std::min(lsgs_sweep_waves(0, ncu), lsgs_sweep_waves(1, ncu)) : 0;
     c->lsgs_waves = std::max(1, lcap);  // wide levels: every co-resident wave
   }
@@ -581,6 +581,16 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     for (int a = c->lev_off[l]; a < c->lev_off[l + 1]; a += 12) chunk_lo.push_back(a);  // 12 cells per wave
   c->nchunks = (int)chunk_lo.size();
   chunk_lo.push_back(nc);
+  {
+    // 1.75 waves per chunk of a mean level, to the nearest whole number of
+    // waves per CU, at least four per CU and at most every co-resident wave:
+    // narrow levels (128^3 channel, ~460 chunks) are fastest at 4 per CU,
+    // the 8M-cell prism grid (~1170 chunks) at 8 (10 per CU polls more and
+    // is 2.5 % slower)
+    const long want = 7L * c->nchunks / (4L * std::max(1, nlev));
+    const long per_cu = std::max(4L, (want + sweep_ncu / 2) / std::max(1, sweep_ncu));
+    c->sweep_waves = (int)std::max(1L, std::min((long)sweep_cap, per_cu * sweep_ncu));
+  }
   std::vector<int> fdep_off(c->nchunks + 1, 0), fdep, bdep_off(c->nchunks + 1, 0), bdep;
   {
     std::vector<int> chunk_of(nc);

@@ -1144,6 +1144,12 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
   const int gbase = 5 * (q < CPW ? q : 0);
+  // component i of each neighbour's X and row i of the diagonal factors,
+  // shared by the cell's five lanes after the wait
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ F al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  F *aw = al + (q < CPW ? q : 0) * 25;
   for (int t = blockIdx.x; t < nchunks; t += nwaves) {
     const int ch = BACK ? nchunks - 1 - t : t;
     const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
@@ -1158,8 +1164,8 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
     int nb[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)cc * MAXR + j];
-    F mrow[MAXR][5], a[25];
-    double xv[MAXR][5];
+    F mrow[MAXR][5], a5[5];
+    double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
@@ -1168,12 +1174,11 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
 #pragma unroll
-    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu[(size_t)cc * 25 + i2];
+    for (int k = 0; k < 5; ++k) a5[k] = lu[(size_t)cc * 25 + 5 * i + k];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j >= nup) : (j < nup);
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj) xv[j][jj] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + jj) : 0.0;
+      x1[j] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + i) : 0.0;
     }
     // turbulence scalar sc = i of this cell (lanes i < ns)
     const bool sact = act && i < sr.ns;
@@ -1197,11 +1202,17 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       }
     }
     if (!BACK) {
+      if (q < CPW) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nup) xw[j * 5 + i] = x1[j];
+      }
+      __syncthreads();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
 #pragma unroll
         for (int jj = 0; jj < 5; ++jj)
-          if (j < nup) tsum += mrow[j][jj] * xv[j][jj];
+          if (j < nup) tsum += mrow[j][jj] * xw[j * 5 + jj];
     }
     // wait for the chunks this one depends on
     const int d0 = dep_off[ch], nd = dep_off[ch + 1] - d0;
@@ -1219,15 +1230,21 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool post = BACK ? (j < nup) : (j >= nup);
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj)
-        if (post) xv[j][jj] = ldx<true>(X + (size_t)nb[j] * 5 + jj);
+      if (post) x1[j] = ldx<true>(X + (size_t)nb[j] * 5 + i);
     }
+    if (q < CPW) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (BACK || j >= nup) xw[j * 5 + i] = x1[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj)
-        if (BACK || j >= nup) tsum += mrow[j][jj] * xv[j][jj];
+        if (BACK || j >= nup) tsum += mrow[j][jj] * xw[j * 5 + jj];
     if (act)
       for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
         const F *M = off + (size_t)(m0 + j) * 25;
@@ -1253,7 +1270,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
     for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
     if (act) {
       double x[5];
-      lusolve5<F>(a, pk, v, x);
+      lusolve5<F>(aw, pk, v, x);
       const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
       stx<true>(X + (size_t)c * 5 + i, xi);
     }
@@ -1289,6 +1306,12 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
   const int gbase = 5 * (q < CPW ? q : 0);
+  // component i of each neighbour's X and row i of the diagonal factors,
+  // shared by the cell's five lanes after the wait
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ DG al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  DG *aw = al + (q < CPW ? q : 0) * 25;
   for (int t = blockIdx.x; t < nchunks; t += nwaves) {
     const int ch = BACK ? nchunks - 1 - t : t;
     const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
@@ -1305,8 +1328,8 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)cc * MAXR + j];
     F mrow[MAXR][5];
-    DG a[25];
-    double xv[MAXR][5];
+    DG a5[5];
+    double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       int jr = j + nup;
@@ -1317,22 +1340,16 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
 #pragma unroll
-    for (int i2 = 0; i2 < 25; ++i2) a[i2] = diag[(size_t)cc * 25 + i2];
+    for (int k = 0; k < 5; ++k) a5[k] = diag[(size_t)cc * 25 + 5 * i + k];
     // values this half-sweep does not write: upper (forward) / lower (backward), and pads
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj) xv[j][jj] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + jj) : 0.0;
+      x1[j] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + i) : 0.0;
     }
     double xold = 0.0;
     if constexpr (BACK) {
       if (xt) xold = ldx<true>(X + (size_t)cc * 5 + i);
-#pragma unroll
-      for (int j = 0; j < MAXR; ++j)
-#pragma unroll
-        for (int jj = 0; jj < 5; ++jj)
-          if (j < nlow) tsum += mrow[j][jj] * xv[j][jj];
     }
     // turbulence scalar sc = i of this cell
     const bool sact = act && i < ls.ns && ((ls.mask >> i) & 1);
@@ -1358,6 +1375,19 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
           if (j < nlow) sres -= sxv[j] * sco[j];
       }
     }
+    if constexpr (BACK) {  // the lower terms are final since the forward sweep
+      if (q < CPW) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nlow) xw[j * 5 + i] = x1[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+          if (j < nlow) tsum += mrow[j][jj] * xw[j * 5 + jj];
+    }
     const int d0 = dep_off[ch], nd = dep_off[ch + 1] - d0;
     const int dl = lane < nd ? dep[d0 + lane] : -1;
     int spins = 0;
@@ -1373,15 +1403,21 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj)
-        if (post) xv[j][jj] = ldx<true>(X + (size_t)nb[j] * 5 + jj);
+      if (post) x1[j] = ldx<true>(X + (size_t)nb[j] * 5 + i);
     }
+    if (q < CPW) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (!BACK || j >= nlow) xw[j * 5 + i] = x1[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj)
-        if (!BACK || j >= nlow) tsum += mrow[j][jj] * xv[j][jj];
+        if (!BACK || j >= nlow) tsum += mrow[j][jj] * xw[j * 5 + jj];
     if (fact)
       for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
         int jr = j + nup;
@@ -1416,7 +1452,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         if (xt) xt[(size_t)c * 5 + i] = xold;
       }
       double x[5];
-      lusolve5<DG>(a, PIV_ID, v, x);
+      lusolve5<DG>(aw, PIV_ID, v, x);
       const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
       stx<true>(X + (size_t)c * 5 + i, xi);
     }
