@@ -1018,64 +1018,14 @@ __global__ void __launch_bounds__(TPB) k_cell_assemble(DevGrid g, Prm p, const d
 // X entry), so every block and neighbour load of a row issues at once.
 // ---------------------------------------------------------------------------
 // Five lanes per cell: lane i of the group owns equation i.  Each lane
-// accumulates row i of every block of the row against the neighbour's whole
-// X (the five lanes read the same X words), the five partial sums are
-// exchanged with cross-lane reads, and every lane of the group repeats the
-// small LU solve and keeps its own component.  The summation order of each
-// t_i is the one-lane order (blocks in row order, columns ascending), so the
-// result is bitwise the same as the sequential sweep.
-
-template <typename F, bool COHERENT>
-__device__ __forceinline__ void sgs_cell5(const DevGrid &g, int c, int i, int gbase, bool act,
-                                          const F *__restrict__ off, const F *__restrict__ lu,
-                                          const int *__restrict__ piv, const F *__restrict__ B,
-                                          double *__restrict__ X) {
-  double t = 0.0;
-  int pk = 0;
-  F a[25];
-  if (act) {
-    t = -(double)B[(size_t)c * 5 + i];
-    pk = piv[c];
-    const int m0 = g.m_off[c];
-    const int len = g.m_off[c + 1] - m0;
-    int nb[MAXR];
-#pragma unroll
-    for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)c * MAXR + j];
-    F mrow[MAXR][5];
-    double xv[MAXR][5];
-#pragma unroll
-    for (int j = 0; j < MAXR; ++j) {
-      const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
-      const F *M = off + (size_t)slot * 25;
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj) {
-        mrow[j][jj] = M[i + 5 * jj];
-        xv[j][jj] = ldx<COHERENT>(X + (size_t)nb[j] * 5 + jj);
-      }
-    }
-#pragma unroll
-    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu[(size_t)c * 25 + i2];
-#pragma unroll
-    for (int j = 0; j < MAXR; ++j)
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
-    for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
-      const F *M = off + (size_t)(m0 + j) * 25;
-      const double *xn = X + (size_t)g.m_nbr[m0 + j] * 5;
-#pragma unroll
-      for (int jj = 0; jj < 5; ++jj) t += M[i + 5 * jj] * ldx<COHERENT>(xn + jj);
-    }
-  }
-  double v[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) v[k] = -__shfl(t, gbase + k, 64);
-  if (act) {
-    double x[5];
-    lusolve5<F>(a, pk, v, x);
-    const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
-    stx<COHERENT>(X + (size_t)c * 5 + i, xi);
-  }
-}
+// loads row i of every block of the row, component i of every neighbour's X
+// and row i of the diagonal factors; the group shares the X components and
+// the factors through LDS, each lane accumulates its row against the
+// neighbours' whole X, the five partial sums are exchanged with cross-lane
+// reads, and every lane of the group repeats the small LU solve and keeps
+// its own component.  The summation order of each t_i is the one-lane order
+// (blocks in row order, columns ascending), so the result is bitwise the
+// same as the sequential sweep.
 
 // one level of a forward or backward sweep; every cell of a level depends only
 // on cells of other levels
@@ -1083,11 +1033,65 @@ template <typename F>
 __global__ void __launch_bounds__(64) k_sgs_level(DevGrid g, int lo, int hi, const F *__restrict__ off,
                                                   const F *__restrict__ lu, const int *__restrict__ piv,
                                                   const F *__restrict__ B, double *__restrict__ X) {
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ F al[CPW * 25];
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
+  const int gbase = 5 * (q < CPW ? q : 0);
   const int c = lo + blockIdx.x * CPW + q;
   const bool act = q < CPW && c < hi;
-  sgs_cell5<F, false>(g, c, i, 5 * (q < CPW ? q : 0), act, off, lu, piv, B, X);
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  F *aw = al + (q < CPW ? q : 0) * 25;
+  double t = 0.0;
+  int pk = 0, m0 = 0, len = 0;
+  F mrow[MAXR][5];
+  if (act) {
+    t = -(double)B[(size_t)c * 5 + i];
+    pk = piv[c];
+    m0 = g.m_off[c];
+    len = g.m_off[c + 1] - m0;
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)c * MAXR + j];
+    double x1[MAXR];
+    F a5[5];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+      x1[j] = X[(size_t)nb[j] * 5 + i];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a5[k] = lu[(size_t)c * 25 + 5 * i + k];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) xw[j * 5 + i] = x1[j];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xw[j * 5 + jj];
+    for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+      const F *M = off + (size_t)(m0 + j) * 25;
+      const double *xn = X + (size_t)g.m_nbr[m0 + j] * 5;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) t += M[i + 5 * jj] * xn[jj];
+    }
+  }
+  double v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = -__shfl(t, gbase + k, 64);
+  if (act) {
+    double x[5];
+    lusolve5<F>(aw, pk, v, x);
+    const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+    X[(size_t)c * 5 + i] = xi;
+  }
 }
 
 // One whole half-sweep in a single cooperative launch of W one-wave
