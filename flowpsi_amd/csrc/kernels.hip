@@ -1502,20 +1502,24 @@ __global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restr
   const int k = lo + blockIdx.x * CPW + q;
   const bool act = q < CPW && k < hi;
   const int gbase = 5 * (q < CPW ? q : 0);
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ double al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  double *aw = al + (q < CPW ? q : 0) * 25;
   double t = 0.0;
-  double a[25];
-  int c = 0;
+  int c = 0, m0 = 0, len = 0, nup = 0;
+  F mrow[MAXR][5];
   if (act) {
     c = cells ? cells[k] : k;
     t = -(double)B[(size_t)c * 5 + i];
-    const int m0 = g.m_off[c];
-    const int len = g.m_off[c + 1] - m0;
-    const int nup = g.m_nup[c];
+    m0 = g.m_off[c];
+    len = g.m_off[c + 1] - m0;
+    nup = g.m_nup[c];
     int nb[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)c * MAXR + j];
-    F mrow[MAXR][5];
-    double xv[MAXR][5];
+    double x1[MAXR];
+    double a5[5];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       int jr = j + nup;
@@ -1523,17 +1527,22 @@ __global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restr
       const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
       const F *M = off + (size_t)slot * 25;
 #pragma unroll
-      for (int jj = 0; jj < 5; ++jj) {
-        mrow[j][jj] = M[i + 5 * jj];
-        xv[j][jj] = X[(size_t)nb[j] * 5 + jj];
-      }
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+      x1[j] = X[(size_t)nb[j] * 5 + i];
     }
 #pragma unroll
-    for (int i2 = 0; i2 < 25; ++i2) a[i2] = lu64[(size_t)c * 25 + i2];
+    for (int kk = 0; kk < 5; ++kk) a5[kk] = lu64[(size_t)c * 25 + 5 * i + kk];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) xw[j * 5 + i] = x1[j];
+#pragma unroll
+    for (int kk = 0; kk < 5; ++kk) aw[5 * i + kk] = a5[kk];
+  }
+  __syncthreads();
+  if (act) {
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
-      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xw[j * 5 + jj];
     for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
       int jr = j + nup;
       if (jr >= len) jr -= len;
@@ -1549,7 +1558,7 @@ __global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restr
   if (act) {
     if constexpr (BACK) xt[(size_t)c * 5 + i] = X[(size_t)c * 5 + i];
     double x[5];
-    lusolve5<double>(a, PIV_ID, v, x);
+    lusolve5<double>(aw, PIV_ID, v, x);
     const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
     X[(size_t)c * 5 + i] = xi;
   }
@@ -1588,20 +1597,24 @@ __global__ void __launch_bounds__(64) k_fsgs_level(DevGrid g, const int *__restr
   const int k = lo + blockIdx.x * CPW + q;
   const bool act = q < CPW && k < hi;
   const int gbase = 5 * (q < CPW ? q : 0);
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ F al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  F *aw = al + (q < CPW ? q : 0) * 25;
   double t = 0.0;
-  F a[25];
-  int c = 0;
+  int c = 0, m0 = 0, len = 0, nup = 0;
+  F mrow[MAXR][5];
   if (act) {
     c = cells ? cells[k] : k;
     t = -(double)B[(size_t)c * 5 + i];
-    const int m0 = g.m_off[c];
-    const int len = g.m_off[c + 1] - m0;
-    const int nup = g.m_nup[c];
+    m0 = g.m_off[c];
+    len = g.m_off[c + 1] - m0;
+    nup = g.m_nup[c];
     int nb[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)c * MAXR + j];
-    F mrow[MAXR][5];
-    double xv[MAXR][5];
+    double x1[MAXR];
+    F a5[5];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       int jr = j + nup;
@@ -1609,17 +1622,22 @@ __global__ void __launch_bounds__(64) k_fsgs_level(DevGrid g, const int *__restr
       const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
       const F *M = off + (size_t)slot * 25;
 #pragma unroll
-      for (int jj = 0; jj < 5; ++jj) {
-        mrow[j][jj] = M[i + 5 * jj];
-        xv[j][jj] = X[(size_t)nb[j] * 5 + jj];
-      }
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+      x1[j] = X[(size_t)nb[j] * 5 + i];
     }
 #pragma unroll
-    for (int i2 = 0; i2 < 25; ++i2) a[i2] = luf[(size_t)c * 25 + i2];
+    for (int kk = 0; kk < 5; ++kk) a5[kk] = luf[(size_t)c * 25 + 5 * i + kk];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) xw[j * 5 + i] = x1[j];
+#pragma unroll
+    for (int kk = 0; kk < 5; ++kk) aw[5 * i + kk] = a5[kk];
+  }
+  __syncthreads();
+  if (act) {
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
-      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xv[j][jj];
+      for (int jj = 0; jj < 5; ++jj) t += mrow[j][jj] * xw[j * 5 + jj];
     for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
       int jr = j + nup;
       if (jr >= len) jr -= len;
@@ -1634,7 +1652,7 @@ __global__ void __launch_bounds__(64) k_fsgs_level(DevGrid g, const int *__restr
   for (int kk = 0; kk < 5; ++kk) v[kk] = -__shfl(t, gbase + kk, 64);
   if (act) {
     double x[5];
-    lusolve5<F>(a, PIV_ID, v, x);
+    lusolve5<F>(aw, PIV_ID, v, x);
     const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
     X[(size_t)c * 5 + i] = xi;
   }
