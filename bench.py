@@ -195,7 +195,9 @@ def main():
         if tiled:
             kname = "k_sgs_tiles"          # one cooperative launch per half-sweep
         elif a.sweep == "persistent":
-            kname = "k_sgs_sweep"          # one cooperative launch per half-sweep
+            # one cooperative launch per half-sweep: flag-free for the fluid
+            # alone, with flags when the turbulence scalars ride along
+            kname = "k_sgs_flow" if prm.get("turb", "none") == "none" else "k_sgs_sweep"
         else:
             avg_s /= info["nlevels"]       # nlevels launches per half-sweep
             bytes_launch /= info["nlevels"]

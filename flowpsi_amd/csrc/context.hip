@@ -652,6 +652,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
       (rc = dalloc(c, &s.maxev, (size_t)nf)) || (rc = dalloc(c, &s.src, 5 * (size_t)nc)) ||
       (rc = dalloc(c, &s.dtau, (size_t)nc)) || (rc = dalloc(c, &s.piv, (size_t)nc)) ||
       (rc = dalloc(c, &s.X, 5 * ((size_t)nc + nper + 1))) ||
+      (rc = dalloc(c, &s.X2, 5 * ((size_t)nc + nper + 1))) ||
       (rc = dalloc(c, &s.npart, 12 * (size_t)norm_parts(g))) || (rc = dalloc(c, &s.nout, 12)) ||
       (rc = dalloc(c, &s.cflag, (size_t)std::max(c->nchunks, 1))) || (rc = dalloc(c, &s.serr, 1)) ||
       (rc = dalloc(c, &s.eta, (size_t)nc)) ||
@@ -679,6 +680,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
   s.B = tmp;
   HIPCHK(c, hipMemsetAsync(s.vflux, 0, 4 * (size_t)nf * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.X, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(s.X2, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(c->Dg, 0, 5 * ((size_t)nc + nper + 1) * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(s.cflag, 0, (size_t)std::max(c->nchunks, 1) * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(s.serr, 0, sizeof(int), c->stream));
@@ -1225,7 +1227,22 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0) {
     // a wait is bounded: far longer than any real one (~1e6 polls of a few
     // hundred ns each), so only a broken schedule reaches it
     const int spin_limit = 1 << 22;
-    for (int it = 0; it < nsweeps; ++it)
+    if (!ns) {
+      // fluid only: flag-free half-sweeps alternating between X and X2; an
+      // even number of them leaves the result in X
+      double *cur = c->s.X, *oth = c->s.X2;
+      for (int it = 0; it < nsweeps; ++it)
+        for (int dir = 0; dir < 2; ++dir) {
+          launch_sweep_prep(c->g, cur, oth, c->stream);
+          hipEvent_t e = prof_begin(c);
+          const hipError_t he = launch_sgs_flow(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, spin_limit, c->s.serr,
+                                                c->sweep_waves, c->s, cur, oth, c->stream);
+          prof_end(c, FPSI_K_SGS_SWEEP, e);
+          if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
+          std::swap(cur, oth);
+        }
+    }
+    for (int it = 0; it < (ns ? nsweeps : 0); ++it)
       for (int dir = 0; dir < 2; ++dir) {
         launch_ghost(c->g, c->s, c->stream);
         if (ns) launch_ghost1(c->g, c->nscal(), c->s.sX, c->stream);

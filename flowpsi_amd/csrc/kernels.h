@@ -23,6 +23,11 @@ int sgs_sweep_waves(int fp32, int ncu);
 hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo,
                             const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
                             int nwaves, int ns, long nnz, DevState &s, hipStream_t st);
+// the fluid-only dataflow half-sweep without flags: reads Xs, writes Xt,
+// which launch_sweep_prep has filled with sentinels (ghosts in both buffers)
+hipError_t launch_sgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
+                           int *err, int nwaves, DevState &s, const double *Xs, double *Xt, hipStream_t st);
+void launch_sweep_prep(const DevGrid &g, double *Xs, double *Xt, hipStream_t st);
 // line SGS (fsgs = 0) or FSGS (fsgs = 1) half-sweep as one cooperative
 // dataflow launch over the SGS chunks (valid when the breadth-first levels
 // are the SGS levels)

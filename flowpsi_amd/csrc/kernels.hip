@@ -1284,6 +1284,159 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
   }
 }
 
+// The same half-sweep without flags, for the fluid system alone.  X is
+// double-buffered: the source buffer Xs holds the values of the previous
+// half-sweep and the target Xt starts as the sentinel X_SENT (a signalling
+// NaN, which no arithmetic produces) on every interior row, with the lagged
+// periodic ghosts and the zero pad row in place.  A lane reads component i
+// of each neighbour the sweep has not reached (upper forward, lower
+// backward) from Xs, and polls component i of each neighbour it has reached
+// in Xt until no lane of the wave sees a sentinel: an aligned 8-byte store
+// is seen whole, so a non-sentinel word is that neighbour's final value.
+// The wait then costs one poll round trip on the neighbours' own words, and
+// the producer neither waits for its stores nor publishes a flag.  Chunks,
+// order and summation are those of k_sgs_sweep, so the result is bitwise
+// the same; the wave holding the smallest unfinished chunk always proceeds,
+// and a wait longer than spin_limit polls (or an error raised by another
+// wave) ends the launch with *err set.
+constexpr unsigned long long X_SENT = 0x7FF45EE5DEAD0001ull;
+
+__device__ __forceinline__ bool is_sent(double v) {
+  return (unsigned long long)__double_as_longlong(v) == X_SENT;
+}
+
+template <typename F, bool BACK>
+__global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
+                                                 int spin_limit, int *__restrict__ err, const F *__restrict__ off,
+                                                 const F *__restrict__ lu, const int *__restrict__ piv,
+                                                 const F *__restrict__ B, const double *__restrict__ Xs,
+                                                 double *__restrict__ Xt) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int gbase = 5 * (q < CPW ? q : 0);
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ F al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  F *aw = al + (q < CPW ? q : 0) * 25;
+  for (int t = blockIdx.x; t < nchunks; t += nwaves) {
+    const int ch = BACK ? nchunks - 1 - t : t;
+    const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
+    const int c = lo + q;
+    const bool act = q < CPW && c < hi;
+    const int cc = act ? c : lo;
+    double tsum = -(double)B[(size_t)cc * 5 + i];
+    const int pk = piv[cc];
+    const int m0 = g.m_off[cc];
+    const int len = g.m_off[cc + 1] - m0;
+    const int nup = g.m_nup[cc];
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)cc * MAXR + j];
+    F mrow[MAXR][5], a5[5];
+    double x1[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const int slot = len > 0 ? m0 + (j < len ? j : len - 1) : 0;
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a5[k] = lu[(size_t)cc * 25 + 5 * i + k];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool pre = BACK ? (j >= nup) : (j < nup);
+      x1[j] = pre ? ldx<true>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
+    }
+    if (!BACK) {  // forward, the upper terms come first
+      if (q < CPW) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nup) xw[j * 5 + i] = x1[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+          if (j < nup) tsum += mrow[j][jj] * xw[j * 5 + jj];
+    }
+    int spins = 0;
+    while (true) {
+      bool ready = true;
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = BACK ? (j < nup) : (j >= nup);
+        if (post) {
+          x1[j] = ldx<true>(Xt + (size_t)nb[j] * 5 + i);
+          ready = ready && !is_sent(x1[j]);
+        }
+      }
+      if (__all(ready)) break;
+      if (++spins > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    }
+    if (q < CPW) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (BACK || j >= nup) xw[j * 5 + i] = x1[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (BACK || j >= nup) tsum += mrow[j][jj] * xw[j * 5 + jj];
+    if (act)
+      for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+        const F *M = off + (size_t)(m0 + j) * 25;
+        const bool post = BACK ? (j < nup) : (j >= nup);
+        const double *xn = (post ? (const double *)Xt : Xs) + (size_t)g.m_nbr[m0 + j] * 5;
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj) {
+          double xv = ldx<true>(xn + jj);
+          for (int sp = 0; post && is_sent(xv); ++sp) {
+            if (sp > spin_limit) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              xv = 0.0;
+              break;
+            }
+            xv = ldx<true>(xn + jj);
+          }
+          tsum += M[i + 5 * jj] * xv;
+        }
+      }
+    double v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
+    if (act) {
+      double x[5];
+      lusolve5<F>(aw, pk, v, x);
+      const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+      stx<true>(Xt + (size_t)c * 5 + i, xi);
+    }
+  }
+}
+
+// the target buffer of a flag-free half-sweep: sentinels on the interior
+// rows; the lagged periodic ghosts of the latest values in both buffers
+__global__ void k_sweep_prep(DevGrid g, double *__restrict__ Xs, double *__restrict__ Xt) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = 5L * g.nc;
+  if (e < n) Xt[e] = __longlong_as_double((long long)X_SENT);
+  if (e < 5L * g.nper) {
+    const int pi = (int)(e / 5), k = (int)(e - 5L * pi);
+    const double v = Xs[(size_t)g.frc[g.nfi + pi] * 5 + k];
+    Xs[(size_t)(g.nc + pi) * 5 + k] = v;
+    Xt[(size_t)(g.nc + pi) * 5 + k] = v;
+  }
+}
+
 // Line-SGS / FSGS half-sweep as one cooperative dataflow launch: the
 // chunks, dependency lists and flags of k_sgs_sweep (valid when the
 // breadth-first levels are the SGS levels), the per-cell update of
@@ -2543,6 +2696,14 @@ int sgs_sweep_waves(int fp32, int ncu) {
            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sgs_sweep<double, true>, 64, 0);
   if (e != hipSuccess) return 0;
   per_cu = std::min(a, b);
+  // the flag-free sweep runs with the same wave count
+  e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_sgs_flow<float, false>, 64, 0)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_sgs_flow<double, false>, 64, 0);
+  if (e != hipSuccess) return 0;
+  e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sgs_flow<float, true>, 64, 0)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sgs_flow<double, true>, 64, 0);
+  if (e != hipSuccess) return 0;
+  per_cu = std::min(per_cu, std::min(a, b));
   if (per_cu <= 0) return 0;
   return per_cu * ncu;
 }
@@ -2589,6 +2750,19 @@ hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunk
   const void *fn = fp32 ? (backward ? (const void *)k_sgs_sweep<float, true> : (const void *)k_sgs_sweep<float, false>)
                         : (backward ? (const void *)k_sgs_sweep<double, true> : (const void *)k_sgs_sweep<double, false>);
   return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
+}
+hipError_t launch_sgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
+                           int *err, int nwaves, DevState &s, const double *Xs, double *Xt, hipStream_t st) {
+  int nw = std::max(1, std::min(nwaves, nchunks));
+  void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &spin_limit, &err, (void *)&s.off, (void *)&s.lu,
+                  (void *)&s.piv, (void *)&s.B, (void *)&Xs, (void *)&Xt};
+  const void *fn = fp32 ? (backward ? (const void *)k_sgs_flow<float, true> : (const void *)k_sgs_flow<float, false>)
+                        : (backward ? (const void *)k_sgs_flow<double, true> : (const void *)k_sgs_flow<double, false>);
+  return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
+}
+void launch_sweep_prep(const DevGrid &g, double *Xs, double *Xt, hipStream_t st) {
+  const long n = 5L * std::max(g.nc, g.nper);
+  hipLaunchKernelGGL(k_sweep_prep, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, g, Xs, Xt);
 }
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);

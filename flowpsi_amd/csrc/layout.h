@@ -62,6 +62,7 @@ struct DevState {
   int *piv;       // [nc] packed 3 bits per row
   void *B;        // [nc][5] real_fj
   double *X;      // [nc + nper + 1][5], last entry stays zero
+  double *X2;     // second buffer of the flag-free dataflow sweep, same shape
   double *npart;  // norm partials
   double *nout;   // final norms
   int *cflag;     // [nchunks] epoch each sweep chunk last finished (dataflow sweep)
