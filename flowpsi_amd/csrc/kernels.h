@@ -26,8 +26,10 @@ hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunk
 // the fluid-only dataflow half-sweep without flags: reads Xs, writes Xt,
 // which launch_sweep_prep has filled with sentinels (ghosts in both buffers)
 hipError_t launch_sgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
-                           int *err, int nwaves, DevState &s, const double *Xs, double *Xt, hipStream_t st);
-void launch_sweep_prep(const DevGrid &g, double *Xs, double *Xt, hipStream_t st);
+                           int *err, int nwaves, DevState &s, double *Xs, double *Xt, hipStream_t st);
+// fill = 1 before the first half-sweep of a solve; the sweep leaves the
+// sentinels in its source buffer for the next one
+void launch_sweep_prep(const DevGrid &g, int fill, double *Xs, double *Xt, hipStream_t st);
 // line SGS (fsgs = 0) or FSGS (fsgs = 1) half-sweep as one cooperative
 // dataflow launch over the SGS chunks (valid when the breadth-first levels
 // are the SGS levels)

@@ -1294,7 +1294,9 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 // in Xt until no lane of the wave sees a sentinel: an aligned 8-byte store
 // is seen whole, so a non-sentinel word is that neighbour's final value.
 // The wait then costs one poll round trip on the neighbours' own words, and
-// the producer neither waits for its stores nor publishes a flag.  Chunks,
+// the producer neither waits for its stores nor publishes a flag; after its
+// store it writes the sentinel over its own word of Xs, which no unfinished
+// cell still reads, so that buffer is ready as the next half-sweep's target.  Chunks,
 // order and summation are those of k_sgs_sweep, so the result is bitwise
 // the same; the wave holding the smallest unfinished chunk always proceeds,
 // and a wait longer than spin_limit polls (or an error raised by another
@@ -1309,7 +1311,7 @@ template <typename F, bool BACK>
 __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                  int spin_limit, int *__restrict__ err, const F *__restrict__ off,
                                                  const F *__restrict__ lu, const int *__restrict__ piv,
-                                                 const F *__restrict__ B, const double *__restrict__ Xs,
+                                                 const F *__restrict__ B, double *__restrict__ Xs,
                                                  double *__restrict__ Xt) {
   const int lane = threadIdx.x;
   const int q = lane / 5, i = lane - 5 * q;
@@ -1419,16 +1421,22 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
       lusolve5<F>(aw, pk, v, x);
       const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
       stx<true>(Xt + (size_t)c * 5 + i, xi);
+      // every cell that reads this word of Xs (a neighbour the sweep had not
+      // reached when it ran) finished before this cell's poll succeeded, so
+      // the word becomes the next half-sweep's sentinel
+      stx<true>(Xs + (size_t)c * 5 + i, __longlong_as_double((long long)X_SENT));
     }
   }
 }
 
 // the target buffer of a flag-free half-sweep: sentinels on the interior
-// rows; the lagged periodic ghosts of the latest values in both buffers
-__global__ void k_sweep_prep(DevGrid g, double *__restrict__ Xs, double *__restrict__ Xt) {
+// rows (fill, needed before the first half-sweep of a solve: each later one
+// finds them left by the previous half-sweep); the lagged periodic ghosts
+// of the latest values in both buffers
+__global__ void k_sweep_prep(DevGrid g, int fill, double *__restrict__ Xs, double *__restrict__ Xt) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long n = 5L * g.nc;
-  if (e < n) Xt[e] = __longlong_as_double((long long)X_SENT);
+  if (fill && e < n) Xt[e] = __longlong_as_double((long long)X_SENT);
   if (e < 5L * g.nper) {
     const int pi = (int)(e / 5), k = (int)(e - 5L * pi);
     const double v = Xs[(size_t)g.frc[g.nfi + pi] * 5 + k];
@@ -2752,7 +2760,7 @@ hipError_t launch_sgs_sweep(const DevGrid &g, int fp32, int backward, int nchunk
   return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
 }
 hipError_t launch_sgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
-                           int *err, int nwaves, DevState &s, const double *Xs, double *Xt, hipStream_t st) {
+                           int *err, int nwaves, DevState &s, double *Xs, double *Xt, hipStream_t st) {
   int nw = std::max(1, std::min(nwaves, nchunks));
   void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &spin_limit, &err, (void *)&s.off, (void *)&s.lu,
                   (void *)&s.piv, (void *)&s.B, (void *)&Xs, (void *)&Xt};
@@ -2760,9 +2768,9 @@ hipError_t launch_sgs_flow(const DevGrid &g, int fp32, int backward, int nchunks
                         : (backward ? (const void *)k_sgs_flow<double, true> : (const void *)k_sgs_flow<double, false>);
   return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
 }
-void launch_sweep_prep(const DevGrid &g, double *Xs, double *Xt, hipStream_t st) {
-  const long n = 5L * std::max(g.nc, g.nper);
-  hipLaunchKernelGGL(k_sweep_prep, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, g, Xs, Xt);
+void launch_sweep_prep(const DevGrid &g, int fill, double *Xs, double *Xt, hipStream_t st) {
+  const long n = 5L * (fill ? std::max(g.nc, g.nper) : g.nper);
+  if (n > 0) hipLaunchKernelGGL(k_sweep_prep, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, g, fill, Xs, Xt);
 }
 void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st) {
   if (g.nper) hipLaunchKernelGGL(k_ghost, dim3(nblk(g.nper)), dim3(TPB), 0, st, g, s.X);
