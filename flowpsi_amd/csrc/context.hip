@@ -73,6 +73,9 @@ struct fpsi_ctx {
   const int *lsgs_cells = nullptr;
   bool lsgs_alloc = false;
   double *lsgs_host = nullptr;  // pinned mirror of DevState::lst
+  // pinned readback slots: [0, 12) step norms, [12] linear residual, [13]
+  // the sweep error word (as int), so several copies share one sync
+  double *rb_host = nullptr;
   int lsgs_iters = 0;
   double lsgs_w = 0, lsgs_resid = 0;
   int sa_lsgs_iters[2] = {0, 0};  // per turbulence scalar
@@ -172,6 +175,7 @@ extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void *p : ctx->allocs) hipFree(p);
   if (ctx->lsgs_host) hipHostFree(ctx->lsgs_host);
+  if (ctx->rb_host) hipHostFree(ctx->rb_host);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -564,6 +568,7 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
 
   // ---- device upload
   HIPCHK(c, hipStreamCreate(&c->stream));
+  HIPCHK(c, hipHostMalloc((void **)&c->rb_host, 16 * sizeof(double), hipHostMallocDefault));
   int sweep_ncu = 1, sweep_cap = 0;
   {
     int dev = 0, ncu = 0;
@@ -1215,7 +1220,8 @@ extern "C" int fpsi_get_system(fpsi_ctx *c, void *fjp, void *fjm, void *D, int *
 
 // ns > 0 (dataflow mode only): the turbulence scalar systems are swept in
 // the same launches as the fluid system
-static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0) {
+// check = false leaves the dataflow error word to the caller's next readback
+static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = true) {
   const int nlev = (int)c->lev_off.size() - 1;
   if (c->tiled()) {
     for (int it = 0; it < nsweeps; ++it)
@@ -1260,7 +1266,7 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0) {
         prof_end(c, FPSI_K_SGS_SWEEP, e);
         if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
       }
-    if (nsweeps > 0) {
+    if (nsweeps > 0 && check) {
       int err = 0;
       HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1679,14 +1685,17 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
 // ---------------------------------------------------------------------------
 // linear residual, tolerance-driven solve, learned guess
 // ---------------------------------------------------------------------------
+// the sweep error word travels with the residual, so sweeps enqueued
+// without their own check (solve_tol) are checked here
 static int linres(fpsi_ctx *c, double *out) {
   launch_ghost(c->g, c->s, c->stream);  // the periodic couplings read current ghosts
   launch_linres(c->g, c->fp32, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
-  double v = 0;
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(&v, c->lr_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->rb_host + 12, c->lr_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->rb_host + 13, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  *out = std::sqrt(v);
+  if (*(const int *)(c->rb_host + 13)) return fail(c, FPSI_ERR_HIP, "dataflow sweep: a dependency wait exceeded its limit");
+  *out = std::sqrt(c->rb_host[12]);
   return 0;
 }
 
@@ -1762,7 +1771,7 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
     }
   }
   while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
-    if (int rc = enqueue_sweeps(c, 1)) return rc;
+    if (int rc = enqueue_sweeps(c, 1, 0, false)) return rc;  // linres checks the sweep
     ++s.sweeps;
     if (int rc = linres(c, &rcur)) return rc;
   }
@@ -1788,14 +1797,15 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
   if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
-    double o[12];
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(o, c->s.nout, sizeof(o), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (hist) norms_from(o, &hist[s]);
-    if (o[0] + o[1] + o[2] != o[0] + o[1] + o[2]) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
+    // the norms arrive with the linear solve's first residual readback; a
+    // NaN step is refused before its update
+    HIPCHK(c, hipMemcpyAsync(c->rb_host, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     fpsi_lin_stats st{};
     if (int rc = solve_tol(c, rtol, max_sweeps, use_guess, &st)) return rc;
+    const double *o = c->rb_host;
+    if (hist) norms_from(o, &hist[s]);
+    if (o[0] + o[1] + o[2] != o[0] + o[1] + o[2]) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
     if (lin) lin[s] = st;
     // turbulence scalars: the reference's fixed-sweep scalar solve
     if (c->kp.turb)
