@@ -1302,11 +1302,11 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 // The wait then costs one poll round trip on the neighbours' own words, and
 // the producer neither waits for its stores nor publishes a flag; after its
 // store it writes the sentinel over its own word of Xs, which no unfinished
-// cell still reads, so that buffer is ready as the next half-sweep's target.  Chunks,
-// order and summation are those of k_sgs_sweep, so the result is bitwise
-// the same; the wave holding the smallest unfinished chunk always proceeds,
-// and a wait longer than spin_limit polls (or an error raised by another
-// wave) ends the launch with *err set.
+// cell still reads, so that buffer is ready as the next half-sweep's
+// target.  Chunks, order and summation are those of k_sgs_sweep, so the
+// result is bitwise the same; the wave holding the smallest unfinished chunk
+// always proceeds, and a wait longer than spin_limit polls (or an error
+// raised by another wave) ends the launch with *err set.
 constexpr unsigned long long X_SENT = 0x7FF45EE5DEAD0001ull;
 
 __device__ __forceinline__ bool is_sent(double v) {
