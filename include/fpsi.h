@@ -56,8 +56,10 @@ enum { FPSI_TIME_STEADY = 0, FPSI_TIME_BDF2 = 1, FPSI_TIME_KEC = 2 };
 /* How a Gauss-Seidel half-sweep runs (same result bit for bit):
  * PER_LEVEL  one launch per dependency level (default);
  * PERSISTENT one cooperative launch per half-sweep run as a dataflow: each
- *            chunk of <= 12 cells waits only for the chunks holding its
- *            dependencies (per-chunk flags);
+ *            chunk of <= 12 cells waits only for the cells it depends on
+ *            (the fluid alone: by polling their values in a second X
+ *            buffer that starts as sentinels; with turbulence scalars in
+ *            the launch: per-chunk flags);
  * TILED      one cooperative launch over tiles of <= 32 cells whose levels run
  *            in LDS (needs a grid whose cell boxes form an acyclic tile graph;
  *            fpsi_info.sweep_tiles > 0). */
