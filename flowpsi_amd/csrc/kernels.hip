@@ -1017,6 +1017,17 @@ __global__ void __launch_bounds__(TPB) k_cell_assemble(DevGrid g, Prm p, const d
 // Rows are read through a fixed-width neighbour list (pads point at a zero
 // X entry), so every block and neighbour load of a row issues at once.
 // ---------------------------------------------------------------------------
+// LDS exchange inside a one-wave workgroup: a wave's LDS operations
+// complete in issue order, so a lane's LDS read sees every LDS write the
+// wave issued before it; the wavefront-scope fences only keep the compiler
+// from moving memory operations across this point.  Unlike __syncthreads it
+// does not wait for the wave's outstanding global loads and stores.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Five lanes per cell: lane i of the group owns equation i.  Each lane
 // loads row i of every block of the row, component i of every neighbour's X
 // and row i of the diagonal factors; the group shares the X components and
@@ -1075,7 +1086,7 @@ __global__ void __launch_bounds__(64) k_sgs_level(DevGrid g, int lo, int hi, con
 #pragma unroll
     for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
   }
-  __syncthreads();
+  wave_sync();
   if (act) {
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
@@ -1217,7 +1228,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
         for (int j = 0; j < MAXR; ++j)
           if (j < nup) xw[j * 5 + i] = x1[j];
       }
-      __syncthreads();
+      wave_sync();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1249,7 +1260,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
       for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1362,7 +1373,7 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
         for (int j = 0; j < MAXR; ++j)
           if (j < nup) xw[j * 5 + i] = x1[j];
       }
-      __syncthreads();
+      wave_sync();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1394,7 +1405,7 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
 #pragma unroll
       for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1552,7 +1563,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         for (int j = 0; j < MAXR; ++j)
           if (j < nlow) xw[j * 5 + i] = x1[j];
       }
-      __syncthreads();
+      wave_sync();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1583,7 +1594,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
       for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
 #pragma unroll
@@ -1704,7 +1715,7 @@ __global__ void __launch_bounds__(64) k_lsgs_level(DevGrid g, const int *__restr
 #pragma unroll
     for (int kk = 0; kk < 5; ++kk) aw[5 * i + kk] = a5[kk];
   }
-  __syncthreads();
+  wave_sync();
   if (act) {
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
@@ -1799,7 +1810,7 @@ __global__ void __launch_bounds__(64) k_fsgs_level(DevGrid g, const int *__restr
 #pragma unroll
     for (int kk = 0; kk < 5; ++kk) aw[5 * i + kk] = a5[kk];
   }
-  __syncthreads();
+  wave_sync();
   if (act) {
 #pragma unroll
     for (int j = 0; j < MAXR; ++j)
