@@ -1,6 +1,6 @@
 """Parity of the HIP Newton step (through the C ABI) against the CPU oracle.
 
-Tolerances (BASELINE north star): per-face fluxes and per-cell residuals to
+Tolerances: per-face fluxes and per-cell residuals to
 1e-12 relative, the state after N Newton steps to 1e-10 relative, in FP64.
 A per-cell residual is a sum of nearly cancelling face fluxes, so its error
 is measured against the sum of the magnitudes of the fluxes that enter it.
