@@ -62,7 +62,8 @@ class Params(C.Structure):
                 ("etaT", C.c_double), ("PLimPC", C.c_double), ("turb", C.c_int), ("Prt", C.c_double),
                 ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double),
                 ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double),
-                ("pbias_control", C.c_double)]
+                ("pbias_control", C.c_double), ("sweep_spin_limit", C.c_int),
+                ("temporal_damping", C.c_double)]
 
 
 class Norms(C.Structure):
@@ -166,6 +167,8 @@ def make_params(prm: dict) -> Params:
     s.newton_iter = int(prm.get("newton_iter", 6))
     s.pseudo_cflmax = float(prm.get("pseudo_cflmax", 10.0))
     s.pbias_control = float(prm.get("pbias_control", 0.0))
+    s.sweep_spin_limit = int(prm.get("sweep_spin_limit", 0))  # <= 0: the library's bound
+    s.temporal_damping = float(prm.get("temporal_damping", 0.0))  # temporalDamping (<= 0: not set)
     return s
 
 

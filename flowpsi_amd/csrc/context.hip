@@ -118,6 +118,32 @@ static int fail(fpsi_ctx *c, int code, const std::string &m) {
       return fail(ctx, FPSI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// A dependency wait of a dataflow sweep ran past its spin limit: the launch
+// ended early and X holds a partial result.  The error word is cleared so
+// that later launches start clean, and the system must be assembled again
+// (which rewrites the Jacobi start; the next flag-free solve refills its
+// sentinels) before the next solve.
+static int sweep_failed(fpsi_ctx *c, const char *what) {
+  hipMemsetAsync(c->s.serr, 0, sizeof(int), c->stream);
+  hipStreamSynchronize(c->stream);
+  c->assembled = false;
+  return fail(c, FPSI_ERR_HIP, std::string(what) + ": a dependency wait exceeded its limit");
+}
+
+// read the dataflow error word after the sweeps enqueued so far
+static int check_sweep_error(fpsi_ctx *c, const char *what) {
+  int err = 0;
+  if (hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, FPSI_ERR_HIP, std::string(what) + ": reading the sweep error word failed");
+  return err ? sweep_failed(c, what) : 0;
+}
+
+// polls of one dependency wait before a dataflow sweep gives up: far longer
+// than any real wait (~1e6 polls of a few hundred ns each), so only a broken
+// schedule reaches it; fpsi_params.sweep_spin_limit overrides it
+static int spin_limit(const fpsi_ctx *c) { return c->prm.sweep_spin_limit > 0 ? c->prm.sweep_spin_limit : 1 << 22; }
+
 template <typename T>
 static int dalloc(fpsi_ctx *c, T **p, size_t n) {
   void *v = nullptr;
@@ -842,6 +868,7 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
   k.unsteady = p->time_mode == FPSI_TIME_KEC ? 2 : p->time_mode == FPSI_TIME_BDF2 ? 1 : 0;
   k.pseudo_cflmax = p->pseudo_cflmax > 0 ? p->pseudo_cflmax : 10.0;
   k.pbias_ctl = p->pbias_control;
+  k.tdamp = p->temporal_damping;
   c->fp32 = p->precision == FPSI_PREC_FP32;
   c->have_params = true;
   c->assembled = false;
@@ -1235,9 +1262,7 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = tru
     return 0;
   }
   if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
-    // a wait is bounded: far longer than any real one (~1e6 polls of a few
-    // hundred ns each), so only a broken schedule reaches it
-    const int spin_limit = 1 << 22;
+    const int spin = spin_limit(c);
     if (!ns) {
       // fluid only: flag-free half-sweeps alternating between X and X2; an
       // even number of them leaves the result in X
@@ -1246,7 +1271,7 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = tru
         for (int dir = 0; dir < 2; ++dir) {
           launch_sweep_prep(c->g, it == 0 && dir == 0, cur, oth, c->stream);
           hipEvent_t e = prof_begin(c);
-          const hipError_t he = launch_sgs_flow(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, spin_limit, c->s.serr,
+          const hipError_t he = launch_sgs_flow(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, spin, c->s.serr,
                                                 c->flow_waves, c->s, cur, oth, c->stream);
           prof_end(c, FPSI_K_SGS_SWEEP, e);
           if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
@@ -1261,17 +1286,12 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = tru
         const int epoch = ++c->flow_epoch;
         const hipError_t he =
             launch_sgs_sweep(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, dir ? c->bdep_off : c->fdep_off,
-                             dir ? c->bdep : c->fdep, c->s.cflag, epoch, spin_limit, c->s.serr, c->sweep_waves,
+                             dir ? c->bdep : c->fdep, c->s.cflag, epoch, spin, c->s.serr, c->sweep_waves,
                              ns, c->nnz, c->s, c->stream);
         prof_end(c, FPSI_K_SGS_SWEEP, e);
         if (he != hipSuccess) return fail(c, FPSI_ERR_HIP, std::string("cooperative sweep launch: ") + hipGetErrorString(he));
       }
-    if (nsweeps > 0 && check) {
-      int err = 0;
-      HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      if (err) return fail(c, FPSI_ERR_HIP, "dataflow sweep: a dependency wait exceeded its limit");
-    }
+    if (nsweeps > 0 && check) return check_sweep_error(c, "dataflow sweep");
     return 0;
   }
   for (int it = 0; it < nsweeps; ++it) {
@@ -1304,7 +1324,6 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
   // one cooperative dataflow launch per half-sweep where the levels allow
   // (bitwise equal to the level launches), else one launch per level
   const bool flow = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT && c->lsgs_flow;
-  const int spin_limit = 1 << 22;
   for (int it = 0; it < maxiter; ++it) {
     launch_ghost(c->g, c->s, c->stream);
     for (int dir = 0; dir < 2; ++dir) {
@@ -1312,7 +1331,7 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
       if (flow) {
         const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
                                                 dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, spin_limit, c->s.serr, c->lsgs_waves, 1, 0, 0,
+                                                ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, 1, 0, 0,
                                                 c->nnz, c->s, c->stream);
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
@@ -1336,12 +1355,7 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
     c->lsgs_iters = (int)c->lsgs_host[3];
     if (c->lsgs_host[4] != 0.0) break;
   }
-  if (flow) {
-    int err = 0;
-    HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (err) return fail(c, FPSI_ERR_HIP, "dataflow line sweep: a dependency wait exceeded its limit");
-  }
+  if (flow) return check_sweep_error(c, "dataflow line sweep");
   return 0;
 }
 
@@ -1371,7 +1385,7 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
       hipEvent_t e = prof_begin(c);
       const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
                                               dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                              ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, fluid ? 1 : 0, ns,
+                                              ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, fluid ? 1 : 0, ns,
                                               mask, c->nnz, c->s, c->stream);
       prof_end(c, FPSI_K_SGS_SWEEP, e);
       if (he != hipSuccess)
@@ -1399,11 +1413,7 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
         if (h[4] != 0.0) mask &= ~(1 << sc);
       }
   }
-  int err = 0;
-  HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (err) return fail(c, FPSI_ERR_HIP, "dataflow line sweep: a dependency wait exceeded its limit");
-  return 0;
+  return check_sweep_error(c, "dataflow line sweep");
 }
 
 // FSGSBlockedSSolve (src/linearSolverFSGS.loci:1598-1708): X from zero, then
@@ -1420,7 +1430,7 @@ static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
       if (flow) {
         const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 1, dir, c->nchunks, c->chunk_lo,
                                                 dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, 1 << 22, c->s.serr, c->lsgs_waves, 1, 0, 0, c->nnz,
+                                                ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, 1, 0, 0, c->nnz,
                                                 c->s, c->stream);
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative FSGS sweep launch: ") + hipGetErrorString(he));
@@ -1436,12 +1446,7 @@ static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
       prof_end(c, FPSI_K_SGS_SWEEP, e);
     }
   }
-  if (flow && nsweeps > 0) {
-    int err = 0;
-    HIPCHK(c, hipMemcpyAsync(&err, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (err) return fail(c, FPSI_ERR_HIP, "dataflow FSGS sweep: a dependency wait exceeded its limit");
-  }
+  if (flow && nsweeps > 0) return check_sweep_error(c, "dataflow FSGS sweep");
   return 0;
 }
 
@@ -1694,7 +1699,7 @@ static int linres(fpsi_ctx *c, double *out) {
   HIPCHK(c, hipMemcpyAsync(c->rb_host + 12, c->lr_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->rb_host + 13, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (*(const int *)(c->rb_host + 13)) return fail(c, FPSI_ERR_HIP, "dataflow sweep: a dependency wait exceeded its limit");
+  if (*(const int *)(c->rb_host + 13)) return sweep_failed(c, "dataflow sweep");
   *out = std::sqrt(c->rb_host[12]);
   return 0;
 }

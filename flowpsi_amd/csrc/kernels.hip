@@ -683,6 +683,7 @@ __global__ void __launch_bounds__(TPB) k_face_jac(DevGrid g, Prm p, const double
     for (int i = 0; i < 25; ++i) blk[i] = (F)0;
     roe_jacobian<F>(blk, ra, an, area, p.gamma, p.Rt, 1);
     if (p.viscous) viscous_jacobian<F>(blk, 1.0, an, area, dist, mu_f, k_f, uf, vf);
+    if (p.tdamp > 0) damping_jacobian<F>(blk, 1.0, L, an, area, p.Pamb, p.Rt, p.gamma, us_n, p.tdamp);
     if (!per) store_blk(off + (size_t)g.slot_cr[f] * 25, blk, true);
     else store_blk(pblk + (size_t)(f - g.nfi) * 25, blk, false);
     // fjm
@@ -690,6 +691,7 @@ __global__ void __launch_bounds__(TPB) k_face_jac(DevGrid g, Prm p, const double
     for (int i = 0; i < 25; ++i) blk[i] = (F)0;
     roe_jacobian<F>(blk, ra, an, area, p.gamma, p.Rt, -1);
     if (p.viscous) viscous_jacobian<F>(blk, -1.0, an, area, dist, mu_f, k_f, uf, vf);
+    if (p.tdamp > 0) damping_jacobian<F>(blk, -1.0, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, p.tdamp);
     store_blk(off + (size_t)g.slot_cl[f] * 25, blk, false);
   } else {
     const int b = f - g.bnd_begin;
@@ -1641,6 +1643,256 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
     __builtin_amdgcn_s_waitcnt(0x0F70);
     if (lane == 0) __hip_atomic_store(flag + ch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Line-SGS half-sweep as one cooperative dataflow launch without flags, for
+// the fluid system and the turbulence scalars of its fused loops.  X is
+// double-buffered.  The forward half-sweep reads the previous iterate of
+// the neighbours it has not reached (upper) from Xa and polls the ones it
+// has reached (lower) in Xb, whose interior rows hold the sentinel X_SENT
+// when it starts; it writes Xb and then the sentinel over its own word of
+// Xa: every cell that reads that word (a lower neighbour) finished before
+// this cell's poll succeeded.  The backward half-sweep reads its lower
+// neighbours' forward values from Xb (final since the forward sweep) and
+// polls its upper neighbours in Xa; it leaves Xb alone, so Xb keeps X after
+// the forward sweep, the relaxation's xt (k_lsgs_relax_flow forms
+// X = w Xa + (1 - w) Xb in Xa and refills Xb with sentinels).  A turbulence
+// scalar does the same with its own pair (ls.X, sXb) and keeps its own xt
+// (the iterate before the forward sweep) as k_lsgs_sweep does.  Periodic
+// ghost rows and the zero pad row are never sentinels: ghosts are upper
+// entries, read from Xa in both directions; pads are read from the source
+// buffer of either direction, so Xb's pad row must hold zero.  Chunks,
+// order and summation are those of k_lsgs_sweep, so the result is bitwise
+// the same; the wave holding the smallest unfinished chunk always proceeds,
+// and a wait longer than spin_limit polls (or an error raised by another
+// wave) ends the launch with *err set.  A system that is not in the launch
+// (fluid = 0, or its scalar's mask bit clear) polls and stores nothing.
+template <typename F, typename DG, bool BACK>
+__global__ void __launch_bounds__(64) k_lsgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
+                                                  int spin_limit, int *__restrict__ err, const F *__restrict__ off,
+                                                  const DG *__restrict__ diag, const F *__restrict__ B,
+                                                  double *__restrict__ Xa, double *__restrict__ Xb, int fluid,
+                                                  LineScalars ls, double *__restrict__ sXb) {
+  const int lane = threadIdx.x;
+  const int q = lane / 5, i = lane - 5 * q;
+  const int gbase = 5 * (q < CPW ? q : 0);
+  __shared__ double xl[CPW * MAXR * 5];
+  __shared__ DG al[CPW * 25];
+  double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
+  DG *aw = al + (q < CPW ? q : 0) * 25;
+  const double *Xs = BACK ? Xb : Xa;  // values this half-sweep does not write
+  double *Xt = BACK ? Xa : Xb;        // the half-sweep's results, polled
+  const double sent = __longlong_as_double((long long)X_SENT);
+  for (int t = blockIdx.x; t < nchunks; t += nwaves) {
+    const int ch = BACK ? nchunks - 1 - t : t;
+    const int lo = chunk_lo[ch], hi = chunk_lo[ch + 1];
+    const int c = lo + q;
+    const bool act = q < CPW && c < hi;
+    const bool fact = act && fluid;
+    const int cc = act ? c : lo;
+    double tsum = -(double)B[(size_t)cc * 5 + i];
+    const int m0 = g.m_off[cc];
+    const int len = g.m_off[cc + 1] - m0;
+    const int nup = g.m_nup[cc];
+    const int nlow = len - nup;  // rotated rows: lower entries j < nlow, then upper
+    int nb[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)cc * MAXR + j];
+    F mrow[MAXR][5];
+    DG a5[5];
+    double x1[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      int jr = j + nup;
+      if (jr >= len) jr -= len;
+      const int slot = j < len ? m0 + jr : (len > 0 ? m0 : 0);
+      const F *M = off + (size_t)slot * 25;
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a5[k] = diag[(size_t)cc * 25 + 5 * i + k];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
+      x1[j] = pre ? ldx<true>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
+    }
+    // turbulence scalar sc = i of this cell
+    const bool sact = act && i < ls.ns && ((ls.mask >> i) & 1);
+    const int sc = sact ? i : 0;
+    const size_t xs = (size_t)g.nc + g.nper + 1;
+    const double *soff = ls.mask ? ls.off + (size_t)sc * ls.nnz + m0 : nullptr;
+    double *sxa = ls.mask ? ls.X + sc * xs : nullptr;
+    double *sxb = ls.mask ? sXb + sc * xs : nullptr;
+    const double *sxs = BACK ? sxb : sxa;
+    double *sxt = BACK ? sxa : sxb;
+    double sres = 0.0, sold = 0.0, sco[MAXR], sxv[MAXR];
+    if (sact) {
+      sres = ls.B[(size_t)sc * g.nc + cc];
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        sco[j] = j < len ? soff[jr] : 0.0;
+        const bool pre = j < len && (BACK ? (j < nlow) : (j >= nlow));
+        sxv[j] = pre ? ldx<true>(sxs + nb[j]) : 0.0;
+      }
+      if (!BACK) sold = ldx<true>(sxa + cc);
+      if (BACK) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nlow) sres -= sxv[j] * sco[j];
+      }
+    }
+    if constexpr (BACK) {  // the lower terms are final since the forward sweep
+      if (q < CPW) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+          if (j < nlow) xw[j * 5 + i] = x1[j];
+      }
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj)
+          if (j < nlow) tsum += mrow[j][jj] * xw[j * 5 + jj];
+    }
+    // the wait: poll the words of the reached neighbours in the target buffers
+    int spins = 0;
+    while (true) {
+      bool ready = true;
+      if (fluid) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j) {
+          const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
+          if (post) {
+            x1[j] = ldx<true>(Xt + (size_t)nb[j] * 5 + i);
+            ready = ready && !is_sent(x1[j]);
+          }
+        }
+      }
+      if (sact) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j) {
+          const bool post = j < len && (BACK ? (j >= nlow) : (j < nlow));
+          if (post) {
+            sxv[j] = ldx<true>(sxt + nb[j]);
+            ready = ready && !is_sent(sxv[j]);
+          }
+        }
+      }
+      if (__all(ready)) break;
+      if (++spins > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    }
+    if (q < CPW) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (!BACK || j >= nlow) xw[j * 5 + i] = x1[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j)
+#pragma unroll
+      for (int jj = 0; jj < 5; ++jj)
+        if (!BACK || j >= nlow) tsum += mrow[j][jj] * xw[j * 5 + jj];
+    if (fact)
+      for (int j = MAXR; j < len; ++j) {  // rows wider than MAXR
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        const F *M = off + (size_t)(m0 + jr) * 25;
+        const bool post = BACK ? (j >= nlow) : (j < nlow);
+        const double *xn = (post ? (const double *)Xt : Xs) + (size_t)g.m_nbr[m0 + jr] * 5;
+#pragma unroll
+        for (int jj = 0; jj < 5; ++jj) {
+          double xv = ldx<true>(xn + jj);
+          for (int sp = 0; post && is_sent(xv); ++sp) {
+            if (sp > spin_limit) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              xv = 0.0;
+              break;
+            }
+            xv = ldx<true>(xn + jj);
+          }
+          tsum += M[i + 5 * jj] * xv;
+        }
+      }
+    if (sact) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j)
+        if (j < len && (!BACK || j >= nlow)) sres -= sxv[j] * sco[j];
+      for (int j = MAXR; j < len; ++j) {
+        int jr = j + nup;
+        if (jr >= len) jr -= len;
+        const bool post = BACK ? (j >= nlow) : (j < nlow);
+        const double *xp = (post ? (const double *)sxt : sxs) + g.m_nbr[m0 + jr];
+        double xv = ldx<true>(xp);
+        for (int sp = 0; post && is_sent(xv); ++sp) {
+          if (sp > spin_limit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            xv = 0.0;
+            break;
+          }
+          xv = ldx<true>(xp);
+        }
+        sres -= xv * soff[jr];
+      }
+      if (!BACK) ls.xt[(size_t)sc * g.nc + c] = sold;
+      stx<true>(sxt + c, ls.dinv[(size_t)sc * g.nc + c] * sres);
+      if (!BACK) stx<true>(sxa + c, sent);
+    }
+    double v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = -__shfl(tsum, gbase + k, 64);
+    if (fact) {
+      double x[5];
+      lusolve5<DG>(aw, PIV_ID, v, x);
+      const double xi = i == 0 ? x[0] : i == 1 ? x[1] : i == 2 ? x[2] : i == 3 ? x[3] : x[4];
+      stx<true>(Xt + (size_t)c * 5 + i, xi);
+      if (!BACK) stx<true>(Xa + (size_t)c * 5 + i, sent);
+    }
+  }
+}
+
+// Relaxation after a flag-free line-SGS sweep (src/linearSolverLSGS.loci:
+// 1868-1874 for the fluid, :1330-1340 for a scalar): X = w Xa + (1 - w) xt
+// into Xa with the block sums of (X - xt)^2, as k_lsgs_relax; then the
+// sentinel into `fill`, the forward half-sweep's target of the next sweep
+// (the fluid's xt is that buffer itself, read before it is overwritten).
+__global__ void k_lsgs_relax_flow(int n, const double *__restrict__ st, double *__restrict__ Xa,
+                                  const double *xt, double *fill, double *__restrict__ part) {
+  __shared__ double sh[TPB];
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  const double w = st[0];
+  double r2 = 0.0;
+  if (i < n) {
+    const double xo = xt[i];
+    const double x = w * Xa[i] + (1. - w) * xo;
+    Xa[i] = x;
+    fill[i] = __longlong_as_double((long long)X_SENT);
+    const double r = x - xo;
+    r2 = r * r;
+  }
+  sh[threadIdx.x] = r2;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// the forward target of a flag-free line-SGS solve: sentinels on the n
+// interior words, zero on the pad row (npad words at `pad`)
+__global__ void k_flow_fill(long n, double *__restrict__ X, long npad, double *__restrict__ pad) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) X[e] = __longlong_as_double((long long)X_SENT);
+  if (e < npad) pad[e] = 0.0;
 }
 
 // periodic ghost copies, src/linearSolverSGS.loci:124-127 and :215-219

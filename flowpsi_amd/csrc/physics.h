@@ -31,6 +31,7 @@ struct Prm {
   int unsteady;           // 0 steady, 1 BDF2, 2 KEC
   double pseudo_cflmax;
   double pbias_ctl;       // timePressureBiasControl (KEC)
+  double tdamp;           // temporalDamping beta (<= 0: not set)
 };
 
 struct V3 {
@@ -405,6 +406,46 @@ __device__ inline void viscous_jacobian(F *Fj, double s, const V3 &an, double sa
   acc(FJ(4, 3), s * (.5 * vf[2]));
   const double factor = k_f * sada / dist;
   acc(FJ(4, 0), s * factor);
+#undef FJ
+}
+
+// Temporal-damping Jacobian, src/fluxInviscid.loci:794-866: damping_fjp
+// (s = +1, left face state) / damping_fjm (s = -1, right face state) add
+// beta (|u.n - us_n| + a) A/2 dQ/dq of the face state; the intermediates are
+// held in the Jacobian precision F as the rule body declares them.
+template <typename F>
+__device__ inline void damping_jacobian(F *Fj, double s, const Q5 &Q, const V3 &an, double area, double pamb,
+                                        double Rt, double gamma, double us_n, double beta) {
+  const F coef = (F)(0.5 * area);
+  const F gm1 = (F)(gamma - 1);
+  const F rr = (F)((Q.pg + pamb) / (Rt * Q.T));
+  const F drdp = (F)(1. / (Rt * Q.T));
+  const F drdT = (F)(-(double)rr / Q.T);
+  const F uta = (F)dot(Q.u, an);
+  const F ut = (F)((double)uta - us_n);
+  const F ar = (F)sqrt(gamma * Rt * Q.T);
+  const F aut = ut < (F)0 ? -ut : ut;
+  const F sum = aut + ar;
+  const double coefd = beta * (double)sum * (double)coef;
+  const double uu = dot(Q.u, Q.u);
+  const double dT = (double)drdT, dp = (double)drdp, r = (double)rr;
+#define FJ(i, j) Fj[(i) + 5 * (j)]
+  acc(FJ(0, 0), s * (coefd * dT));
+  acc(FJ(1, 0), s * (coefd * dT * Q.u.x));
+  acc(FJ(2, 0), s * (coefd * dT * Q.u.y));
+  acc(FJ(3, 0), s * (coefd * dT * Q.u.z));
+  acc(FJ(4, 0), s * (coefd * dT * 0.5 * uu));
+  acc(FJ(0, 4), s * (coefd * dp));
+  acc(FJ(1, 4), s * (coefd * dp * Q.u.x));
+  acc(FJ(2, 4), s * (coefd * dp * Q.u.y));
+  acc(FJ(3, 4), s * (coefd * dp * Q.u.z));
+  acc(FJ(4, 4), s * (coefd * (1. / (double)gm1 + dp * 0.5 * uu)));
+  acc(FJ(1, 1), s * (coefd * r));
+  acc(FJ(2, 2), s * (coefd * r));
+  acc(FJ(3, 3), s * (coefd * r));
+  acc(FJ(4, 1), s * (coefd * r * Q.u.x));
+  acc(FJ(4, 2), s * (coefd * r * Q.u.y));
+  acc(FJ(4, 3), s * (coefd * r * Q.u.z));
 #undef FJ
 }
 

@@ -529,9 +529,10 @@ def prism_hex_box(nx=8, nz=8, n_prism=6, n_hex=6, first=1e-3, growth=1.2, H=1.0,
     return g
 
 
-def bl_box(nx=8, ny=12, nz=8, aspect=1e3, L=1.0, seed=5):
+def bl_box(nx=8, ny=12, nz=8, aspect=1e3, L=1.0, seed=5, side=None):
     """Boundary-layer hex grid: wall at y = 0, geometric stretching in y from a
-    first cell of height L/nx / aspect, farfield elsewhere."""
+    first cell of height L/nx / aspect, farfield elsewhere; `side` sets the
+    type of the spanwise planes z = 0, L (default farfield)."""
     dx = L / nx
     h1 = dx / aspect
     # growth r with h1 (r^ny - 1)/(r - 1) = L
@@ -545,7 +546,8 @@ def bl_box(nx=8, ny=12, nz=8, aspect=1e3, L=1.0, seed=5):
     xs = np.linspace(0.0, L, nx + 1)
     zs = np.linspace(0.0, L, nz + 1)
     X = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"))
-    bc = {"xmin": FARFIELD, "xmax": FARFIELD, "ymin": WALL, "ymax": FARFIELD, "zmin": FARFIELD, "zmax": FARFIELD}
+    zs_bc = FARFIELD if side is None else side
+    bc = {"xmin": FARFIELD, "xmax": FARFIELD, "ymin": WALL, "ymax": FARFIELD, "zmin": zs_bc, "zmax": zs_bc}
     g = structured_hex(X, bc, name=f"bl{nx}x{ny}x{nz}")
     g.meta.update(kind="bl", aspect=aspect, growth=r)
     return g

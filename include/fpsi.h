@@ -129,6 +129,14 @@ typedef struct {
   int newton_iter;
   double pseudo_cflmax;
   double pbias_control;
+  /* polls of one dependency wait of a dataflow sweep before it gives up and
+   * the call returns FPSI_ERR_HIP (<= 0: 1 << 22, far longer than any real
+   * wait); after such an error the system must be assembled again */
+  int sweep_spin_limit;
+  /* temporalDamping beta (src/fluxInviscid.loci:1224-1251): when > 0 the
+   * face Jacobians fjp / fjm of interior and periodic faces gain the
+   * damping_fjp / damping_fjm terms (:794-866); <= 0: not set */
+  double temporal_damping;
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);

@@ -37,7 +37,8 @@ class OgParams(C.Structure):
                 ("sa_cv1", C.c_double), ("sa_cb2", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double),
                 ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double), ("k_bc", C.c_double),
                 ("w_bc", C.c_double), ("unsteady", C.c_int), ("newton_iter", C.c_int),
-                ("pseudo_cflmax", C.c_double), ("pbias_control", C.c_double)]
+                ("pseudo_cflmax", C.c_double), ("pbias_control", C.c_double),
+                ("temporal_damping", C.c_double)]
 
 
 class OgNorms(C.Structure):
@@ -139,6 +140,7 @@ def params(p: dict) -> OgParams:
     s.newton_iter = int(p.get("newton_iter", 6))
     s.pseudo_cflmax = float(p.get("pseudo_cflmax", 10.0))
     s.pbias_control = float(p.get("pbias_control", 0.0))  # timePressureBiasControl
+    s.temporal_damping = float(p.get("temporal_damping", 0.0))  # temporalDamping (<= 0: not set)
     return s
 
 

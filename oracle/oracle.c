@@ -1216,6 +1216,41 @@ static void visc_jac(real_fj *F, real s, const real *an, real sada, real dist, r
   addfj(&M_(F, ei, 0), s * factor);
 }
 
+/* Temporal-damping Jacobian damping_fjp (s = +1, left face state) /
+ * damping_fjm (s = -1, right face state), src/fluxInviscid.loci:794-866:
+ * beta (|u.n - us_n| + a) A/2 times dQ/dq of the face state, with the
+ * intermediates held in real_fj as the rule body declares them. */
+static void damping_jac(real_fj *fj, real s, real pg, real T, const real *U, const real *an, real area,
+                        real pamb, real Rt, real gamma, real us_n, real beta) {
+  const real_fj coef = (real_fj)(0.5 * area);
+  const real_fj gm1 = (real_fj)(gamma - 1);
+  const real_fj rr = (real_fj)((pg + pamb) / (Rt * T));
+  const real_fj drdp = (real_fj)(1. / (Rt * T));
+  const real_fj drdT = (real_fj)(-rr / T);
+  const real_fj uta = (real_fj)dot3(U, an);
+  const real_fj ut = (real_fj)(uta - us_n);
+  const real_fj ar = (real_fj)sqrt(gamma * Rt * T);
+  const real_fj aut = (real_fj)fabs(ut);
+  const real coefd = beta * (real_fj)(aut + ar) * coef;
+  const real uu = dot3(U, U);
+  addfj(&M_(fj, 0, 0), s * (coefd * drdT));
+  addfj(&M_(fj, 1, 0), s * (coefd * drdT * U[0]));
+  addfj(&M_(fj, 2, 0), s * (coefd * drdT * U[1]));
+  addfj(&M_(fj, 3, 0), s * (coefd * drdT * U[2]));
+  addfj(&M_(fj, 4, 0), s * (coefd * drdT * 0.5 * uu));
+  addfj(&M_(fj, 0, 4), s * (coefd * drdp));
+  addfj(&M_(fj, 1, 4), s * (coefd * drdp * U[0]));
+  addfj(&M_(fj, 2, 4), s * (coefd * drdp * U[1]));
+  addfj(&M_(fj, 3, 4), s * (coefd * drdp * U[2]));
+  addfj(&M_(fj, 4, 4), s * (coefd * (1. / gm1 + drdp * 0.5 * uu)));
+  addfj(&M_(fj, 1, 1), s * (coefd * rr));
+  addfj(&M_(fj, 2, 2), s * (coefd * rr));
+  addfj(&M_(fj, 3, 3), s * (coefd * rr));
+  addfj(&M_(fj, 4, 1), s * (coefd * rr * U[0]));
+  addfj(&M_(fj, 4, 2), s * (coefd * rr * U[1]));
+  addfj(&M_(fj, 4, 3), s * (coefd * rr * U[2]));
+}
+
 /* Boundary-face Jacobian block bj (real_fj, zeroed by the caller), i.e. the
  * face's whole contribution to srcJ of its interior cell:
  *   farfield src/bcInflow.loci:357-375, wall src/bcViscousWall.loci:230-240,
@@ -1389,6 +1424,12 @@ void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const 
         const real dist = dot3(an, dv);
         visc_jac(Mm, -1.0, an, g->area[f], dist, mu_f, k_f, uf, vf);
         visc_jac(P, 1.0, an, g->area[f], dist, mu_f, k_f, uf, vf);
+      }
+      if (p->temporal_damping > 0) { /* src/fluxInviscid.loci:1230-1251 */
+        damping_jac(P, 1.0, qL[4], qL[0], UL, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n,
+                    p->temporal_damping);
+        damping_jac(Mm, -1.0, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n,
+                    p->temporal_damping);
       }
     } else {
       boundary_block(g, p, q, qf, grad, phi, f, eta_p[g->cl[f]], bblk + 25 * (size_t)f);

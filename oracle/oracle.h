@@ -69,6 +69,7 @@ typedef struct {
   int newton_iter;        /* newtonMaxIter: Newton iterations per time step */
   double pseudo_cflmax;   /* pseudo-time CFL of the BDF2 preconditioning blend */
   double pbias_control;   /* timePressureBiasControl (KEC) */
+  double temporal_damping; /* temporalDamping beta (src/fluxInviscid.loci:1224-1251); <= 0: not set */
 } og_params;
 
 typedef struct {
