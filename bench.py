@@ -387,7 +387,7 @@ def main():
             gb, qb, pb = workloads.box(32, sweep="persistent")
             cname = "Euler box 32^3 (32768 cells), Roe, CFL 10, SGS x5 (fixed paths)"
         log(f"time to 1e-8 drop on the {cname}")
-        conv = converge.compare(gb, qb, pb)
+        conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), log=log)
         conv["workload"] = cname
         log("converge: " + ", ".join(f"{k}={v.get('time_to_target_s')}" for k, v in conv.items()
                                      if isinstance(v, dict) and "time_to_target_s" in v))

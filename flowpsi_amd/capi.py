@@ -24,7 +24,7 @@ FLUX = {"hllc": 0, "roe": 1}
 LIMITER = {"none": 0, "venkatakrishnan": 1, "zero": 2, "barth": 3}
 PRECISION = {"fp64": 0, "fp32": 1}
 SOLVER = {"sgs": 0, "lsgs": 1, "fsgs": 2}
-SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2}
+SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2, "persistent_flags": 3}
 TURB = {"none": 0, "sa": 1, "sst": 2}
 TIME_SCHEME = {"bdf2": 1, "kec": 2}
 NSCALAR = {"none": 0, "sa": 1, "sst": 2}  # turbulence scalars after the 5 fluid variables
@@ -63,7 +63,7 @@ class Params(C.Structure):
                 ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double),
                 ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double),
                 ("pbias_control", C.c_double), ("sweep_spin_limit", C.c_int),
-                ("temporal_damping", C.c_double)]
+                ("temporal_damping", C.c_double), ("sweep_waves_per_cu", C.c_int)]
 
 
 class Norms(C.Structure):
@@ -169,6 +169,7 @@ def make_params(prm: dict) -> Params:
     s.pbias_control = float(prm.get("pbias_control", 0.0))
     s.sweep_spin_limit = int(prm.get("sweep_spin_limit", 0))  # <= 0: the library's bound
     s.temporal_damping = float(prm.get("temporal_damping", 0.0))  # temporalDamping (<= 0: not set)
+    s.sweep_waves_per_cu = int(prm.get("sweep_waves_per_cu", 0))  # <= 0: every co-resident wave
     return s
 
 

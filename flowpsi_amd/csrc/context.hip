@@ -41,6 +41,8 @@ struct fpsi_ctx {
   // SGS levels (every cell in the same level, so the same dependency graph)
   bool lsgs_flow = false;
   int lsgs_waves = 0;
+  int lsgs_fwaves = 0;      // waves of the flag-free line-SGS sweep
+  int ncu = 1;              // compute units of the device
   int max_width = 0;
   double vtot = 0;
   DevGrid g{};
@@ -54,9 +56,16 @@ struct fpsi_ctx {
   // profiling: event pairs per kernel group, resolved at fpsi_profile_read
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  struct TimedLaunch {
+    int grp;
+    hipEvent_t start, end;
+    double bytes;  // algorithmic bytes of this launch (< 0: the group's fixed model)
+  };
+  std::vector<TimedLaunch> ev_pending;
   double prof_ms[FPSI_K_NGROUPS] = {0};
   long prof_n[FPSI_K_NGROUPS] = {0};
+  double prof_b[FPSI_K_NGROUPS] = {0};   // summed per-launch bytes
+  long prof_nb[FPSI_K_NGROUPS] = {0};    // launches that carried their own bytes
   // linear residual scratch
   double *lr_parts = nullptr, *lr_bsum = nullptr, *lr_out = nullptr, *Xj = nullptr, *Dg = nullptr;
   // learned guess: graph and geometry (new cell order), buffers, samples
@@ -64,7 +73,8 @@ struct fpsi_ctx {
   std::vector<float> mlp_geo;
   bool mlp_alloc = false, mlp_norm = false;
   MlpBufs mb{};
-  float *mlp_xs[4] = {nullptr, nullptr, nullptr, nullptr}, *mlp_ys[4] = {nullptr, nullptr, nullptr, nullptr};
+  static constexpr int MLP_SLOTS = 16;  // harvested samples kept (the oldest is overwritten)
+  float *mlp_xs[MLP_SLOTS] = {}, *mlp_ys[MLP_SLOTS] = {};
   float *mlp_xcur = nullptr;
   int mlp_nsamp = 0, mlp_next = 0, mlp_step = 0;
   // line SGS: levels of the breadth-first line order (new cell numbering);
@@ -142,6 +152,18 @@ static int check_sweep_error(fpsi_ctx *c, const char *what) {
 // polls of one dependency wait before a dataflow sweep gives up: far longer
 // than any real wait (~1e6 polls of a few hundred ns each), so only a broken
 // schedule reaches it; fpsi_params.sweep_spin_limit overrides it
+// both dataflow modes: flag-free (PERSISTENT) or with per-chunk flags
+static bool dataflow(const fpsi_ctx *c) {
+  return c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT || c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT_FLAGS;
+}
+
+// co-resident waves of a dataflow line-SGS launch: sweep_waves_per_cu when
+// set (capped at what fits), else every co-resident wave (wide levels)
+static int line_waves(const fpsi_ctx *c, int cap) {
+  const int per_cu = c->prm.sweep_waves_per_cu;
+  return per_cu > 0 ? std::max(1, std::min(cap, per_cu * c->ncu)) : cap;
+}
+
 static int spin_limit(const fpsi_ctx *c) { return c->prm.sweep_spin_limit > 0 ? c->prm.sweep_spin_limit : 1 << 22; }
 
 template <typename T>
@@ -186,18 +208,33 @@ static hipEvent_t prof_begin(fpsi_ctx *c) {
   if (e) hipEventRecord(e, c->stream);
   return e;
 }
-static void prof_end(fpsi_ctx *c, int grp, hipEvent_t start) {
+static void prof_end(fpsi_ctx *c, int grp, hipEvent_t start, double bytes = -1.0) {
   if (!c->prof || !start) return;
   hipEvent_t e = ev_get(c);
   if (!e) return;
   hipEventRecord(e, c->stream);
-  c->ev_pending.push_back({grp, {start, e}});
+  c->ev_pending.push_back({grp, start, e, bytes});
+}
+
+// Algorithmic bytes of one flag-free line-SGS half-sweep (each array once,
+// neighbour gathers cache-perfect), from this context's buffer sizes: the
+// fluid's block rows (row offset and neighbour index per slot), FP64
+// diagonal factors, B, X read and written, and in the forward direction the
+// sentinel written over the source word; per turbulence scalar in the
+// launch its row coefficients, B, 1/D, X read and written, and forward its
+// xt and sentinel.
+static double sweep_bytes(const fpsi_ctx *c, bool fluid, int nscal, int backward) {
+  const double nc = c->nc, nnz = c->nnz, ps = 8, pj = (double)c->fj_bytes();
+  double b = nc * 8 /* m_off, m_nup */ + nc * 6 * 4 /* padded neighbour list */;
+  if (fluid) b += nnz * 25 * pj + nc * (25 * 8 + 5 * pj + 2 * 5 * ps + (backward ? 0 : 5 * ps));
+  b += nscal * (nnz * ps + nc * (ps + ps + 2 * ps + (backward ? 0 : 2 * ps)));
+  return b;
 }
 
 extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  for (auto &p : ctx->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+  for (auto &p : ctx->ev_pending) { hipEventDestroy(p.start); hipEventDestroy(p.end); }
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
   for (void *p : ctx->allocs) hipFree(p);
   if (ctx->lsgs_host) hipHostFree(ctx->lsgs_host);
@@ -601,9 +638,12 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     sweep_ncu = ncu;
+    c->ncu = ncu;
     sweep_cap = sgs_sweep_waves(1, ncu) > 0 ? std::min(sgs_sweep_waves(0, ncu), sgs_sweep_waves(1, ncu)) : 0;
     const int lcap = lsgs_sweep_waves(1, ncu) > 0 ? std::min(lsgs_sweep_waves(0, ncu), lsgs_sweep_waves(1, ncu)) : 0;
     c->lsgs_waves = std::max(1, lcap);  // wide levels: every co-resident wave
+    const int fcap = lsgs_flow_waves(1, ncu) > 0 ? std::min(lsgs_flow_waves(0, ncu), lsgs_flow_waves(1, ncu)) : 0;
+    c->lsgs_fwaves = std::max(1, fcap);
   }
   DevGrid &g = c->g;
   g.nc = nc; g.nf = nf; g.nfi = nfi; g.nper = nper; g.nb = c->nb; g.bnd_begin = nfi + nper;
@@ -809,9 +849,10 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     }
   }
   if (p->sgs_iter < 0) return fail(c, FPSI_ERR_ARG, "gauss_seidel_iter < 0");
-  if (p->sweep_mode == FPSI_SWEEP_PERSISTENT && !c->have_flow)
+  if ((p->sweep_mode == FPSI_SWEEP_PERSISTENT || p->sweep_mode == FPSI_SWEEP_PERSISTENT_FLAGS) && !c->have_flow)
     return fail(c, FPSI_ERR_ARG, "no dataflow sweep on this grid (a sweep chunk has more than 64 dependencies)");
   if (p->sweep_mode != FPSI_SWEEP_PERSISTENT && p->sweep_mode != FPSI_SWEEP_PER_LEVEL &&
+      p->sweep_mode != FPSI_SWEEP_PERSISTENT_FLAGS &&
       p->sweep_mode != FPSI_SWEEP_TILED)
     return fail(c, FPSI_ERR_ARG, "unknown sweep mode");
   if (p->sweep_mode == FPSI_SWEEP_TILED && !c->have_tiles)
@@ -1108,9 +1149,10 @@ static int sa_alloc(fpsi_ctx *c) {
   if ((rc = dalloc(c, &s.tmu, nc)) || (rc = dalloc(c, &s.saf, 12 * nf)) || (rc = dalloc(c, &s.scell, 8 * nc)) ||
       (rc = dalloc(c, &s.soff, 2 * nnz)) || (rc = dalloc(c, &s.ssrc, 2 * nc)) ||
       (rc = dalloc(c, &s.sdinv, 2 * nc)) || (rc = dalloc(c, &s.sB, 2 * nc)) || (rc = dalloc(c, &s.sX, 2 * xs)) ||
-      (rc = dalloc(c, &s.sxt, 2 * nc)))
+      (rc = dalloc(c, &s.sX2, 2 * xs)) || (rc = dalloc(c, &s.sxt, 2 * nc)))
     return rc;
   HIPCHK(c, hipMemset(s.sX, 0, 2 * xs * sizeof(double)));
+  HIPCHK(c, hipMemset(s.sX2, 0, 2 * xs * sizeof(double)));
   HIPCHK(c, hipMemset(s.soff, 0, 2 * nnz * sizeof(double)));
   c->sa_alloc = true;
   return 0;
@@ -1261,9 +1303,10 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = tru
       }
     return 0;
   }
-  if (c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT) {
+  if (dataflow(c)) {
     const int spin = spin_limit(c);
-    if (!ns) {
+    const bool flags = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT_FLAGS;
+    if (!ns && !flags) {
       // fluid only: flag-free half-sweeps alternating between X and X2; an
       // even number of them leaves the result in X
       double *cur = c->s.X, *oth = c->s.X2;
@@ -1278,7 +1321,7 @@ static int enqueue_sweeps(fpsi_ctx *c, int nsweeps, int ns = 0, bool check = tru
           std::swap(cur, oth);
         }
     }
-    for (int it = 0; it < (ns ? nsweeps : 0); ++it)
+    for (int it = 0; it < (ns || flags ? nsweeps : 0); ++it)
       for (int dir = 0; dir < 2; ++dir) {
         launch_ghost(c->g, c->s, c->stream);
         if (ns) launch_ghost1(c->g, c->nscal(), c->s.sX, c->stream);
@@ -1321,20 +1364,32 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
   c->lsgs_iters = 0;
   c->lsgs_w = p.lsgs_relax;
   c->lsgs_resid = 0;
-  // one cooperative dataflow launch per half-sweep where the levels allow
-  // (bitwise equal to the level launches), else one launch per level
+  // one cooperative flag-free dataflow launch per half-sweep where the
+  // levels allow (bitwise equal to the level launches), else one launch per
+  // level
   const bool flow = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT && c->lsgs_flow;
+  const bool flags = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT_FLAGS && c->lsgs_flow;
+  if (flow) launch_lsgs_flow_fill(c->g, c->s, 1, 0, c->stream);
   for (int it = 0; it < maxiter; ++it) {
     launch_ghost(c->g, c->s, c->stream);
     for (int dir = 0; dir < 2; ++dir) {
       hipEvent_t e = prof_begin(c);
       if (flow) {
-        const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
-                                                dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, 1, 0, 0,
-                                                c->nnz, c->s, c->stream);
+        const hipError_t he = launch_lsgs_flow(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, spin_limit(c), c->s.serr,
+                                               line_waves(c, c->lsgs_fwaves), 1, 0, 0, c->nnz, c->s, c->stream);
+        prof_end(c, FPSI_K_SGS_SWEEP, e, sweep_bytes(c, true, 0, dir));
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
+        continue;
+      } else if (flags) {
+        const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
+                                                dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
+                                                ++c->flow_epoch, spin_limit(c), c->s.serr,
+                                                line_waves(c, c->lsgs_waves), 1, 0, 0, c->nnz, c->s, c->stream);
+        prof_end(c, FPSI_K_SGS_SWEEP, e, sweep_bytes(c, true, 0, dir));
+        if (he != hipSuccess)
+          return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
+        continue;
       } else if (dir == 0) {
         for (int l = 0; l < nl; ++l)
           launch_lsgs_level(c->g, c->fp32, 0, c->lsgs_cells, c->lsgs_lev_off[l], c->lsgs_lev_off[l + 1], c->s,
@@ -1346,7 +1401,8 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
       }
       prof_end(c, FPSI_K_SGS_SWEEP, e);
     }
-    launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    if (flow) launch_lsgs_relax_flow(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    else launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1355,7 +1411,7 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
     c->lsgs_iters = (int)c->lsgs_host[3];
     if (c->lsgs_host[4] != 0.0) break;
   }
-  if (flow) return check_sweep_error(c, "dataflow line sweep");
+  if (flow || flags) return check_sweep_error(c, "dataflow line sweep");
   return 0;
 }
 
@@ -1364,9 +1420,21 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
 // (ghost refresh, forward, backward, relaxation and stopping test with its
 // own state slot), so the bits are those of the separate solves; a system
 // leaves the launches once its test stops it.
-static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
+// Stopping rule of the fluid's line-SGS loop run to a linear-residual
+// tolerance (in place of its update-norm test): stop once ||B - A X|| <=
+// rtol r0 or after max_sweeps relaxed symmetric sweeps.
+struct LinTol {
+  double rtol, r0;
+  int max_sweeps;
+  int sweeps;
+  double rfinal;
+};
+
+static int linres(fpsi_ctx *c, double *out);
+
+static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter, LinTol *tol = nullptr, bool scalars = true) {
   const fpsi_params &p = c->prm;
-  const int ns = c->nscal();
+  const int ns = scalars ? c->nscal() : 0;
   for (int slot = 0; slot <= ns; ++slot) launch_lsgs_init(c->s, p.lsgs_relax, c->stream, slot);
   c->lsgs_iters = 0;
   c->lsgs_w = p.lsgs_relax;
@@ -1376,24 +1444,39 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
     c->sa_lsgs_w[sc] = p.lsgs_relax;
     c->sa_lsgs_resid[sc] = 0;
   }
-  bool fluid = true;
+  bool fluid = !tol || (tol->max_sweeps > 0 && !(tol->rfinal <= tol->rtol * tol->r0));
+  const int fmax = tol ? tol->max_sweeps : maxiter;
   int mask = (1 << ns) - 1;
-  for (int it = 0; it < maxiter && (fluid || mask); ++it) {
+  const bool flags = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT_FLAGS;
+  if (!flags) launch_lsgs_flow_fill(c->g, c->s, 1, ns, c->stream);
+  for (int it = 0; (fluid && it < fmax) || (mask && it < maxiter); ++it) {
+    if (it >= maxiter) mask = 0;
+    if (it >= fmax) fluid = false;
     if (fluid) launch_ghost(c->g, c->s, c->stream);
     if (mask) launch_ghost1(c->g, ns, c->s.sX, c->stream);
+    int nact = 0;
+    for (int sc = 0; sc < ns; ++sc) nact += mask >> sc & 1;
     for (int dir = 0; dir < 2; ++dir) {
       hipEvent_t e = prof_begin(c);
-      const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo,
-                                              dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                              ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, fluid ? 1 : 0, ns,
-                                              mask, c->nnz, c->s, c->stream);
-      prof_end(c, FPSI_K_SGS_SWEEP, e);
+      const hipError_t he =
+          flags ? launch_lsgs_sweep(c->g, c->fp32, 0, dir, c->nchunks, c->chunk_lo, dir ? c->bdep_off : c->fdep_off,
+                                    dir ? c->bdep : c->fdep, c->s.cflag, ++c->flow_epoch, spin_limit(c), c->s.serr,
+                                    line_waves(c, c->lsgs_waves), fluid ? 1 : 0, ns, mask, c->nnz, c->s, c->stream)
+                : launch_lsgs_flow(c->g, c->fp32, dir, c->nchunks, c->chunk_lo, spin_limit(c), c->s.serr,
+                                   line_waves(c, c->lsgs_fwaves), fluid ? 1 : 0, ns, mask, c->nnz, c->s, c->stream);
+      prof_end(c, FPSI_K_SGS_SWEEP, e, sweep_bytes(c, fluid, nact, dir));
       if (he != hipSuccess)
         return fail(c, FPSI_ERR_HIP, std::string("cooperative line sweep launch: ") + hipGetErrorString(he));
     }
-    if (fluid) launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    if (fluid) {
+      if (flags) launch_lsgs_relax(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+      else launch_lsgs_relax_flow(c->g, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream);
+    }
     for (int sc = 0; sc < ns; ++sc)
-      if (mask >> sc & 1) launch_sa_lsgs_relax(c->g, sc, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream, 1 + sc);
+      if (mask >> sc & 1) {
+        if (flags) launch_sa_lsgs_relax(c->g, sc, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream, 1 + sc);
+        else launch_sa_lsgs_relax_flow(c->g, sc, it, p.lsgs_abstol, p.lsgs_reltol, c->s, c->stream, 1 + sc);
+      }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->lsgs_host, c->s.lst, 8 * (1 + ns) * sizeof(double), hipMemcpyDeviceToHost,
                              c->stream));
@@ -1402,7 +1485,13 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
       c->lsgs_w = c->lsgs_host[0];
       c->lsgs_resid = c->lsgs_host[2];
       c->lsgs_iters = (int)c->lsgs_host[3];
-      if (c->lsgs_host[4] != 0.0) fluid = false;
+      if (tol) {
+        ++tol->sweeps;
+        if (int rc = linres(c, &tol->rfinal)) return rc;
+        if (tol->rfinal <= tol->rtol * tol->r0) fluid = false;
+      } else if (c->lsgs_host[4] != 0.0) {
+        fluid = false;
+      }
     }
     for (int sc = 0; sc < ns; ++sc)
       if (mask >> sc & 1) {
@@ -1421,7 +1510,7 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter) {
 // unpivoted Jacobian-precision factors, periodic ghosts refreshed before each.
 static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
   const int nl = (int)c->lsgs_lev_off.size() - 1;
-  const bool flow = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT && c->lsgs_flow;
+  const bool flow = dataflow(c) && c->lsgs_flow;
   launch_zero_d(5 * ((long)c->nc + c->nper + 1), c->s.X, c->stream);
   for (int it = 0; it < nsweeps; ++it) {
     launch_ghost(c->g, c->s, c->stream);
@@ -1430,7 +1519,8 @@ static int enqueue_fsgs(fpsi_ctx *c, int nsweeps) {
       if (flow) {
         const hipError_t he = launch_lsgs_sweep(c->g, c->fp32, 1, dir, c->nchunks, c->chunk_lo,
                                                 dir ? c->bdep_off : c->fdep_off, dir ? c->bdep : c->fdep, c->s.cflag,
-                                                ++c->flow_epoch, spin_limit(c), c->s.serr, c->lsgs_waves, 1, 0, 0, c->nnz,
+                                                ++c->flow_epoch, spin_limit(c), c->s.serr,
+                                                line_waves(c, c->lsgs_waves), 1, 0, 0, c->nnz,
                                                 c->s, c->stream);
         if (he != hipSuccess)
           return fail(c, FPSI_ERR_HIP, std::string("cooperative FSGS sweep launch: ") + hipGetErrorString(he));
@@ -1465,6 +1555,16 @@ static int enqueue_solve(fpsi_ctx *c, int nsweeps) {
 
 static int need_sgs(fpsi_ctx *c) {
   if (c->line_order()) return fail(c, FPSI_ERR_STATE, "this call needs linear_solver = FPSI_SOLVER_SGS");
+  return 0;
+}
+
+// the tolerance-driven solve and the learned guess: block SGS, or line SGS
+// run as the dataflow sweep (its fused loop carries the tolerance test)
+static int need_tol_solver(fpsi_ctx *c) {
+  if (c->fsgs()) return fail(c, FPSI_ERR_STATE, "this call needs linear_solver = FPSI_SOLVER_SGS or FPSI_SOLVER_LSGS");
+  if (c->lsgs() && !(dataflow(c) && c->lsgs_flow))
+    return fail(c, FPSI_ERR_STATE, "a line-SGS solve to a tolerance needs the dataflow sweep on a grid whose "
+                                   "breadth-first levels are its Gauss-Seidel levels");
   return 0;
 }
 
@@ -1504,7 +1604,7 @@ static int enqueue_iteration(fpsi_ctx *c, double *dh) {
   HIPCHK(c, hipMemcpyAsync(dh, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   // turbulence scalars ride in the fluid's dataflow sweep when both use SGS,
   // or line SGS on levels the dataflow covers
-  const bool persistent = c->prm.sweep_mode == FPSI_SWEEP_PERSISTENT;
+  const bool persistent = dataflow(c);
   const bool fuse = c->kp.turb && !c->line_order() && persistent;
   const bool lfuse = c->kp.turb && c->lsgs() && persistent && c->lsgs_flow;
   if (lfuse) {
@@ -1640,9 +1740,9 @@ extern "C" int fpsi_synchronize(fpsi_ctx *c) {
 extern "C" int fpsi_profile_enable(fpsi_ctx *c, int on) {
   if (!c) return FPSI_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.second.first); c->ev_pool.push_back(p.second.second); }
+  for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.start); c->ev_pool.push_back(p.end); }
   c->ev_pending.clear();
-  for (int k = 0; k < FPSI_K_NGROUPS; ++k) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+  for (int k = 0; k < FPSI_K_NGROUPS; ++k) { c->prof_ms[k] = 0; c->prof_n[k] = 0; c->prof_b[k] = 0; c->prof_nb[k] = 0; }
   c->prof = on != 0;
   return 0;
 }
@@ -1652,11 +1752,12 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (auto &p : c->ev_pending) {
     float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, p.second.first, p.second.second));
-    c->prof_ms[p.first] += ms;
-    c->prof_n[p.first] += 1;
-    c->ev_pool.push_back(p.second.first);
-    c->ev_pool.push_back(p.second.second);
+    HIPCHK(c, hipEventElapsedTime(&ms, p.start, p.end));
+    c->prof_ms[p.grp] += ms;
+    c->prof_n[p.grp] += 1;
+    if (p.bytes >= 0) { c->prof_b[p.grp] += p.bytes; c->prof_nb[p.grp] += 1; }
+    c->ev_pool.push_back(p.start);
+    c->ev_pool.push_back(p.end);
   }
   c->ev_pending.clear();
   // algorithmic bytes of one launch of each group (each array once, gathers
@@ -1682,7 +1783,9 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
   for (int k = 0; k < FPSI_K_NGROUPS; ++k) {
     out->ms[k] = c->prof_ms[k];
     out->launches[k] = c->prof_n[k];
-    out->bytes[k] = b[k];
+    // launches that carried their own byte count (the line-SGS sweeps, whose
+    // systems leave the launches as their tests stop them) report its mean
+    out->bytes[k] = c->prof_nb[k] == c->prof_n[k] && c->prof_n[k] > 0 ? c->prof_b[k] / c->prof_n[k] : b[k];
   }
   return 0;
 }
@@ -1694,7 +1797,7 @@ extern "C" int fpsi_profile_read(fpsi_ctx *c, fpsi_profile *out) {
 // without their own check (solve_tol) are checked here
 static int linres(fpsi_ctx *c, double *out) {
   launch_ghost(c->g, c->s, c->stream);  // the periodic couplings read current ghosts
-  launch_linres(c->g, c->fp32, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
+  launch_linres(c->g, c->fp32, c->line_order(), c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->rb_host + 12, c->lr_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->rb_host + 13, c->s.serr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1707,7 +1810,6 @@ static int linres(fpsi_ctx *c, double *out) {
 extern "C" int fpsi_linear_residual(fpsi_ctx *c, double *norm) {
   if (!c || !norm) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
-  if (int rc = need_sgs(c)) return rc;
   return linres(c, norm);
 }
 
@@ -1718,24 +1820,36 @@ extern "C" int fpsi_linear_residual(fpsi_ctx *c, double *norm) {
 static int mlp_set_norm(fpsi_ctx *c) {
   MlpBufs m = c->mb;
   const int nc = c->nc;
-  float sums[16];
+  float sums[32];
   mlp_feature_stats(m, nc, m.loss, c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sums, m.loss, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  float nrm[16];
+  float nrm[32];
+  for (int k = 0; k < 32; ++k) nrm[k] = 0.f;
   for (int k = 0; k < 10; ++k) nrm[k] = std::sqrt(sums[k] / (float)nc) + 1e-30f;
   for (int k = 10; k < 15; ++k) nrm[k] = sums[k] / (float)nc;
-  nrm[15] = 0.f;
+  for (int k = 16; k < 26; ++k) nrm[k] = std::sqrt(sums[k] / (float)nc) + 1e-30f;
   HIPCHK(c, hipMemcpyAsync(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
   c->mlp_norm = true;
+  return 0;
+}
+
+// raw features of the assembled system with X = the plain start: X0, B, the
+// diagonal factors, and r0 = B - A X0 (into Dg) with its Jacobi correction
+static int mlp_raw(fpsi_ctx *c) {
+  const bool line = c->line_order();
+  launch_linres(c->g, c->fp32, line, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream, c->Dg);
+  mlp_features(c->mb, c->nc, c->fp32, c->s.X, c->s.B, line ? (const void *)c->s.lu64 : c->s.lu, line ? 1 : 0,
+               line ? nullptr : c->s.piv, c->Dg, c->stream);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
 static int mlp_guess(fpsi_ctx *c) {
   MlpBufs m = c->mb;
   const int nc = c->nc;
-  mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
+  if (int rc = mlp_raw(c)) return rc;
   if (int rc = mlp_set_norm(c)) return rc;
   mlp_normalise(m, nc, c->mlp_xcur, c->stream);
   m.x = c->mlp_xcur;
@@ -1745,7 +1859,10 @@ static int mlp_guess(fpsi_ctx *c) {
   return 0;
 }
 
-static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st) {
+// scalars: with line SGS, also run the turbulence scalars' own line-SGS
+// loops in the same launches (a Newton step); block SGS leaves them alone
+static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st,
+                     bool scalars = false) {
   fpsi_lin_stats s{};
   double rcur = 0;
   if (int rc = linres(c, &s.r0)) return rc;
@@ -1760,7 +1877,8 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
     // than the plain start
     launch_sub_d(5 * (long)c->nc, c->s.X, c->Xj, c->Dg, c->stream);
     launch_ghost_of(c->g, c->Dg, c->stream);
-    launch_linres_pair(c->g, c->fp32, c->s, c->Xj, c->Dg, c->lr_parts, c->lr_bsum, c->lr_out, c->stream);
+    launch_linres_pair(c->g, c->fp32, c->line_order(), c->s, c->Xj, c->Dg, c->lr_parts, c->lr_bsum, c->lr_out,
+                       c->stream);
     double h3[3] = {0, 0, 0};
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(h3, c->lr_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1775,10 +1893,17 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
       launch_copy_d(n, c->Xj, c->s.X, c->stream);  // fall back to the plain start
     }
   }
-  while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
-    if (int rc = enqueue_sweeps(c, 1, 0, false)) return rc;  // linres checks the sweep
-    ++s.sweeps;
-    if (int rc = linres(c, &rcur)) return rc;
+  if (c->line_order()) {
+    LinTol t{rtol, s.r0, max_sweeps, 0, rcur};
+    if (int rc = enqueue_lsgs_fused(c, c->prm.lsgs_maxiter, &t, scalars)) return rc;
+    s.sweeps = t.sweeps;
+    rcur = t.rfinal;
+  } else {
+    while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
+      if (int rc = enqueue_sweeps(c, 1, 0, false)) return rc;  // linres checks the sweep
+      ++s.sweeps;
+      if (int rc = linres(c, &rcur)) return rc;
+    }
   }
   s.rfinal = rcur;
   if (st) *st = s;
@@ -1787,7 +1912,7 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
 
 extern "C" int fpsi_linear_solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st) {
   if (!c) return FPSI_ERR_ARG;
-  if (int rc = need_sgs(c)) return rc;
+  if (int rc = need_tol_solver(c)) return rc;
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   const int rc = solve_tol(c, rtol, max_sweeps, use_guess, st);
   c->assembled = false;
@@ -1798,7 +1923,7 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
                                     fpsi_norms *hist, fpsi_lin_stats *lin) {
   if (!c) return FPSI_ERR_ARG;
   if (int rc = check_ready(c)) return rc;
-  if (int rc = need_sgs(c)) return rc;
+  if (int rc = need_tol_solver(c)) return rc;
   if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
   for (int s = 0; s < nsteps; ++s) {
     enqueue_assemble(c);
@@ -1807,13 +1932,14 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
     // NaN step is refused before its update
     HIPCHK(c, hipMemcpyAsync(c->rb_host, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     fpsi_lin_stats st{};
-    if (int rc = solve_tol(c, rtol, max_sweeps, use_guess, &st)) return rc;
+    if (int rc = solve_tol(c, rtol, max_sweeps, use_guess, &st, true)) return rc;
     const double *o = c->rb_host;
     if (hist) norms_from(o, &hist[s]);
     if (o[0] + o[1] + o[2] != o[0] + o[1] + o[2]) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
     if (lin) lin[s] = st;
-    // turbulence scalars: the reference's fixed-sweep scalar solve
-    if (c->kp.turb)
+    // turbulence scalars: the reference's own scalar solves (under line SGS
+    // they ran in the fluid's launches)
+    if (c->kp.turb && !c->line_order())
       if (int rc = enqueue_sa_solve(c, -1, c->prm.sgs_iter)) return rc;
     launch_update(c->g, c->kp, c->s, c->stream);
     if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
@@ -1859,7 +1985,7 @@ extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
     const float *d_zero = nullptr;
     if ((rc = upload(c, &d_off, c->mlp_nb_off)) || (rc = upload(c, &d_idx, c->mlp_nb_idx)) ||
         (rc = upload(c, &d_geo, c->mlp_geo)) || (rc = upload(c, &d_mask, mask)) || (rc = upload(c, &d_zero, zero64)) ||
-        (rc = dalloc(c, &m.raw, 16 * nc)) || (rc = dalloc(c, &m.norm, 16)) || (rc = dalloc(c, &m.h0, 64 * nc)) ||
+        (rc = dalloc(c, &m.raw, 32 * nc)) || (rc = dalloc(c, &m.norm, 32)) || (rc = dalloc(c, &m.h0, 64 * nc)) ||
         (rc = dalloc(c, &m.a1, 64 * nc)) || (rc = dalloc(c, &m.h1, 64 * nc)) || (rc = dalloc(c, &m.a2, 64 * nc)) ||
         (rc = dalloc(c, &m.h2, 64 * nc)) || (rc = dalloc(c, &m.out, 16 * nc)) || (rc = dalloc(c, &m.dout, 32 * nc)) ||
         (rc = dalloc(c, &m.g0, 64 * nc)) || (rc = dalloc(c, &m.g1, 64 * nc)) || (rc = dalloc(c, &m.g2, 64 * nc)) ||
@@ -1868,11 +1994,10 @@ extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
         (rc = dalloc(c, &m.av, (size_t)L.n)) || (rc = dalloc(c, &m.wb, (size_t)L.n)) ||
         (rc = dalloc(c, &m.wt1, 64 * 128)) || (rc = dalloc(c, &m.wt2, 64 * 128)) || (rc = dalloc(c, &m.wt3, 64 * 32)) ||
         (rc = dalloc(c, &m.wpart, (size_t)np * 64 * 128)) || (rc = dalloc(c, &m.bpart, (size_t)np * 64)) ||
-        (rc = dalloc(c, &m.lpart, (size_t)std::max(norm_parts(c->g), 1) * 16 + 16)) || (rc = dalloc(c, &m.loss, 16)) ||
+        (rc = dalloc(c, &m.lpart, (size_t)std::max(mlp_parts((int)nc), std::max(norm_parts(c->g), 1)) * 32 + 32)) ||
+        (rc = dalloc(c, &m.loss, 32)) ||
         (rc = dalloc(c, &c->mlp_xcur, 32 * nc)))
       return rc;
-    for (int s = 0; s < 4; ++s)
-      if ((rc = dalloc(c, &c->mlp_xs[s], 32 * nc)) || (rc = dalloc(c, &c->mlp_ys[s], 8 * nc))) return rc;
     m.nb_off = d_off;
     m.nb_idx = d_idx;
     m.geo = d_geo;
@@ -1887,7 +2012,7 @@ extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
   auto fill = [&](int off, int rows, int cols, double scale) {
     for (int i = 0; i < rows * cols; ++i) w[off + i] = (float)(scale * gauss(st));
   };
-  fill(L.w0, 64, 32, std::sqrt(2.0 / 19.0));
+  fill(L.w0, 64, 32, std::sqrt(2.0 / 29.0));  // 29 live input features
   fill(L.w1, 64, 128, std::sqrt(2.0 / 128.0));
   fill(L.w2, 64, 128, std::sqrt(2.0 / 128.0));
   fill(L.w3, 5, 64, 0.1 * std::sqrt(1.0 / 64.0));
@@ -1922,21 +2047,31 @@ extern "C" int fpsi_mlp_set_params(fpsi_ctx *c, const float *p) {
 
 extern "C" int fpsi_mlp_harvest(fpsi_ctx *c, int target_sweeps, int *slot) {
   if (!c) return FPSI_ERR_ARG;
-  if (int rc = need_sgs(c)) return rc;
+  if (int rc = need_tol_solver(c)) return rc;
   if (!c->mlp_alloc) return fail(c, FPSI_ERR_STATE, "fpsi_mlp_init not called");
   if (!c->assembled) return fail(c, FPSI_ERR_STATE, "system not assembled");
   const int nc = c->nc;
   MlpBufs m = c->mb;
-  mlp_features(m, nc, c->fp32, c->s.X, c->s.B, c->s.lu, c->stream);
-  if (int rc = mlp_set_norm(c)) return rc;
   const int sl = c->mlp_next;
+  if (!c->mlp_xs[sl]) {  // sample slots are allocated as they are first filled
+    if (int rc = dalloc(c, &c->mlp_xs[sl], 32 * (size_t)nc)) return rc;
+    if (int rc = dalloc(c, &c->mlp_ys[sl], 8 * (size_t)nc)) return rc;
+  }
+  if (int rc = mlp_raw(c)) return rc;
+  if (int rc = mlp_set_norm(c)) return rc;
   mlp_normalise(m, nc, c->mlp_xs[sl], c->stream);
-  if (int rc = enqueue_sweeps(c, target_sweeps)) return rc;
+  // the target: the solver's own iterate after target_sweeps sweeps
+  if (c->line_order()) {
+    LinTol t{0.0, 1.0, target_sweeps, 0, 1.0};
+    if (int rc = enqueue_lsgs_fused(c, c->prm.lsgs_maxiter, &t, false)) return rc;
+  } else if (int rc = enqueue_sweeps(c, target_sweeps)) {
+    return rc;
+  }
   mlp_target(m, nc, c->s.X, c->mlp_ys[sl], c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->mlp_next = (sl + 1) % 4;
-  c->mlp_nsamp = std::min(c->mlp_nsamp + 1, 4);
+  c->mlp_next = (sl + 1) % fpsi_ctx::MLP_SLOTS;
+  c->mlp_nsamp = std::min(c->mlp_nsamp + 1, (int)fpsi_ctx::MLP_SLOTS);
   c->assembled = false;
   if (slot) *slot = sl;
   return 0;

@@ -34,6 +34,19 @@ void launch_sweep_prep(const DevGrid &g, int fill, double *Xs, double *Xt, hipSt
 // dataflow launch over the SGS chunks (valid when the breadth-first levels
 // are the SGS levels)
 int lsgs_sweep_waves(int fp32, int ncu);
+// line SGS half-sweep as a flag-free dataflow (X and X2, sX and sX2
+// double-buffered; see k_lsgs_flow); before the first forward half-sweep of
+// a solve launch_lsgs_flow_fill puts sentinels into the forward targets,
+// and launch_lsgs_relax_flow / launch_sa_lsgs_relax_flow relax into X (sX)
+// and refill them
+int lsgs_flow_waves(int fp32, int ncu);
+hipError_t launch_lsgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
+                            int *err, int nwaves, int fluid, int ns, int smask, long nnz, DevState &s,
+                            hipStream_t st);
+void launch_lsgs_flow_fill(const DevGrid &g, DevState &s, int fluid, int ns, hipStream_t st);
+void launch_lsgs_relax_flow(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st);
+void launch_sa_lsgs_relax_flow(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
+                               hipStream_t st, int slot);
 // fluid = 0 skips the block rows; bit sc of smask sweeps turbulence scalar sc
 // (line SGS rows, xt kept before the forward sweep) in the same launch
 hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
@@ -43,8 +56,11 @@ void launch_ghost(const DevGrid &g, DevState &s, hipStream_t st);
 void launch_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // KEC: the time step's level n+1 from its last Newton iterate
 void launch_kec_final(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
-void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
-                   hipStream_t st);
+// linear residual norm of the current X; rawd: the diagonal array holds the
+// unfactored fluid_D (line SGS, FSGS) rather than pivoted LU factors; rout
+// (nullable) receives r = B - A X per cell [nc][5]
+void launch_linres(const DevGrid &g, int fp32, int rawd, DevState &s, double *parts, double *bsum, double *out,
+                   hipStream_t st, double *rout = nullptr);
 void launch_copy_d(long n, const double *a, double *b, hipStream_t st);
 // turbulence models: eddy viscosity (after the gradients)
 void launch_tmu(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
@@ -61,8 +77,8 @@ void launch_sa_lsgs_relax(const DevGrid &g, int sc, int iter, double abstol, dou
                           hipStream_t st, int slot = 0);
 void launch_sa_update(const DevGrid &g, const Prm &p, DevState &s, hipStream_t st);
 // weighted r.r, r.q, q.q of r = B - A X0, q = A D into out[0..2] (needs 3 * nblk(nc) parts)
-void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
-                        double *bsum, double *out, hipStream_t st);
+void launch_linres_pair(const DevGrid &g, int fp32, int rawd, DevState &s, const double *X0, const double *D,
+                        double *parts, double *bsum, double *out, hipStream_t st);
 void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream_t st);
 void launch_axpy_d(long n, const double *a, double alpha, const double *d, double *x, hipStream_t st);
 // periodic ghost entries V[nc + i] = V[partner cell], 5-wide rows

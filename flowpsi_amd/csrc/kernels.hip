@@ -1201,7 +1201,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j >= nup) : (j < nup);
-      x1[j] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + i) : 0.0;
+      x1[j] = pre ? ldx<false>(X + (size_t)nb[j] * 5 + i) : 0.0;
     }
     // turbulence scalar sc = i of this cell (lanes i < ns)
     const bool sact = act && i < sr.ns;
@@ -1216,7 +1216,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       for (int j = 0; j < MAXR; ++j) {
         sco[j] = j < len ? soff[j] : 0.0;
         const bool pre = BACK ? (j >= nup) : (j < nup);
-        sxv[j] = (pre && j < len) ? ldx<true>(sx + nb[j]) : 0.0;
+        sxv[j] = (pre && j < len) ? ldx<false>(sx + nb[j]) : 0.0;
       }
       if (!BACK) {
 #pragma unroll
@@ -1367,7 +1367,7 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j >= nup) : (j < nup);
-      x1[j] = pre ? ldx<true>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
+      x1[j] = pre ? ldx<false>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
     }
     if (!BACK) {  // forward, the upper terms come first
       if (q < CPW) {
@@ -1529,11 +1529,11 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
-      x1[j] = pre ? ldx<true>(X + (size_t)nb[j] * 5 + i) : 0.0;
+      x1[j] = pre ? ldx<false>(X + (size_t)nb[j] * 5 + i) : 0.0;
     }
     double xold = 0.0;
     if constexpr (BACK) {
-      if (xt) xold = ldx<true>(X + (size_t)cc * 5 + i);
+      if (xt) xold = ldx<false>(X + (size_t)cc * 5 + i);
     }
     // turbulence scalar sc = i of this cell
     const bool sact = act && i < ls.ns && ((ls.mask >> i) & 1);
@@ -1550,9 +1550,9 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         if (jr >= len) jr -= len;
         sco[j] = j < len ? soff[jr] : 0.0;
         const bool pre = j < len && (BACK ? (j < nlow) : (j >= nlow));
-        sxv[j] = pre ? ldx<true>(sx + nb[j]) : 0.0;
+        sxv[j] = pre ? ldx<false>(sx + nb[j]) : 0.0;
       }
-      if (!BACK) sold = ldx<true>(sx + cc);
+      if (!BACK) sold = ldx<false>(sx + cc);
       if (BACK) {
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
@@ -1668,7 +1668,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 // wave) ends the launch with *err set.  A system that is not in the launch
 // (fluid = 0, or its scalar's mask bit clear) polls and stores nothing.
 template <typename F, typename DG, bool BACK>
-__global__ void __launch_bounds__(64) k_lsgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
+__global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                   int spin_limit, int *__restrict__ err, const F *__restrict__ off,
                                                   const DG *__restrict__ diag, const F *__restrict__ B,
                                                   double *__restrict__ Xa, double *__restrict__ Xb, int fluid,
@@ -1715,7 +1715,7 @@ __global__ void __launch_bounds__(64) k_lsgs_flow(DevGrid g, int nchunks, int nw
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
-      x1[j] = pre ? ldx<true>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
+      x1[j] = pre ? ldx<false>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
     }
     // turbulence scalar sc = i of this cell
     const bool sact = act && i < ls.ns && ((ls.mask >> i) & 1);
@@ -1735,9 +1735,9 @@ __global__ void __launch_bounds__(64) k_lsgs_flow(DevGrid g, int nchunks, int nw
         if (jr >= len) jr -= len;
         sco[j] = j < len ? soff[jr] : 0.0;
         const bool pre = j < len && (BACK ? (j < nlow) : (j >= nlow));
-        sxv[j] = pre ? ldx<true>(sxs + nb[j]) : 0.0;
+        sxv[j] = pre ? ldx<false>(sxs + nb[j]) : 0.0;
       }
-      if (!BACK) sold = ldx<true>(sxa + cc);
+      if (!BACK) sold = ldx<false>(sxa + cc);
       if (BACK) {
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
@@ -2997,6 +2997,61 @@ int lsgs_sweep_waves(int fp32, int ncu) {
   const int b = fp32 ? lsgs_waves_t<float, float>(ncu) : a;
   return std::min(a, b);
 }
+template <typename F>
+static int lsgs_flow_waves_t(int ncu) {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_lsgs_flow<F, double, false>, 64, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_lsgs_flow<F, double, true>, 64, 0) != hipSuccess) return 0;
+  const int per_cu = std::min(a, b);
+  return per_cu > 0 ? per_cu * ncu : 0;
+}
+int lsgs_flow_waves(int fp32, int ncu) { return fp32 ? lsgs_flow_waves_t<float>(ncu) : lsgs_flow_waves_t<double>(ncu); }
+hipError_t launch_lsgs_flow(const DevGrid &g, int fp32, int backward, int nchunks, const int *chunk_lo, int spin_limit,
+                            int *err, int nwaves, int fluid, int ns, int smask, long nnz, DevState &s,
+                            hipStream_t st) {
+  int nw = std::max(1, std::min(nwaves, nchunks));
+  LineScalars ls{ns, smask, nnz, s.soff, s.sdinv, s.sB, s.sX, s.sxt};
+  double *sXb = s.sX2;
+  if (!smask) {
+    ls = LineScalars{0, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+    sXb = nullptr;
+  }
+  void *args[] = {(void *)&g, &nchunks, &nw, &chunk_lo, &spin_limit, &err, (void *)&s.off, (void *)&s.lu64,
+                  (void *)&s.B, (void *)&s.X, (void *)&s.X2, &fluid, (void *)&ls, (void *)&sXb};
+  const void *fn;
+  if (fp32) fn = backward ? (const void *)k_lsgs_flow<float, double, true> : (const void *)k_lsgs_flow<float, double, false>;
+  else fn = backward ? (const void *)k_lsgs_flow<double, double, true> : (const void *)k_lsgs_flow<double, double, false>;
+  return hipLaunchCooperativeKernel(fn, dim3(nw), dim3(64), args, 0, st);
+}
+void launch_lsgs_flow_fill(const DevGrid &g, DevState &s, int fluid, int ns, hipStream_t st) {
+  const size_t xs = (size_t)g.nc + g.nper + 1;
+  if (fluid) {
+    const long n = 5L * g.nc;
+    hipLaunchKernelGGL(k_flow_fill, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, s.X2, 5L,
+                       s.X2 + 5 * (xs - 1));
+  }
+  for (int sc = 0; sc < ns; ++sc) {
+    const long n = g.nc;
+    hipLaunchKernelGGL(k_flow_fill, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, s.sX2 + sc * xs, 1L,
+                       s.sX2 + sc * xs + (xs - 1));
+  }
+}
+void launch_lsgs_relax_flow(const DevGrid &g, int iter, double abstol, double reltol, DevState &s, hipStream_t st) {
+  const int n = g.nc * 5, np = lsgs_parts(g);
+  hipLaunchKernelGGL(k_lsgs_relax_flow, dim3(np), dim3(TPB), 0, st, n, (const double *)s.lst, s.X,
+                     (const double *)s.X2, s.X2, s.lparts);
+  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)s.lparts, s.lst, iter, abstol,
+                     reltol);
+}
+void launch_sa_lsgs_relax_flow(const DevGrid &g, int sc, int iter, double abstol, double reltol, DevState &s,
+                               hipStream_t st, int slot) {
+  const int np = nblk(g.nc);
+  const size_t nc = g.nc, xs = nc + g.nper + 1;
+  double *lst = s.lst + 8 * slot, *parts = s.lparts + (size_t)slot * lsgs_parts(g);
+  hipLaunchKernelGGL(k_lsgs_relax_flow, dim3(np), dim3(TPB), 0, st, g.nc, (const double *)lst, s.sX + sc * xs,
+                     (const double *)(s.sxt + sc * nc), s.sX2 + sc * xs, parts);
+  hipLaunchKernelGGL(k_lsgs_check, dim3(1), dim3(TPB), 0, st, np, (const double *)parts, lst, iter, abstol, reltol);
+}
 hipError_t launch_lsgs_sweep(const DevGrid &g, int fp32, int fsgs, int backward, int nchunks, const int *chunk_lo,
                              const int *dep_off, const int *dep, int *flag, int epoch, int spin_limit, int *err,
                              int nwaves, int fluid, int ns, int smask, long nnz, DevState &s, hipStream_t st) {
@@ -3105,20 +3160,24 @@ __global__ void k_sum_parts_d(int nparts, int len, const double *__restrict__ pa
   out[i] = s;
 }
 
-template <typename F>
-__global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__restrict__ off,
-                         const F *__restrict__ lu, const int *__restrict__ piv, const F *__restrict__ B,
-                         const double *__restrict__ X, double *__restrict__ part) {
-  __shared__ double sh[TPB];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (c < g.nc) {
-    double x[5], y[5], z[5], r[5];
+// D x of one cell: from the packed pivoted LU factors (P D = L U) under
+// block SGS, or from the unfactored block (RAWD: line SGS and FSGS keep
+// fluid_D itself in this array)
+template <typename F, bool RAWD>
+__device__ __forceinline__ void diag_times(const F *__restrict__ lu, int pk, int c, const double x[5], double out[5]) {
+  F a[25];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) x[k] = X[(size_t)c * 5 + k];
-    F a[25];
+  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+  if constexpr (RAWD) {
 #pragma unroll
-    for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+    for (int i = 0; i < 5; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) s += a[i + 5 * j] * x[j];
+      out[i] = s;
+    }
+  } else {
+    double y[5], z[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       double s = 0.0;
@@ -3133,32 +3192,58 @@ __global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__
       for (int j = 0; j < i; ++j) s += a[i + 5 * j] * y[j];
       z[i] = s;
     }
-    const int pk = piv[c];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) r[k] = (double)B[(size_t)c * 5 + k];
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       const int p = (pk >> (3 * i)) & 7;
 #pragma unroll
       for (int k = 0; k < 5; ++k)
-        if (k == p) r[k] -= z[i];
+        if (k == p) out[k] += z[i];
     }
-    const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
-    for (int j = m0; j < m1; ++j) {
-      const F *M = off + (size_t)j * 25;
-      const double *xn = X + (size_t)g.m_nbr[j] * 5;
+  }
+}
+
+// (A V)_c = D_c V_c + sum over the row of M_j V_nbr(j)
+template <typename F, bool RAWD>
+__device__ __forceinline__ void apply_a(const DevGrid &g, const F *__restrict__ off, const F *__restrict__ lu,
+                                        int pk, int c, const double *__restrict__ V, double out[5]) {
+  double x[5];
 #pragma unroll
-      for (int jj = 0; jj < 5; ++jj) {
-        const double xv = xn[jj];
+  for (int k = 0; k < 5; ++k) x[k] = V[(size_t)c * 5 + k];
+  diag_times<F, RAWD>(lu, pk, c, x, out);
+  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
+  for (int j = m0; j < m1; ++j) {
+    const F *M = off + (size_t)j * 25;
+    const double *xn = V + (size_t)g.m_nbr[j] * 5;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) r[i] -= M[i + 5 * jj] * xv;
-      }
+    for (int jj = 0; jj < 5; ++jj) {
+      const double xv = xn[jj];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) out[i] += M[i + 5 * jj] * xv;
     }
+  }
+}
+
+// block sums of the weighted norm of r = B - A X (per-component weights
+// 1 / rms(B_k)^2); r itself into rout[nc][5] when rout is not null
+template <typename F, bool RAWD>
+__global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__restrict__ off,
+                         const F *__restrict__ lu, const int *__restrict__ piv, const F *__restrict__ B,
+                         const double *__restrict__ X, double *__restrict__ part, double *__restrict__ rout) {
+  __shared__ double sh[TPB];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (c < g.nc) {
+    double ax[5];
+    apply_a<F, RAWD>(g, off, lu, RAWD ? 0 : piv[c], c, X, ax);
     const double ncd = (double)g.nc;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
+      const double r = (double)B[(size_t)c * 5 + k] - ax[k];
+      if (rout) rout[(size_t)c * 5 + k] = r;
       const double sB = sqrt(bsum[k] / ncd) + 1e-300;
-      const double e = r[k] / sB;
+      const double e = r / sB;
       acc += e * e;
     }
   }
@@ -3174,53 +3259,7 @@ __global__ void k_linres(DevGrid g, const double *__restrict__ bsum, const F *__
 // Weighted inner products of r = B - A X0 and q = A D (same operator and
 // per-component weights 1/rms(B_k)^2 as k_linres): part[3 * block + {0,1,2}]
 // = block sums of r.r, r.q, q.q.  Both vectors need current ghost copies.
-template <typename F>
-__device__ __forceinline__ void apply_a(const DevGrid &g, const F *__restrict__ off, const F *__restrict__ lu,
-                                        int pk, int c, const double *__restrict__ V, double out[5]) {
-  double x[5], y[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) x[k] = V[(size_t)c * 5 + k];
-  F a[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = i; j < 5; ++j) s += a[i + 5 * j] * x[j];
-    y[i] = s;
-  }
-  double z[5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    double s = y[i];
-#pragma unroll
-    for (int j = 0; j < i; ++j) s += a[i + 5 * j] * y[j];
-    z[i] = s;
-  }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) out[k] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    const int p = (pk >> (3 * i)) & 7;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (k == p) out[k] += z[i];
-  }
-  const int m0 = g.m_off[c], m1 = g.m_off[c + 1];
-  for (int j = m0; j < m1; ++j) {
-    const F *M = off + (size_t)j * 25;
-    const double *xn = V + (size_t)g.m_nbr[j] * 5;
-#pragma unroll
-    for (int jj = 0; jj < 5; ++jj) {
-      const double xv = xn[jj];
-#pragma unroll
-      for (int i = 0; i < 5; ++i) out[i] += M[i + 5 * jj] * xv;
-    }
-  }
-}
-
-template <typename F>
+template <typename F, bool RAWD>
 __global__ void k_linres_pair(DevGrid g, const double *__restrict__ bsum, const F *__restrict__ off,
                               const F *__restrict__ lu, const int *__restrict__ piv, const F *__restrict__ B,
                               const double *__restrict__ X0, const double *__restrict__ D,
@@ -3229,10 +3268,10 @@ __global__ void k_linres_pair(DevGrid g, const double *__restrict__ bsum, const 
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rq = 0.0, qq = 0.0;
   if (c < g.nc) {
-    const int pk = piv[c];
+    const int pk = RAWD ? 0 : piv[c];
     double ax[5], ad[5];
-    apply_a<F>(g, off, lu, pk, c, X0, ax);
-    apply_a<F>(g, off, lu, pk, c, D, ad);
+    apply_a<F, RAWD>(g, off, lu, pk, c, X0, ax);
+    apply_a<F, RAWD>(g, off, lu, pk, c, D, ad);
     const double ncd = (double)g.nc;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -3272,22 +3311,25 @@ __global__ void k_copy_d(long n, const double *__restrict__ a, double *__restric
 }
 
 // linear residual norm squared into out[0]; needs the ghost copies current
-void launch_linres(const DevGrid &g, int fp32, DevState &s, double *parts, double *bsum, double *out,
-                   hipStream_t st) {
+template <typename F, bool RAWD>
+static void linres_t(const DevGrid &g, DevState &s, double *parts, double *bsum, double *rout, hipStream_t st) {
   const int nb = nblk(g.nc);
+  hipLaunchKernelGGL(k_b_stats<F>, dim3(nb), dim3(TPB), 0, st, g.nc, (const F *)s.B, parts);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+  hipLaunchKernelGGL((k_linres<F, RAWD>), dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum, (const F *)s.off,
+                     (const F *)s.lu, (const int *)s.piv, (const F *)s.B, (const double *)s.X, parts, rout);
+}
+void launch_linres(const DevGrid &g, int fp32, int rawd, DevState &s, double *parts, double *bsum, double *out,
+                   hipStream_t st, double *rout) {
   launch_ghost(g, s, st);
   if (fp32) {
-    hipLaunchKernelGGL(k_b_stats<float>, dim3(nb), dim3(TPB), 0, st, g.nc, (const float *)s.B, parts);
-    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
-    hipLaunchKernelGGL(k_linres<float>, dim3(nb), dim3(TPB), 0, st, g, bsum, (const float *)s.off,
-                       (const float *)s.lu, s.piv, (const float *)s.B, s.X, parts);
+    if (rawd) linres_t<float, true>(g, s, parts, bsum, rout, st);
+    else linres_t<float, false>(g, s, parts, bsum, rout, st);
   } else {
-    hipLaunchKernelGGL(k_b_stats<double>, dim3(nb), dim3(TPB), 0, st, g.nc, (const double *)s.B, parts);
-    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
-    hipLaunchKernelGGL(k_linres<double>, dim3(nb), dim3(TPB), 0, st, g, bsum, (const double *)s.off,
-                       (const double *)s.lu, s.piv, (const double *)s.B, s.X, parts);
+    if (rawd) linres_t<double, true>(g, s, parts, bsum, rout, st);
+    else linres_t<double, false>(g, s, parts, bsum, rout, st);
   }
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 1, parts, out);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nblk(g.nc), 1, parts, out);
 }
 
 int lsgs_parts(const DevGrid &g) { return (int)(((long)g.nc * 5 + TPB - 1) / TPB); }
@@ -3342,21 +3384,25 @@ void launch_lsgs_relax(const DevGrid &g, int iter, double abstol, double reltol,
 // slot > 0: a turbulence scalar's own relaxation state and partial sums
 // (lst + 8 slot, lparts + slot x lsgs_parts), so its loop can run beside the fluid's
 
-void launch_linres_pair(const DevGrid &g, int fp32, DevState &s, const double *X0, const double *D, double *parts,
-                        double *bsum, double *out, hipStream_t st) {
+template <typename F, bool RAWD>
+static void linres_pair_t(const DevGrid &g, DevState &s, const double *X0, const double *D, double *parts,
+                          double *bsum, hipStream_t st) {
   const int nb = nblk(g.nc);
+  hipLaunchKernelGGL(k_b_stats<F>, dim3(nb), dim3(TPB), 0, st, g.nc, (const F *)s.B, parts);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+  hipLaunchKernelGGL((k_linres_pair<F, RAWD>), dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
+                     (const F *)s.off, (const F *)s.lu, (const int *)s.piv, (const F *)s.B, X0, D, parts);
+}
+void launch_linres_pair(const DevGrid &g, int fp32, int rawd, DevState &s, const double *X0, const double *D,
+                        double *parts, double *bsum, double *out, hipStream_t st) {
   if (fp32) {
-    hipLaunchKernelGGL(k_b_stats<float>, dim3(nb), dim3(TPB), 0, st, g.nc, (const float *)s.B, parts);
-    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
-    hipLaunchKernelGGL(k_linres_pair<float>, dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
-                       (const float *)s.off, (const float *)s.lu, s.piv, (const float *)s.B, X0, D, parts);
+    if (rawd) linres_pair_t<float, true>(g, s, X0, D, parts, bsum, st);
+    else linres_pair_t<float, false>(g, s, X0, D, parts, bsum, st);
   } else {
-    hipLaunchKernelGGL(k_b_stats<double>, dim3(nb), dim3(TPB), 0, st, g.nc, (const double *)s.B, parts);
-    hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
-    hipLaunchKernelGGL(k_linres_pair<double>, dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
-                       (const double *)s.off, (const double *)s.lu, s.piv, (const double *)s.B, X0, D, parts);
+    if (rawd) linres_pair_t<double, true>(g, s, X0, D, parts, bsum, st);
+    else linres_pair_t<double, false>(g, s, X0, D, parts, bsum, st);
   }
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 3, parts, out);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nblk(g.nc), 3, parts, out);
 }
 void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream_t st) {
   hipLaunchKernelGGL(k_sub_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, a, b, d);

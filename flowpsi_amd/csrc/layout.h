@@ -80,6 +80,7 @@ struct DevState {
   double *sdinv;  // [ns][nc] 1 / D
   double *sB;     // [ns][nc]
   double *sX;     // [ns][nc + nper + 1] scalar solutions with ghosts and the zero entry
+  double *sX2;    // second buffer of the flag-free line-SGS sweep, same shape
   double *sxt;    // [ns][nc] scalar line SGS: each cell's value before the forward sweep
   void *luf;      // [nc][25] FSGS: diagonal factored without pivots in the Jacobian precision
   // line SGS (allocated when that solver is selected)

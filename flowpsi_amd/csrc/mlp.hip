@@ -13,6 +13,7 @@
 // Every reduction (weight gradients, loss, feature scales) is a fixed-order
 // two-pass reduction, so training is bitwise reproducible.
 #include "mlp.h"
+#include "blocks.h"
 
 #include <algorithm>
 
@@ -325,13 +326,15 @@ __global__ void k_transpose_bf16(int K, const unsigned short *__restrict__ W, un
 // ---------------------------------------------------------------------------
 // features and guess
 // ---------------------------------------------------------------------------
-// raw per-cell quantities: X0 (Jacobi start), B, log|U_kk| of the LU
-template <typename F>
-__global__ void k_feat_raw(int N, const double *__restrict__ X, const F *__restrict__ B,
-                           const F *__restrict__ lu, float *__restrict__ raw) {
+// raw per-cell quantities [N][32]: X0 (the plain start), B, log|U_kk| of the
+// diagonal factors (0..14); then (k_feat_resid) the residual of the plain
+// start r0 = B - A X0 and the Jacobi correction D^-1 r0 (16..25)
+template <typename FB, typename FL>
+__global__ void k_feat_raw(int N, const double *__restrict__ X, const FB *__restrict__ B,
+                           const FL *__restrict__ lu, float *__restrict__ raw) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  float *o = raw + (size_t)c * 16;
+  float *o = raw + (size_t)c * 32;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     o[k] = (float)X[(size_t)c * 5 + k];
@@ -341,16 +344,42 @@ __global__ void k_feat_raw(int N, const double *__restrict__ X, const F *__restr
   o[15] = 0.f;
 }
 
-// feature scales: part[p][0..15] = sum of squares (0..9) and sums (10..14)
+// r0 and D^-1 r0 from the diagonal factors (pivoted under block SGS, piv
+// null: unpivoted FP64 factors of line SGS)
+template <typename FL>
+__global__ void k_feat_resid(int N, const double *__restrict__ r, const FL *__restrict__ lu,
+                             const int *__restrict__ piv, float *__restrict__ raw) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double b[5], x[5];
+  FL a[25];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) b[k] = r[(size_t)c * 5 + k];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = lu[(size_t)c * 25 + i];
+  lusolve5<FL>(a, piv ? piv[c] : PIV_ID, b, x);
+  float *o = raw + (size_t)c * 32;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    o[16 + k] = (float)b[k];
+    o[21 + k] = (float)x[k];
+  }
+#pragma unroll
+  for (int k = 26; k < 32; ++k) o[k] = 0.f;
+}
+
+// feature scales: part[p][0..31] = sums of squares (X0, B, r0, D^-1 r0)
+// and sums (log|U_kk|)
+__device__ __forceinline__ bool feat_squared(int k) { return k < 10 || (k >= 16 && k < 26); }
 __global__ void k_feat_stats(int N, const float *__restrict__ raw, float *__restrict__ part) {
-  __shared__ float sh[TPB][16];
+  __shared__ float sh[TPB][32];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < 32; ++k) {
     float v = 0.f;
-    if (c < N && k < 15) {
-      const float x = raw[(size_t)c * 16 + k];
-      v = k < 10 ? x * x : x;
+    if (c < N && k != 15 && k < 26) {
+      const float x = raw[(size_t)c * 32 + k];
+      v = feat_squared(k) ? x * x : x;
     }
     sh[threadIdx.x][k] = v;
   }
@@ -358,29 +387,32 @@ __global__ void k_feat_stats(int N, const float *__restrict__ raw, float *__rest
   for (int w = TPB / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w)
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + w][k];
+      for (int k = 0; k < 32; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + w][k];
     __syncthreads();
   }
-  if (threadIdx.x < 16) part[(size_t)blockIdx.x * 16 + threadIdx.x] = sh[0][threadIdx.x];
+  if (threadIdx.x < 32) part[(size_t)blockIdx.x * 32 + threadIdx.x] = sh[0][threadIdx.x];
 }
 
-// x[c] = normalised features; norm: sx[5], sb[5], mlu[5]
+// x[c] = normalised features: X0/sx, B/sb, log|U_kk| - mean, geometry (4),
+// r0/sr, D^-1 r0/sd
 __global__ void k_feat(int N, const float *__restrict__ raw, const float *__restrict__ geo,
                        const float *__restrict__ norm, float *__restrict__ x) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  const float *r = raw + (size_t)c * 16;
+  const float *r = raw + (size_t)c * 32;
   float *o = x + (size_t)c * 32;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     o[k] = r[k] / norm[k];
     o[5 + k] = r[5 + k] / norm[5 + k];
     o[10 + k] = r[10 + k] - norm[10 + k];
+    o[19 + k] = r[16 + k] / norm[16 + k];
+    o[24 + k] = r[21 + k] / norm[21 + k];
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) o[15 + k] = geo[(size_t)c * 4 + k];
 #pragma unroll
-  for (int k = 19; k < 32; ++k) o[k] = 0.f;
+  for (int k = 29; k < 32; ++k) o[k] = 0.f;
 }
 
 // training target y = (dq* - X0) / sx
@@ -390,7 +422,7 @@ __global__ void k_target(int N, const double *__restrict__ Xs, const float *__re
   if (c >= N) return;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    y[(size_t)c * 8 + k] = k < 5 ? (float)((Xs[(size_t)c * 5 + k] - (double)raw[(size_t)c * 16 + k]) / norm[k]) : 0.f;
+    y[(size_t)c * 8 + k] = k < 5 ? (float)((Xs[(size_t)c * 5 + k] - (double)raw[(size_t)c * 32 + k]) / norm[k]) : 0.f;
 }
 
 // X_guess = X0 + sx * out
@@ -400,7 +432,7 @@ __global__ void k_apply_guess(int N, const float *__restrict__ out, const float 
   if (c >= N) return;
 #pragma unroll
   for (int k = 0; k < 5; ++k)
-    X[(size_t)c * 5 + k] = (double)raw[(size_t)c * 16 + k] + (double)norm[k] * (double)out[(size_t)c * 16 + k];
+    X[(size_t)c * 5 + k] = (double)raw[(size_t)c * 32 + k] + (double)norm[k] * (double)out[(size_t)c * 16 + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -484,18 +516,23 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
 }  // namespace fpsi
 
 namespace fpsi {
-void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const void *B, const void *lu,
-                  hipStream_t st) {
-  if (fp32)
-    hipLaunchKernelGGL(k_feat_raw<float>, dim3(nblk(N)), dim3(TPB), 0, st, N, X, (const float *)B,
-                       (const float *)lu, m.raw);
+void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const void *B, const void *lu, int lu_f64,
+                  const int *piv, const double *r0, hipStream_t st) {
+  const dim3 gb(nblk(N)), tb(TPB);
+  if (fp32 && !lu_f64)
+    hipLaunchKernelGGL((k_feat_raw<float, float>), gb, tb, 0, st, N, X, (const float *)B, (const float *)lu, m.raw);
+  else if (fp32)
+    hipLaunchKernelGGL((k_feat_raw<float, double>), gb, tb, 0, st, N, X, (const float *)B, (const double *)lu, m.raw);
   else
-    hipLaunchKernelGGL(k_feat_raw<double>, dim3(nblk(N)), dim3(TPB), 0, st, N, X, (const double *)B,
-                       (const double *)lu, m.raw);
+    hipLaunchKernelGGL((k_feat_raw<double, double>), gb, tb, 0, st, N, X, (const double *)B, (const double *)lu, m.raw);
+  if (fp32 && !lu_f64)
+    hipLaunchKernelGGL(k_feat_resid<float>, gb, tb, 0, st, N, r0, (const float *)lu, piv, m.raw);
+  else
+    hipLaunchKernelGGL(k_feat_resid<double>, gb, tb, 0, st, N, r0, (const double *)lu, piv, m.raw);
 }
-void mlp_feature_stats(const MlpBufs &m, int N, float *out16, hipStream_t st) {
+void mlp_feature_stats(const MlpBufs &m, int N, float *out32, hipStream_t st) {
   hipLaunchKernelGGL(k_feat_stats, dim3(nblk(N)), dim3(TPB), 0, st, N, m.raw, m.lpart);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, nblk(N), 16, m.lpart, out16);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, nblk(N), 32, m.lpart, out32);
 }
 void mlp_normalise(const MlpBufs &m, int N, float *x, hipStream_t st) {
   hipLaunchKernelGGL(k_feat, dim3(nblk(N)), dim3(TPB), 0, st, N, m.raw, m.geo, m.norm, x);

@@ -126,15 +126,18 @@ def c4_hybrid(nx=8, nz=8, n_prism=8, n_hex=8, seed=4, **over):
     return g, q, prm
 
 
-def c5_bl(nx=16, ny=60, nz=16, aspect=1e3, seed=5, side=None, **over):
+def c5_bl(nx=16, ny=60, nz=16, aspect=1e3, seed=5, side=grids.SYMMETRY, **over):
     """Boundary-layer hex grid (wall at y = 0, first cell aspect ratio
     `aspect`, geometric growth to y = 1); Menter SST, M = 0.6 along x,
     Re = 5e6 per metre; k and omega from the inflow defaults with seeded
     noise plus the near-wall 6 nu / (beta1 d^2); line SGS; Venkatakrishnan
     limiter (Kl = 1), CFL 100 with flowPsi's default urelax = 0.2
-    (src/solverTimestep.loci:281-283), under which the grid converges to
-    machine level (with urelax unset, CFL 10 crawls: a 3e-4 drop after 800
-    steps)."""
+    (src/solverTimestep.loci:281-283).  The spanwise planes z = 0, 1 are
+    symmetry planes (`side`): with farfield there, the characteristic
+    boundary switches between its inflow and outflow states where the wall
+    meets it (w ~ 0) and the residual stalls near 3e-6; with symmetry the
+    grid converges to machine level (with urelax unset, CFL 10 crawls: a
+    3e-4 drop after 800 steps)."""
     import numpy as np_
     g = grids.bl_box(nx, ny, nz, aspect, side=side)
     g.meta["dist"] = grids.wall_distance(g)

@@ -63,7 +63,10 @@ enum { FPSI_TIME_STEADY = 0, FPSI_TIME_BDF2 = 1, FPSI_TIME_KEC = 2 };
  * TILED      one cooperative launch over tiles of <= 32 cells whose levels run
  *            in LDS (needs a grid whose cell boxes form an acyclic tile graph;
  *            fpsi_info.sweep_tiles > 0). */
-enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1, FPSI_SWEEP_TILED = 2 };
+enum { FPSI_SWEEP_PER_LEVEL = 0, FPSI_SWEEP_PERSISTENT = 1, FPSI_SWEEP_TILED = 2,
+       /* PERSISTENT_FLAGS: the dataflow with per-chunk flags for every system
+        * (the flag-free form polls the neighbours' values themselves) */
+       FPSI_SWEEP_PERSISTENT_FLAGS = 3 };
 
 /* Grid: the Loci FVM facts cl, cr, area{n,sada}, facecenter, vol,
  * cellcenter, pmap (used at src/fluxInviscid.loci:1170-1188,
@@ -137,6 +140,9 @@ typedef struct {
    * face Jacobians fjp / fjm of interior and periodic faces gain the
    * damping_fjp / damping_fjm terms (:794-866); <= 0: not set */
   double temporal_damping;
+  /* co-resident waves per compute unit of a line-SGS / FSGS dataflow
+   * launch (<= 0: every wave that fits) */
+  int sweep_waves_per_cu;
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);
@@ -349,7 +355,7 @@ int fpsi_mlp_get_params(fpsi_ctx *ctx, float *p);
 int fpsi_mlp_set_params(fpsi_ctx *ctx, const float *p);
 /* On an assembled system: record features and, as the target, the SGS
  * solution after target_sweeps sweeps (left in dfluid); *slot receives the
- * sample slot (the oldest of 4 is overwritten). */
+ * sample slot (the oldest of 16 is overwritten). */
 int fpsi_mlp_harvest(fpsi_ctx *ctx, int target_sweeps, int *slot);
 /* Adam over all recorded samples, epochs passes, mean loss of the last pass. */
 int fpsi_mlp_train(fpsi_ctx *ctx, int epochs, float lr, double *loss);

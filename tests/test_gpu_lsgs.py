@@ -114,15 +114,22 @@ def test_newton_steps(case):
         assert np.allclose(tot, tot_r, rtol=1e-8)
 
 
-def test_sgs_only_calls_refuse_lsgs():
+def test_tolerance_calls_under_line_solvers():
+    """The linear residual is defined under every solver; a solve to a
+    tolerance needs block SGS or line SGS run as the dataflow sweep (FSGS
+    and line SGS in level launches refuse it)."""
     g, q, prm = workloads.box(4)
-    s = Solver(g, dict(prm, solver="lsgs"))
+    s = Solver(g, dict(prm, solver="lsgs"))  # level launches
+    s.set_state(q)
+    s.assemble()
+    assert s.linear_residual() > 0
+    with pytest.raises(FpsiError):
+        s.linear_solve_tol(0.1, 5)
+    s = Solver(g, dict(prm, solver="fsgs", sweep="persistent"))
     s.set_state(q)
     s.assemble()
     with pytest.raises(FpsiError):
         s.linear_solve_tol(0.1, 5)
-    with pytest.raises(FpsiError):
-        s.linear_residual()
 
 
 def test_sgs_on_renumbered_grid():
