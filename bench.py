@@ -297,7 +297,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                          "(sharding the cells over GPUs is not implemented)"}))
         sys.exit(2)
 
     import numpy as np
@@ -387,7 +387,13 @@ def main():
             gb, qb, pb = workloads.box(32, sweep="persistent")
             cname = "Euler box 32^3 (32768 cells), Roe, CFL 10, SGS x5 (fixed paths)"
         log(f"time to 1e-8 drop on the {cname}")
-        conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), log=log)
+        if a.workload == "c5":
+            # line SGS: the tolerance paths stop once the linear residual has
+            # halved; fp32_max3 caps flowPsi's own loop at LSGSMaxIter = 3
+            conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), rtol=0.5, log=log,
+                                    paths=("fp64_ref", "fp32_ref", "fp32_max3", "fp32_tol", "fp32_nn"))
+        else:
+            conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), log=log)
         conv["workload"] = cname
         log("converge: " + ", ".join(f"{k}={v.get('time_to_target_s')}" for k, v in conv.items()
                                      if isinstance(v, dict) and "time_to_target_s" in v))

@@ -14,6 +14,9 @@ relative to the first Newton step.  Paths, all from the same initial state:
              is kept whenever the guess does not lower the linear residual)
   fp64_tol   FP64 with the same tolerance (separates precision from the
              solve's stopping rule)
+  fp32_max<k> FP32 with the workload's own solve capped at k sweeps
+             (LSGSMaxIter = k under line SGS, gauss_seidel_iter = k under
+             block SGS): flowPsi's options, no tolerance and no guess
 
 Every path is clocked directly: one Newton step per call, the host checks
 the drop after each, and the clock stops at the step whose residual first
@@ -101,16 +104,19 @@ def train_guess(g, q, prm, harvest_steps=8, target_sweeps=30, epochs=200, lr=2e-
 
 def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref_drop=1e-11,
             paths=("fp64_ref", "fp32_ref", "fp32_tol", "fp32_nn", "fp64_tol"), log=None):
-    """The paths on one workload.  rtol defaults to flowPsi's own relative
-    linear tolerance, LSGSRelTol = 0.1 (src/linearSolverLSGS.loci:96-107)."""
+    """The paths on one workload; rtol is the relative linear-residual
+    tolerance of the _tol and _nn paths."""
     say = log or (lambda m: None)
     out = {"target": target, "rtol": rtol, "max_sweeps": max_sweeps, "ref_drop": ref_drop}
     states = {}
     runs = {}
     for name in paths:
-        prec = name.split("_")[0]
+        prec, kind = name.split("_", 1)
         p = dict(prm, precision=prec)
-        tol = (rtol, max_sweeps) if name.endswith(("_tol", "_nn")) else None
+        if kind.startswith("max"):
+            k = int(kind[3:])
+            p.update(lsgs_maxiter=k) if prm.get("solver") == "lsgs" else p.update(sgs_iter=k)
+        tol = (rtol, max_sweeps) if kind in ("tol", "nn") else None
         tr = None
         solver = None
         if name.endswith("_nn"):

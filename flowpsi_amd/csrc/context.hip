@@ -76,6 +76,8 @@ struct fpsi_ctx {
   static constexpr int MLP_SLOTS = 16;  // harvested samples kept (the oldest is overwritten)
   float *mlp_xs[MLP_SLOTS] = {}, *mlp_ys[MLP_SLOTS] = {};
   float *mlp_xcur = nullptr;
+  double *mlp_xprev = nullptr;  // [nc][5] the last Newton update (zero before the first)
+  int tol_sweeps_last = 0;       // sweeps the last solve to a tolerance ran
   int mlp_nsamp = 0, mlp_next = 0, mlp_step = 0;
   // line SGS: levels of the breadth-first line order (new cell numbering);
   // lsgs_cells is null when every level is a contiguous cell range
@@ -927,6 +929,7 @@ extern "C" int fpsi_set_state(fpsi_ctx *c, const double *q) {
   HIPCHK(c, hipMemcpyAsync(c->s.q, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
+  c->tol_sweeps_last = 0;  // a new run: tolerance solves check from the first sweep
   c->assembled = false;
   c->have_prev = c->in_step = false;  // q{n-1} = q until set otherwise
   return 0;
@@ -1423,11 +1426,16 @@ static int enqueue_lsgs(fpsi_ctx *c, int maxiter) {
 // Stopping rule of the fluid's line-SGS loop run to a linear-residual
 // tolerance (in place of its update-norm test): stop once ||B - A X|| <=
 // rtol r0 or after max_sweeps relaxed symmetric sweeps.
+// The residual costs about a quarter of a symmetric sweep, so it is first
+// evaluated after check_from sweeps (one fewer than the previous solve of
+// the context needed; a solve that meets the tolerance earlier runs those
+// extra sweeps).
 struct LinTol {
   double rtol, r0;
   int max_sweeps;
   int sweeps;
   double rfinal;
+  int check_from;
 };
 
 static int linres(fpsi_ctx *c, double *out);
@@ -1487,8 +1495,10 @@ static int enqueue_lsgs_fused(fpsi_ctx *c, int maxiter, LinTol *tol = nullptr, b
       c->lsgs_iters = (int)c->lsgs_host[3];
       if (tol) {
         ++tol->sweeps;
-        if (int rc = linres(c, &tol->rfinal)) return rc;
-        if (tol->rfinal <= tol->rtol * tol->r0) fluid = false;
+        if (tol->sweeps >= tol->check_from || tol->sweeps >= fmax) {
+          if (int rc = linres(c, &tol->rfinal)) return rc;
+          if (tol->rfinal <= tol->rtol * tol->r0) fluid = false;
+        }
       } else if (c->lsgs_host[4] != 0.0) {
         fluid = false;
       }
@@ -1835,12 +1845,13 @@ static int mlp_set_norm(fpsi_ctx *c) {
   return 0;
 }
 
-// raw features of the assembled system with X = the plain start: X0, B, the
-// diagonal factors, and r0 = B - A X0 (into Dg) with its Jacobi correction
+// raw features of the assembled system with X = the plain start: X0, the
+// previous Newton update, the diagonal factors, and r0 = B - A X0 (into Dg)
+// with its Jacobi correction
 static int mlp_raw(fpsi_ctx *c) {
   const bool line = c->line_order();
   launch_linres(c->g, c->fp32, line, c->s, c->lr_parts, c->lr_bsum, c->lr_out, c->stream, c->Dg);
-  mlp_features(c->mb, c->nc, c->fp32, c->s.X, c->s.B, line ? (const void *)c->s.lu64 : c->s.lu, line ? 1 : 0,
+  mlp_features(c->mb, c->nc, c->fp32, c->s.X, c->mlp_xprev, line ? (const void *)c->s.lu64 : c->s.lu, line ? 1 : 0,
                line ? nullptr : c->s.piv, c->Dg, c->stream);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -1893,19 +1904,24 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
       launch_copy_d(n, c->Xj, c->s.X, c->stream);  // fall back to the plain start
     }
   }
+  const int check_from = std::max(1, c->tol_sweeps_last - 1);
   if (c->line_order()) {
-    LinTol t{rtol, s.r0, max_sweeps, 0, rcur};
+    LinTol t{rtol, s.r0, max_sweeps, 0, rcur, check_from};
     if (int rc = enqueue_lsgs_fused(c, c->prm.lsgs_maxiter, &t, scalars)) return rc;
     s.sweeps = t.sweeps;
     rcur = t.rfinal;
   } else {
     while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
-      if (int rc = enqueue_sweeps(c, 1, 0, false)) return rc;  // linres checks the sweep
-      ++s.sweeps;
+      // linres checks the sweeps' error word
+      const int k = std::max(1, std::min(check_from - s.sweeps, max_sweeps - s.sweeps));
+      if (int rc = enqueue_sweeps(c, k, 0, false)) return rc;
+      s.sweeps += k;
       if (int rc = linres(c, &rcur)) return rc;
     }
   }
+  c->tol_sweeps_last = s.sweeps;
   s.rfinal = rcur;
+  if (c->mlp_alloc) launch_copy_d(5 * (long)c->nc, c->s.X, c->mlp_xprev, c->stream);
   if (st) *st = s;
   return 0;
 }
@@ -1996,7 +2012,7 @@ extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
         (rc = dalloc(c, &m.wpart, (size_t)np * 64 * 128)) || (rc = dalloc(c, &m.bpart, (size_t)np * 64)) ||
         (rc = dalloc(c, &m.lpart, (size_t)std::max(mlp_parts((int)nc), std::max(norm_parts(c->g), 1)) * 32 + 32)) ||
         (rc = dalloc(c, &m.loss, 32)) ||
-        (rc = dalloc(c, &c->mlp_xcur, 32 * nc)))
+        (rc = dalloc(c, &c->mlp_xcur, 32 * nc)) || (rc = dalloc(c, &c->mlp_xprev, 5 * nc)))
       return rc;
     m.nb_off = d_off;
     m.nb_idx = d_idx;
@@ -2018,6 +2034,7 @@ extern "C" int fpsi_mlp_init(fpsi_ctx *c, unsigned long long seed) {
   fill(L.w3, 5, 64, 0.1 * std::sqrt(1.0 / 64.0));
   HIPCHK(c, hipMemcpyAsync(m.w, w.data(), L.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(m.am, 0, L.n * sizeof(float), c->stream));
+  launch_zero_d(5 * (long)nc, c->mlp_xprev, c->stream);
   HIPCHK(c, hipMemsetAsync(m.av, 0, L.n * sizeof(float), c->stream));
   mlp_refresh_bf16(m, c->stream);
   HIPCHK(c, hipGetLastError());
@@ -2062,12 +2079,13 @@ extern "C" int fpsi_mlp_harvest(fpsi_ctx *c, int target_sweeps, int *slot) {
   mlp_normalise(m, nc, c->mlp_xs[sl], c->stream);
   // the target: the solver's own iterate after target_sweeps sweeps
   if (c->line_order()) {
-    LinTol t{0.0, 1.0, target_sweeps, 0, 1.0};
+    LinTol t{0.0, 1.0, target_sweeps, 0, 1.0, target_sweeps};
     if (int rc = enqueue_lsgs_fused(c, c->prm.lsgs_maxiter, &t, false)) return rc;
   } else if (int rc = enqueue_sweeps(c, target_sweeps)) {
     return rc;
   }
   mlp_target(m, nc, c->s.X, c->mlp_ys[sl], c->stream);
+  launch_copy_d(5 * (long)nc, c->s.X, c->mlp_xprev, c->stream);  // the next step's previous update
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mlp_next = (sl + 1) % fpsi_ctx::MLP_SLOTS;

@@ -15,8 +15,8 @@ struct MlpBufs {
   // graph: face neighbours of each cell (interior faces and periodic partners)
   const int *nb_off, *nb_idx;
   const float *geo;      // [N][4] shape factors and log volume
-  float *raw;            // [N][32] X0, B, log|U_kk|, -, r0 = B - A X0, D^-1 r0
-  float *norm;           // [32] sx[5], sb[5], mean log|U_kk|[5], -, sr[5], sd[5]
+  float *raw;            // [N][32] X0, previous update, log|U_kk|, -, r0 = B - A X0, D^-1 r0
+  float *norm;           // [32] sx[5], sp[5], mean log|U_kk|[5], -, sr[5], sd[5]
   float *x;              // [N][32] features of the current input
   float *y;              // [N][8] target
   float *h0, *a1, *h1, *a2, *h2, *out;  // activations
@@ -31,11 +31,12 @@ struct MlpBufs {
 
 void mlp_forward(const MlpBufs &m, int N, hipStream_t st);
 void mlp_backward(const MlpBufs &m, int N, hipStream_t st);
-// raw features of the current system: X0 = X, B and the diagonal factors lu
-// (real_fj packed pivoted LU with piv, or with lu_f64 the unpivoted FP64
-// factors and piv null), and r0 = B - A X0 [N][5]
-void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const void *B, const void *lu, int lu_f64,
-                  const int *piv, const double *r0, hipStream_t st);
+// raw features of the current system: X0 = X, the previous Newton update
+// Xprev [N][5], the diagonal factors lu (real_fj packed pivoted LU with piv,
+// or with lu_f64 the unpivoted FP64 factors and piv null), and
+// r0 = B - A X0 [N][5]
+void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const double *Xprev, const void *lu,
+                  int lu_f64, const int *piv, const double *r0, hipStream_t st);
 void mlp_feature_stats(const MlpBufs &m, int N, float *out32, hipStream_t st);
 void mlp_normalise(const MlpBufs &m, int N, float *x, hipStream_t st);
 void mlp_target(const MlpBufs &m, int N, const double *Xs, float *y, hipStream_t st);

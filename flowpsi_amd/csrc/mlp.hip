@@ -1,4 +1,4 @@
-// Learned initial guess for the block-SGS solve: a two-hop message-passing
+// Learned initial guess for the block-SGS / line-SGS solve: a two-hop message-passing
 // MLP of width 64 with bf16 GEMM operands on MFMA (v_mfma_f32_16x16x32_bf16,
 // fp32 accumulation), fp32 activations and fp32 master weights.
 //
@@ -326,11 +326,12 @@ __global__ void k_transpose_bf16(int K, const unsigned short *__restrict__ W, un
 // ---------------------------------------------------------------------------
 // features and guess
 // ---------------------------------------------------------------------------
-// raw per-cell quantities [N][32]: X0 (the plain start), B, log|U_kk| of the
-// diagonal factors (0..14); then (k_feat_resid) the residual of the plain
-// start r0 = B - A X0 and the Jacobi correction D^-1 r0 (16..25)
-template <typename FB, typename FL>
-__global__ void k_feat_raw(int N, const double *__restrict__ X, const FB *__restrict__ B,
+// raw per-cell quantities [N][32]: X0 (the plain start), the previous
+// Newton update, log|U_kk| of the diagonal factors (0..14); then
+// (k_feat_resid) the residual of the plain start r0 = B - A X0 and the
+// Jacobi correction D^-1 r0 (16..25)
+template <typename FL>
+__global__ void k_feat_raw(int N, const double *__restrict__ X, const double *__restrict__ Xprev,
                            const FL *__restrict__ lu, float *__restrict__ raw) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -338,7 +339,7 @@ __global__ void k_feat_raw(int N, const double *__restrict__ X, const FB *__rest
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     o[k] = (float)X[(size_t)c * 5 + k];
-    o[5 + k] = (float)B[(size_t)c * 5 + k];
+    o[5 + k] = (float)Xprev[(size_t)c * 5 + k];
     o[10 + k] = logf(fabsf((float)lu[(size_t)c * 25 + k * 6]) + 1e-30f);
   }
   o[15] = 0.f;
@@ -368,7 +369,7 @@ __global__ void k_feat_resid(int N, const double *__restrict__ r, const FL *__re
   for (int k = 26; k < 32; ++k) o[k] = 0.f;
 }
 
-// feature scales: part[p][0..31] = sums of squares (X0, B, r0, D^-1 r0)
+// feature scales: part[p][0..31] = sums of squares (X0, previous update, r0, D^-1 r0)
 // and sums (log|U_kk|)
 __device__ __forceinline__ bool feat_squared(int k) { return k < 10 || (k >= 16 && k < 26); }
 __global__ void k_feat_stats(int N, const float *__restrict__ raw, float *__restrict__ part) {
@@ -393,8 +394,10 @@ __global__ void k_feat_stats(int N, const float *__restrict__ raw, float *__rest
   if (threadIdx.x < 32) part[(size_t)blockIdx.x * 32 + threadIdx.x] = sh[0][threadIdx.x];
 }
 
-// x[c] = normalised features: X0/sx, B/sb, log|U_kk| - mean, geometry (4),
-// r0/sr, D^-1 r0/sd
+// x[c] = normalised features: X0, the previous update and D^-1 r0 in units
+// of the RMS of D^-1 r0 per equation (the unit of the target, so that the
+// solution's dependence on them is the same map for every system), r0/sr,
+// log|U_kk| - mean, geometry (4)
 __global__ void k_feat(int N, const float *__restrict__ raw, const float *__restrict__ geo,
                        const float *__restrict__ norm, float *__restrict__ x) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,11 +406,12 @@ __global__ void k_feat(int N, const float *__restrict__ raw, const float *__rest
   float *o = x + (size_t)c * 32;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    o[k] = r[k] / norm[k];
-    o[5 + k] = r[5 + k] / norm[5 + k];
+    const float sd = norm[21 + k];
+    o[k] = r[k] / sd;
+    o[5 + k] = r[5 + k] / sd;
     o[10 + k] = r[10 + k] - norm[10 + k];
     o[19 + k] = r[16 + k] / norm[16 + k];
-    o[24 + k] = r[21 + k] / norm[21 + k];
+    o[24 + k] = r[21 + k] / sd;
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) o[15 + k] = geo[(size_t)c * 4 + k];
@@ -415,24 +419,26 @@ __global__ void k_feat(int N, const float *__restrict__ raw, const float *__rest
   for (int k = 29; k < 32; ++k) o[k] = 0.f;
 }
 
-// training target y = (dq* - X0) / sx
+// training target y = (dq* - X0) / sd, in units of the Jacobi correction's
+// RMS per equation
 __global__ void k_target(int N, const double *__restrict__ Xs, const float *__restrict__ raw,
                          const float *__restrict__ norm, float *__restrict__ y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    y[(size_t)c * 8 + k] = k < 5 ? (float)((Xs[(size_t)c * 5 + k] - (double)raw[(size_t)c * 32 + k]) / norm[k]) : 0.f;
+    y[(size_t)c * 8 + k] =
+        k < 5 ? (float)((Xs[(size_t)c * 5 + k] - (double)raw[(size_t)c * 32 + k]) / norm[21 + k]) : 0.f;
 }
 
-// X_guess = X0 + sx * out
+// X_guess = X0 + sd * out
 __global__ void k_apply_guess(int N, const float *__restrict__ out, const float *__restrict__ raw,
                               const float *__restrict__ norm, double *__restrict__ X) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
 #pragma unroll
   for (int k = 0; k < 5; ++k)
-    X[(size_t)c * 5 + k] = (double)raw[(size_t)c * 32 + k] + (double)norm[k] * (double)out[(size_t)c * 16 + k];
+    X[(size_t)c * 5 + k] = (double)raw[(size_t)c * 32 + k] + (double)norm[21 + k] * (double)out[(size_t)c * 16 + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -516,19 +522,16 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
 }  // namespace fpsi
 
 namespace fpsi {
-void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const void *B, const void *lu, int lu_f64,
-                  const int *piv, const double *r0, hipStream_t st) {
+void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const double *Xprev, const void *lu,
+                  int lu_f64, const int *piv, const double *r0, hipStream_t st) {
   const dim3 gb(nblk(N)), tb(TPB);
-  if (fp32 && !lu_f64)
-    hipLaunchKernelGGL((k_feat_raw<float, float>), gb, tb, 0, st, N, X, (const float *)B, (const float *)lu, m.raw);
-  else if (fp32)
-    hipLaunchKernelGGL((k_feat_raw<float, double>), gb, tb, 0, st, N, X, (const float *)B, (const double *)lu, m.raw);
-  else
-    hipLaunchKernelGGL((k_feat_raw<double, double>), gb, tb, 0, st, N, X, (const double *)B, (const double *)lu, m.raw);
-  if (fp32 && !lu_f64)
+  if (fp32 && !lu_f64) {
+    hipLaunchKernelGGL(k_feat_raw<float>, gb, tb, 0, st, N, X, Xprev, (const float *)lu, m.raw);
     hipLaunchKernelGGL(k_feat_resid<float>, gb, tb, 0, st, N, r0, (const float *)lu, piv, m.raw);
-  else
+  } else {
+    hipLaunchKernelGGL(k_feat_raw<double>, gb, tb, 0, st, N, X, Xprev, (const double *)lu, m.raw);
     hipLaunchKernelGGL(k_feat_resid<double>, gb, tb, 0, st, N, r0, (const double *)lu, piv, m.raw);
+  }
 }
 void mlp_feature_stats(const MlpBufs &m, int N, float *out32, hipStream_t st) {
   hipLaunchKernelGGL(k_feat_stats, dim3(nblk(N)), dim3(TPB), 0, st, N, m.raw, m.lpart);

@@ -54,10 +54,17 @@ def _read_var(f, name):
 def write(dirpath, case, solver, ncycle=0, stime=0.0):
     """Write the solver's state (and, time-accurate, level n-1) as restart
     files of case `case` under dirpath."""
-    os.makedirs(dirpath, exist_ok=True)
     prm = solver.prm
-    g = solver.grid
-    q = solver.get_state()
+    qm1 = None
+    if prm.get("unsteady", False) and prm.get("time_scheme", "bdf2") == "bdf2":
+        _, qm1 = solver.get_time_levels()
+    write_state(dirpath, case, solver.grid, prm, solver.get_state(), qm1, ncycle, stime)
+
+
+def write_state(dirpath, case, g, prm, q, qm1=None, ncycle=0, stime=0.0):
+    """Write a state q [nvar][nc] (and level n-1, qm1) of grid g as restart
+    files of case `case` under dirpath."""
+    os.makedirs(dirpath, exist_ok=True)
     pamb = float(prm.get("Pambient", 0.0))
     cc = np.ascontiguousarray(g.cc.T)
     _write_file(os.path.join(dirpath, f"flowVars_{case}"), {
@@ -67,8 +74,7 @@ def write(dirpath, case, solver, ncycle=0, stime=0.0):
     names = SCALARS[prm.get("turb", "none")]
     for j, x in enumerate(names):
         _write_file(os.path.join(dirpath, f"{x}_{case}"), {x: ("store", q[5 + j])})
-    if prm.get("unsteady", False) and prm.get("time_scheme", "bdf2") == "bdf2":
-        _, qm1 = solver.get_time_levels()
+    if qm1 is not None:
         _write_file(os.path.join(dirpath, f"flowVarsPrev_{case}"), {
             "temperature": ("store", qm1[0]), "gagePressure": ("store", qm1[4]), "u": ("vect3d", qm1[1:4].T),
             "Pambient": ("param", [pamb])})

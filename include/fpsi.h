@@ -335,9 +335,13 @@ typedef struct {
   double alpha;    /* step taken along the learned correction (X0 + alpha (guess - X0)) */
 } fpsi_lin_stats;
 
-/* SGS from the Jacobi start, or from the learned guess when use_guess is set
- * and the guess has a smaller residual than the Jacobi start (otherwise the
- * plain start is kept), sweeping until ||r|| <= rtol * r0 or max_sweeps. */
+/* Block SGS (or, with linear_solver = FPSI_SOLVER_LSGS and the dataflow
+ * sweep mode, relaxed line-SGS sweeps) from the Jacobi start, or from the
+ * learned guess when use_guess is set and the guess has a smaller residual
+ * than the Jacobi start (otherwise the plain start is kept), sweeping until
+ * ||r|| <= rtol * r0 or max_sweeps.  The residual costs about a quarter of a
+ * symmetric sweep, so it is first evaluated one sweep before the count the
+ * previous tolerance solve since fpsi_set_state needed. */
 int fpsi_linear_solve_tol(fpsi_ctx *ctx, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st);
 
 /* Steady Newton steps whose linear solves run to a tolerance (see above);
@@ -346,9 +350,11 @@ int fpsi_newton_adaptive(fpsi_ctx *ctx, int nsteps, double rtol, int max_sweeps,
                          fpsi_norms *hist, fpsi_lin_stats *lin);
 
 /* Learned initial guess: a two-hop message-passing MLP of width 64 (bf16
- * GEMM operands on MFMA, fp32 accumulation) mapping per-cell features
- * (Jacobi start, right-hand side, LU pivots, face geometry) to a correction
- * of the Jacobi start.  No counterpart in the reference. */
+ * GEMM operands on MFMA, fp32 accumulation) mapping per-cell features (the
+ * Jacobi start X0, the previous Newton update, the diagonal factors' pivots,
+ * the residual r0 = B - A X0 of the start and its Jacobi correction D^-1 r0,
+ * face geometry) to a correction of the Jacobi start, under block SGS and
+ * under line SGS.  No counterpart in the reference. */
 int fpsi_mlp_num_params(void);
 int fpsi_mlp_init(fpsi_ctx *ctx, unsigned long long seed);
 int fpsi_mlp_get_params(fpsi_ctx *ctx, float *p);

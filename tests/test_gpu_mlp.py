@@ -38,23 +38,27 @@ def test_forward_and_gradient_match_numpy():
 
 
 def test_training_reduces_loss_and_guess_saves_sweeps():
-    g, s = _solver(12)
+    """Trained on (system, 30-sweep solution) pairs from eight Newton steps
+    of the 16^3 Euler box, the guess saves sweeps on the four steps after
+    them: strictly fewer symmetric sweeps to the same linear tolerance, at
+    least two fewer over the four steps (measured: 24 plain, 20 learned)."""
+    g, s = _solver(16)
     s.mlp_init(seed=7)
-    # harvest from the first Newton steps (targets: 30 sweeps), advance the state
-    for _ in range(3):
+    s.newton(5)  # past the start transient
+    for _ in range(8):
         s.assemble()
         s.mlp_harvest(30)
         s.update()
     l0 = s.mlp_train(1, lr=0.0)
     l1 = s.mlp_train(300, lr=2e-3)
     print(f"\nloss before {l0:.4e} after {l1:.4e}")
-    assert l1 < 0.7 * l0
+    assert l1 < 0.5 * l0
     # held-out steps: same linear tolerance with and without the guess
     plain, learned = [], []
     q = s.get_state()
     for use in (False, True):
         s.set_state(q)
-        _, lin = s.newton_adaptive(3, 1e-3, 60, use_guess=use)
+        _, lin = s.newton_adaptive(4, 1e-3, 60, use_guess=use)
         (learned if use else plain).append(lin)
     sw_plain = sum(l["sweeps"] for l in plain[0])
     sw_learn = sum(l["sweeps"] for l in learned[0])
@@ -64,9 +68,33 @@ def test_training_reduces_loss_and_guess_saves_sweeps():
           f"{[round(l['alpha'], 3) for l in learned[0]]}")
     for l in learned[0]:
         assert l["rguess"] <= l["r0"] * (1 + 1e-12)
+        assert l["used_guess"] == 1
     for l in plain[0] + learned[0]:
         assert l["rfinal"] <= 1e-3 * l["r0"] or l["sweeps"] == 60
-    assert sw_learn <= sw_plain
+    assert sw_learn <= sw_plain - 2
+
+
+def test_guess_under_line_sgs():
+    """The same network under line SGS (the boundary-layer SST workload,
+    dataflow sweeps): harvest, training and the guess run; the guess is never
+    worse than the line-Jacobi start and the solve meets its tolerance."""
+    g, q, prm = workloads.c5_bl(nx=10, ny=24, nz=8, sweep="persistent", precision="fp32")
+    s = Solver(g, prm)
+    s.set_state(q)
+    s.mlp_init(seed=5)
+    for _ in range(4):
+        s.assemble()
+        s.mlp_harvest(20)
+        s.update()
+    l0 = s.mlp_train(1, lr=0.0)
+    l1 = s.mlp_train(100, lr=2e-3)
+    assert l1 < l0
+    _, lin = s.newton_adaptive(3, 0.3, 40, use_guess=True)
+    for l in lin:
+        assert l["rguess"] <= l["r0"] * (1 + 1e-12)
+        assert l["rfinal"] <= 0.3 * l["r0"] or l["sweeps"] == 40
+    print(f"\nline SGS guess: r_guess/r0 {[round(l['rguess'] / l['r0'], 3) for l in lin]}, "
+          f"sweeps {[l['sweeps'] for l in lin]}")
 
 
 def test_bad_guess_is_never_worse_than_plain_start():
