@@ -28,12 +28,14 @@ SWEEP = {"per_level": 0, "persistent": 1, "tiled": 2, "persistent_flags": 3}
 TURB = {"none": 0, "sa": 1, "sst": 2}
 TIME_SCHEME = {"bdf2": 1, "kec": 2}
 NSCALAR = {"none": 0, "sa": 1, "sst": 2}  # turbulence scalars after the 5 fluid variables
+NOALLOC_MAX_STEPS = 4096  # FPSI_NOALLOC_MAX_STEPS
 ERRORS = {-1: "FPSI_ERR_ARG", -2: "FPSI_ERR_HIP", -3: "FPSI_ERR_NAN", -4: "FPSI_ERR_STATE", -5: "FPSI_ERR_LU"}
 
 # every entry point declared in include/fpsi.h
 EXPORTS = ["fpsi_create", "fpsi_destroy", "fpsi_last_error", "fpsi_get_info", "fpsi_set_params",
            "fpsi_set_state", "fpsi_get_state", "fpsi_residual", "fpsi_gradients", "fpsi_assemble",
            "fpsi_get_system", "fpsi_linear_solve", "fpsi_update", "fpsi_newton_step",
+           "fpsi_newton_step_noalloc",
            "fpsi_synchronize", "fpsi_profile_enable", "fpsi_profile_read", "fpsi_linear_residual",
            "fpsi_linear_solve_tol", "fpsi_newton_adaptive", "fpsi_mlp_num_params", "fpsi_mlp_init",
            "fpsi_mlp_get_params", "fpsi_mlp_set_params", "fpsi_mlp_harvest", "fpsi_mlp_train",
@@ -346,6 +348,14 @@ class Solver:
     def newton(self, nsteps):
         hist = (Norms * max(nsteps, 1))()
         self._chk(self.L.fpsi_newton_step(self.h, int(nsteps), hist), "fpsi_newton_step")
+        return [hist[i].as_dict() for i in range(nsteps)]
+
+    def newton_noalloc(self, nsteps):
+        """newton() through the entry point that allocates nothing: the
+        norm history sits in the library's fixed per-GPU array
+        (at most NOALLOC_MAX_STEPS steps per call)."""
+        hist = (Norms * max(nsteps, 1))()
+        self._chk(self.L.fpsi_newton_step_noalloc(self.h, int(nsteps), hist), "fpsi_newton_step_noalloc")
         return [hist[i].as_dict() for i in range(nsteps)]
 
     # -- time-accurate BDF2

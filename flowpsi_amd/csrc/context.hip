@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -19,9 +20,21 @@
 
 using namespace fpsi;
 
+// Norm history of fpsi_newton_step_noalloc: one array per GPU in the code
+// object, FPSI_NOALLOC_MAX_STEPS steps of 12 doubles.  Host code never uses
+// the name (a host-side stand-in): the device address of each GPU's copy is
+// looked up once per device, with that device current, when the first
+// context is created on it, and kept in scratch_dev.
+static __device__ __align__(256) double scratch[12 * FPSI_NOALLOC_MAX_STEPS];
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
 struct fpsi_ctx {
   std::string err;
   hipStream_t stream = nullptr;
+  double *scratch = nullptr;  // this device's copy of `scratch`
+  double *hist_host = nullptr;  // pinned, 12 x 64 doubles: norm history read-back
   int nc = 0, nf = 0, nfi = 0, nper = 0, nb = 0, nnz = 0;
   std::vector<int> cperm;   // new -> old cell
   std::vector<int> icperm;  // old -> new cell
@@ -241,6 +254,7 @@ extern "C" void fpsi_destroy(fpsi_ctx *ctx) {
   for (void *p : ctx->allocs) hipFree(p);
   if (ctx->lsgs_host) hipHostFree(ctx->lsgs_host);
   if (ctx->rb_host) hipHostFree(ctx->rb_host);
+  if (ctx->hist_host) hipHostFree(ctx->hist_host);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -631,13 +645,29 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
     }
   }
 
-  // ---- device upload
+  // ---- device upload (a failure from here on leaves the context to the
+  // caller for fpsi_last_error and fpsi_destroy)
+  *out = c;
   HIPCHK(c, hipStreamCreate(&c->stream));
   HIPCHK(c, hipHostMalloc((void **)&c->rb_host, 16 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(c, hipHostMalloc((void **)&c->hist_host, 12 * 64 * sizeof(double), hipHostMallocDefault));
   int sweep_ncu = 1, sweep_cap = 0;
   {
     int dev = 0, ncu = 0;
     HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+      *out = c;
+      return fail(c, FPSI_ERR_ARG, "device ordinal beyond the norm-history table");
+    }
+    {
+      std::lock_guard<std::mutex> lk(scratch_mu);
+      if (!scratch_dev[dev]) {
+        double *p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = p;
+      }
+      c->scratch = scratch_dev[dev];
+    }
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     sweep_ncu = ncu;
     c->ncu = ncu;
@@ -1611,7 +1641,7 @@ extern "C" int fpsi_update(fpsi_ctx *c) {
 // solves of the fluid and the turbulence scalars, then the updates
 static int enqueue_iteration(fpsi_ctx *c, double *dh) {
   enqueue_assemble(c);
-  HIPCHK(c, hipMemcpyAsync(dh, c->s.nout, 12 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  launch_copy_d(12, c->s.nout, dh, c->stream);
   // turbulence scalars ride in the fluid's dataflow sweep when both use SGS,
   // or line SGS on levels the dataflow covers
   const bool persistent = dataflow(c);
@@ -1645,21 +1675,26 @@ static void enqueue_finish_step(fpsi_ctx *c) {
 
 // n Newton iterations (per time step when levels: then q{n,0} = q{n} and the
 // levels shift after the step's `levels` iterations); hist[n] nullable
-static int run_iterations(fpsi_ctx *c, int n, int levels, fpsi_norms *hist) {
-  std::vector<double> h(12 * (size_t)std::max(n, 1));
-  double *dh = nullptr;
-  if (n > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
+static int enqueue_iterations(fpsi_ctx *c, int n, int levels, double *dh) {
   for (int s = 0; s < n; ++s) {
     if (levels && s % levels == 0) {  // q{n,0} = q{n}; q{n-1} = q unless set
       c->in_step = false;
       enqueue_levels(c);
     }
-    if (int rc = enqueue_iteration(c, dh + 12 * (size_t)s)) {
-      hipStreamSynchronize(c->stream);
-      hipFree(dh);
-      return rc;
-    }
+    if (int rc = enqueue_iteration(c, dh + 12 * (size_t)s)) return rc;
     if (levels && (s + 1) % levels == 0) enqueue_finish_step(c);
+  }
+  return 0;
+}
+
+static int run_iterations(fpsi_ctx *c, int n, int levels, fpsi_norms *hist) {
+  std::vector<double> h(12 * (size_t)std::max(n, 1));
+  double *dh = nullptr;
+  if (n > 0) HIPCHK(c, hipMallocAsync((void **)&dh, h.size() * sizeof(double), c->stream));
+  if (int rc = enqueue_iterations(c, n, levels, dh)) {
+    hipStreamSynchronize(c->stream);
+    hipFree(dh);
+    return rc;
   }
   HIPCHK(c, hipGetLastError());
   if (n > 0) {
@@ -1682,6 +1717,41 @@ extern "C" int fpsi_newton_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
   if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
   if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
   return run_iterations(c, nsteps, 0, hist);
+}
+
+// fpsi_newton_step with the history in this device's `scratch` copy: the
+// copy kernel of each iteration writes all 12 norms of its step before they
+// are read, so nothing there needs clearing between calls; the history
+// comes back 64 steps at a time through the pinned hist_host
+extern "C" int fpsi_newton_step_noalloc(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {
+  if (!c) return FPSI_ERR_ARG;
+  if (int rc = check_ready(c)) return rc;
+  if (nsteps < 0) return fail(c, FPSI_ERR_ARG, "nsteps < 0");
+  if (nsteps > FPSI_NOALLOC_MAX_STEPS)
+    return fail(c, FPSI_ERR_ARG, "nsteps above FPSI_NOALLOC_MAX_STEPS (4096): the norm history would not fit");
+  if (c->bdf2()) return fail(c, FPSI_ERR_STATE, "time-accurate time_mode: use fpsi_unsteady_step");
+  if (c->prof) return fail(c, FPSI_ERR_STATE, "profiling creates events on demand: disable it for the no-allocation step");
+  if (!c->scratch) return fail(c, FPSI_ERR_STATE, "norm-history array not resolved for this device");
+  if (int rc = enqueue_iterations(c, nsteps, 0, c->scratch)) {
+    hipStreamSynchronize(c->stream);
+    return rc;
+  }
+  HIPCHK(c, hipGetLastError());
+  c->assembled = false;
+  for (int s0 = 0; s0 < nsteps; s0 += 64) {
+    const int m = std::min(64, nsteps - s0);
+    HIPCHK(c, hipMemcpyAsync(c->hist_host, c->scratch + 12 * (size_t)s0, 12 * (size_t)m * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < m; ++k) {
+      const double *h = c->hist_host + 12 * k;
+      if (hist) norms_from(h, &hist[s0 + k]);
+      if (h[0] + h[1] + h[2] != h[0] + h[1] + h[2])
+        return fail(c, FPSI_ERR_NAN, "NAN residual at Newton iteration " + std::to_string(s0 + k));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 extern "C" int fpsi_unsteady_step(fpsi_ctx *c, int nsteps, fpsi_norms *hist) {

@@ -267,6 +267,19 @@ int fpsi_update(fpsi_ctx *ctx);
  * residual evaluated at the start of each step. */
 int fpsi_newton_step(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, serving loops that forbid hipMalloc):
+ * fpsi_newton_step with the per-step norm history held in a fixed device
+ * array of FPSI_NOALLOC_MAX_STEPS steps (12 doubles each, 384 KiB per GPU,
+ * held until the process ends) instead of a per-call allocation; the call
+ * allocates no device or host memory on success.  nsteps above the maximum
+ * is refused (FPSI_ERR_ARG), and so is a context with profiling on (its
+ * events are created on demand).  The array is one per GPU, shared by every
+ * context there and keeping the last call's contents: calls that overlap on
+ * one GPU (several host threads) race. */
+#define FPSI_NOALLOC_MAX_STEPS 4096
+int fpsi_newton_step_noalloc(fpsi_ctx *ctx, int nsteps, fpsi_norms *hist);
+
 /* Time-accurate steps (FPSI_TIME_BDF2, FPSI_TIME_KEC).  The context holds the
  * state q at time level n and the level before it, q{n-1} (read by BDF2
  * only); fpsi_set_state makes q{n-1} = q (the reference's start "constant in
