@@ -181,8 +181,9 @@ def kernel_name(prof, prm, sweep, info, precision):
     kname = dom
     solver = prm.get("solver", "sgs")
     if dom == "sgs_sweep" and solver in ("lsgs", "fsgs"):
-        if sweep == "persistent" and info.get("lsgs_dataflow"):
-            kname = "k_lsgs_sweep"
+        if sweep in ("persistent", "persistent_flags") and info.get("lsgs_dataflow"):
+            # line SGS: flag-free (persistent) or chunk flags; FSGS always flags
+            kname = "k_lsgs_flow" if (sweep == "persistent" and solver == "lsgs") else "k_lsgs_sweep"
         else:
             avg_s /= info["lsgs_nlevels"]
             nbytes /= info["lsgs_nlevels"]
@@ -192,6 +193,8 @@ def kernel_name(prof, prm, sweep, info, precision):
             kname = "k_sgs_tiles"
         elif sweep == "persistent":
             kname = "k_sgs_flow" if prm.get("turb", "none") == "none" else "k_sgs_sweep"
+        elif sweep == "persistent_flags":
+            kname = "k_sgs_sweep"
         else:
             avg_s /= info["nlevels"]
             nbytes /= info["nlevels"]
@@ -275,8 +278,9 @@ def main():
                          "grids are extra lines")
     ap.add_argument("--n", type=int, default=None, help="cells per axis (c5: x and z; default 577)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--sweep", default=None, choices=["per_level", "tiled", "persistent"],
-                    help="default persistent (the dataflow); per_level for c3, whose six dependency levels "
+    ap.add_argument("--sweep", default=None, choices=["per_level", "tiled", "persistent", "persistent_flags"],
+                    help="default persistent (the flag-free dataflow); persistent_flags (chunk flags, the "
+                         "faster line-SGS dataflow) for c5; per_level for c3, whose six dependency levels "
                          "of ~690k cells each fill the GPU in one launch per level")
     ap.add_argument("--time-scheme", default="steady", choices=["steady", "bdf2", "kec"],
                     help="bdf2/kec: time-accurate dual time stepping; a step is one time step of "
@@ -312,7 +316,7 @@ def main():
     t0 = time.time()
     g, q, prm, wname, tag = build_workload(a)
     if a.sweep is None:
-        a.sweep = "per_level" if a.workload == "c3" else "persistent"
+        a.sweep = {"c3": "per_level", "c5": "persistent_flags"}.get(a.workload, "persistent")
     prm = dict(prm, precision=a.precision, sweep=a.sweep)
     unsteady = a.time_scheme != "steady"
     iters = a.newton_iter if unsteady else 1
@@ -390,7 +394,7 @@ def main():
         if a.workload == "c5":
             # line SGS: the tolerance paths stop once the linear residual has
             # halved; fp32_max3 caps flowPsi's own loop at LSGSMaxIter = 3
-            conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), rtol=0.5, log=log,
+            conv = converge.compare(gb, qb, dict(pb, sweep=a.sweep), rtol=0.5, log=log,
                                     paths=("fp64_ref", "fp32_ref", "fp32_max3", "fp32_tol", "fp32_nn"))
         else:
             conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), log=log)

@@ -70,14 +70,21 @@ class _Run:
             self.lin += l
         return np.sqrt(_tot(self.hist[-1]) / _tot(self.hist[0]))
 
-    def run_to(self, target, max_steps):
+    def run_to(self, target, max_steps, say=None, every_s=30.0):
+        """Newton steps until the drop is below target (or max_steps in
+        all); say(msg), when given, hears the progress every every_s
+        seconds (the host's drop check already synchronises each step)."""
         self.s.synchronize()
         t0 = time.perf_counter()
+        t_say = t0
         d = 1.0
         while len(self.hist) < max_steps:
             d = self.step()
             if d < target:
                 break
+            if say is not None and time.perf_counter() - t_say > every_s:
+                t_say = time.perf_counter()
+                say(f"  step {len(self.hist)}: drop {d:.2e} after {t_say - t0:.0f}s")
         self.s.synchronize()
         return time.perf_counter() - t0, d
 
@@ -103,13 +110,17 @@ def train_guess(g, q, prm, harvest_steps=8, target_sweeps=30, epochs=200, lr=2e-
 
 
 def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref_drop=1e-11,
-            paths=("fp64_ref", "fp32_ref", "fp32_tol", "fp32_nn", "fp64_tol"), log=None):
-    """The paths on one workload; rtol is the relative linear-residual
-    tolerance of the _tol and _nn paths."""
+            paths=("fp64_ref", "fp32_ref", "fp32_tol", "fp32_nn", "fp64_tol"), log=None,
+            continue_paths=True, train_kw=None):
+    """The paths on one workload, one solver alive at a time; rtol is the
+    relative linear-residual tolerance of the _tol and _nn paths.  The
+    first path is the reference: it runs on to ref_drop before the others
+    start.  continue_paths=False skips continuing the others to ref_drop
+    (their converged-state error is then not reported).  train_kw goes to
+    train_guess (harvest_steps, target_sweeps, epochs, lr)."""
     say = log or (lambda m: None)
     out = {"target": target, "rtol": rtol, "max_sweeps": max_sweeps, "ref_drop": ref_drop}
-    states = {}
-    runs = {}
+    qref = None
     for name in paths:
         prec, kind = name.split("_", 1)
         p = dict(prm, precision=prec)
@@ -120,10 +131,10 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
         tr = None
         solver = None
         if name.endswith("_nn"):
-            solver, tr = train_guess(g, q, p)
+            solver, tr = train_guess(g, q, p, **(train_kw or {}))
             say(f"{name}: trained in {tr['train_time_s']:.1f}s, loss {tr['loss_before']:.3e} -> {tr['loss_after']:.3e}")
         r = _Run(g, q, p, tol, name.endswith("_nn"), solver)
-        t, d = r.run_to(target, max_steps)
+        t, d = r.run_to(target, max_steps, say)
         n = len(r.hist) - 1 if d < target else None
         res = dict(steps_to_target=n, time_to_target_s=t if n is not None else None, final_drop=float(d),
                    ms_per_step=1e3 * t / max(len(r.hist), 1))
@@ -140,23 +151,22 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
             res["training"] = tr
             if res["time_to_target_s"] is not None:
                 res["time_to_target_incl_training_s"] = res["time_to_target_s"] + tr["train_time_s"]
-        states[name] = r.s.get_state()
-        say(f"{name}: {n} steps, {res['time_to_target_s']} s, drop {d:.2e}")
+        q_target = r.s.get_state()
+        say(f"{name}: {n} steps, {res['time_to_target_s']} s, drop {d:.2e}; {res}")
         out[name] = res
-        runs[name] = r
-    # the reference converged state: the FP64 plain path continued
-    base = "fp64_ref" if "fp64_ref" in runs else paths[0]
-    rb = runs[base]
-    rb.run_to(ref_drop, max_steps)
-    qref = rb.s.get_state()
-    say(f"reference: {base} continued to drop {np.sqrt(_tot(rb.hist[-1]) / _tot(rb.hist[0])):.2e} "
-        f"after {len(rb.hist)} steps")
-    for name, r in runs.items():
-        out[name]["state_err_at_target_vs_converged_fp64"] = state_err(states[name], qref)
-        if name != base:
-            r.run_to(ref_drop, max_steps)
-        out[name]["state_err_converged_vs_converged_fp64"] = state_err(r.s.get_state(), qref)
+        if qref is None:  # the reference converged state: this path continued
+            r.run_to(ref_drop, max_steps, say)
+            qref = r.s.get_state()
+            out["reference"] = dict(path=name, steps=len(r.hist),
+                                    drop=float(np.sqrt(_tot(r.hist[-1]) / _tot(r.hist[0]))))
+            say(f"reference: {name} continued to drop {out['reference']['drop']:.2e} after {len(r.hist)} steps")
+            res["state_err_converged_vs_converged_fp64"] = 0.0
+        elif continue_paths:
+            r.run_to(ref_drop, max_steps, say)
+            res["state_err_converged_vs_converged_fp64"] = state_err(r.s.get_state(), qref)
+        res["state_err_at_target_vs_converged_fp64"] = state_err(q_target, qref)
         r.s.close()
+        del r, q_target
     t64 = out.get("fp64_ref", {}).get("time_to_target_s")
     for name in paths:
         tn = out[name].get("time_to_target_s")

@@ -1584,6 +1584,9 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    // from the wait to the publish this chunk is on the dependency chain:
+    // it goes ahead of the SIMD's waves still loading their own chunks
+    __builtin_amdgcn_s_setprio(3);
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
@@ -1642,6 +1645,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);
     if (lane == 0) __hip_atomic_store(flag + ch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_setprio(0);
   }
 }
 

@@ -1,5 +1,6 @@
 """Line SGS and FSGS as one cooperative dataflow launch per half-sweep
-(sweep = persistent) against one launch per level: the same bits for the
+(sweep = persistent, flag-free; persistent_flags, chunk flags) against one
+launch per level: the same bits for the
 solution, the line-SGS iteration count, relaxation and update norm, and
 the state after Newton steps, in FP64 and FP32 blocks, on grids whose
 breadth-first levels are the SGS levels (where the dataflow runs; with a
@@ -35,7 +36,7 @@ def test_dataflow_line_sweeps_bitwise(name, prec, solver):
     prm = dict(prm, solver=solver, precision=prec, lsgs_maxiter=6, sgs_iter=4, lsgs_reltol=1e-3)
     print(f"\n[{name} {solver} {prec}] nc={g.nc}", flush=True)
     out = {}
-    for sweep in ("per_level", "persistent"):
+    for sweep in ("per_level", "persistent", "persistent_flags"):
         s = Solver(g, dict(prm, sweep=sweep))
         info = s.info()
         s.set_state(q)
@@ -51,9 +52,10 @@ def test_dataflow_line_sweeps_bitwise(name, prec, solver):
     if name not in ("tets", "hybrid"):
         assert out["persistent"][4]["lsgs_dataflow"] == 1
     dq0, st0, q0, h0, _, t0 = out["per_level"]
-    dq1, st1, q1, h1, _, t1 = out["persistent"]
-    assert t0 == t1
-    assert np.array_equal(dq0, dq1)
-    assert st0 == st1
-    assert np.array_equal(q0, q1)
-    assert h0 == h1
+    for mode in ("persistent", "persistent_flags"):
+        dq1, st1, q1, h1, _, t1 = out[mode]
+        assert t0 == t1, mode
+        assert np.array_equal(dq0, dq1), mode
+        assert st0 == st1, mode
+        assert np.array_equal(q0, q1), mode
+        assert h0 == h1, mode
