@@ -27,7 +27,9 @@ grid; the training time is reported beside the path and added in
 
 Final states are compared with an FP64 reference run continued to a drop
 of `ref_drop` (well below the target): at the step each path reaches the
-target, and after each path is itself continued to `ref_drop`.
+target, and after each path is itself continued to `ref_drop`.  Each
+path's state at the target is also compared with the first path's state at
+its target (the precision's own error at the same residual level).
 """
 from __future__ import annotations
 
@@ -120,7 +122,7 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
     train_guess (harvest_steps, target_sweeps, epochs, lr)."""
     say = log or (lambda m: None)
     out = {"target": target, "rtol": rtol, "max_sweeps": max_sweeps, "ref_drop": ref_drop}
-    qref = None
+    qref = q_base = None
     for name in paths:
         prec, kind = name.split("_", 1)
         p = dict(prm, precision=prec)
@@ -154,7 +156,10 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
         q_target = r.s.get_state()
         say(f"{name}: {n} steps, {res['time_to_target_s']} s, drop {d:.2e}; {res}")
         out[name] = res
+        if q_base is not None:
+            res["state_err_at_target_vs_first_path_at_target"] = state_err(q_target, q_base)
         if qref is None:  # the reference converged state: this path continued
+            q_base = q_target.copy()
             r.run_to(ref_drop, max_steps, say)
             qref = r.s.get_state()
             out["reference"] = dict(path=name, steps=len(r.hist),
