@@ -52,6 +52,14 @@ class GridDesc(C.Structure):
                 ("pshift", _dp)]
 
 
+MAX_BC_SETS = 8  # FPSI_MAX_BC_SETS
+
+
+class BcSet(C.Structure):
+    _fields_ = [("T", C.c_double), ("pg", C.c_double), ("u", C.c_double * 3), ("p0", C.c_double),
+                ("T0", C.c_double), ("mdot", C.c_double)]
+
+
 class Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("Rtilde", C.c_double), ("Pambient", C.c_double),
                 ("flux", C.c_int), ("limiter", C.c_int), ("Kl", C.c_double), ("viscous", C.c_int),
@@ -65,7 +73,8 @@ class Params(C.Structure):
                 ("nut_bc", C.c_double), ("k_bc", C.c_double), ("w_bc", C.c_double),
                 ("time_mode", C.c_int), ("newton_iter", C.c_int), ("pseudo_cflmax", C.c_double),
                 ("pbias_control", C.c_double), ("sweep_spin_limit", C.c_int),
-                ("temporal_damping", C.c_double), ("sweep_waves_per_cu", C.c_int)]
+                ("temporal_damping", C.c_double), ("sweep_waves_per_cu", C.c_int),
+                ("bc", BcSet * MAX_BC_SETS)]
 
 
 class Norms(C.Structure):
@@ -172,6 +181,17 @@ def make_params(prm: dict) -> Params:
     s.sweep_spin_limit = int(prm.get("sweep_spin_limit", 0))  # <= 0: the library's bound
     s.temporal_damping = float(prm.get("temporal_damping", 0.0))  # temporalDamping (<= 0: not set)
     s.sweep_waves_per_cu = int(prm.get("sweep_waves_per_cu", 0))  # <= 0: every co-resident wave
+    # boundary parameter sets: bc_sets[k] = dict(T, pg or p, u, p0, T0, mdot)
+    # for the faces whose btype carries set k (grids.bc_code)
+    for k, b in enumerate(prm.get("bc_sets", ())):
+        d = s.bc[k]
+        d.T = float(b.get("T", 0.0))
+        d.pg = float(b["pg"]) if "pg" in b else float(b.get("p", 0.0)) - s.Pambient
+        u = b.get("u", (0.0, 0.0, 0.0))
+        d.u[:] = (float(u), 0.0, 0.0) if np.isscalar(u) else tuple(float(x) for x in u)
+        d.p0 = float(b.get("p0", 0.0))
+        d.T0 = float(b.get("T0", 0.0))
+        d.mdot = float(b.get("mdot", 0.0))
     return s
 
 

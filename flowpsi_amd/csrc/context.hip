@@ -33,6 +33,9 @@ static double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
 struct fpsi_ctx {
   std::string err;
   hipStream_t stream = nullptr;
+  double bc_area[FPSI_MAX_BC_SETS] = {};  // summed area of each parameter set's fixed-mass faces
+  bool has_set_bc = false;                // some face is of a type REFLECTING .. FIXED_MASS
+  double *bcs = nullptr;                  // device [FPSI_MAX_BC_SETS][8]: T, pg, u, p0, T0, massFlux
   double *scratch = nullptr;  // this device's copy of `scratch`
   double *hist_host = nullptr;  // pinned, 12 x 64 doubles: norm history read-back
   int nc = 0, nf = 0, nfi = 0, nper = 0, nb = 0, nnz = 0;
@@ -286,9 +289,12 @@ extern "C" int fpsi_create(fpsi_ctx **out, const fpsi_grid_desc *gd) {
         return fail(c, FPSI_ERR_ARG, "periodic face without a matching partner at face " + std::to_string(f));
       }
       kind[f] = 1;
-    } else if (t == FPSI_BC_FARFIELD || t == FPSI_BC_WALL || t == FPSI_BC_SYMMETRY) {
+    } else if (FPSI_BC_TYPE(t) >= FPSI_BC_FARFIELD && FPSI_BC_TYPE(t) <= FPSI_BC_FIXED_MASS &&
+               FPSI_BC_TYPE(t) != FPSI_BC_PERIODIC && FPSI_BC_SET(t) < FPSI_MAX_BC_SETS && (t >> 16) == 0) {
       if (r != -1) { *out = c; return fail(c, FPSI_ERR_ARG, "boundary face must have cr = -1"); }
       kind[f] = 2;
+      if (FPSI_BC_TYPE(t) == FPSI_BC_FIXED_MASS) c->bc_area[FPSI_BC_SET(t)] += gd->area[f];
+      if (FPSI_BC_TYPE(t) >= FPSI_BC_REFLECTING) c->has_set_bc = true;
     } else {
       *out = c;
       return fail(c, FPSI_ERR_ARG, "unsupported boundary type at face " + std::to_string(f));
@@ -865,6 +871,8 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
     return fail(c, FPSI_ERR_ARG, "unknown turbulence model");
   if (p->turb != FPSI_TURB_NONE) {
     if (!p->viscous) return fail(c, FPSI_ERR_ARG, "a turbulence model needs viscous = 1");
+    if (c->has_set_bc)
+      return fail(c, FPSI_ERR_ARG, "turbulence scalars are not built for the boundary types REFLECTING .. FIXED_MASS");
     if (int rc = sa_alloc(c)) return rc;
   }
   if (p->linear_solver == FPSI_SOLVER_LSGS) {
@@ -904,9 +912,24 @@ extern "C" int fpsi_set_params(fpsi_ctx *c, const fpsi_params *p) {
       c->bdf2_alloc = true;
     }
   }
+  {  // boundary parameter sets; fixedMass's massFlux = mdot / boundary area (src/bcFixedMass.loci:230-232)
+    double h[FPSI_MAX_BC_SETS * 8];
+    for (int i = 0; i < FPSI_MAX_BC_SETS; ++i) {
+      const fpsi_bc_set &b = p->bc[i];
+      double *o = h + 8 * i;
+      o[0] = b.T; o[1] = b.pg; o[2] = b.u[0]; o[3] = b.u[1]; o[4] = b.u[2];
+      o[5] = b.p0; o[6] = b.T0;
+      o[7] = c->bc_area[i] > 0 ? b.mdot / c->bc_area[i] : 0.0;
+    }
+    if (!c->bcs)
+      if (int rc = dalloc(c, &c->bcs, (size_t)FPSI_MAX_BC_SETS * 8)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->bcs, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   c->prm = *p;
   c->newton_iter = p->newton_iter > 0 ? p->newton_iter : 6;
   Prm &k = c->kp;
+  k.bcs = c->bcs;
   k.gamma = p->gamma; k.Rt = p->Rtilde; k.Pamb = p->Pambient;
   k.flux = p->flux; k.limiter = p->limiter; k.Kl = p->Kl; k.viscous = p->viscous;
   k.a1 = p->suth[0]; k.a2 = p->suth[1]; k.a3 = p->suth[2];

@@ -37,7 +37,8 @@ __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *
   i.u = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
   i.pg = q[4 * nc + c];
   Q5 o;
-  const int t = g.fbt[f];
+  const int t = g.fbt[f] & 0xFF;
+  const double *bs = p.bcs + 8 * (g.fbt[f] >> 8);  // the face's parameter set (types 5-12)
   if (t == 1) {  // farfield
     const double us_n = 0.0, eta_pf = eta_face(p, i.T, i.u);
     const double uit = dot(i.u, n) - us_n;
@@ -58,13 +59,54 @@ __global__ void k_bface(DevGrid g, Prm p, const double *__restrict__ q, double *
     o.u = {0.0, 0.0, 0.0};
     o.pg = i.pg;
     o.T = p.Twall;
-  } else {  // symmetry
+  } else if (t == 3) {  // symmetry
     const double uit = dot(i.u, n);
     o.u = {i.u.x - uit * n.x, i.u.y - uit * n.y, i.u.z - uit * n.z};
     const double rho = (i.pg + p.Pamb) / (p.Rt * i.T);
     const double U = sqrt(dot(i.u, i.u));
     o.pg = i.pg + 0.5 * rho * U * uit;
     o.T = i.T;
+  } else if (t == 5) {  // reflecting, src/bcImpermeable.loci:102-117
+    reflecting_state(o, i, (i.pg + p.Pamb) / (p.Rt * i.T), n, 0.0, p.Pamb);
+  } else if (t == 6 || t == 7) {  // inflow (src/bcInflow.loci:377-402), supersonic inflow (:270-278)
+    Q5 r0;
+    r0.T = bs[0];
+    r0.pg = bs[1];
+    r0.u = {bs[2], bs[3], bs[4]};
+    o = r0;
+    const double uit = dot(i.u, n) - 0.0;
+    const double a0 = sqrt(p.gamma * p.Rt * r0.T);
+    if (t == 6 && !(uit > 0) && !(-uit > a0)) cv_inflow(o, i, r0, n, 0.0, p.Pamb, p.gamma, p.Rt, eta_face(p, i.T, i.u));
+  } else if (t == 8) {  // outflow with a fixed pressure, src/bcOutflow.loci:206-240
+    const double pref = bs[1];
+    const double uit = dot(i.u, n) - 0.0;
+    const double ai = sqrt(p.gamma * p.Rt * i.T);
+    const double rhoi = (i.pg + p.Pamb) / (p.Rt * i.T);
+    if (uit >= 0) {
+      o = i;
+      if (!(uit > ai)) {
+        o.pg = pref;
+        const double pa = o.pg + p.Pamb;
+        o.T = pa / (p.Rt * rhoi);
+      }
+    } else {  // reversed flow
+      o.u = {0.0 * n.x, 0.0 * n.y, 0.0 * n.z};
+      o.pg = i.pg - .05 * (i.pg + p.Pamb);
+      o.T = i.T;
+    }
+  } else if (t == 9) {  // supersonic outflow, src/bcOutflow.loci:127-141
+    o = i;
+    const double ai = sqrt(p.gamma * p.Rt * i.T);
+    const double ut = dot(n, i.u) - 0.0;
+    if (ut < ai) o.pg -= (i.pg + p.Pamb) * .1;
+  } else if (t == 10) {  // extrapolate, src/bcOutflow.loci:84-94
+    o = i;
+  } else if (t == 11) {  // isentropic inflow, src/bcIsentropic.loci:146-166
+    const V3 d = {-1. * n.x, -1. * n.y, -1. * n.z};
+    p0t0_state(o, bs[6], bs[5], p.gamma, p.Rt, i, d, p.Pamb);
+  } else {  // fixed mass, src/bcFixedMass.loci:140-153
+    const V3 d = {-1. * n.x, -1. * n.y, -1. * n.z};
+    fixed_mass_state(o, i.pg + p.Pamb, i.T, bs[7], bs[6], p.gamma, p.Rt, d, n, p.Pamb);
   }
   const int nb = g.nb;
   qf[b] = o.T;
@@ -261,7 +303,7 @@ __device__ __forceinline__ double tmu_face(const DevGrid &g, const double *__res
     const double vl = g.vol[c], vr = g.vol[o];
     return (tmu[o] * vl + tmu[c] * vr) / (vr + vl);
   }
-  return g.fbt[f] == 2 ? 0.0 : tmu[c];
+  return (g.fbt[f] & 0xFF) == 2 ? 0.0 : tmu[c];
 }
 // muTotal_f, kconductTotal (src/fluxViscous.loci:182-189,275-278)
 __device__ __forceinline__ void add_tmu(const DevGrid &g, const Prm &p, const double *__restrict__ tmu, int f,
@@ -315,7 +357,7 @@ __device__ inline void viscous_face(const DevGrid &g, const Prm &p, const double
     uf->y = qf[2 * nb + b];
     uf->z = qf[3 * nb + b];
     if (!need_flux) return;
-    if (g.fbt[f] == 3) {
+    if ((g.fbt[f] & 0xFF) == 3) {
       vf[0] = vf[1] = vf[2] = vf[3] = 0.0;
       return;
     }
@@ -431,14 +473,23 @@ __global__ void __launch_bounds__(TPB) k_face_flux(DevGrid g, Prm p, const doubl
     mev = dmax(mevl, mevr);
   } else {
     const int b = f - g.bnd_begin;
-    const int t = g.fbt[f];
+    const int t = g.fbt[f] & 0xFF;
     Q5 R;
     R.T = qf[b];
     R.u = {qf[nb + b], qf[2 * nb + b], qf[3 * nb + b]};
     R.pg = qf[4 * nb + b];
-    if (t == 1) {
-      if (p.flux == 1) roe_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
-      else hllc_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+    if (t == 1 || (t >= 6 && t <= 10)) {  // Riemann flux against the face state
+      // supersonic inflow uses the face's Eta_pf (src/bcInflow.loci:280-289)
+      const double eb = t == 7 ? eta_face(p, q[c], V3{q[nc + c], q[2 * nc + c], q[3 * nc + c]}) : eta;
+      if (p.flux == 1) roe_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eb);
+      else hllc_flux(fl, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eb);
+    } else if (t == 5) {  // reflecting: against the mirrored left state, no mass flux
+      const Q5 Rm = mirror(L, an, us_n);
+      if (p.flux == 1) roe_flux(fl, L, Rm, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+      else hllc_flux(fl, L, Rm, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+      fl[0] = 0;
+    } else if (t == 11 || t == 12) {  // the face state's Euler flux
+      euler_flux(fl, R, an, area, p.Pamb, p.Rt, p.gamma, us_n);
     } else if (t == 2) {
       const double pg = R.pg;
       fl[0] = 0;
@@ -695,21 +746,30 @@ __global__ void __launch_bounds__(TPB) k_face_jac(DevGrid g, Prm p, const double
     store_blk(off + (size_t)g.slot_cl[f] * 25, blk, false);
   } else {
     const int b = f - g.bnd_begin;
-    const int t = g.fbt[f];
+    const int t = g.fbt[f] & 0xFF;
 #pragma unroll
     for (int i = 0; i < 25; ++i) blk[i] = (F)0;
     Q5 R;
     R.T = qf[b];
     R.u = {qf[nb + b], qf[2 * nb + b], qf[3 * nb + b]};
     R.pg = qf[4 * nb + b];
-    if (t == 1) {
+    if (t == 1 || (t >= 6 && t <= 10)) {  // srcJ -= fjp(left, face state)
+      const double eb = t == 7 ? eta_face(p, q[c], V3{q[nc + c], q[2 * nc + c], q[3 * nc + c]}) : eta;
       F tmp[25];
 #pragma unroll
       for (int i = 0; i < 25; ++i) tmp[i] = (F)0;
-      const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, eta);
+      const RoeAvg ra = roe_average(L, R, an, p.Pamb, p.Rt, p.gamma, us_n, eb);
       roe_jacobian<F>(tmp, ra, an, area, p.gamma, p.Rt, 1);
 #pragma unroll
       for (int i = 0; i < 25; ++i) blk[i] -= tmp[i];
+    } else if (t == 5) {
+      reflecting_jacobian<F>(blk, L, an, area, p, us_n, eta);
+    } else if (t == 11 || t == 12) {
+      Q5 ci;
+      ci.T = q[c];
+      ci.u = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+      ci.pg = q[4 * nc + c];
+      bc_state_jacobian<F>(blk, t, ci, p.bcs + 8 * (g.fbt[f] >> 8), an, area, p, us_n);
     } else if (t == 2) {
       acc(blk[1 + 5 * 4], -(an.x * area));
       acc(blk[2 + 5 * 4], -(an.y * area));
@@ -2433,7 +2493,7 @@ __global__ void k_sa_face(DevGrid g, Prm p, const double *__restrict__ q, const 
     v[2] = gdn * area;
     double bj = 0.0;
     bj -= dmax(mdot, 0.0);
-    if (g.fbt[f] != 3) {
+    if ((g.fbt[f] & 0xFF) != 3) {
       const double C = (mu_f + (1. + cb2) * rho_f * Xf) * area;
       const double dd = dvec[0] * an[0] + dvec[1] * an[1] + dvec[2] * an[2];
       bj += -C / (dd * sig);

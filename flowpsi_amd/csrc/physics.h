@@ -32,6 +32,8 @@ struct Prm {
   double pseudo_cflmax;
   double pbias_ctl;       // timePressureBiasControl (KEC)
   double tdamp;           // temporalDamping beta (<= 0: not set)
+  // boundary parameter sets [set][8]: T, pg, ux, uy, uz, p0, T0, massFlux
+  const double *bcs;
 };
 
 struct V3 {
@@ -354,6 +356,204 @@ __device__ inline void cv_outflow(Q5 &b, const Q5 &i, const Q5 &r0s, const V3 &n
   const double rhob = r0 + (p - i.pg) * (sigma - uit * BetaM) / (Beta * a0 * a0 * (sigma + uit * BetaM));
   b.T = (p + Pambient) / (Rtilde * rhob);
   b.pg = p;
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions with a parameter set of their own (include/fpsi.h
+// FPSI_BC_REFLECTING .. FIXED_MASS)
+// ---------------------------------------------------------------------------
+// inviscidFlux (include/inviscidFlux.h:25-45): the Euler flux of one state
+__device__ inline void euler_flux(double *f, const Q5 &s, const V3 &an, double area, double Pamb, double Rt,
+                                  double gamma, double us_n) {
+  const double gm1 = gamma - 1;
+  const double rho = (s.pg + Pamb) / (Rt * s.T);
+  const double h0 = Rt * s.T * gamma / gm1 + 0.5 * dot(s.u, s.u);
+  const double e0 = h0 - Rt * s.T;
+  const double uta = dot(s.u, an);
+  const double ut = uta - us_n;
+  const double mdot = area * rho * ut;
+  const double pa = s.pg + Pamb;
+  f[0] = mdot;
+  f[1] = mdot * s.u.x + area * s.pg * an.x;
+  f[2] = mdot * s.u.y + area * s.pg * an.y;
+  f[3] = mdot * s.u.z + area * s.pg * an.z;
+  f[4] = mdot * e0 + area * pa * uta;
+}
+
+// reflecting face values, src/bcImpermeable.loci:102-117 (rho_i the cell's density)
+__device__ inline void reflecting_state(Q5 &o, const Q5 &i, double rho_i, const V3 &n, double us_n, double Pamb) {
+  const double uit = dot(i.u, n) - us_n;
+  o.u = {i.u.x - uit * n.x, i.u.y - uit * n.y, i.u.z - uit * n.z};
+  const double Pmin = 0.5 * (i.pg + Pamb) - Pamb;
+  o.pg = dmax(Pmin, i.pg + 0.5 * rho_i * uit * fabs(uit));
+  o.T = i.T;
+}
+
+// the mirrored right state of the reflecting flux, src/bcImpermeable.loci:119-130
+__device__ inline Q5 mirror(const Q5 &l, const V3 &n, double us_n) {
+  const double uit = dot(l.u, n) - us_n;
+  Q5 r;
+  r.T = l.T;
+  r.pg = l.pg;
+  r.u = {l.u.x - 2. * uit * n.x, l.u.y - 2. * uit * n.y, l.u.z - 2. * uit * n.z};
+  return r;
+}
+
+// fixed_p0t0_bc and total_pressure_bc_imp, src/bcIsentropic.loci:66-144:
+// the boundary state from p0, T0 and the outgoing Riemann invariant along
+// the flow direction d (the inward normal), with no rigid rotation
+__device__ inline void p0t0_state(Q5 &o, double T0, double p0, double gamma, double Rt, const Q5 &i, const V3 &d,
+                                  double Pamb) {
+  const double gm1 = gamma - 1.;
+  const double a0 = sqrt(gamma * Rt * T0);
+  const double ai = sqrt(gamma * Rt * i.T);
+  const double uit = dot(i.u, d);
+  double ab, unb;
+  {
+    const double rim = uit - 2. * ai / gm1;
+    const double a = (2. / gm1 + 1);
+    const double b = (2. * rim);
+    const double c = (0.5 * gm1 * rim * rim - a0 * a0);
+    const double D = b * b - 4. * a * c;
+    if (D < 0) {
+      ab = a0;
+    } else {
+      ab = (-b + sqrt(D)) / (2. * a);
+      ab = dmax(ab, .25 * a0);
+    }
+    unb = rim + 2. * ab / gm1;
+  }
+  const double maxMach = 1.0;
+  V3 ub = {unb * d.x + 0.0, unb * d.y + 0.0, unb * d.z + 0.0};
+  double M = sqrt(dot(ub, ub)) / ab;
+  if (M > maxMach) {
+    M = maxMach;
+    const double utot = M * a0 / sqrt(1 + .5 * gm1 * M * M);
+    const double un = sqrt(dmax(utot * utot - 0.0, 0.0));
+    ub = {un * d.x + 0.0, un * d.y + 0.0, un * d.z + 0.0};
+  }
+  double Tb = T0 / (1 + .5 * gm1 * M * M);
+  double pb = p0 * pow(1. + .5 * gm1 * M * M, -gamma / gm1);
+  double f = (Tb - i.T) / dmax(i.T, Tb) - .1;
+  if (f > 0) {
+    f = pow(f, .5);
+    ab = (1 - f) * ab + f * a0;
+    unb = (1 - f) * unb + f * uit;
+    ub = {unb * d.x + 0.0, unb * d.y + 0.0, unb * d.z + 0.0};
+    double M2 = sqrt(dot(ub, ub)) / ab;
+    if (M2 > maxMach) {
+      M2 = maxMach;
+      const double utot = M2 * a0 / sqrt(1 + .5 * gm1 * M2 * M2);
+      const double un = sqrt(dmax(utot * utot - 0.0, 0.0));
+      ub = {un * d.x + 0.0, un * d.y + 0.0, un * d.z + 0.0};
+    }
+    Tb = T0 / (1 + .5 * gm1 * M2 * M2);
+    pb = p0 * pow(1. + .5 * gm1 * M2 * M2, -gamma / gm1);
+  }
+  o.u = ub;
+  o.T = Tb;
+  o.pg = pb - Pamb;
+}
+
+// mass_flowrate_bc, src/bcFixedMass.loci:85-136: the temperature that
+// carries mass flux mflux at the cell's pressure pin with total
+// temperature Tstag (fixed-point iteration from the cell's temperature),
+// flowing along d (the inward normal)
+__device__ inline void fixed_mass_state(Q5 &o, double pin, double Tin, double mflux, double Tstag, double gamma,
+                                        double Rt, const V3 &d, const V3 &n, double Pamb) {
+  double T = Tin;
+  const double eps = Tin * 1e-9;
+  for (int iter = 0; iter < 1000; ++iter) {
+    const double rho = pin / (Rt * T);
+    const double a = sqrt(gamma * Rt * T);
+    const double u = mflux / rho;
+    const double M = u / a;
+    const double lamda = sqrt(((gamma + 1.0) * M * M) / ((gamma - 1.0) * M * M + 2));
+    const double p0 = pin * pow((1. - ((gamma - 1.) / (gamma + 1.)) * lamda * lamda), -gamma / (gamma - 1.0));
+    const double rho0 = p0 / (Rt * Tstag);
+    const double rho_1 = rho0 * pow((1. - (gamma - 1.0) / (gamma + 1.0) * lamda * lamda), 1.0 / (gamma - 1.0));
+    const double T_1 = pin / (rho_1 * Rt);
+    const double dT = T_1 - T;
+    if (fabs(dT) < eps) break;
+    T = T_1;
+  }
+  const double rhob = pin / (Rt * T);
+  const double vel = mflux / rhob;
+  const double sc = -vel / dot(d, n);
+  o.u = {d.x * sc + 0.0 + 0.0, d.y * sc + 0.0 + 0.0, d.z * sc + 0.0 + 0.0};
+  o.T = T;
+  o.pg = pin - Pamb;
+}
+
+// inviscidRiemannFlux: the flux option's Riemann flux
+__device__ inline void riemann_flux(double *f, const Q5 &L, const Q5 &R, const V3 &an, double area, const Prm &p,
+                                    double us_n, double eta) {
+  if (p.flux == 1) roe_flux(f, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+  else hllc_flux(f, L, R, an, area, p.Pamb, p.Rt, p.gamma, us_n, eta);
+}
+
+// srcJ of a reflecting face by one-sided differences of its flux in the
+// reconstructed left state, momentum and energy rows
+// (src/bcImpermeable.loci:132-181, the NUMERICAL form the file compiles)
+template <typename F>
+__device__ inline void reflecting_jacobian(F *blk, const Q5 &L, const V3 &an, double area, const Prm &p,
+                                           double us_n, double eta) {
+  double f0[5];
+  riemann_flux(f0, L, mirror(L, an, us_n), an, area, p, us_n, eta);
+  const double a2 = p.gamma * p.Rt * L.T;
+  const double eps = 1e-5;
+  const double deltas[5] = {L.T * eps, sqrt(a2) * eps, sqrt(a2) * eps, sqrt(a2) * eps, a2 * eps};
+  double vals[5] = {L.T, L.u.x, L.u.y, L.u.z, L.pg};
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const double tmp = vals[j];
+    vals[j] += deltas[j];
+    Q5 l;
+    l.T = vals[0];
+    l.u = {vals[1], vals[2], vals[3]};
+    l.pg = vals[4];
+    double f1[5];
+    riemann_flux(f1, l, mirror(l, an, us_n), an, area, p, us_n, eta);
+    const double rh = 1. / (tmp - vals[j]);
+#pragma unroll
+    for (int i = 1; i < 5; ++i) acc(blk[i + 5 * j], -((f0[i] - f1[i]) * rh));
+    vals[j] = tmp;
+  }
+}
+
+// srcJ of an isentropic-inflow (kind 11) or fixed-mass (kind 12) face by
+// one-sided differences of the face state's Euler flux in the cell's
+// primitives (src/bcIsentropic.loci:171-247, src/bcFixedMass.loci:166-228)
+template <typename F>
+__device__ inline void bc_state_jacobian(F *blk, int kind, const Q5 &ci, const double *bs, const V3 &an, double area,
+                                         const Prm &p, double us_n) {
+  const V3 d = {-1. * an.x, -1. * an.y, -1. * an.z};
+  const double eps = 1e-5;
+  const double ai = sqrt(p.gamma * p.Rt * ci.T);
+  const double deltas[5] = {ci.T * eps, ai * eps, ai * eps, ai * eps, ai * ai * eps};
+  double vals[5] = {ci.T, ci.u.x, ci.u.y, ci.u.z, ci.pg};
+  Q5 o;
+  if (kind == 11) p0t0_state(o, bs[6], bs[5], p.gamma, p.Rt, ci, d, p.Pamb);
+  else fixed_mass_state(o, ci.pg + p.Pamb, ci.T, bs[7], bs[6], p.gamma, p.Rt, d, an, p.Pamb);
+  double f0[5];
+  euler_flux(f0, o, an, area, p.Pamb, p.Rt, p.gamma, us_n);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const double tmp = vals[j];
+    vals[j] += deltas[j];
+    Q5 s;
+    s.T = vals[0];
+    s.u = {vals[1], vals[2], vals[3]};
+    s.pg = vals[4];
+    if (kind == 11) p0t0_state(o, bs[6], bs[5], p.gamma, p.Rt, s, d, p.Pamb);
+    else fixed_mass_state(o, vals[4] + p.Pamb, vals[0], bs[7], bs[6], p.gamma, p.Rt, d, an, p.Pamb);
+    double f1[5];
+    euler_flux(f1, o, an, area, p.Pamb, p.Rt, p.gamma, us_n);
+    const double rh = 1. / (tmp - vals[j]);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) acc(blk[i + 5 * j], -((f0[i] - f1[i]) * rh));
+    vals[j] = tmp;
+  }
 }
 
 // Sutherland law, src/fluxViscous.loci:112-119 and :137-143

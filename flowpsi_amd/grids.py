@@ -23,6 +23,19 @@ from dataclasses import dataclass, field
 import numpy as np
 
 INTERIOR, FARFIELD, WALL, SYMMETRY, PERIODIC = 0, 1, 2, 3, 4
+# boundary types with a parameter set of their own (include/fpsi.h)
+REFLECTING, INFLOW, SUPERSONIC_INFLOW, OUTFLOW, SUPERSONIC_OUTFLOW, EXTRAPOLATE = 5, 6, 7, 8, 9, 10
+ISENTROPIC_INFLOW, FIXED_MASS = 11, 12
+
+
+def bc_code(btype, bc_set=0):
+    """btype of a boundary face of type `btype` reading parameter set
+    `bc_set` (bits 8-15)"""
+    return int(btype) | (int(bc_set) << 8)
+
+
+def bc_type(btype):
+    return np.asarray(btype) & 0xFF
 BC_NAMES = {INTERIOR: "interior", FARFIELD: "farfield", WALL: "viscousWall",
             SYMMETRY: "symmetry", PERIODIC: "periodic"}
 

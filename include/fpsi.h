@@ -36,9 +36,21 @@ enum {
   FPSI_ERR_LU = -5       /* singular diagonal block */
 };
 
-/* boundary types of a face (btype) */
+/* boundary types of a face (btype; bits 8-15 of a boundary face's btype
+ * select its parameter set fpsi_params.bc[], see below) */
 enum { FPSI_BC_INTERIOR = 0, FPSI_BC_FARFIELD = 1, FPSI_BC_WALL = 2,
-       FPSI_BC_SYMMETRY = 3, FPSI_BC_PERIODIC = 4 };
+       FPSI_BC_SYMMETRY = 3, FPSI_BC_PERIODIC = 4,
+       FPSI_BC_REFLECTING = 5,          /* src/bcImpermeable.loci:102-181 */
+       FPSI_BC_INFLOW = 6,              /* src/bcInflow.loci:377-430 */
+       FPSI_BC_SUPERSONIC_INFLOW = 7,   /* src/bcInflow.loci:270-308 */
+       FPSI_BC_OUTFLOW = 8,             /* src/bcOutflow.loci:174-275 (option p) */
+       FPSI_BC_SUPERSONIC_OUTFLOW = 9,  /* src/bcOutflow.loci:127-171 */
+       FPSI_BC_EXTRAPOLATE = 10,        /* src/bcOutflow.loci:84-125 */
+       FPSI_BC_ISENTROPIC_INFLOW = 11,  /* src/bcIsentropic.loci:66-257 */
+       FPSI_BC_FIXED_MASS = 12 };       /* src/bcFixedMass.loci:85-237 */
+#define FPSI_BC_TYPE(btype) ((btype) & 0xFF)
+#define FPSI_BC_SET(btype) (((btype) >> 8) & 0xFF)
+#define FPSI_MAX_BC_SETS 8
 enum { FPSI_FLUX_HLLC = 0, FPSI_FLUX_ROE = 1 };
 enum { FPSI_LIM_NONE = 0, FPSI_LIM_VENKATAKRISHNAN = 1, FPSI_LIM_ZERO = 2, FPSI_LIM_BARTH = 3 };
 enum { FPSI_PREC_FP64 = 0, FPSI_PREC_FP32 = 1 };
@@ -86,6 +98,18 @@ typedef struct {
   const double *cc;     /* [3][nc] cell centre */
   const double *pshift; /* [3][nf] periodic ghost centre = cc[cl[pface]] + pshift */
 } fpsi_grid_desc;
+
+/* Reference values of one boundary (the BC options of
+ * src/bcInputs.loci:188-220 after their unit conversion): temperatureRef_BC,
+ * gagePressureRef_BC (p - Pambient) and uRef_BC for inflow, supersonic
+ * inflow and (pg only) outflow; p0 and T0 for isentropic inflow; mdot and
+ * T0 for fixed mass (massFlux = mdot / the summed area of the set's
+ * fixed-mass faces, :230-232: give each such boundary its own set). */
+typedef struct {
+  double T, pg, u[3];
+  double p0, T0;
+  double mdot;
+} fpsi_bc_set;
 
 /* Parameters, with flowPsi's option names and code defaults:
  * gamma/molecularMass/p0 src/solverEoS.loci:27-47; flux include/inviscidFlux.h:82-90;
@@ -143,6 +167,10 @@ typedef struct {
   /* co-resident waves per compute unit of a line-SGS / FSGS dataflow
    * launch (<= 0: every wave that fits) */
   int sweep_waves_per_cu;
+  /* the parameter sets of the boundaries of types REFLECTING ..
+   * FIXED_MASS: a face reads bc[FPSI_BC_SET(btype)] (farfield keeps pg0,
+   * T0, u0 above).  These types are inviscid or laminar only. */
+  fpsi_bc_set bc[FPSI_MAX_BC_SETS];
 } fpsi_params;
 
 /* src/solverResidual.loci:31-112 (struct residual in include/flowTypes.h:517-523);

@@ -38,7 +38,7 @@ class OgParams(C.Structure):
                 ("sa_sigma", C.c_double), ("Prt", C.c_double), ("nut_bc", C.c_double), ("k_bc", C.c_double),
                 ("w_bc", C.c_double), ("unsteady", C.c_int), ("newton_iter", C.c_int),
                 ("pseudo_cflmax", C.c_double), ("pbias_control", C.c_double),
-                ("temporal_damping", C.c_double)]
+                ("temporal_damping", C.c_double), ("bcs", (C.c_double * 8) * 8)]
 
 
 class OgNorms(C.Structure):
@@ -141,6 +141,13 @@ def params(p: dict) -> OgParams:
     s.pseudo_cflmax = float(p.get("pseudo_cflmax", 10.0))
     s.pbias_control = float(p.get("pbias_control", 0.0))  # timePressureBiasControl
     s.temporal_damping = float(p.get("temporal_damping", 0.0))  # temporalDamping (<= 0: not set)
+    # boundary parameter sets: bc_sets[k] = dict(T, pg or p, u, p0, T0, mdot)
+    for k, b in enumerate(p.get("bc_sets", ())):
+        u = b.get("u", (0.0, 0.0, 0.0))
+        u = (float(u), 0.0, 0.0) if np.isscalar(u) else tuple(float(x) for x in u)
+        pg = float(b["pg"]) if "pg" in b else float(b.get("p", 0.0)) - s.Pambient
+        s.bcs[k][:] = (float(b.get("T", 0.0)), pg, *u, float(b.get("p0", 0.0)), float(b.get("T0", 0.0)),
+                       float(b.get("mdot", 0.0)))
     return s
 
 

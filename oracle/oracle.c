@@ -41,6 +41,8 @@ static inline void subfj(real_fj *a, real v) { *a = (real_fj)((real)*a - v); }
  * pragma is inert in the serial builds the tests use.  Sweeps stay
  * sequential, as in the reference. */
 #define OG_PAR _Pragma("omp parallel for schedule(static)")
+#define OG_BT(g, f) ((g)->btype[f] & 0xFF)        /* boundary type of a face */
+#define OG_SET(g, f) (((g)->btype[f] >> 8) & 0xFF) /* its parameter set */
 
 int og_real_fj_bytes(void) { return (int)sizeof(real_fj); }
 
@@ -388,7 +390,7 @@ static void build_adj(const og_grid *g, og_adj *a) {
   a->up_off = calloc(nc + 1, sizeof(int));
   a->bd_off = calloc(nc + 1, sizeof(int));
   for (int f = 0; f < nf; ++f) {
-    int t = g->btype[f];
+    int t = OG_BT(g, f);
     if (t == OG_INTERIOR) {
       a->up_off[g->cl[f] + 1]++;
       a->lo_off[g->cr[f] + 1]++;
@@ -408,7 +410,7 @@ static void build_adj(const og_grid *g, og_adj *a) {
   a->bd = malloc(sizeof(int) * (a->bd_off[nc] + 1));
   int *lp = calloc(nc, sizeof(int)), *upp = calloc(nc, sizeof(int)), *bp = calloc(nc, sizeof(int));
   for (int f = 0; f < nf; ++f) {
-    int t = g->btype[f];
+    int t = OG_BT(g, f);
     if (t == OG_INTERIOR) {
       int c = g->cl[f];
       a->up[a->up_off[c] + upp[c]++] = f;
@@ -443,7 +445,7 @@ static inline void ccenter(const og_grid *g, int c, real *x) {
 /* cell on the cr side of an interior or periodic face, and its centre
  * (src/solverPeriodic.loci:31-35: ghost centre = transformed partner centre) */
 static inline int rcell(const og_grid *g, int f, real *x) {
-  if (g->btype[f] == OG_PERIODIC) {
+  if (OG_BT(g, f) == OG_PERIODIC) {
     int c = g->cl[g->pface[f]];
     ccenter(g, c, x);
     for (int d = 0; d < 3; ++d) x[d] += g->pshift[d * g->nf + f];
@@ -461,7 +463,7 @@ static inline int nq_of(const og_params *p) { return p->turb == 2 ? 7 : p->turb 
 static void stencil_nb(const og_grid *g, const double *q, const double *qf, int c, int f, int nq,
                        real *x, real *v) {
   int nc = g->nc, nf = g->nf;
-  int t = g->btype[f];
+  int t = OG_BT(g, f);
   if (t == OG_INTERIOR) {
     int o = (g->cl[f] == c) ? g->cr[f] : g->cl[f];
     ccenter(g, o, x);
@@ -523,7 +525,7 @@ static void eta_impl(const og_grid *g, const og_params *p, const double *q, cons
     mref[c] = sqrt(dot3(U, U)) / a[c];
   }
   for (int f = 0; f < nf; ++f) {
-    const int t = g->btype[f];
+    const int t = OG_BT(g, f);
     if (t != OG_INTERIOR && t != OG_PERIODIC) continue;
     const int l = g->cl[f];
     const int r = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
@@ -559,7 +561,7 @@ static void eta_impl(const og_grid *g, const og_params *p, const double *q, cons
   for (int pass = 0; pass < 2; ++pass) {
     for (int c = 0; c < nc; ++c) dst[c] = src[c];
     for (int f = 0; f < nf; ++f) {
-      const int t = g->btype[f];
+      const int t = OG_BT(g, f);
       if (t != OG_INTERIOR && t != OG_PERIODIC) continue;
       const int l = g->cl[f];
       const int r = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
@@ -591,6 +593,152 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
 }
 
 /* ------------------------------------------------------------------ */
+/* boundary conditions with parameter sets (types 5-12)                */
+/* ------------------------------------------------------------------ */
+
+/* inviscidFlux, include/inviscidFlux.h:25-45 */
+static void euler_flux_(real *iflux, real pg, real T, const real *U, const real *an, real area,
+                        real pambient, real Rt, real gamma, real Us_n) {
+  real gm1 = gamma - 1;
+  real rho = (pg + pambient) / (Rt * T);
+  real h0 = Rt * T * gamma / gm1 + 0.5 * dot3(U, U);
+  const real e0 = h0 - Rt * T;
+  const real uta = dot3(U, an);
+  const real ut = uta - Us_n;
+  real mdot = area * rho * ut;
+  real pabs = pg + pambient;
+  iflux[0] = mdot;
+  iflux[1] = mdot * U[0] + area * pg * an[0];
+  iflux[2] = mdot * U[1] + area * pg * an[1];
+  iflux[3] = mdot * U[2] + area * pg * an[2];
+  iflux[4] = mdot * e0 + area * pabs * uta;
+}
+
+/* the option's Riemann flux (inviscidRiemannFlux) */
+static void riemann_(const og_params *p, real *fl, real pgl, real Tl, const real *ul, real pgr, real Tr,
+                     const real *ur, const real *an, real area, real us_n, real eta) {
+  if (p->flux == 1) og_roe_flux(fl, pgl, Tl, ul, pgr, Tr, ur, an, area, p->Pambient, p->Rt, p->gamma, us_n, eta);
+  else og_hllc_flux(fl, pgl, Tl, ul, pgr, Tr, ur, an, area, p->Pambient, p->Rt, p->gamma, us_n, eta);
+}
+
+/* src/bcImpermeable.loci:119-130: the flux against the mirrored left state */
+static void reflecting_flux_(const og_params *p, real *fl, real pgl, real Tl, const real *ul, const real *n,
+                             real area, real us_n, real eta) {
+  const real uit = dot3(ul, n) - us_n;
+  real ur[3];
+  for (int k = 0; k < 3; ++k) ur[k] = ul[k] - 2. * uit * n[k];
+  riemann_(p, fl, pgl, Tl, ul, pgl, Tl, ur, n, area, us_n, eta);
+}
+
+/* fixed_p0t0_bc, src/bcIsentropic.loci:66-89 */
+static void fixed_p0t0_bc_(real *unb, real *ab, real uni, real ai, real a0, real gamma0) {
+  const real gm1 = gamma0 - 1.;
+  const real rim = uni - 2. * ai / gm1;
+  real a = (2. / gm1 + 1);
+  real b = (2. * rim);
+  real c = (0.5 * gm1 * rim * rim - a0 * a0);
+  real D = b * b - 4. * a * c;
+  if (D < 0) {
+    *ab = a0;
+    *unb = rim + 2. * (*ab) / gm1;
+    return;
+  }
+  *ab = (-b + sqrt(D)) / (2. * a);
+  *ab = maxr(*ab, .25 * a0);
+  *unb = rim + 2. * (*ab) / gm1;
+}
+
+/* total_pressure_bc_imp, src/bcIsentropic.loci:91-144, no rigid rotation */
+static void total_pressure_bc_(real *pb, real *Tb, real *ub, real T0, real p0, real gamma0, real a0,
+                               const real *ui, real ai, real ti, const real *n) {
+  const real gm1 = gamma0 - 1.;
+  const real uit = dot3(ui, n);
+  real unb = uit, ab = a0;
+  fixed_p0t0_bc_(&unb, &ab, uit, ai, a0, gamma0);
+  for (int k = 0; k < 3; ++k) ub[k] = unb * n[k] + 0.0;
+  real M = sqrt(dot3(ub, ub)) / ab;
+  const real maxMach = 1.0;
+  if (M > maxMach) {
+    M = maxMach;
+    const real utot = M * a0 / sqrt(1 + .5 * gm1 * M * M);
+    const real un = sqrt(maxr(utot * utot - 0.0, 0.0));
+    for (int k = 0; k < 3; ++k) ub[k] = un * n[k] + 0.0;
+  }
+  *Tb = T0 / (1 + .5 * gm1 * M * M);
+  *pb = p0 * pow(1. + .5 * gm1 * M * M, -gamma0 / gm1);
+  real f = (*Tb - ti) / maxr(ti, *Tb) - .1;
+  if (f > 0) {
+    f = pow(f, .5);
+    ab = (1 - f) * ab + f * a0;
+    unb = (1 - f) * unb + f * uit;
+    for (int k = 0; k < 3; ++k) ub[k] = unb * n[k] + 0.0;
+    real M2 = sqrt(dot3(ub, ub)) / ab;
+    if (M2 > maxMach) {
+      M2 = maxMach;
+      const real utot = M2 * a0 / sqrt(1 + .5 * gm1 * M2 * M2);
+      const real un = sqrt(maxr(utot * utot - 0.0, 0.0));
+      for (int k = 0; k < 3; ++k) ub[k] = un * n[k] + 0.0;
+    }
+    *Tb = T0 / (1 + .5 * gm1 * M2 * M2);
+    *pb = p0 * pow(1. + .5 * gm1 * M2 * M2, -gamma0 / gm1);
+  }
+}
+
+/* mass_flowrate_bc, src/bcFixedMass.loci:85-136 */
+static void mass_flowrate_bc_(real *ub, real *Tb, real *Pb, real pin, real Tin, real massflow,
+                              const real *flow_dir, real Tstag, real Pambient, real gamma, real Rtilde,
+                              const real *n) {
+  real T = Tin;
+  const real eps = Tin * 1e-9;
+  for (int iter = 0; iter < 1000; ++iter) {
+    const real rho = pin / (Rtilde * T);
+    real a = sqrt(gamma * Rtilde * T);
+    real u = massflow / rho;
+    real M = u / a;
+    real lamda = sqrt(((gamma + 1.0) * M * M) / ((gamma - 1.0) * M * M + 2));
+    real p0 = pin * pow((1. - ((gamma - 1.) / (gamma + 1.)) * lamda * lamda), -gamma / (gamma - 1.0));
+    real rho0 = p0 / (Rtilde * Tstag);
+    const real rho_1 = rho0 * pow((1. - (gamma - 1.0) / (gamma + 1.0) * lamda * lamda), 1.0 / (gamma - 1.0));
+    real T_1 = pin / (rho_1 * Rtilde);
+    real dT = T_1 - T;
+    if (fabs(dT) < eps) break;
+    T = T_1;
+  }
+  const real rhob = pin / (Rtilde * T);
+  const real vel = massflow / rhob;
+  const real sc = -vel / dot3(flow_dir, n);
+  for (int k = 0; k < 3; ++k) ub[k] = flow_dir[k] * sc + 0.0 + 0.0; /* += rigid_u, += us_n n (zero) */
+  *Tb = T;
+  *Pb = pin - Pambient;
+}
+
+/* massFlux of a set: mdot over the summed area of its fixed-mass faces
+ * (src/bcFixedMass.loci:230-232: boundary_area of the boundary), summed in
+ * face order */
+static real set_mass_flux(const og_grid *g, const og_params *p, int set) {
+  real area = 0;
+  for (int f = 0; f < g->nf; ++f)
+    if (OG_BT(g, f) == OG_FIXED_MASS && OG_SET(g, f) == set) area += g->area[f];
+  return area > 0 ? p->bcs[set][7] / area : 0.0;
+}
+
+/* face state of an isentropic-inflow or fixed-mass face from a cell state
+ * (T, u, pg); src/bcIsentropic.loci:146-166, src/bcFixedMass.loci:140-153 */
+static void state_bc_(const og_params *p, int t, const double *bs, real mflux, real Ti, const real *ui, real pgi,
+                      const real *n, real *Tb, real *ub, real *pgb) {
+  const real d[3] = {-1. * n[0], -1. * n[1], -1. * n[2]}; /* flow_direction = -area.n */
+  if (t == OG_ISENTROPIC_INFLOW) {
+    const real a0 = sqrt(p->gamma * p->Rt * bs[6]);
+    const real ai = sqrt(p->gamma * p->Rt * Ti);
+    real pb;
+    total_pressure_bc_(&pb, Tb, ub, bs[6], bs[5], p->gamma, a0, ui, ai, Ti, d);
+    *pgb = pb - p->Pambient;
+  } else {
+    mass_flowrate_bc_(ub, Tb, pgb, pgi + p->Pambient, Ti, mflux, d, bs[6], p->Pambient, p->gamma, p->Rt, n);
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* boundary-face states                                                */
 /* ------------------------------------------------------------------ */
 
@@ -598,9 +746,11 @@ void og_eta(const og_grid *g, const og_params *p, const double *q, double *eta_p
  * (stationary wall, us = 0); symmetry: src/bcImpermeable.loci:48-64 */
 void og_face_states(const og_grid *g, const og_params *p, const double *q, double *qf) {
   int nc = g->nc, nf = g->nf;
+  real mflux[8];
+  for (int k = 0; k < 8; ++k) mflux[k] = set_mass_flux(g, p, k);
   OG_PAR
   for (int f = 0; f < nf; ++f) {
-    int t = g->btype[f];
+    int t = OG_BT(g, f);
     if (t == OG_INTERIOR || t == OG_PERIODIC) continue;
     int c = g->cl[f];
     real n[3];
@@ -632,7 +782,7 @@ void og_face_states(const og_grid *g, const og_params *p, const double *q, doubl
       ub[0] = ub[1] = ub[2] = 0.0;
       pgb = pgi;
       Tb = p->Twall;
-    } else { /* symmetry */
+    } else if (t == OG_SYMMETRY) {
       const real us_n = 0.0;
       const real uit = dot3(ui, n) - us_n;
       for (int k = 0; k < 3; ++k) ub[k] = ui[k] - uit * n[k];
@@ -640,6 +790,59 @@ void og_face_states(const og_grid *g, const og_params *p, const double *q, doubl
       const real U = sqrt(dot3(ui, ui));
       pgb = pgi + 0.5 * rho * U * uit;
       Tb = Ti;
+    } else if (t == OG_REFLECTING) { /* src/bcImpermeable.loci:102-117 */
+      const real us_n = 0.0;
+      const real uit = dot3(ui, n) - us_n;
+      for (int k = 0; k < 3; ++k) ub[k] = ui[k] - uit * n[k];
+      const real rho = (pgi + p->Pambient) / (p->Rt * Ti);
+      const real Pmin = 0.5 * (pgi + p->Pambient) - p->Pambient;
+      pgb = maxr(Pmin, pgi + 0.5 * rho * uit * fabs(uit));
+      Tb = Ti;
+    } else if (t == OG_INFLOW || t == OG_SUPERSONIC_INFLOW) { /* src/bcInflow.loci:270-278, 377-402 */
+      const double *bs = p->bcs[OG_SET(g, f)];
+      const real us_n = 0.0;
+      const real u0[3] = {bs[2], bs[3], bs[4]};
+      const real uit = dot3(ui, n) - us_n;
+      const real a0 = sqrt(p->gamma * p->Rt * bs[0]);
+      pgb = bs[1]; Tb = bs[0]; ub[0] = u0[0]; ub[1] = u0[1]; ub[2] = u0[2];
+      if (t == OG_INFLOW && !(uit > 0) && !(-uit > a0))
+        og_cv_inflow(&pgb, &Tb, ub, pgi, Ti, ui, bs[1], bs[0], u0, n, us_n, p->Pambient, p->gamma, p->Rt,
+                     og_eta_face(p, Ti, ui));
+    } else if (t == OG_OUTFLOW) { /* src/bcOutflow.loci:206-240, option p */
+      const double *bs = p->bcs[OG_SET(g, f)];
+      const real us_n = 0.0;
+      const real pref = bs[1];
+      const real uit = dot3(ui, n) - us_n;
+      const real ai = sqrt(p->gamma * p->Rt * Ti);
+      const real rhoi = (pgi + p->Pambient) / (p->Rt * Ti);
+      if (uit >= 0) {
+        if (uit > ai) {
+          for (int k = 0; k < 3; ++k) ub[k] = ui[k];
+          Tb = Ti;
+          pgb = pgi;
+        } else {
+          pgb = pref;
+          for (int k = 0; k < 3; ++k) ub[k] = ui[k];
+          const real pa = pgb + p->Pambient;
+          Tb = pa / (p->Rt * rhoi);
+        }
+      } else {
+        for (int k = 0; k < 3; ++k) ub[k] = us_n * n[k];
+        pgb = pgi - .05 * (pgi + p->Pambient);
+        Tb = Ti;
+      }
+    } else if (t == OG_SUPERSONIC_OUTFLOW) { /* src/bcOutflow.loci:127-141 */
+      pgb = pgi;
+      const real ai = sqrt(p->gamma * p->Rt * Ti);
+      const real ut = dot3(n, ui) - 0.0;
+      if (ut < ai) pgb -= (pgi + p->Pambient) * .1;
+      Tb = Ti;
+      for (int k = 0; k < 3; ++k) ub[k] = ui[k];
+    } else if (t == OG_EXTRAPOLATE) { /* src/bcOutflow.loci:84-94 */
+      pgb = pgi; Tb = Ti;
+      for (int k = 0; k < 3; ++k) ub[k] = ui[k];
+    } else { /* isentropic inflow, fixed mass */
+      state_bc_(p, t, p->bcs[OG_SET(g, f)], mflux[OG_SET(g, f)], Ti, ui, pgi, n, &Tb, ub, &pgb);
     }
     if (p->turb == 2) {
       /* k_f, w_f: inflow/outflow switch on u_f.n (turbulence/KOmega/sst_viscous.loci:131-144),
@@ -691,7 +894,7 @@ static void lsq_inverse(const og_grid *g, const og_adj *a, int c, real *Ai /* 6:
     for (int j = lists[L][0]; j < lists[L][1]; ++j) {
       int f = arr[L][j];
       real x[3];
-      int t = g->btype[f];
+      int t = OG_BT(g, f);
       if (t == OG_INTERIOR) {
         int o = (g->cl[f] == c) ? g->cr[f] : g->cl[f];
         ccenter(g, o, x);
@@ -824,7 +1027,7 @@ static inline real suth_k(const og_params *p, real T) { return p->suth[3] * pow(
 static void face_grad(const og_grid *g, const double *q, const double *qf, const double *grad,
                       int f, int k, real *gout) {
   int nc = g->nc, nf = g->nf;
-  int t = g->btype[f];
+  int t = OG_BT(g, f);
   int c = g->cl[f];
   real xl[3];
   ccenter(g, c, xl);
@@ -865,7 +1068,7 @@ static real tmu_cell(const og_grid *g, const og_params *p, const double *q, cons
 /* tmu_f: volume-weighted interior average, the interior cell on boundary
  * faces, zero on no-slip walls (src/fluxViscous.loci:192-201) */
 static real tmu_face(const og_grid *g, const og_params *p, const double *q, const double *grad, int f) {
-  const int t = g->btype[f];
+  const int t = OG_BT(g, f);
   const int c = g->cl[f];
   if (t == OG_INTERIOR || t == OG_PERIODIC) {
     real xr[3];
@@ -882,7 +1085,7 @@ static real tmu_face(const og_grid *g, const og_params *p, const double *q, cons
 static void viscous_face(const og_grid *g, const og_params *p, const double *q, const double *qf,
                          const double *grad, int f, real *mu_f, real *k_f, real *uf, real *vflux) {
   int nc = g->nc, nf = g->nf;
-  int t = g->btype[f];
+  int t = OG_BT(g, f);
   real an[3];
   fnormal(g, f, an);
   const real sada = g->area[f];
@@ -952,7 +1155,7 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
   eta_impl(g, p, q, grad, eta_p);
   OG_PAR
   for (int f = 0; f < nf; ++f) {
-    int t = g->btype[f];
+    int t = OG_BT(g, f);
     real an[3], xf[3], xl[3];
     fnormal(g, f, an);
     fcenter(g, f, xf);
@@ -967,10 +1170,15 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
     real qL[5];
     recon(g, p, q, grad, phi, c, xl, xf, qL);
     real UL[3] = {qL[1], qL[2], qL[3]};
-    if (t == OG_INTERIOR || t == OG_PERIODIC || t == OG_FARFIELD) {
+    const int riemann_face = t == OG_FARFIELD || (t >= OG_INFLOW && t <= OG_EXTRAPOLATE);
+    if (t == OG_INTERIOR || t == OG_PERIODIC || riemann_face) {
       real qR[5];
-      if (t == OG_FARFIELD) {
+      if (riemann_face) {
         for (int k = 0; k < 5; ++k) qR[k] = qf[k * nf + f];
+        if (t == OG_SUPERSONIC_INFLOW) { /* Eta_pf, src/bcInflow.loci:280-289 */
+          const real Uc[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+          eta = og_eta_face(p, q[c], Uc);
+        }
       } else {
         real xr[3];
         int o = rcell(g, f, xr);
@@ -981,6 +1189,12 @@ void og_fluxes(const og_grid *g, const og_params *p, const double *q, const doub
         og_roe_flux(fl, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta);
       else
         og_hllc_flux(fl, qL[4], qL[0], UL, qR[4], qR[0], UR, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n, eta);
+    } else if (t == OG_REFLECTING) { /* src/bcImpermeable.loci:119-131 */
+      reflecting_flux_(p, fl, qL[4], qL[0], UL, an, g->area[f], us_n, eta);
+      fl[0] = 0;
+    } else if (t == OG_ISENTROPIC_INFLOW || t == OG_FIXED_MASS) { /* the face state's Euler flux */
+      const real uf[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+      euler_flux_(fl, qf[4 * nf + f], qf[f], uf, an, g->area[f], p->Pambient, p->Rt, p->gamma, us_n);
     } else if (t == OG_WALL) {
       const real pg = qf[4 * nf + f];
       const real A = g->area[f];
@@ -1258,24 +1472,83 @@ static void damping_jac(real_fj *fj, real s, real pg, real T, const real *U, con
 static void boundary_block(const og_grid *g, const og_params *p, const double *q, const double *qf,
                            const double *grad, const double *phi, int f, real eta_c, real_fj *bj) {
   int nc = g->nc, nf = g->nf;
-  int t = g->btype[f];
+  int t = OG_BT(g, f);
   int c = g->cl[f];
   real an[3], xf[3], xl[3];
   fnormal(g, f, an);
   fcenter(g, f, xf);
   ccenter(g, c, xl);
   const real sada = g->area[f], us_n = 0.0;
-  if (t == OG_FARFIELD) {
+  if (t == OG_FARFIELD || (t >= OG_INFLOW && t <= OG_EXTRAPOLATE)) {
+    /* srcJ -= fjp(left, face state): farfield; inflow, supersonic inflow
+     * (src/bcInflow.loci:291-308,415-430), outflow, supersonic outflow,
+     * extrapolate (src/bcOutflow.loci:107-125,154-171,254-274) */
     real qL[5];
     recon(g, p, q, grad, phi, c, xl, xf, qL);
     real UL[3] = {qL[1], qL[2], qL[3]};
     real UR[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
+    real eta = eta_c;
+    if (t == OG_SUPERSONIC_INFLOW) {
+      const real Uc[3] = {q[nc + c], q[2 * nc + c], q[3 * nc + c]};
+      eta = og_eta_face(p, q[c], Uc);
+    }
     real_fj fjp[25];
     for (int i = 0; i < 25; ++i) fjp[i] = 0;
     roe_jacobian(fjp, qL[4], qL[0], UL, qf[4 * nf + f], qf[f], UR, an, sada, p->Pambient, p->Rt,
-                 p->gamma, us_n, eta_c, 1);
+                 p->gamma, us_n, eta, 1);
     for (int i = 0; i < 5; ++i)
       for (int j = 0; j < 5; ++j) M_(bj, i, j) -= M_(fjp, i, j);
+  } else if (t == OG_REFLECTING) {
+    /* src/bcImpermeable.loci:132-181 (NUMERICAL): one-sided differences
+     * of the flux in the reconstructed left state, rows 1-4 */
+    real qL[5];
+    recon(g, p, q, grad, phi, c, xl, xf, qL);
+    real Tl = qL[0], Pgl = qL[4], ul[3] = {qL[1], qL[2], qL[3]};
+    real f0[5], f1[5], vals[5], deltas[5];
+    reflecting_flux_(p, f0, Pgl, Tl, ul, an, sada, us_n, eta_c);
+    const real a2 = p->gamma * p->Rt * Tl;
+    const real eps = 1e-5;
+    deltas[0] = Tl * eps;
+    deltas[1] = sqrt(a2) * eps;
+    deltas[2] = deltas[1];
+    deltas[3] = deltas[1];
+    deltas[4] = a2 * eps;
+    vals[0] = Tl; vals[1] = ul[0]; vals[2] = ul[1]; vals[3] = ul[2]; vals[4] = Pgl;
+    for (int j = 0; j < 5; ++j) {
+      real tmp = vals[j];
+      vals[j] += deltas[j];
+      real u2[3] = {vals[1], vals[2], vals[3]};
+      reflecting_flux_(p, f1, vals[4], vals[0], u2, an, sada, us_n, eta_c);
+      const real rh = 1. / (tmp - vals[j]);
+      for (int i = 1; i < 5; ++i) subfj(&M_(bj, i, j), (f0[i] - f1[i]) * rh);
+      vals[j] = tmp;
+    }
+  } else if (t == OG_ISENTROPIC_INFLOW || t == OG_FIXED_MASS) {
+    /* src/bcIsentropic.loci:171-247, src/bcFixedMass.loci:166-228:
+     * one-sided differences of the face state's Euler flux in the cell's
+     * primitives, all rows */
+    const double *bs = p->bcs[OG_SET(g, f)];
+    const real mflux = t == OG_FIXED_MASS ? set_mass_flux(g, p, OG_SET(g, f)) : 0.0;
+    real vals[5] = {q[c], q[nc + c], q[2 * nc + c], q[3 * nc + c], q[4 * nc + c]};
+    real Tb, pgb, ub[3], f0[5], f1[5], deltas[5];
+    state_bc_(p, t, bs, mflux, vals[0], vals + 1, vals[4], an, &Tb, ub, &pgb);
+    euler_flux_(f0, pgb, Tb, ub, an, sada, p->Pambient, p->Rt, p->gamma, us_n);
+    const real eps = 1e-5;
+    const real ai = sqrt(p->gamma * p->Rt * vals[0]);
+    deltas[0] = vals[0] * eps;
+    deltas[1] = ai * eps;
+    deltas[2] = deltas[1];
+    deltas[3] = deltas[1];
+    deltas[4] = ai * ai * eps;
+    for (int j = 0; j < 5; ++j) {
+      real tmp = vals[j];
+      vals[j] += deltas[j];
+      state_bc_(p, t, bs, mflux, vals[0], vals + 1, vals[4], an, &Tb, ub, &pgb);
+      euler_flux_(f1, pgb, Tb, ub, an, sada, p->Pambient, p->Rt, p->gamma, us_n);
+      const real rh = 1. / (tmp - vals[j]);
+      for (int i = 0; i < 5; ++i) subfj(&M_(bj, i, j), (f0[i] - f1[i]) * rh);
+      vals[j] = tmp;
+    }
   } else if (t == OG_WALL) {
     const int mi = 1, ei = 4;
     real uf[3] = {qf[nf + f], qf[2 * nf + f], qf[3 * nf + f]};
@@ -1402,7 +1675,7 @@ void og_assemble_t(const og_grid *g, const og_params *p, const double *q, const 
   for (int f = 0; f < nf; ++f) {
     real_fj *P = fjp + 25 * (size_t)f, *Mm = fjm + 25 * (size_t)f;
     for (int i = 0; i < 25; ++i) P[i] = Mm[i] = 0;
-    int t = g->btype[f];
+    int t = OG_BT(g, f);
     if (t == OG_INTERIOR || t == OG_PERIODIC) {
       const int rc = t == OG_INTERIOR ? g->cr[f] : g->cl[g->pface[f]];
       const real eta = maxr(eta_p[g->cl[f]], eta_p[rc]); /* :1196-1220 */
@@ -1611,7 +1884,7 @@ void og_sgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v,
   for (int it = 0; it < nsweeps; ++it) {
     for (int pass = 0; pass < 2; ++pass) {
       for (int f = 0; f < nf; ++f)
-        if (g->btype[f] == OG_PERIODIC)
+        if (OG_BT(g, f) == OG_PERIODIC)
           for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
       for (int ii = 0; ii < nc; ++ii) {
         int c = pass == 0 ? ii : nc - 1 - ii;
@@ -1619,7 +1892,7 @@ void og_sgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v,
         for (int k = 0; k < 5; ++k) t[k] = -(real)B[5 * (size_t)c + k];
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           int f = a.up[j];
-          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          const real *xn = OG_BT(g, f) == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
           dotprod_accum(fjm + 25 * (size_t)f, xn, t); /* fluid_U = fjm */
         }
         for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
@@ -1847,7 +2120,7 @@ static void sa_core(const og_params *p, real T, real pg, real nut, real vortMag,
  * state on boundaries (src/fluxViscous.loci:121-133,210-215, solverEoS.loci:53) */
 static void face_rho_mu(const og_grid *g, const og_params *p, const double *q, const double *qf, int f,
                         real *rho_f, real *mu_f) {
-  const int nc = g->nc, nf = g->nf, t = g->btype[f], c = g->cl[f];
+  const int nc = g->nc, nf = g->nf, t = OG_BT(g, f), c = g->cl[f];
   if (t == OG_INTERIOR || t == OG_PERIODIC) {
     real xr[3];
     const int o = rcell(g, f, xr);
@@ -1956,7 +2229,7 @@ void og_sa_assemble_t(const og_grid *g, const og_params *p, const double *q, con
   double *bJ = calloc(nf, sizeof(double)); /* boundary srcJ contribution to ci */
   double *cJl = calloc(nf, sizeof(double)), *cJr = calloc(nf, sizeof(double)); /* cb2 diagonal parts */
   for (int f = 0; f < nf; ++f) {
-    const int t = g->btype[f], c = g->cl[f];
+    const int t = OG_BT(g, f), c = g->cl[f];
     real an[3], xf[3], xl[3];
     fnormal(g, f, an);
     fcenter(g, f, xf);
@@ -2262,7 +2535,7 @@ void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, co
   /* face pass 1: area grad_f(k).n and its omega-weighted form */
   double *gkn = calloc(nf, sizeof(double)), *wgkn = calloc(nf, sizeof(double));
   for (int f = 0; f < nf; ++f) {
-    const int t = g->btype[f], c = g->cl[f];
+    const int t = OG_BT(g, f), c = g->cl[f];
     real an[3], gk[3];
     fnormal(g, f, an);
     face_grad(g, q, qf, grad, f, 5, gk);
@@ -2315,7 +2588,7 @@ void og_sst_assemble_t(const og_grid *g, const og_params *p, const double *q, co
     const int kv = 5 + sc;
     double *Fp = fjp + (size_t)sc * nf, *Fm = fjm + (size_t)sc * nf;
     for (int f = 0; f < nf; ++f) {
-      const int t = g->btype[f], c = g->cl[f];
+      const int t = OG_BT(g, f), c = g->cl[f];
       real an[3], xf[3], xl[3], gx[3];
       fnormal(g, f, an);
       fcenter(g, f, xf);
@@ -2411,13 +2684,13 @@ void og_sa_sgs(const og_grid *g, int nsweeps, const double *fjp, const double *f
   for (int it = 0; it < nsweeps; ++it)
     for (int pass = 0; pass < 2; ++pass) {
       for (int f = 0; f < nf; ++f)
-        if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+        if (OG_BT(g, f) == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
       for (int ii = 0; ii < nc; ++ii) {
         const int c = pass == 0 ? ii : nc - 1 - ii;
         real t = -B[c];
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           const int f = a.up[j];
-          const real xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          const real xn = OG_BT(g, f) == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
           t += xn * fjm[f];
         }
         for (int j = a.lo_off[c]; j < a.lo_off[c + 1]; ++j) {
@@ -2454,7 +2727,7 @@ int og_sa_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, doub
   int iter = 0;
   for (iter = 0; iter < maxiter; ++iter) {
     for (int f = 0; f < nf; ++f)
-      if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+      if (OG_BT(g, f) == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
     for (int pass = 0; pass < 2; ++pass)
       for (int ii = 0; ii < nc; ++ii) {
         const int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
@@ -2465,7 +2738,7 @@ int og_sa_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, doub
         }
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           const int f = a.up[j];
-          const double xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          const double xn = OG_BT(g, f) == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
           r -= xn * fjm[f];
         }
         if (pass == 0) xt[c] = X[c];
@@ -2507,7 +2780,7 @@ void og_sa_fsgs(const og_grid *g, int nsweeps, const double *fjp, const double *
   for (int c = 0; c < nc; ++c) X[c] = 0.0;
   for (int it = 0; it < nsweeps; ++it) {
     for (int f = 0; f < nf; ++f)
-      if (g->btype[f] == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
+      if (OG_BT(g, f) == OG_PERIODIC) Xg[f] = X[g->cl[g->pface[f]]];
     for (int pass = 0; pass < 2; ++pass)
       for (int ii = 0; ii < nc; ++ii) {
         const int c = pass == 0 ? order[ii] : order[nc - 1 - ii];
@@ -2518,7 +2791,7 @@ void og_sa_fsgs(const og_grid *g, int nsweeps, const double *fjp, const double *
         }
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           const int f = a.up[j];
-          const double xn = g->btype[f] == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
+          const double xn = OG_BT(g, f) == OG_PERIODIC ? Xg[f] : X[g->cr[f]];
           r -= xn * fjm[f];
         }
         X[c] = Dinv[c] * r;
@@ -2560,7 +2833,7 @@ int og_lsgs_order(const og_grid *g, int *order) {
       }
       for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
         int f = a.up[j];
-        if (g->btype[f] != OG_INTERIOR) continue; /* ghosts do not join the search */
+        if (OG_BT(g, f) != OG_INTERIOR) continue; /* ghosts do not join the search */
         int o = g->cr[f];
         if (!vis[o]) { vis[o] = 1; nxt[nn++] = o; }
       }
@@ -2665,7 +2938,7 @@ int og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double 
   int iter = 0;
   for (iter = 0; iter < maxiter; ++iter) {
     for (int f = 0; f < nf; ++f)
-      if (g->btype[f] == OG_PERIODIC)
+      if (OG_BT(g, f) == OG_PERIODIC)
         for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
     for (int pass = 0; pass < 2; ++pass) {
       for (int ii = 0; ii < nc; ++ii) {
@@ -2680,7 +2953,7 @@ int og_lsgs(const og_grid *g, int maxiter, double abstol, double reltol, double 
         }
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           int f = a.up[j];
-          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          const real *xn = OG_BT(g, f) == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
           dotprod_accum(fjm + 25 * (size_t)f, xn, t);
         }
         for (int k = 0; k < 5; ++k) {
@@ -2733,7 +3006,7 @@ void og_fsgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v
   for (size_t i = 0; i < 5 * (size_t)nc; ++i) X[i] = 0.0;
   for (int it = 0; it < nsweeps; ++it) {
     for (int f = 0; f < nf; ++f)
-      if (g->btype[f] == OG_PERIODIC)
+      if (OG_BT(g, f) == OG_PERIODIC)
         for (int k = 0; k < 5; ++k) Xg[5 * (size_t)f + k] = X[5 * (size_t)g->cl[g->pface[f]] + k];
     for (int pass = 0; pass < 2; ++pass)
       for (int ii = 0; ii < nc; ++ii) {
@@ -2748,7 +3021,7 @@ void og_fsgs(const og_grid *g, int nsweeps, const void *fjp_v, const void *fjm_v
         }
         for (int j = a.up_off[c]; j < a.up_off[c + 1]; ++j) {
           int f = a.up[j];
-          const real *xn = g->btype[f] == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
+          const real *xn = OG_BT(g, f) == OG_PERIODIC ? Xg + 5 * (size_t)f : X + 5 * (size_t)g->cr[f];
           dotprod_accum(fjm + 25 * (size_t)f, xn, t);
         }
         for (int k = 0; k < 5; ++k) v[k] = -t[k];

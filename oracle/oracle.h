@@ -27,7 +27,11 @@
 extern "C" {
 #endif
 
-enum { OG_INTERIOR = 0, OG_FARFIELD = 1, OG_WALL = 2, OG_SYMMETRY = 3, OG_PERIODIC = 4 };
+enum { OG_INTERIOR = 0, OG_FARFIELD = 1, OG_WALL = 2, OG_SYMMETRY = 3, OG_PERIODIC = 4,
+       OG_REFLECTING = 5, OG_INFLOW = 6, OG_SUPERSONIC_INFLOW = 7, OG_OUTFLOW = 8,
+       OG_SUPERSONIC_OUTFLOW = 9, OG_EXTRAPOLATE = 10, OG_ISENTROPIC_INFLOW = 11, OG_FIXED_MASS = 12 };
+/* a boundary face's btype: its type in bits 0-7, its parameter set
+ * (og_params.bcs) in bits 8-15 */
 
 typedef struct {
   int nc, nf;
@@ -70,6 +74,9 @@ typedef struct {
   double pseudo_cflmax;   /* pseudo-time CFL of the BDF2 preconditioning blend */
   double pbias_control;   /* timePressureBiasControl (KEC) */
   double temporal_damping; /* temporalDamping beta (src/fluxInviscid.loci:1224-1251); <= 0: not set */
+  /* boundary parameter sets (src/bcInputs.loci:188-220): T, pg (p - Pambient),
+   * u[3], p0, T0, mdot (fixedMass: massFlux = mdot / the set's face area) */
+  double bcs[8][8];
 } og_params;
 
 typedef struct {
