@@ -282,6 +282,9 @@ def main():
                     help="default persistent (the flag-free dataflow); persistent_flags (chunk flags, the "
                          "faster line-SGS dataflow) for c5; per_level for c3, whose six dependency levels "
                          "of ~690k cells each fill the GPU in one launch per level")
+    ap.add_argument("--sweep-fp32", default="persistent", choices=["per_level", "persistent", "persistent_flags"],
+                    help="c5: the sweep of the FP32 paths of the time-to-1e-8 comparison (the flag-free dataflow "
+                         "is the faster line-SGS form with FP32 blocks, the flag-based one with FP64 blocks)")
     ap.add_argument("--time-scheme", default="steady", choices=["steady", "bdf2", "kec"],
                     help="bdf2/kec: time-accurate dual time stepping; a step is one time step of "
                          "--newton-iter Newton iterations and the value counts Newton iterations")
@@ -395,7 +398,8 @@ def main():
             # line SGS: the tolerance paths stop once the linear residual has
             # halved; fp32_max3 caps flowPsi's own loop at LSGSMaxIter = 3
             conv = converge.compare(gb, qb, dict(pb, sweep=a.sweep), rtol=0.5, log=log,
-                                    paths=("fp64_ref", "fp32_ref", "fp32_max3", "fp32_tol", "fp32_nn"))
+                                    paths=("fp64_ref", "fp32_ref", "fp32_max3", "fp32_tol", "fp32_nn"),
+                                    sweep_fp32=a.sweep_fp32)
         else:
             conv = converge.compare(gb, qb, dict(pb, sweep="persistent"), log=log)
         conv["workload"] = cname

@@ -113,19 +113,23 @@ def train_guess(g, q, prm, harvest_steps=8, target_sweeps=30, epochs=200, lr=2e-
 
 def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref_drop=1e-11,
             paths=("fp64_ref", "fp32_ref", "fp32_tol", "fp32_nn", "fp64_tol"), log=None,
-            continue_paths=True, train_kw=None):
+            continue_paths=True, train_kw=None, sweep_fp32=None):
     """The paths on one workload, one solver alive at a time; rtol is the
     relative linear-residual tolerance of the _tol and _nn paths.  The
     first path is the reference: it runs on to ref_drop before the others
     start.  continue_paths=False skips continuing the others to ref_drop
     (their converged-state error is then not reported).  train_kw goes to
-    train_guess (harvest_steps, target_sweeps, epochs, lr)."""
+    train_guess (harvest_steps, target_sweeps, epochs, lr).  sweep_fp32,
+    when given, is the sweep mode of the FP32 paths (each precision runs
+    its faster dataflow form; every form gives the same bits)."""
     say = log or (lambda m: None)
     out = {"target": target, "rtol": rtol, "max_sweeps": max_sweeps, "ref_drop": ref_drop}
     qref = q_base = None
     for name in paths:
         prec, kind = name.split("_", 1)
         p = dict(prm, precision=prec)
+        if prec == "fp32" and sweep_fp32:
+            p["sweep"] = sweep_fp32
         if kind.startswith("max"):
             k = int(kind[3:])
             p.update(lsgs_maxiter=k) if prm.get("solver") == "lsgs" else p.update(sgs_iter=k)
@@ -139,7 +143,7 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
         t, d = r.run_to(target, max_steps, say)
         n = len(r.hist) - 1 if d < target else None
         res = dict(steps_to_target=n, time_to_target_s=t if n is not None else None, final_drop=float(d),
-                   ms_per_step=1e3 * t / max(len(r.hist), 1))
+                   ms_per_step=1e3 * t / max(len(r.hist), 1), sweep=p.get("sweep"))
         if r.lin:
             sw = np.array([x["sweeps"] for x in r.lin])
             res["mean_sweeps"] = float(sw.mean())
@@ -180,3 +184,76 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
     tnn = out.get("fp32_nn", {}).get("time_to_target_s")
     out["speedup_fp32_nn_vs_fp64"] = (t64 / tnn) if (t64 and tnn) else None
     return out
+
+
+def guess_gain(g, q, prm, transient=5, harvest_steps=2, target_sweeps=30, epochs=100, lr=2e-3, eval_steps=4,
+               rtol=1e-3, max_sweeps=60, seed=1, log=None):
+    """What the trained network gains on Newton steps it was not trained on.
+
+    After `transient` plain Newton steps, `harvest_steps` steps harvest
+    (system, `target_sweeps`-sweep solution) pairs, one per cell each, and
+    the network trains `epochs` epochs on them.  From the state after the
+    harvest, `eval_steps` Newton steps then run with the fluid's sweeps
+    stopped at a linear residual of `rtol` x the plain start's, first from
+    the learned guess and then again from the same state without it (the
+    guess reads the previous step's solution, so the learned run goes
+    first).  Reported per path: the sweeps of each step, their total, the
+    device time per step (host clock around synchronised calls), and for
+    the guess the linear residual of the guess over the plain start's."""
+    say = log or (lambda m: None)
+    s = Solver(g, prm)
+    try:
+        s.set_state(q)
+        s.mlp_init(seed)
+        s.newton(transient)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(harvest_steps):
+            s.assemble()
+            s.mlp_harvest(target_sweeps)
+            s.update()
+        loss0 = s.mlp_train(1, lr=0.0)
+        loss = s.mlp_train(epochs, lr=lr)
+        s.synchronize()
+        out = dict(cells=g.nc, harvested_pairs=g.nc * harvest_steps, rtol=rtol,
+                   training=dict(train_time_s=time.perf_counter() - t0, loss_before=loss0, loss_after=loss,
+                                 harvest_steps=harvest_steps, target_sweeps=target_sweeps, epochs=epochs))
+        say(f"trained on {g.nc * harvest_steps} pairs in {out['training']['train_time_s']:.1f}s, "
+            f"loss {loss0:.3e} -> {loss:.3e}")
+        qh = s.get_state()
+        for use in (True, False):
+            s.set_state(qh)
+            s.synchronize()
+            c0 = time.perf_counter()
+            _, lin = s.newton_adaptive(eval_steps, rtol, max_sweeps, use_guess=use)
+            s.synchronize()
+            dt = time.perf_counter() - c0
+            r = dict(sweeps=[x["sweeps"] for x in lin], total_sweeps=int(sum(x["sweeps"] for x in lin)),
+                     ms_per_step=1e3 * dt / eval_steps,
+                     reached=[bool(x["rfinal"] <= rtol * x["r0"]) for x in lin])
+            if use:
+                r["used_guess"] = [int(x["used_guess"]) for x in lin]
+                r["rguess_over_r0"] = [float(x["rguess"] / x["r0"]) for x in lin]
+            out["learned" if use else "plain"] = r
+            say(f"{'learned' if use else 'plain'}: {r}")
+        out["sweeps_saved"] = out["plain"]["total_sweeps"] - out["learned"]["total_sweeps"]
+    finally:
+        s.close()
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    from . import workloads
+
+    # the learned guess on the full 8M-cell prism-layer + hex grid (SA, block SGS)
+    t0 = time.time()
+    g, q, prm = workloads.c4_hybrid(nx=200, nz=200, n_prism=30, n_hex=140)
+    lg = lambda m: print(f"[guess c4] {m}", file=sys.stderr, flush=True)
+    lg(f"grid {g.nc} cells built in {time.time() - t0:.1f}s")
+    for prec in sys.argv[1:] or ["fp32"]:
+        r = guess_gain(g, q, dict(prm, precision=prec, sweep="persistent"), log=lg)
+        r.update(workload="c4", precision=prec)
+        print(json.dumps(r), flush=True)

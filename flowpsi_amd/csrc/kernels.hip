@@ -1308,7 +1308,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
       }
-      __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_s_sleep(4);
     }
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
@@ -1386,6 +1386,27 @@ __device__ __forceinline__ bool is_sent(double v) {
   return (unsigned long long)__double_as_longlong(v) == X_SENT;
 }
 
+// The first stage of a flag-free wait: poll one word per lane (p, or
+// nothing when p is null), with a pause between polls, until no lane sees
+// the sentinel.  The word stands in for the lane's whole set of reached
+// neighbour words: the full poll that follows finds the others written, or
+// nearly so, and while the producers are still levels away the waiting
+// waves issue a few short polls instead of full gathers.  spins counts
+// polls towards the caller's spin limit; false: the limit was passed here
+// (*err set) or another wave raised *err.
+__device__ __forceinline__ bool wait_witness(const double *p, int &spins, int spin_limit, int *err) {
+  while (true) {
+    const bool rdy = p ? !is_sent(ldx<true>(p)) : true;
+    if (__all(rdy)) return true;
+    if (++spins > spin_limit) {
+      if (threadIdx.x == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 template <typename F, bool BACK>
 __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwaves, const int *__restrict__ chunk_lo,
                                                  int spin_limit, int *__restrict__ err, const F *__restrict__ off,
@@ -1443,6 +1464,13 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
           if (j < nup) tsum += mrow[j][jj] * xw[j * 5 + jj];
     }
     int spins = 0;
+    {
+      const double *wp = nullptr;  // the first reached neighbour's word
+#pragma unroll
+      for (int j = MAXR - 1; j >= 0; --j)
+        if (BACK ? (j < nup) : (j >= nup)) wp = Xt + (size_t)nb[j] * 5 + i;
+      if (!wait_witness(wp, spins, spin_limit, err)) return;
+    }
     while (true) {
       bool ready = true;
 #pragma unroll
@@ -1459,6 +1487,7 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
         return;
       }
       if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+      __builtin_amdgcn_s_sleep(1);
     }
     if (q < CPW) {
 #pragma unroll
@@ -1642,7 +1671,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
       }
-      __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_s_sleep(4);
     }
     // from the wait to the publish this chunk is on the dependency chain:
     // it goes ahead of the SIMD's waves still loading their own chunks
@@ -1823,6 +1852,13 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
     }
     // the wait: poll the words of the reached neighbours in the target buffers
     int spins = 0;
+    {
+      const double *wp = nullptr;  // the first reached neighbour's fluid word
+#pragma unroll
+      for (int j = MAXR - 1; j >= 0; --j)
+        if (BACK ? (j >= nlow && j < len) : (j < nlow)) wp = Xt + (size_t)nb[j] * 5 + i;
+      if (!wait_witness(fluid ? wp : nullptr, spins, spin_limit, err)) return;
+    }
     while (true) {
       bool ready = true;
       if (fluid) {
@@ -1851,6 +1887,7 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         return;
       }
       if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+      __builtin_amdgcn_s_sleep(1);
     }
     if (q < CPW) {
 #pragma unroll
