@@ -279,12 +279,10 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="cells per axis (c5: x and z; default 577)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--sweep", default=None, choices=["per_level", "tiled", "persistent", "persistent_flags"],
-                    help="default persistent (the flag-free dataflow); persistent_flags (chunk flags, the "
-                         "faster line-SGS dataflow) for c5; per_level for c3, whose six dependency levels "
-                         "of ~690k cells each fill the GPU in one launch per level")
-    ap.add_argument("--sweep-fp32", default="persistent", choices=["per_level", "persistent", "persistent_flags"],
-                    help="c5: the sweep of the FP32 paths of the time-to-1e-8 comparison (the flag-free dataflow "
-                         "is the faster line-SGS form with FP32 blocks, the flag-based one with FP64 blocks)")
+                    help="default persistent (the flag-free dataflow); per_level for c3, whose six dependency "
+                         "levels of ~690k cells each fill the GPU in one launch per level")
+    ap.add_argument("--sweep-fp32", default=None, choices=["per_level", "persistent", "persistent_flags"],
+                    help="c5: the sweep of the FP32 paths of the time-to-1e-8 comparison (default: --sweep)")
     ap.add_argument("--time-scheme", default="steady", choices=["steady", "bdf2", "kec"],
                     help="bdf2/kec: time-accurate dual time stepping; a step is one time step of "
                          "--newton-iter Newton iterations and the value counts Newton iterations")
@@ -319,7 +317,7 @@ def main():
     t0 = time.time()
     g, q, prm, wname, tag = build_workload(a)
     if a.sweep is None:
-        a.sweep = {"c3": "per_level", "c5": "persistent_flags"}.get(a.workload, "persistent")
+        a.sweep = {"c3": "per_level"}.get(a.workload, "persistent")
     prm = dict(prm, precision=a.precision, sweep=a.sweep)
     unsteady = a.time_scheme != "steady"
     iters = a.newton_iter if unsteady else 1

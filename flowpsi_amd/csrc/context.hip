@@ -1963,8 +1963,13 @@ static int mlp_guess(fpsi_ctx *c) {
   return 0;
 }
 
-// scalars: with line SGS, also run the turbulence scalars' own line-SGS
-// loops in the same launches (a Newton step); block SGS leaves them alone
+// scalars: also solve the turbulence scalars with their own solver (a
+// Newton step).  Under line SGS their line-SGS loops run in the fluid's
+// launches; under block SGS with the dataflow sweep their
+// gauss_seidel_iter sweeps ride in the fluid's first sweeps (the rest, if
+// the fluid stops earlier, follow on their own), otherwise they are swept
+// after the fluid.  Either way the scalars' bits are those of their
+// separate solve.
 static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st,
                      bool scalars = false) {
   fpsi_lin_stats s{};
@@ -2004,13 +2009,24 @@ static int solve_tol(fpsi_ctx *c, double rtol, int max_sweeps, int use_guess, fp
     s.sweeps = t.sweeps;
     rcur = t.rfinal;
   } else {
+    const bool turb = scalars && c->kp.turb;
+    const int ns = turb && dataflow(c) && !c->tiled() ? c->nscal() : 0;
+    int srem = ns ? c->prm.sgs_iter : 0;  // scalar sweeps still to ride along
     while (rcur > rtol * s.r0 && s.sweeps < max_sweeps) {
       // linres checks the sweeps' error word
       const int k = std::max(1, std::min(check_from - s.sweeps, max_sweeps - s.sweeps));
-      if (int rc = enqueue_sweeps(c, k, 0, false)) return rc;
+      const int kf = std::min(k, srem);
+      if (kf)
+        if (int rc = enqueue_sweeps(c, kf, ns, false)) return rc;
+      if (k > kf)
+        if (int rc = enqueue_sweeps(c, k - kf, 0, false)) return rc;
+      srem -= kf;
       s.sweeps += k;
       if (int rc = linres(c, &rcur)) return rc;
     }
+    if (srem) enqueue_sa_sweeps(c, 0, ns, srem);
+    if (turb && !ns)
+      if (int rc = enqueue_sa_solve(c, -1, c->prm.sgs_iter)) return rc;
   }
   c->tol_sweeps_last = s.sweeps;
   s.rfinal = rcur;
@@ -2046,10 +2062,6 @@ extern "C" int fpsi_newton_adaptive(fpsi_ctx *c, int nsteps, double rtol, int ma
     if (hist) norms_from(o, &hist[s]);
     if (o[0] + o[1] + o[2] != o[0] + o[1] + o[2]) return fail(c, FPSI_ERR_NAN, "NAN residual at Newton step " + std::to_string(s));
     if (lin) lin[s] = st;
-    // turbulence scalars: the reference's own scalar solves (under line SGS
-    // they ran in the fluid's launches)
-    if (c->kp.turb && !c->line_order())
-      if (int rc = enqueue_sa_solve(c, -1, c->prm.sgs_iter)) return rc;
     launch_update(c->g, c->kp, c->s, c->stream);
     if (c->kp.turb) launch_sa_update(c->g, c->kp, c->s, c->stream);
   }

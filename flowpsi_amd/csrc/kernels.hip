@@ -1247,7 +1247,7 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
     int nb[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)cc * MAXR + j];
-    F mrow[MAXR][5], a5[5];
+    F mrow[MAXR][5];
     double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
@@ -1256,8 +1256,11 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
+    // row i of the diagonal factors goes to LDS before the wait: no registers held across it
+    if (q < CPW) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) a5[k] = lu[(size_t)cc * 25 + 5 * i + k];
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = lu[(size_t)cc * 25 + 5 * i + k];
+    }
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j >= nup) : (j < nup);
@@ -1319,8 +1322,6 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
         if (BACK || j >= nup) xw[j * 5 + i] = x1[j];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
     }
     wave_sync();
 #pragma unroll
@@ -1434,7 +1435,7 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
     int nb[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.nbr_pad[(size_t)cc * MAXR + j];
-    F mrow[MAXR][5], a5[5];
+    F mrow[MAXR][5];
     double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
@@ -1443,8 +1444,11 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
+    // row i of the diagonal factors goes to LDS before the wait: no registers held across it
+    if (q < CPW) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) a5[k] = lu[(size_t)cc * 25 + 5 * i + k];
+      for (int k = 0; k < 5; ++k) aw[5 * i + k] = lu[(size_t)cc * 25 + 5 * i + k];
+    }
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j >= nup) : (j < nup);
@@ -1493,8 +1497,6 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
         if (BACK || j >= nup) xw[j * 5 + i] = x1[j];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
     }
     wave_sync();
 #pragma unroll
@@ -1601,7 +1603,6 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)cc * MAXR + j];
     F mrow[MAXR][5];
-    DG a5[5];
     double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
@@ -1612,6 +1613,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
+    DG a5[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) a5[k] = diag[(size_t)cc * 25 + 5 * i + k];
     // values this half-sweep does not write: upper (forward) / lower (backward), and pads
@@ -1771,8 +1773,16 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
   const int gbase = 5 * (q < CPW ? q : 0);
   __shared__ double xl[CPW * MAXR * 5];
   __shared__ DG al[CPW * 25];
+  // With FP64 blocks the diagonal row and the scalar row coefficients wait
+  // for the dependencies in LDS rather than registers (no spills at 3
+  // waves per SIMD); with FP32 blocks registers are free and the LDS
+  // stores would wait for the loads before the polls are issued.
+  constexpr bool STAGE = sizeof(F) == 8;
+  // scalar i's row coefficients of the cell (lanes i < ns) when staged
+  __shared__ double sl[CPW * 5 * MAXR];
   double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
   DG *aw = al + (q < CPW ? q : 0) * 25;
+  double *sw = sl + ((q < CPW ? q : 0) * 5 + i) * MAXR;
   const double *Xs = BACK ? Xb : Xa;  // values this half-sweep does not write
   double *Xt = BACK ? Xa : Xb;        // the half-sweep's results, polled
   const double sent = __longlong_as_double((long long)X_SENT);
@@ -1792,7 +1802,6 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) nb[j] = g.lnbr_pad[(size_t)cc * MAXR + j];
     F mrow[MAXR][5];
-    DG a5[5];
     double x1[MAXR];
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
@@ -1803,8 +1812,16 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
 #pragma unroll
       for (int jj = 0; jj < 5; ++jj) mrow[j][jj] = M[i + 5 * jj];
     }
+    DG a5[5];
+    if constexpr (STAGE) {  // row i of the diagonal factors into LDS before the wait
+      if (q < CPW) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) a5[k] = diag[(size_t)cc * 25 + 5 * i + k];
+        for (int k = 0; k < 5; ++k) aw[5 * i + k] = diag[(size_t)cc * 25 + 5 * i + k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a5[k] = diag[(size_t)cc * 25 + 5 * i + k];
+    }
 #pragma unroll
     for (int j = 0; j < MAXR; ++j) {
       const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
@@ -1826,7 +1843,9 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
       for (int j = 0; j < MAXR; ++j) {
         int jr = j + nup;
         if (jr >= len) jr -= len;
-        sco[j] = j < len ? soff[jr] : 0.0;
+        const double co = j < len ? soff[jr] : 0.0;
+        if constexpr (STAGE) sw[j] = co;
+        else sco[j] = co;
         const bool pre = j < len && (BACK ? (j < nlow) : (j >= nlow));
         sxv[j] = pre ? ldx<false>(sxs + nb[j]) : 0.0;
       }
@@ -1834,7 +1853,7 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
       if (BACK) {
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
-          if (j < nlow) sres -= sxv[j] * sco[j];
+          if (j < nlow) sres -= sxv[j] * (STAGE ? sw[j] : sco[j]);
       }
     }
     if constexpr (BACK) {  // the lower terms are final since the forward sweep
@@ -1893,8 +1912,10 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
         if (!BACK || j >= nlow) xw[j * 5 + i] = x1[j];
+      if constexpr (!STAGE) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+        for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
+      }
     }
     wave_sync();
 #pragma unroll
@@ -1926,7 +1947,7 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
     if (sact) {
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
-        if (j < len && (!BACK || j >= nlow)) sres -= sxv[j] * sco[j];
+        if (j < len && (!BACK || j >= nlow)) sres -= sxv[j] * (STAGE ? sw[j] : sco[j]);
       for (int j = MAXR; j < len; ++j) {
         int jr = j + nup;
         if (jr >= len) jr -= len;
@@ -3253,12 +3274,22 @@ __global__ void k_b_stats(int nc, const F *__restrict__ B, double *__restrict__ 
 }
 
 // out[i] = sum_p part[p][i] in block order (one thread per i)
-__global__ void k_sum_parts_d(int nparts, int len, const double *__restrict__ part, double *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
+// out[i] = sum_p part[p][i] for i < len, one workgroup per component:
+// thread t sums parts t, t + TPB, ... in order, then a fixed-order tree
+// over the workgroup, so the result does not depend on timing
+__global__ void __launch_bounds__(TPB) k_sum_parts_d(int nparts, int len, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[TPB];
+  const int i = blockIdx.x;
   double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * len + i];
-  out[i] = s;
+  for (int p = threadIdx.x; p < nparts; p += TPB) s += part[(size_t)p * len + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = sh[0];
 }
 
 // D x of one cell: from the packed pivoted LU factors (P D = L U) under
@@ -3416,7 +3447,7 @@ template <typename F, bool RAWD>
 static void linres_t(const DevGrid &g, DevState &s, double *parts, double *bsum, double *rout, hipStream_t st) {
   const int nb = nblk(g.nc);
   hipLaunchKernelGGL(k_b_stats<F>, dim3(nb), dim3(TPB), 0, st, g.nc, (const F *)s.B, parts);
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(5), dim3(TPB), 0, st, nb, 5, parts, bsum);
   hipLaunchKernelGGL((k_linres<F, RAWD>), dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum, (const F *)s.off,
                      (const F *)s.lu, (const int *)s.piv, (const F *)s.B, (const double *)s.X, parts, rout);
 }
@@ -3430,7 +3461,7 @@ void launch_linres(const DevGrid &g, int fp32, int rawd, DevState &s, double *pa
     if (rawd) linres_t<double, true>(g, s, parts, bsum, rout, st);
     else linres_t<double, false>(g, s, parts, bsum, rout, st);
   }
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nblk(g.nc), 1, parts, out);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(TPB), 0, st, nblk(g.nc), 1, parts, out);
 }
 
 int lsgs_parts(const DevGrid &g) { return (int)(((long)g.nc * 5 + TPB - 1) / TPB); }
@@ -3490,7 +3521,7 @@ static void linres_pair_t(const DevGrid &g, DevState &s, const double *X0, const
                           double *bsum, hipStream_t st) {
   const int nb = nblk(g.nc);
   hipLaunchKernelGGL(k_b_stats<F>, dim3(nb), dim3(TPB), 0, st, g.nc, (const F *)s.B, parts);
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nb, 5, parts, bsum);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(5), dim3(TPB), 0, st, nb, 5, parts, bsum);
   hipLaunchKernelGGL((k_linres_pair<F, RAWD>), dim3(nb), dim3(TPB), 0, st, g, (const double *)bsum,
                      (const F *)s.off, (const F *)s.lu, (const int *)s.piv, (const F *)s.B, X0, D, parts);
 }
@@ -3503,7 +3534,7 @@ void launch_linres_pair(const DevGrid &g, int fp32, int rawd, DevState &s, const
     if (rawd) linres_pair_t<double, true>(g, s, X0, D, parts, bsum, st);
     else linres_pair_t<double, false>(g, s, X0, D, parts, bsum, st);
   }
-  hipLaunchKernelGGL(k_sum_parts_d, dim3(1), dim3(64), 0, st, nblk(g.nc), 3, parts, out);
+  hipLaunchKernelGGL(k_sum_parts_d, dim3(3), dim3(TPB), 0, st, nblk(g.nc), 3, parts, out);
 }
 void launch_sub_d(long n, const double *a, const double *b, double *d, hipStream_t st) {
   hipLaunchKernelGGL(k_sub_d, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, a, b, d);

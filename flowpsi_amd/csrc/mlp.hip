@@ -194,7 +194,26 @@ __global__ void __launch_bounds__(256) k_wgrad(int N, const float *__restrict__ 
   if (lane < 16) pb[o] = ((bsum + q1) + q2) + q3;
 }
 
-// sum partials in block order: out[i] = sum_p part[p][i]
+// out[i] = sum_p part[p][i] for a few components with many partials, one
+// workgroup per component: thread t sums parts t, t + TPB, ... in order,
+// then a fixed-order tree over the workgroup
+__global__ void __launch_bounds__(256) k_sum_parts_wg(int nparts, int len, const float *__restrict__ part,
+                                                      float *__restrict__ out) {
+  __shared__ float sh[256];
+  const int i = blockIdx.x;
+  float s = 0.f;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += part[(size_t)p * len + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = sh[0];
+}
+
+// sum partials in block order: out[i] = sum_p part[p][i] (one thread per
+// component, for many components with few partials each)
 __global__ void k_sum_parts(int nparts, int len, const float *__restrict__ part, float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
@@ -481,7 +500,7 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
   const int gb = nblk(N, 64);
   const int np = mlp_parts(N);
   hipLaunchKernelGGL(k_loss, dim3(nblk(N)), dim3(TPB), 0, st, N, m.out, m.y, m.dout, m.lpart);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, nblk(N), 1, m.lpart, m.loss);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(1), dim3(256), 0, st, nblk(N), 1, m.lpart, m.loss);
   // transposed bf16 weights for the input gradients
   hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk(64 * 128)), dim3(TPB), 0, st, 128, m.wb + L.w1, m.wt1);
   hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk(64 * 128)), dim3(TPB), 0, st, 128, m.wb + L.w2, m.wt2);
@@ -489,7 +508,7 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
   hipLaunchKernelGGL((k_wgrad<64, 16>), dim3(np), dim3(256), 0, st, N, m.dout, 32, m.h2, 64,
                      (const float *)nullptr, m.wpart, m.bpart);
   hipLaunchKernelGGL(k_sum_parts, dim3(nblk(16 * 64)), dim3(TPB), 0, st, np, 16 * 64, m.wpart, m.grad + L.w3);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, np, 16, m.bpart, m.grad + L.b3);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(16), dim3(256), 0, st, np, 16, m.bpart, m.grad + L.b3);
   // dh2 = W3^T dout: a 64-wide output from the 16-wide gradient, done as a
   // dense layer over the transposed (zero-padded) W3
   hipLaunchKernelGGL(k_w3t, dim3(nblk(64 * 32)), dim3(TPB), 0, st, m.wb + L.w3, m.wt3);
@@ -499,7 +518,7 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
   // layer 2
   hipLaunchKernelGGL((k_wgrad<128, 64>), dim3(np), dim3(256), 0, st, N, m.g2, 64, m.h1, 64, m.a2, m.wpart, m.bpart);
   hipLaunchKernelGGL(k_sum_parts, dim3(nblk(64 * 128)), dim3(TPB), 0, st, np, 64 * 128, m.wpart, m.grad + L.w2);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, np, 64, m.bpart, m.grad + L.b2);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(64), dim3(256), 0, st, np, 64, m.bpart, m.grad + L.b2);
   hipLaunchKernelGGL(k_dense_bwd_in<128>, dim3(gb), dim3(256), 0, st, N, m.g2, m.wt2, m.dcat);
   hipLaunchKernelGGL(k_split, dim3(nblk((long)N * 64)), dim3(TPB), 0, st, N, m.dcat, m.g1, m.da);
   hipLaunchKernelGGL(k_agg_bwd, dim3(nblk((long)N * 16)), dim3(TPB), 0, st, N, m.nb_off, m.nb_idx, m.da, m.g1);
@@ -507,7 +526,7 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
   // layer 1
   hipLaunchKernelGGL((k_wgrad<128, 64>), dim3(np), dim3(256), 0, st, N, m.g1, 64, m.h0, 64, m.a1, m.wpart, m.bpart);
   hipLaunchKernelGGL(k_sum_parts, dim3(nblk(64 * 128)), dim3(TPB), 0, st, np, 64 * 128, m.wpart, m.grad + L.w1);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, np, 64, m.bpart, m.grad + L.b1);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(64), dim3(256), 0, st, np, 64, m.bpart, m.grad + L.b1);
   hipLaunchKernelGGL(k_dense_bwd_in<128>, dim3(gb), dim3(256), 0, st, N, m.g1, m.wt1, m.dcat);
   hipLaunchKernelGGL(k_split, dim3(nblk((long)N * 64)), dim3(TPB), 0, st, N, m.dcat, m.g0, m.da);
   hipLaunchKernelGGL(k_agg_bwd, dim3(nblk((long)N * 16)), dim3(TPB), 0, st, N, m.nb_off, m.nb_idx, m.da, m.g0);
@@ -516,7 +535,7 @@ void mlp_backward(const MlpBufs &m, int N, hipStream_t st) {
   hipLaunchKernelGGL((k_wgrad<32, 64>), dim3(np), dim3(256), 0, st, N, m.g0, 64, m.x, 32,
                      (const float *)nullptr, m.wpart, m.bpart);
   hipLaunchKernelGGL(k_sum_parts, dim3(nblk(64 * 32)), dim3(TPB), 0, st, np, 64 * 32, m.wpart, m.grad + L.w0);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, np, 64, m.bpart, m.grad + L.b0);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(64), dim3(256), 0, st, np, 64, m.bpart, m.grad + L.b0);
 }
 
 }  // namespace fpsi
@@ -535,7 +554,7 @@ void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const doub
 }
 void mlp_feature_stats(const MlpBufs &m, int N, float *out32, hipStream_t st) {
   hipLaunchKernelGGL(k_feat_stats, dim3(nblk(N)), dim3(TPB), 0, st, N, m.raw, m.lpart);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, nblk(N), 32, m.lpart, out32);
+  hipLaunchKernelGGL(k_sum_parts_wg, dim3(32), dim3(256), 0, st, nblk(N), 32, m.lpart, out32);
 }
 void mlp_normalise(const MlpBufs &m, int N, float *x, hipStream_t st) {
   hipLaunchKernelGGL(k_feat, dim3(nblk(N)), dim3(TPB), 0, st, N, m.raw, m.geo, m.norm, x);

@@ -119,3 +119,22 @@ def test_bad_guess_is_never_worse_than_plain_start():
     assert st["rfinal"] <= 1e-3 * st["r0"]
     if not st["used_guess"]:
         assert st["sweeps"] == st0["sweeps"] and st["rfinal"] == st0["rfinal"]
+
+
+def test_guess_gain_on_held_out_steps_of_the_hybrid_grid():
+    """converge.guess_gain on a small prism-layer + hex grid (SA, block SGS,
+    dataflow sweep with the scalar riding in the fluid's sweeps): both paths
+    reach the tolerance on every held-out step, the kept guesses lower the
+    start's residual, and the report adds up."""
+    from flowpsi_amd import converge
+    g, q, prm = workloads.c4_hybrid(nx=10, nz=10, n_prism=8, n_hex=8)
+    r = converge.guess_gain(g, q, dict(prm, precision="fp32", sweep="persistent"), transient=3, harvest_steps=2,
+                            epochs=60, eval_steps=3, rtol=1e-3, max_sweeps=60)
+    print(r)
+    for path in ("learned", "plain"):
+        assert all(r[path]["reached"])
+        assert r[path]["total_sweeps"] == sum(r[path]["sweeps"])
+    for used, ratio in zip(r["learned"]["used_guess"], r["learned"]["rguess_over_r0"]):
+        assert (ratio < 1.0) == bool(used)
+    assert r["sweeps_saved"] == r["plain"]["total_sweeps"] - r["learned"]["total_sweeps"]
+    assert r["training"]["loss_after"] < r["training"]["loss_before"]
