@@ -186,7 +186,7 @@ def compare(g, q, prm, rtol=0.1, max_sweeps=20, target=1e-8, max_steps=3000, ref
     return out
 
 
-def guess_gain(g, q, prm, transient=5, harvest_steps=2, target_sweeps=30, epochs=100, lr=2e-3, eval_steps=4,
+def guess_gain(g, q, prm, transient=5, harvest_steps=4, target_sweeps=30, epochs=300, lr=5e-3, eval_steps=4,
                rtol=1e-3, max_sweeps=60, seed=1, log=None):
     """What the trained network gains on Newton steps it was not trained on.
 

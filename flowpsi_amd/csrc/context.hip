@@ -1957,8 +1957,7 @@ static int mlp_guess(fpsi_ctx *c) {
   if (int rc = mlp_set_norm(c)) return rc;
   mlp_normalise(m, nc, c->mlp_xcur, c->stream);
   m.x = c->mlp_xcur;
-  mlp_forward(m, nc, c->stream);
-  mlp_apply_guess(m, nc, c->s.X, c->stream);
+  mlp_infer_guess(m, nc, c->s.X, c->stream);
   HIPCHK(c, hipGetLastError());
   return 0;
 }

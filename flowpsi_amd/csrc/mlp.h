@@ -30,6 +30,9 @@ struct MlpBufs {
 };
 
 void mlp_forward(const MlpBufs &m, int N, hipStream_t st);
+// the guess X = X0 + sd * out[0:5] from the normalised features m.x (the
+// forward pass of mlp_forward, fused; h0 and h1 are kept, nothing after)
+void mlp_infer_guess(const MlpBufs &m, int N, double *X, hipStream_t st);
 void mlp_backward(const MlpBufs &m, int N, hipStream_t st);
 // raw features of the current system: X0 = X, the previous Newton update
 // Xprev [N][5], the diagonal factors lu (real_fj packed pivoted LU with piv,
@@ -40,7 +43,6 @@ void mlp_features(const MlpBufs &m, int N, int fp32, const double *X, const doub
 void mlp_feature_stats(const MlpBufs &m, int N, float *out32, hipStream_t st);
 void mlp_normalise(const MlpBufs &m, int N, float *x, hipStream_t st);
 void mlp_target(const MlpBufs &m, int N, const double *Xs, float *y, hipStream_t st);
-void mlp_apply_guess(const MlpBufs &m, int N, double *X, hipStream_t st);
 void mlp_adam(const MlpBufs &m, float lr, int t, hipStream_t st);
 void mlp_refresh_bf16(const MlpBufs &m, hipStream_t st);
 

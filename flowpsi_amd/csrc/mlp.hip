@@ -101,6 +101,117 @@ __global__ void __launch_bounds__(256) k_dense_fwd(int N, const float *__restric
   }
 }
 
+// One message-passing hop of the forward pass for the guess (inference):
+// h_out = relu(W [h_in; mean of h_in over face neighbours] + b), with the
+// neighbour mean formed in registers in k_agg's order and arithmetic, so
+// the result is bitwise that of k_agg + k_dense_fwd<128, 64, true> without
+// the mean's round trip through memory.  FINAL keeps the hop's output on
+// chip: it goes through LDS (one 16 x 64 tile per wave) into the output
+// layer out = W3 h + b3 (bitwise k_dense_fwd<64, 16, false>), and the
+// guess X = X0 + sd * out[0:5] is written.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool FINAL>
+__global__ void __launch_bounds__(256) k_hop_fwd(int N, const int *__restrict__ off, const int *__restrict__ nbr,
+                                                 const float *__restrict__ hin, const unsigned short *__restrict__ W,
+                                                 const float *__restrict__ b, float *__restrict__ hout,
+                                                 const unsigned short *__restrict__ W3, const float *__restrict__ b3,
+                                                 const float *__restrict__ raw, const float *__restrict__ norm,
+                                                 double *__restrict__ X) {
+  __shared__ unsigned short sW[64 * 128];
+  __shared__ unsigned short sW3[FINAL ? 16 * 64 : 1];
+  __shared__ float sh[FINAL ? 4 * 16 * 64 : 1];
+  for (int i = threadIdx.x; i < 64 * 128; i += blockDim.x) sW[i] = W[i];
+  if constexpr (FINAL)
+    for (int i = threadIdx.x; i < 16 * 64; i += blockDim.x) sW3[i] = W3[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row0 = (blockIdx.x * 4 + wave) * 16;
+  if (row0 >= N) return;
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  const int cell = min(row0 + r, N - 1);
+  const int j0 = off[cell], j1 = off[cell + 1];
+  const float inv = j1 > j0 ? 1.0f / (float)(j1 - j0) : 0.f;
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 128; kk += 32) {
+    const int k = kk + kq;
+    float v[8];
+    if (k < 64) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = hin[(size_t)cell * 64 + k + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = 0.f;
+      for (int j = j0; j < j1; ++j) {
+        const float *hn = hin + (size_t)nbr[j] * 64 + (k - 64);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] += hn[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] *= inv;
+    }
+    const bf16x8 a = pack8(v);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bf16x8 bb = pack8b(&sW[(t * 16 + r) * 128 + k]);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[t], 0, 0, 0);
+    }
+  }
+  if constexpr (!FINAL) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int o = t * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = row0 + (lane >> 4) * 4 + i;
+        if (row < N) {
+          float y = acc[t][i] + b[o];
+          y = y > 0.f ? y : 0.f;
+          hout[(size_t)row * 64 + o] = y;
+        }
+      }
+    }
+  } else {
+    float *tile = sh + wave * 16 * 64;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int o = t * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float y = acc[t][i] + b[o];
+        tile[((lane >> 4) * 4 + i) * 64 + o] = y > 0.f ? y : 0.f;
+      }
+    }
+    wave_lds_sync();
+    f32x4 o3 = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 64; kk += 32) {
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = tile[r * 64 + kk + kq + i];
+      const bf16x8 a = pack8(v);
+      const bf16x8 bb = pack8b(&sW3[r * 64 + kk + kq]);
+      o3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, o3, 0, 0, 0);
+    }
+    if (r < 5) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = row0 + (lane >> 4) * 4 + i;
+        if (row < N) {
+          const float out = o3[i] + b3[r];
+          X[(size_t)row * 5 + r] = (double)raw[(size_t)row * 32 + r] + (double)norm[21 + r] * (double)out;
+        }
+      }
+    }
+  }
+}
+
 // input gradient: din[N][K] = g[N][64] W[64][K] (W given transposed, WT [K][64])
 template <int K>
 __global__ void __launch_bounds__(256) k_dense_bwd_in(int N, const float *__restrict__ g,
@@ -450,16 +561,6 @@ __global__ void k_target(int N, const double *__restrict__ Xs, const float *__re
         k < 5 ? (float)((Xs[(size_t)c * 5 + k] - (double)raw[(size_t)c * 32 + k]) / norm[21 + k]) : 0.f;
 }
 
-// X_guess = X0 + sd * out
-__global__ void k_apply_guess(int N, const float *__restrict__ out, const float *__restrict__ raw,
-                              const float *__restrict__ norm, double *__restrict__ X) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-#pragma unroll
-  for (int k = 0; k < 5; ++k)
-    X[(size_t)c * 5 + k] = (double)raw[(size_t)c * 32 + k] + (double)norm[21 + k] * (double)out[(size_t)c * 16 + k];
-}
-
 // ---------------------------------------------------------------------------
 // host-side orchestration
 // ---------------------------------------------------------------------------
@@ -492,6 +593,22 @@ void mlp_forward(const MlpBufs &m, int N, hipStream_t st) {
                      m.w + L.b2, m.h2);
   hipLaunchKernelGGL((k_dense_fwd<64, 16, false>), dim3(gb), dim3(256), 0, st, N, m.h2, 64, (const float *)nullptr,
                      m.wb + L.w3, m.w + L.b3, m.out);
+}
+
+// the guess X = X0 + sd * out[0:5] from the normalised features m.x: the
+// forward pass of mlp_forward, bitwise, with the neighbour means in
+// registers and the last hop, the output layer and the guess fused
+void mlp_infer_guess(const MlpBufs &m, int N, double *X, hipStream_t st) {
+  const MlpLayout L = mlp_layout();
+  const int gb = nblk(N, 64);
+  hipLaunchKernelGGL((k_dense_fwd<32, 64, true>), dim3(gb), dim3(256), 0, st, N, m.x, 32, (const float *)nullptr,
+                     m.wb + L.w0, m.w + L.b0, m.h0);
+  hipLaunchKernelGGL(k_hop_fwd<false>, dim3(gb), dim3(256), 0, st, N, m.nb_off, m.nb_idx, (const float *)m.h0,
+                     m.wb + L.w1, m.w + L.b1, m.h1, (const unsigned short *)nullptr, (const float *)nullptr,
+                     (const float *)nullptr, (const float *)nullptr, (double *)nullptr);
+  hipLaunchKernelGGL(k_hop_fwd<true>, dim3(gb), dim3(256), 0, st, N, m.nb_off, m.nb_idx, (const float *)m.h1,
+                     m.wb + L.w2, m.w + L.b2, (float *)nullptr, m.wb + L.w3, m.w + L.b3, (const float *)m.raw,
+                     (const float *)m.norm, X);
 }
 
 // loss + gradient of all parameters into m.grad (fixed-order reductions)
@@ -561,9 +678,6 @@ void mlp_normalise(const MlpBufs &m, int N, float *x, hipStream_t st) {
 }
 void mlp_target(const MlpBufs &m, int N, const double *Xs, float *y, hipStream_t st) {
   hipLaunchKernelGGL(k_target, dim3(nblk(N)), dim3(TPB), 0, st, N, Xs, m.raw, m.norm, y);
-}
-void mlp_apply_guess(const MlpBufs &m, int N, double *X, hipStream_t st) {
-  hipLaunchKernelGGL(k_apply_guess, dim3(nblk(N)), dim3(TPB), 0, st, N, m.out, m.raw, m.norm, X);
 }
 void mlp_adam(const MlpBufs &m, float lr, int t, hipStream_t st) {
   const MlpLayout L = mlp_layout();
