@@ -385,8 +385,11 @@ typedef struct {
  * previous tolerance solve since fpsi_set_state needed. */
 int fpsi_linear_solve_tol(fpsi_ctx *ctx, double rtol, int max_sweeps, int use_guess, fpsi_lin_stats *st);
 
-/* Steady Newton steps whose linear solves run to a tolerance (see above);
- * hist[nsteps] and lin[nsteps] are nullable. */
+/* Steady Newton steps whose fluid linear solves run to a tolerance (see
+ * above); the turbulence scalars keep their own solves (gauss_seidel_iter
+ * block-SGS sweeps, riding in the fluid's first dataflow launches, or
+ * their line-SGS loops in the fluid's line-SGS launches).  hist[nsteps]
+ * and lin[nsteps] are nullable. */
 int fpsi_newton_adaptive(fpsi_ctx *ctx, int nsteps, double rtol, int max_sweeps, int use_guess,
                          fpsi_norms *hist, fpsi_lin_stats *lin);
 
