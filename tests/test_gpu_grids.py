@@ -132,3 +132,29 @@ def test_adaptive_newton_with_turbulence_and_learned_guess():
         assert l["rfinal"] <= 1e-3 * l["r0"] or l["sweeps"] == 60
     qs = s.get_state()
     assert np.all(np.isfinite(qs)) and np.all(qs[5] >= 1e-15)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_boundary_layer_at_its_wall_normal_resolution(prec):
+    """The line of record's settings (SST, line SGS with the k and omega
+    loops in the flag-free dataflow launches, CFL 100, urelax 0.2) on the
+    boundary-layer grid with its 60 wall-normal cells (first-cell aspect
+    1e3), 12 x 10 columns: the state after 5 Newton steps equals the oracle
+    built with the same real_fj to 1e-10 (FP64) and 1e-13 (FP32 blocks, the
+    same float Jacobians and factors, FP64 residual and update)."""
+    g, q, prm = workloads.c5_bl(nx=12, ny=60, nz=10)
+    prm = dict(prm, precision=prec, sweep="persistent")
+    q_r, h_r = oracle.Oracle(g, prm, _fj(prm)).newton(q, 5)
+    s = Solver(g, prm)
+    assert s.info()["lsgs_dataflow"] == 1
+    s.set_state(q)
+    hist = s.newton(5)
+    q_g = s.get_state()
+    s.close()
+    err = (np.abs(q_g - q_r) / np.abs(q_r).max(axis=1, keepdims=True)).max(axis=1)
+    tot = np.array([h["rtrms"] + h["etrms"] + h["mtrms"] for h in hist])
+    tot_r = h_r[:, 0] + h_r[:, 1] + h_r[:, 2]
+    print(f"\nbl 12x60x10 {prec}: state err per row {np.array2string(err, precision=1)}, "
+          f"drop {np.sqrt(tot[-1] / tot[0]):.3e}")
+    assert err.max() < (1e-10 if prec == "fp64" else 1e-13)
+    assert np.allclose(tot, tot_r, rtol=1e-9)
