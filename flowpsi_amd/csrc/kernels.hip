@@ -1272,9 +1272,10 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
     const int sc = sact ? i : 0;
     const double *soff = sr.ns ? sr.off + (size_t)sc * sr.nnz + m0 : nullptr;
     double *sx = sr.ns ? sr.X + sc * xs : nullptr;
-    double ssum = 0.0, sco[MAXR], sxv[MAXR];
+    double ssum = 0.0, sdv = 0.0, sco[MAXR], sxv[MAXR];
     if (sact) {
       ssum = -sr.B[(size_t)sc * g.nc + cc];
+      sdv = sr.dinv[(size_t)sc * g.nc + cc];  // before the wait: off the chain
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) {
         sco[j] = j < len ? soff[j] : 0.0;
@@ -1318,6 +1319,14 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       const bool post = BACK ? (j < nup) : (j >= nup);
       if (post) x1[j] = ldx<true>(X + (size_t)nb[j] * 5 + i);
     }
+    // the scalars' reached neighbours in the same round of loads
+    if (sact) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = BACK ? (j < nup) : (j >= nup);
+        if (post && j < len) sxv[j] = ldx<true>(sx + nb[j]);
+      }
+    }
     if (q < CPW) {
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
@@ -1338,16 +1347,11 @@ __global__ void __launch_bounds__(64) k_sgs_sweep(DevGrid g, int nchunks, int nw
       }
     if (sact) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j) {
-        const bool post = BACK ? (j < nup) : (j >= nup);
-        if (post && j < len) sxv[j] = ldx<true>(sx + nb[j]);
-      }
-#pragma unroll
       for (int j = 0; j < MAXR; ++j)
         if ((BACK || j >= nup) && j < len) ssum += sxv[j] * sco[j];
       for (int j = MAXR; j < len; ++j) ssum += ldx<true>(sx + g.m_nbr[m0 + j]) * soff[j];
       const double sv = -ssum;
-      stx<true>(sx + c, sr.dinv[(size_t)sc * g.nc + c] * sv);
+      stx<true>(sx + c, sdv * sv);
     }
     double v[5];
 #pragma unroll
@@ -1386,6 +1390,15 @@ constexpr unsigned long long X_SENT = 0x7FF45EE5DEAD0001ull;
 __device__ __forceinline__ bool is_sent(double v) {
   return (unsigned long long)__double_as_longlong(v) == X_SENT;
 }
+
+// The value v, opaque to the optimiser: a poll loop recomputes the
+// addresses it derives from v on every iteration (a few integer ops)
+// instead of hoisting them and holding 64-bit addresses across the loop.
+__device__ __forceinline__ unsigned opaque(unsigned v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ bool opaque(bool b) { return opaque(b ? 1u : 0u) != 0u; }
 
 // The first stage of a flag-free wait: poll one word per lane (p, or
 // nothing when p is null), with a pause between polls, until no lane sees
@@ -1454,12 +1467,16 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
       const bool pre = BACK ? (j >= nup) : (j < nup);
       x1[j] = pre ? ldx<false>(Xs + (size_t)nb[j] * 5 + i) : 0.0;
     }
-    if (!BACK) {  // forward, the upper terms come first
-      if (q < CPW) {
+    // the values the wait does not cover go to LDS now, so only the polled
+    // words are held in registers across the wait
+    if (q < CPW) {
 #pragma unroll
-        for (int j = 0; j < MAXR; ++j)
-          if (j < nup) xw[j * 5 + i] = x1[j];
+      for (int j = 0; j < MAXR; ++j) {
+        const bool pre = BACK ? (j >= nup) : (j < nup);
+        if (pre) xw[j * 5 + i] = x1[j];
       }
+    }
+    if (!BACK) {  // forward, the upper terms come first
       wave_sync();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
@@ -1475,16 +1492,22 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
         if (BACK ? (j < nup) : (j >= nup)) wp = Xt + (size_t)nb[j] * 5 + i;
       if (!wait_witness(wp, spins, spin_limit, err)) return;
     }
+    // every poll issues all MAXR loads at once, with no branch around a load
+    // (so no load waits for the one before it): an entry the wait does not
+    // cover rereads its source-buffer word, which its cell overwrites only
+    // after this one has finished.  Offsets are 32-bit and recomputed on
+    // every poll, so no 64-bit addresses are held.
+    double xv[MAXR];
     while (true) {
-      bool ready = true;
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) {
         const bool post = BACK ? (j < nup) : (j >= nup);
-        if (post) {
-          x1[j] = ldx<true>(Xt + (size_t)nb[j] * 5 + i);
-          ready = ready && !is_sent(x1[j]);
-        }
+        const unsigned o = opaque((unsigned)nb[j] * 5u + (unsigned)i);
+        xv[j] = ldx<true>((opaque(post) ? Xt : Xs) + o);
       }
+      bool ready = true;
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]);
       if (__all(ready)) break;
       if (++spins > spin_limit) {
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1495,8 +1518,10 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
     }
     if (q < CPW) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j)
-        if (BACK || j >= nup) xw[j * 5 + i] = x1[j];
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = BACK ? (j < nup) : (j >= nup);
+        if (post) xw[j * 5 + i] = xv[j];
+      }
     }
     wave_sync();
 #pragma unroll
@@ -1511,16 +1536,16 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
         const double *xn = (post ? (const double *)Xt : Xs) + (size_t)g.m_nbr[m0 + j] * 5;
 #pragma unroll
         for (int jj = 0; jj < 5; ++jj) {
-          double xv = ldx<true>(xn + jj);
-          for (int sp = 0; post && is_sent(xv); ++sp) {
+          double xnv = ldx<true>(xn + jj);
+          for (int sp = 0; post && is_sent(xnv); ++sp) {
             if (sp > spin_limit) {
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              xv = 0.0;
+              xnv = 0.0;
               break;
             }
-            xv = ldx<true>(xn + jj);
+            xnv = ldx<true>(xn + jj);
           }
-          tsum += M[i + 5 * jj] * xv;
+          tsum += M[i + 5 * jj] * xnv;
         }
       }
     double v[5];
@@ -1632,9 +1657,10 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
     const size_t xs = (size_t)g.nc + g.nper + 1;
     const double *soff = ls.mask ? ls.off + (size_t)sc * ls.nnz + m0 : nullptr;
     double *sx = ls.mask ? ls.X + sc * xs : nullptr;
-    double sres = 0.0, sold = 0.0, sco[MAXR], sxv[MAXR];
+    double sres = 0.0, sold = 0.0, sdv = 0.0, sco[MAXR], sxv[MAXR];
     if (sact) {
       sres = ls.B[(size_t)sc * g.nc + cc];
+      sdv = ls.dinv[(size_t)sc * g.nc + cc];  // before the wait: off the chain
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) {
         int jr = j + nup;
@@ -1720,7 +1746,7 @@ __global__ void __launch_bounds__(64) k_lsgs_sweep(DevGrid g, int nchunks, int n
         sres -= ldx<true>(sx + g.m_nbr[m0 + jr]) * soff[jr];
       }
       if (!BACK) ls.xt[(size_t)sc * g.nc + c] = sold;
-      stx<true>(sx + c, ls.dinv[(size_t)sc * g.nc + c] * sres);
+      stx<true>(sx + c, sdv * sres);
     }
     double v[5];
 #pragma unroll
@@ -1780,9 +1806,12 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
   constexpr bool STAGE = sizeof(F) == 8;
   // scalar i's row coefficients of the cell (lanes i < ns) when staged
   __shared__ double sl[CPW * 5 * MAXR];
+  // scalar i's neighbour values of the cell (lanes i < ns), held in LDS across the wait
+  __shared__ double svl[CPW * 5 * MAXR];
   double *xw = xl + (q < CPW ? q : 0) * (MAXR * 5);
   DG *aw = al + (q < CPW ? q : 0) * 25;
   double *sw = sl + ((q < CPW ? q : 0) * 5 + i) * MAXR;
+  double *svw = svl + ((q < CPW ? q : 0) * 5 + i) * MAXR;
   const double *Xs = BACK ? Xb : Xa;  // values this half-sweep does not write
   double *Xt = BACK ? Xa : Xb;        // the half-sweep's results, polled
   const double sent = __longlong_as_double((long long)X_SENT);
@@ -1836,9 +1865,10 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
     double *sxb = ls.mask ? sXb + sc * xs : nullptr;
     const double *sxs = BACK ? sxb : sxa;
     double *sxt = BACK ? sxa : sxb;
-    double sres = 0.0, sold = 0.0, sco[MAXR], sxv[MAXR];
+    double sres = 0.0, sold = 0.0, sdv = 0.0, sco[MAXR], sxv[MAXR];
     if (sact) {
       sres = ls.B[(size_t)sc * g.nc + cc];
+      sdv = ls.dinv[(size_t)sc * g.nc + cc];  // before the wait: off the chain
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) {
         int jr = j + nup;
@@ -1855,13 +1885,19 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         for (int j = 0; j < MAXR; ++j)
           if (j < nlow) sres -= sxv[j] * (STAGE ? sw[j] : sco[j]);
       }
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) svw[j] = sxv[j];
+    }
+    // the values the wait does not cover go to LDS now, so only the polled
+    // words are held in registers across the wait
+    if (q < CPW) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool pre = BACK ? (j < nlow || j >= len) : (j >= nlow);
+        if (pre) xw[j * 5 + i] = x1[j];
+      }
     }
     if constexpr (BACK) {  // the lower terms are final since the forward sweep
-      if (q < CPW) {
-#pragma unroll
-        for (int j = 0; j < MAXR; ++j)
-          if (j < nlow) xw[j * 5 + i] = x1[j];
-      }
       wave_sync();
 #pragma unroll
       for (int j = 0; j < MAXR; ++j)
@@ -1878,27 +1914,38 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         if (BACK ? (j >= nlow && j < len) : (j < nlow)) wp = Xt + (size_t)nb[j] * 5 + i;
       if (!wait_witness(fluid ? wp : nullptr, spins, spin_limit, err)) return;
     }
+    // every poll issues all its loads at once (entries the wait does not
+    // cover, and lanes without a scalar, read the zero pad row, never a
+    // sentinel): no branch around a load, so no load waits for the one
+    // before it.  Offsets are 32-bit and recomputed on every poll, so no
+    // 64-bit addresses are held across it.
+    const unsigned zpad = (unsigned)(xs - 1) * 5u + (unsigned)i;
+    const double *sbase = BACK ? ls.X : sXb;  // the scalars' target buffers
+    const unsigned sofs = (unsigned)sc * (unsigned)xs, szpad = sofs + (unsigned)(xs - 1);
+    double xv[MAXR], sv[MAXR];
     while (true) {
       bool ready = true;
       if (fluid) {
 #pragma unroll
         for (int j = 0; j < MAXR; ++j) {
           const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
-          if (post) {
-            x1[j] = ldx<true>(Xt + (size_t)nb[j] * 5 + i);
-            ready = ready && !is_sent(x1[j]);
-          }
+          xv[j] = ldx<true>(Xt + opaque(post ? (unsigned)nb[j] * 5u + (unsigned)i : zpad));
         }
       }
-      if (sact) {
+      if (ls.mask) {
 #pragma unroll
         for (int j = 0; j < MAXR; ++j) {
-          const bool post = j < len && (BACK ? (j >= nlow) : (j < nlow));
-          if (post) {
-            sxv[j] = ldx<true>(sxt + nb[j]);
-            ready = ready && !is_sent(sxv[j]);
-          }
+          const bool post = sact && j < len && (BACK ? (j >= nlow) : (j < nlow));
+          sv[j] = ldx<true>(sbase + opaque(post ? sofs + (unsigned)nb[j] : szpad));
         }
+      }
+      if (fluid) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]);
+      }
+      if (ls.mask) {
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j) ready &= !is_sent(sv[j]);
       }
       if (__all(ready)) break;
       if (++spins > spin_limit) {
@@ -1908,10 +1955,12 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
       if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
       __builtin_amdgcn_s_sleep(1);
     }
-    if (q < CPW) {
+    if (q < CPW && fluid) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j)
-        if (!BACK || j >= nlow) xw[j * 5 + i] = x1[j];
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
+        if (post) xw[j * 5 + i] = xv[j];
+      }
       if constexpr (!STAGE) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) aw[5 * i + k] = a5[k];
@@ -1932,22 +1981,25 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         const double *xn = (post ? (const double *)Xt : Xs) + (size_t)g.m_nbr[m0 + jr] * 5;
 #pragma unroll
         for (int jj = 0; jj < 5; ++jj) {
-          double xv = ldx<true>(xn + jj);
-          for (int sp = 0; post && is_sent(xv); ++sp) {
+          double xnv = ldx<true>(xn + jj);
+          for (int sp = 0; post && is_sent(xnv); ++sp) {
             if (sp > spin_limit) {
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              xv = 0.0;
+              xnv = 0.0;
               break;
             }
-            xv = ldx<true>(xn + jj);
+            xnv = ldx<true>(xn + jj);
           }
-          tsum += M[i + 5 * jj] * xv;
+          tsum += M[i + 5 * jj] * xnv;
         }
       }
     if (sact) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j)
-        if (j < len && (!BACK || j >= nlow)) sres -= sxv[j] * (STAGE ? sw[j] : sco[j]);
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = j < len && (BACK ? (j >= nlow) : (j < nlow));
+        const double xj = post ? sv[j] : svw[j];
+        if (j < len && (!BACK || j >= nlow)) sres -= xj * (STAGE ? sw[j] : sco[j]);
+      }
       for (int j = MAXR; j < len; ++j) {
         int jr = j + nup;
         if (jr >= len) jr -= len;
@@ -1965,7 +2017,7 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         sres -= xv * soff[jr];
       }
       if (!BACK) ls.xt[(size_t)sc * g.nc + c] = sold;
-      stx<true>(sxt + c, ls.dinv[(size_t)sc * g.nc + c] * sres);
+      stx<true>(sxt + c, sdv * sres);
       if (!BACK) stx<true>(sxa + c, sent);
     }
     double v[5];
