@@ -1914,39 +1914,32 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
         if (BACK ? (j >= nlow && j < len) : (j < nlow)) wp = Xt + (size_t)nb[j] * 5 + i;
       if (!wait_witness(fluid ? wp : nullptr, spins, spin_limit, err)) return;
     }
-    // every poll issues all its loads at once (entries the wait does not
-    // cover, and lanes without a scalar, read the zero pad row, never a
-    // sentinel): no branch around a load, so no load waits for the one
-    // before it.  Offsets are 32-bit and recomputed on every poll, so no
-    // 64-bit addresses are held across it.
-    const unsigned zpad = (unsigned)(xs - 1) * 5u + (unsigned)i;
-    const double *sbase = BACK ? ls.X : sXb;  // the scalars' target buffers
-    const unsigned sofs = (unsigned)sc * (unsigned)xs, szpad = sofs + (unsigned)(xs - 1);
+    // every poll issues all its loads before it checks any (no load waits
+    // for the one before it); entries the wait does not cover load nothing
+    // and stay zero, never a sentinel.  Offsets are 32-bit and recomputed
+    // on every poll, so no 64-bit addresses are held across it.
+    const double *sbase = ls.mask ? (BACK ? ls.X : sXb) : Xt;  // the scalars' target buffers
+    const unsigned sofs = (unsigned)sc * (unsigned)xs;
     double xv[MAXR], sv[MAXR];
     while (true) {
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        xv[j] = 0.0;
+        sv[j] = 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = fluid && (BACK ? (j >= nlow && j < len) : (j < nlow));
+        if (post) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
+      }
+#pragma unroll
+      for (int j = 0; j < MAXR; ++j) {
+        const bool post = sact && j < len && (BACK ? (j >= nlow) : (j < nlow));
+        if (post) sv[j] = ldx<true>(sbase + opaque(sofs + (unsigned)nb[j]));
+      }
       bool ready = true;
-      if (fluid) {
 #pragma unroll
-        for (int j = 0; j < MAXR; ++j) {
-          const bool post = BACK ? (j >= nlow && j < len) : (j < nlow);
-          xv[j] = ldx<true>(Xt + opaque(post ? (unsigned)nb[j] * 5u + (unsigned)i : zpad));
-        }
-      }
-      if (ls.mask) {
-#pragma unroll
-        for (int j = 0; j < MAXR; ++j) {
-          const bool post = sact && j < len && (BACK ? (j >= nlow) : (j < nlow));
-          sv[j] = ldx<true>(sbase + opaque(post ? sofs + (unsigned)nb[j] : szpad));
-        }
-      }
-      if (fluid) {
-#pragma unroll
-        for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]);
-      }
-      if (ls.mask) {
-#pragma unroll
-        for (int j = 0; j < MAXR; ++j) ready &= !is_sent(sv[j]);
-      }
+      for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]) & !is_sent(sv[j]);
       if (__all(ready)) break;
       if (++spins > spin_limit) {
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
