@@ -1398,7 +1398,6 @@ __device__ __forceinline__ unsigned opaque(unsigned v) {
   asm volatile("" : "+v"(v));
   return v;
 }
-__device__ __forceinline__ bool opaque(bool b) { return opaque(b ? 1u : 0u) != 0u; }
 
 // The first stage of a flag-free wait: poll one word per lane (p, or
 // nothing when p is null), with a pause between polls, until no lane sees
@@ -1492,18 +1491,18 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
         if (BACK ? (j < nup) : (j >= nup)) wp = Xt + (size_t)nb[j] * 5 + i;
       if (!wait_witness(wp, spins, spin_limit, err)) return;
     }
-    // every poll issues all MAXR loads at once, with no branch around a load
-    // (so no load waits for the one before it): an entry the wait does not
-    // cover rereads its source-buffer word, which its cell overwrites only
-    // after this one has finished.  Offsets are 32-bit and recomputed on
-    // every poll, so no 64-bit addresses are held.
+    // every poll issues all its loads before it checks any (no load waits
+    // for the one before it); entries the wait does not cover load nothing
+    // and stay zero, never a sentinel.  Offsets are 32-bit and recomputed
+    // on every poll, so no 64-bit addresses are held across it.
     double xv[MAXR];
     while (true) {
 #pragma unroll
+      for (int j = 0; j < MAXR; ++j) xv[j] = 0.0;
+#pragma unroll
       for (int j = 0; j < MAXR; ++j) {
         const bool post = BACK ? (j < nup) : (j >= nup);
-        const unsigned o = opaque((unsigned)nb[j] * 5u + (unsigned)i);
-        xv[j] = ldx<true>((opaque(post) ? Xt : Xs) + o);
+        if (post) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
       }
       bool ready = true;
 #pragma unroll
