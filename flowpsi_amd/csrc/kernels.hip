@@ -1399,14 +1399,29 @@ __device__ __forceinline__ unsigned opaque(unsigned v) {
   return v;
 }
 
-// The first stage of a flag-free wait: poll one word per lane (p, or
-// nothing when p is null), with a pause between polls, until no lane sees
-// the sentinel.  The word stands in for the lane's whole set of reached
-// neighbour words: the full poll that follows finds the others written, or
-// nearly so, and while the producers are still levels away the waiting
-// waves issue a few short polls instead of full gathers.  spins counts
-// polls towards the caller's spin limit; false: the limit was passed here
-// (*err set) or another wave raised *err.
+// The reached neighbour a wave is likely to wait for longest: the largest
+// (forward) or smallest (backward) candidate row over its lanes; a lane
+// with no candidate passes -1 (forward) or INT_MAX (backward).  Rows are
+// numbered in sweep order, so that row's producer comes last in the order
+// the waves take their chunks.
+template <bool BACK>
+__device__ __forceinline__ int wave_latest(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int w = __shfl_xor(v, o, 64);
+    v = BACK ? (w < v ? w : v) : (w > v ? w : v);
+  }
+  return v;
+}
+
+// The first stage of a flag-free wait: poll one word (p, the same for every
+// lane, or nothing when p is null), with a pause between polls, until it is
+// no longer the sentinel.  The word is the first component of the wave's
+// latest reached neighbour (wave_latest): the full poll that follows finds
+// the others written, or nearly so, and while the producers are still
+// levels away the waiting wave issues one short request per poll instead
+// of full gathers.  spins counts polls towards the caller's spin limit;
+// false: the limit was passed here (*err set) or another wave raised *err.
 __device__ __forceinline__ bool wait_witness(const double *p, int &spins, int spin_limit, int *err) {
   while (true) {
     const bool rdy = p ? !is_sent(ldx<true>(p)) : true;
@@ -1485,25 +1500,28 @@ __global__ void __launch_bounds__(64) k_sgs_flow(DevGrid g, int nchunks, int nwa
     }
     int spins = 0;
     {
-      const double *wp = nullptr;  // the first reached neighbour's word
+      int latest = BACK ? 0x7fffffff : -1;  // over the reached neighbours, pads excluded
 #pragma unroll
-      for (int j = MAXR - 1; j >= 0; --j)
-        if (BACK ? (j < nup) : (j >= nup)) wp = Xt + (size_t)nb[j] * 5 + i;
-      if (!wait_witness(wp, spins, spin_limit, err)) return;
+      for (int j = 0; j < MAXR; ++j)
+        if (BACK ? (j < nup) : (j >= nup && j < len))
+          latest = BACK ? (nb[j] < latest ? nb[j] : latest) : (nb[j] > latest ? nb[j] : latest);
+      latest = wave_latest<BACK>(latest);
+      const bool any = BACK ? latest != 0x7fffffff : latest >= 0;
+      if (!wait_witness(any ? Xt + (size_t)latest * 5 : nullptr, spins, spin_limit, err)) return;
     }
     // every poll issues all its loads before it checks any (no load waits
-    // for the one before it); entries the wait does not cover load nothing
-    // and stay zero, never a sentinel.  Offsets are 32-bit and recomputed
+    // for the one before it) and loads only the words still holding the
+    // sentinel (a word seen written is final); entries the wait does not
+    // cover load nothing and stay zero.  Offsets are 32-bit and recomputed
     // on every poll, so no 64-bit addresses are held across it.
+    const double sent = __longlong_as_double((long long)X_SENT);
     double xv[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) xv[j] = (BACK ? (j < nup) : (j >= nup)) ? sent : 0.0;
     while (true) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j) xv[j] = 0.0;
-#pragma unroll
-      for (int j = 0; j < MAXR; ++j) {
-        const bool post = BACK ? (j < nup) : (j >= nup);
-        if (post) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
-      }
+      for (int j = 0; j < MAXR; ++j)
+        if (is_sent(xv[j])) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
       bool ready = true;
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]);
@@ -1907,35 +1925,35 @@ __global__ void __launch_bounds__(64, 3) k_lsgs_flow(DevGrid g, int nchunks, int
     // the wait: poll the words of the reached neighbours in the target buffers
     int spins = 0;
     {
-      const double *wp = nullptr;  // the first reached neighbour's fluid word
+      int latest = BACK ? 0x7fffffff : -1;  // the fluid's reached neighbours
 #pragma unroll
-      for (int j = MAXR - 1; j >= 0; --j)
-        if (BACK ? (j >= nlow && j < len) : (j < nlow)) wp = Xt + (size_t)nb[j] * 5 + i;
-      if (!wait_witness(fluid ? wp : nullptr, spins, spin_limit, err)) return;
+      for (int j = 0; j < MAXR; ++j)
+        if (BACK ? (j >= nlow && j < len) : (j < nlow))
+          latest = BACK ? (nb[j] < latest ? nb[j] : latest) : (nb[j] > latest ? nb[j] : latest);
+      latest = wave_latest<BACK>(latest);
+      const bool any = fluid && (BACK ? latest != 0x7fffffff : latest >= 0);
+      if (!wait_witness(any ? Xt + (size_t)latest * 5 : nullptr, spins, spin_limit, err)) return;
     }
     // every poll issues all its loads before it checks any (no load waits
-    // for the one before it); entries the wait does not cover load nothing
-    // and stay zero, never a sentinel.  Offsets are 32-bit and recomputed
+    // for the one before it) and loads only the words still holding the
+    // sentinel (a word seen written is final); entries the wait does not
+    // cover load nothing and stay zero.  Offsets are 32-bit and recomputed
     // on every poll, so no 64-bit addresses are held across it.
     const double *sbase = ls.mask ? (BACK ? ls.X : sXb) : Xt;  // the scalars' target buffers
     const unsigned sofs = (unsigned)sc * (unsigned)xs;
     double xv[MAXR], sv[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; ++j) {
+      xv[j] = fluid && (BACK ? (j >= nlow && j < len) : (j < nlow)) ? sent : 0.0;
+      sv[j] = sact && j < len && (BACK ? (j >= nlow) : (j < nlow)) ? sent : 0.0;
+    }
     while (true) {
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j) {
-        xv[j] = 0.0;
-        sv[j] = 0.0;
-      }
+      for (int j = 0; j < MAXR; ++j)
+        if (is_sent(xv[j])) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
 #pragma unroll
-      for (int j = 0; j < MAXR; ++j) {
-        const bool post = fluid && (BACK ? (j >= nlow && j < len) : (j < nlow));
-        if (post) xv[j] = ldx<true>(Xt + opaque((unsigned)nb[j] * 5u + (unsigned)i));
-      }
-#pragma unroll
-      for (int j = 0; j < MAXR; ++j) {
-        const bool post = sact && j < len && (BACK ? (j >= nlow) : (j < nlow));
-        if (post) sv[j] = ldx<true>(sbase + opaque(sofs + (unsigned)nb[j]));
-      }
+      for (int j = 0; j < MAXR; ++j)
+        if (is_sent(sv[j])) sv[j] = ldx<true>(sbase + opaque(sofs + (unsigned)nb[j]));
       bool ready = true;
 #pragma unroll
       for (int j = 0; j < MAXR; ++j) ready &= !is_sent(xv[j]) & !is_sent(sv[j]);
